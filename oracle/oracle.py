@@ -1,0 +1,213 @@
+"""ctypes front end of the CPU oracle (jfem_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, where it is the checker.  The product library
+never imports or calls it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libjfem_oracle.so")
+_ETYPE = {"Tet10": 10, "Tet4": 4, "Hex8": 8}
+_FTYPE = {"Tri6": 6, "Tri3": 3, "Quad4": 4}
+
+_P = C.c_void_p
+_I64 = C.c_int64
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def _load():
+    if not os.path.exists(_LIB_PATH):
+        build()
+    lib = C.CDLL(_LIB_PATH)
+    lib.orc_element_stiffness.restype = C.c_double
+    lib.orc_element_stiffness.argtypes = [C.c_int, _P, C.c_double, C.c_double, _P]
+    lib.orc_element_internal_force.restype = None
+    lib.orc_element_internal_force.argtypes = [C.c_int, _P, _P, C.c_double, C.c_double, _P]
+    lib.orc_surface_load.restype = None
+    lib.orc_surface_load.argtypes = [C.c_int, _P, _P, _P]
+    lib.orc_volume_rule.restype = C.c_int
+    lib.orc_volume_rule.argtypes = [C.c_int, _P, _P]
+    lib.orc_coo_size.restype = _I64
+    lib.orc_coo_size.argtypes = [_I64, C.c_int]
+    lib.orc_assemble_coo.restype = C.c_double
+    lib.orc_assemble_coo.argtypes = [_P, _I64, C.c_int, _P, _P, _P, _P, _P, _P, C.c_int]
+    lib.orc_coo_to_csr.restype = _I64
+    lib.orc_coo_to_csr.argtypes = [_I64, _I64, _P, _P, _P, _P, _P, _P, _P]
+    lib.orc_load_vector.restype = None
+    lib.orc_load_vector.argtypes = [_I64, _P, _I64, C.c_int, _P, _P, _P]
+    lib.orc_spmv.restype = None
+    lib.orc_spmv.argtypes = [_I64, _P, _P, _P, _P, _P, C.c_int]
+    lib.orc_solve_direct.restype = C.c_int
+    lib.orc_solve_direct.argtypes = [_I64, _P, _P, _P, _P, _P, _P, _P, _P, _P]
+    lib.orc_pcg_bj.restype = C.c_int
+    lib.orc_pcg_bj.argtypes = [_I64, _P, _P, _P, _P, _P, _P, C.c_double, C.c_int, _P, _P, C.c_int]
+    lib.orc_node_to_elem.restype = None
+    lib.orc_node_to_elem.argtypes = [_I64, _I64, C.c_int, _P, _P, _P]
+    lib.orc_matfree_apply.restype = None
+    lib.orc_matfree_apply.argtypes = [_I64, _P, _I64, C.c_int, _P, _P, _P, _P, _P, _P]
+    return lib
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = _load()
+    return _lib
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i64(a):
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
+def volume_rule(etype: str):
+    w = np.zeros(8)
+    xi = np.zeros((8, 3))
+    n = lib().orc_volume_rule(_ETYPE[etype], _p(w), _p(xi))
+    return w[:n].copy(), xi[:n].copy()
+
+
+def element_stiffness(etype: str, Xe, E: float, nu: float):
+    Xe = _f64(Xe)
+    nd = 3 * _ETYPE[etype]
+    Ke = np.zeros((nd, nd))
+    det = lib().orc_element_stiffness(_ETYPE[etype], _p(Xe), E, nu, _p(Ke))
+    return Ke, det
+
+
+def element_internal_force(etype: str, Xe, ue, E: float, nu: float):
+    Xe, ue = _f64(Xe), _f64(ue)
+    fe = np.zeros(3 * _ETYPE[etype])
+    lib().orc_element_internal_force(_ETYPE[etype], _p(Xe), _p(ue), E, nu, _p(fe))
+    return fe
+
+
+def surface_load(ftype: str, Xf, t):
+    Xf, t = _f64(Xf), _f64(t)
+    fe = np.zeros(3 * _FTYPE[ftype])
+    lib().orc_surface_load(_FTYPE[ftype], _p(Xf), _p(t), _p(fe))
+    return fe
+
+
+def assemble_csr(mesh, nthreads: int = 0):
+    """K in CSR with sparse(COO) semantics: (rowptr, colind, vals), int64/f64."""
+    et = _ETYPE[mesh.etype]
+    X, conn = _f64(mesh.X), _i64(mesh.conn)
+    E, nu = _f64(mesh.E), _f64(mesh.nu)
+    m = int(lib().orc_coo_size(mesh.n_el, et))
+    I = np.empty(m, np.int64)
+    J = np.empty(m, np.int64)
+    V = np.empty(m, np.float64)
+    min_det = lib().orc_assemble_coo(_p(X), mesh.n_el, et, _p(conn), _p(E), _p(nu),
+                                     _p(I), _p(J), _p(V), nthreads)
+    if not min_det > 0:
+        raise ValueError(f"inverted element (min detJ = {min_det})")
+    n = mesh.n_dofs
+    rowptr = np.empty(n + 1, np.int64)
+    colind = np.empty(m, np.int64)
+    vals = np.empty(m, np.float64)
+    perm = np.empty(m, np.int64)
+    nnz = int(lib().orc_coo_to_csr(n, m, _p(I), _p(J), _p(V), _p(rowptr), _p(colind), _p(vals), _p(perm)))
+    del I, J, V, perm
+    return rowptr, colind[:nnz].copy(), vals[:nnz].copy()
+
+
+def load_vector(mesh):
+    f = np.zeros(mesh.n_dofs)
+    faces = _i64(mesh.faces)
+    tr = _f64(mesh.face_traction())
+    lib().orc_load_vector(mesh.n_dofs, _p(_f64(mesh.X)), faces.shape[0], _FTYPE[mesh.ftype],
+                          _p(faces), _p(tr), _p(f))
+    return f
+
+
+def spmv(csr, x, nthreads: int = 0):
+    rowptr, colind, vals = csr
+    x = _f64(x)
+    y = np.empty(rowptr.size - 1)
+    lib().orc_spmv(rowptr.size - 1, _p(rowptr), _p(colind), _p(vals), _p(x), _p(y), nthreads)
+    return y
+
+
+def solve_direct(csr, fixed, g, f):
+    rowptr, colind, vals = csr
+    n = rowptr.size - 1
+    fixed = np.ascontiguousarray(fixed, dtype=np.uint8)
+    g, f = _f64(g), _f64(f)
+    u = np.zeros(n)
+    la = np.zeros(n)
+    bw = C.c_int64(0)
+    rc = lib().orc_solve_direct(n, _p(rowptr), _p(colind), _p(vals), _p(fixed), _p(g), _p(f),
+                                _p(u), _p(la), C.byref(bw))
+    if rc != 0:
+        raise RuntimeError(f"oracle direct solve failed (rc={rc})")
+    return u, la, int(bw.value)
+
+
+def pcg_bj(csr, fixed, b, rtol=1e-8, maxit=100000, x0=None, nthreads: int = 0):
+    rowptr, colind, vals = csr
+    n = rowptr.size - 1
+    fixed = np.ascontiguousarray(fixed, dtype=np.uint8)
+    x = np.zeros(n) if x0 is None else _f64(x0).copy()
+    it = C.c_int(0)
+    rel = C.c_double(0)
+    rc = lib().orc_pcg_bj(n, _p(rowptr), _p(colind), _p(vals), _p(fixed), _p(_f64(b)), _p(x),
+                          rtol, maxit, C.byref(it), C.byref(rel), nthreads)
+    return x, int(it.value), float(rel.value), rc
+
+
+def node_to_elem(mesh):
+    ptr = np.empty(mesh.n_nodes + 1, np.int64)
+    data = np.empty(mesh.n_el * mesh.conn.shape[1], np.int64)
+    lib().orc_node_to_elem(mesh.n_nodes, mesh.n_el, mesh.conn.shape[1], _p(_i64(mesh.conn)), _p(ptr), _p(data))
+    return ptr, data
+
+
+def matfree_apply(mesh, u, fixed=None):
+    y = np.empty(mesh.n_dofs)
+    fx = None if fixed is None else np.ascontiguousarray(fixed, dtype=np.uint8)
+    lib().orc_matfree_apply(mesh.n_nodes, _p(_f64(mesh.X)), mesh.n_el, _ETYPE[mesh.etype],
+                            _p(_i64(mesh.conn)), _p(_f64(mesh.E)), _p(_f64(mesh.nu)),
+                            None if fx is None else _p(fx), _p(_f64(u)), _p(y))
+    return y
+
+
+def basis(etype: str, xi, surface: bool = False):
+    """(N, dN) of a reference element at xi."""
+    code = (_FTYPE if surface else _ETYPE)[etype]
+    dim = 2 if surface else 3
+    xi = _f64(xi)
+    N = np.zeros(code)
+    dN = np.zeros((code, dim))
+    lib().orc_basis.argtypes = [C.c_int, C.c_int, _P, _P, _P]
+    lib().orc_basis(code, int(surface), _p(xi), _p(N), _p(dN))
+    return N, dN
+
+
+def surface_rule(ftype: str):
+    w = np.zeros(4)
+    xi = np.zeros((4, 2))
+    lib().orc_surface_rule.argtypes = [C.c_int, _P, _P]
+    n = lib().orc_surface_rule(_FTYPE[ftype], _p(w), _p(xi))
+    return w[:n].copy(), xi[:n].copy()
