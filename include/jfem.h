@@ -1,0 +1,169 @@
+/*
+ * jfem.h -- C ABI of libjfem_hip.so, the MI355X (gfx950) elasticity library.
+ *
+ * The library is the device side of JuliaFEM's linear-elasticity path.  Each
+ * entry point replaces one piece of the reference (paths relative to the
+ * reference repository root); the Julia and Python bindings live in the
+ * package and are shown in INTEGRATION.md.
+ *
+ * Conventions
+ *   - Plain C, no exceptions cross the boundary.  Every int-returning call
+ *     returns JFM_OK (0) or a negative JFM_E* code; jfm_last_error(ctx)
+ *     then describes the failure (valid until the next call on that ctx).
+ *   - Host buffers belong to the caller; the library copies in or out and
+ *     keeps no host pointer after a call returns.  Calls that return host
+ *     data synchronise the context's stream first.
+ *   - Node coordinates X are 3 x n_nodes column-major (Julia's layout; the
+ *     same bytes as an n_nodes x 3 row-major C array).  Connectivity is
+ *     npe x n_el column-major int64 with index_base 1 (Julia) or 0.
+ *   - DOF of (node a, component c) is 3a + c (0-based), i.e. the
+ *     reference's gdofs = 3(node-1)+c (src/assembly/problems.jl:466-478).
+ *   - One context = one device + one HIP stream; a context is not
+ *     thread-safe, distinct contexts are independent.
+ *   - Results are deterministic: no floating-point atomics anywhere; every
+ *     sum has a fixed order (ascending element id for nodal sums, fixed
+ *     reduction trees for dot products), so repeated calls give identical bits.
+ */
+#ifndef JFEM_H
+#define JFEM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct jfm_ctx jfm_ctx;
+
+enum {
+    JFM_OK = 0,
+    JFM_EINVAL = -1,      /* bad argument */
+    JFM_EHIP = -2,        /* HIP runtime error */
+    JFM_ENOTSPD = -3,     /* non-positive pivot / inverted element */
+    JFM_ENOCONV = -4,     /* iteration limit reached */
+    JFM_ESTATE = -5,      /* call out of order (e.g. solve before mesh) */
+    JFM_ENOMEM = -6,      /* device allocation failed */
+    JFM_EUNSUPPORTED = -7 /* element type not supported */
+};
+
+/* K.u operator variants */
+enum {
+    JFM_OP_CSR = 0,      /* assembled K (3x3-block compressed CSR) */
+    JFM_OP_EBE = 1,      /* stored element matrices, deterministic nodal gather */
+    JFM_OP_MATFREE = 2   /* element geometric factors, recompute, nodal gather */
+};
+enum { JFM_FP64 = 0, JFM_FP32 = 1 };
+
+/* solve methods */
+enum {
+    JFM_SOLVE_PCG = 0,       /* FP64 preconditioned CG to rtol */
+    JFM_SOLVE_MIXED_IR = 1   /* FP32 inner PCG inside FP64 iterative refinement */
+};
+/* preconditioners */
+enum {
+    JFM_PREC_BLOCK_JACOBI = 0, /* inverse 3x3 node diagonal blocks */
+    JFM_PREC_LEARNED = 1       /* network-predicted SPD blocks M = C (L L' + eps I) C' */
+};
+
+typedef struct {
+    int32_t method;       /* JFM_SOLVE_* */
+    int32_t op;           /* JFM_OP_* used for the (inner) iteration */
+    int32_t op_outer;     /* JFM_OP_* used for the FP64 outer residual (mixed only) */
+    int32_t precond;      /* JFM_PREC_* */
+    double rtol;          /* target ||f - K u|| / ||f|| over free DOFs */
+    double inner_rtol;    /* inner relative tolerance (mixed only) */
+    int32_t maxit;        /* iteration cap (inner iterations summed for mixed) */
+    int32_t max_outer;    /* refinement-step cap (mixed only) */
+    int32_t check_every;  /* iterations per device batch between host checks */
+    int32_t learned_probe;/* iterations of the learned-vs-block-Jacobi probe; 0 = none */
+} jfm_solve_opts;
+
+typedef struct {
+    int32_t iterations;   /* PCG iterations (inner iterations summed for mixed) */
+    int32_t outer;        /* refinement steps (mixed), 1 for plain PCG */
+    int32_t converged;    /* 1 when the FP64 true residual met rtol */
+    int32_t fallback;     /* 1 when the learned preconditioner was rejected */
+    double relres;        /* final FP64 true relative residual */
+    double solve_ms;      /* device time of the solve (HIP events) */
+    double setup_ms;      /* preconditioner setup time inside the call */
+} jfm_solve_stats;
+
+typedef struct {
+    int64_t n_nodes, n_el, n_dofs, nnz, nnzb, n_faces;
+    int32_t npe, affine;
+    int64_t device_bytes;  /* bytes currently allocated by the context */
+} jfm_info;
+
+/* ---- context --------------------------------------------------------- */
+int jfm_create(int device, jfm_ctx **out);
+void jfm_destroy(jfm_ctx *ctx);
+const char *jfm_last_error(const jfm_ctx *ctx);
+int jfm_get_info(jfm_ctx *ctx, jfm_info *info);
+
+/* ---- problem upload (ext/JuliaFEMCUDAExt.jl:86-217 initialize_gpu_data!) --
+ * Replaces the host-side renumbering and CuArray uploads.  npe = 10 (Tet10),
+ * 4 (Tet4) or 8 (Hex8).  Builds the node->element incidence in ascending
+ * element id (ext/JuliaFEMCUDAExt.jl:179-193), the 3x3-block sparsity
+ * pattern and the per-element geometric factors.  Fails with JFM_ENOTSPD if
+ * any element has detJ <= 0 at a Gauss point (src/basis/math.jl:202 keeps the
+ * signed determinant). */
+int jfm_upload_mesh(jfm_ctx *ctx, int64_t n_nodes, const double *X, int64_t n_el, int32_t npe,
+                    const int64_t *conn, const double *E, const double *nu, int32_t index_base);
+
+/* Dirichlet DOFs and surface traction (src/problems_dirichlet.jl:60-89,
+ * src/problems_elasticity.jl:454-502).  fixed/u_prescribed have n_dofs
+ * entries (u_prescribed may be NULL = zero).  face_conn is
+ * nodes_per_face x n_faces column-major (6 = Tri6, 3 = Tri3, 4 = Quad4),
+ * traction is 3 x n_faces (force per unit area, constant on a face). */
+int jfm_upload_bcs(jfm_ctx *ctx, const uint8_t *fixed, const double *u_prescribed,
+                   int64_t n_faces, int32_t nodes_per_face, const int64_t *face_conn,
+                   const double *traction, int32_t index_base);
+
+/* ---- assembly (assemble!(problem, time), src/assembly/assembly.jl:31-70 ->
+ * src/problems_elasticity.jl:65-81, 203-451; sparse(COO), src/sparse/sparse.jl:54)
+ * CSR sparsity of K over all DOFs, identical to SparseArrays.sparse() of the
+ * reference's COO triplets (duplicates summed, explicit zeros kept, columns
+ * sorted).  K is structurally symmetric, so this is also its CSC pattern.
+ * Pass rowptr/colind = NULL to query nnz only. */
+int jfm_csr_pattern(jfm_ctx *ctx, int64_t *nnz_out, int64_t *rowptr, int64_t *colind, int32_t base);
+
+/* Element stiffness at Gauss points and deterministic scatter into the CSR
+ * values (summed in ascending element id), plus f = f_ext - K u for the
+ * displacement u (u = NULL means zero).  vals (nnz) and f (n_dofs) may be
+ * NULL to keep the results on the device only. */
+int jfm_assemble(jfm_ctx *ctx, const double *u, double *vals, double *f);
+
+/* ---- operator (compute_residual_gpu!, ext/JuliaFEMCUDAExt.jl:442-478) --
+ * y = K x on the free rows, y = 0 on the rows of fixed DOFs; x is used as
+ * given on every DOF.  Host pointers, FP64 in and out; precision selects the
+ * arithmetic of the operator. */
+int jfm_apply(jfm_ctx *ctx, int32_t op, int32_t precision, const double *x, double *y);
+
+/* Average device times (ms) over `reps` back-to-back applications on
+ * device-resident vectors, timed with HIP events on the context's stream:
+ * avg_ms[0] = whole K.u, avg_ms[1] = its first kernel (SpMV, or the element
+ * phase of EBE/MATFREE), avg_ms[2] = the nodal gather (0 for CSR). */
+int jfm_time_apply(jfm_ctx *ctx, int32_t op, int32_t precision, int32_t reps, double *avg_ms);
+
+/* ---- solve (solve!(solver), src/solvers.jl:192-216, 273-344; and
+ * solve_backend!, ext/JuliaFEMCUDAExt.jl:886-916) ---------------------
+ * Solves K u = f on the free DOFs with u = u_prescribed on fixed DOFs.
+ * f = NULL uses the assembled load vector.  u (n_dofs) receives the
+ * displacement; la (n_dofs, may be NULL) receives the reactions
+ * f - K u on fixed DOFs (the reference's Lagrange multipliers la[B]). */
+int jfm_solve(jfm_ctx *ctx, const jfm_solve_opts *opts, const double *f, double *u, double *la,
+              jfm_solve_stats *stats);
+
+/* ---- learned preconditioner (no reference counterpart) ---------------
+ * Per-node MLP 24 -> 64 -> 64 -> 6 (GELU), BF16 weights, FP32 accumulation.
+ * The weight blob is the packed layout described in jfem_learned.h of the
+ * package sources; jfm_learned_blob_bytes() gives its size. */
+size_t jfm_learned_blob_bytes(void);
+int jfm_learned_set_weights(jfm_ctx *ctx, const void *blob, size_t bytes);
+int jfm_learned_get_weights(jfm_ctx *ctx, void *blob, size_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* JFEM_H */

@@ -1,0 +1,833 @@
+// api.hip -- C ABI of libjfem_hip.so: context, problem upload, assembly,
+// operator and solvers.  Declarations and the reference entry point each
+// function replaces are in include/jfem.h.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <thread>
+
+#include "jfem_device.h"
+#include "jfem_learned.h"
+
+using namespace jfem;
+
+// ================================================================ helpers ==
+int jfm_fail(jfm_ctx *ctx, int code, const std::string &msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+int jfm_hip_fail(jfm_ctx *ctx, hipError_t e, const char *what) {
+    return jfm_fail(ctx, JFM_EHIP, std::string("HIP error ") + hipGetErrorString(e) + " in " + what);
+}
+
+template <typename T> T *jfm_alloc(jfm_ctx *ctx, int64_t count) {
+    if (count <= 0) count = 1;
+    void *p = nullptr;
+    const size_t bytes = (size_t)count * sizeof(T);
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    ctx->allocs.push_back(p);
+    ctx->bytes += (int64_t)bytes;
+    return (T *)p;
+}
+
+void jfm_free(jfm_ctx *ctx, void *p) {
+    if (!p) return;
+    auto it = std::find(ctx->allocs.begin(), ctx->allocs.end(), p);
+    if (it != ctx->allocs.end()) ctx->allocs.erase(it);
+    size_t sz = 0;
+    (void)hipMemPtrGetInfo(p, &sz);
+    ctx->bytes -= (int64_t)sz;
+    (void)hipFree(p);
+}
+
+template <typename T> static void free_ptr(jfm_ctx *ctx, T *&p) {
+    jfm_free(ctx, (void *)p);
+    p = nullptr;
+}
+
+#define ALLOC(ctx, ptr, T, n)                                                          \
+    do {                                                                               \
+        (ptr) = jfm_alloc<T>((ctx), (n));                                              \
+        if (!(ptr)) return jfm_fail((ctx), JFM_ENOMEM, "device allocation failed: " #ptr); \
+    } while (0)
+
+#define CHECK_LAUNCH(ctx, call)                           \
+    do {                                                  \
+        hipError_t e_ = (call);                           \
+        if (e_ != hipSuccess) return jfm_hip_fail(ctx, e_, #call); \
+    } while (0)
+
+template <typename F> static void parallel_for(int64_t n, F f) {
+    int nt = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
+    if (n < 4096 || nt == 1) {
+        f(0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    const int64_t chunk = (n + nt - 1) / nt;
+    for (int t = 0; t < nt; ++t) {
+        const int64_t a = t * chunk, b = std::min(n, a + chunk);
+        if (a >= b) break;
+        th.emplace_back([=] { f(a, b); });
+    }
+    for (auto &x : th) x.join();
+}
+
+static int sync(jfm_ctx *ctx) {
+    JFM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return JFM_OK;
+}
+
+// ================================================================ context ==
+extern "C" int jfm_create(int device, jfm_ctx **out) {
+    if (!out) return JFM_EINVAL;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return JFM_EHIP;
+    if (device < 0 || device >= ndev) return JFM_EINVAL;
+    if (hipSetDevice(device) != hipSuccess) return JFM_EHIP;
+    jfm_ctx *ctx = new jfm_ctx();
+    ctx->device = device;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return JFM_EHIP;
+    }
+    ctx->S = jfm_alloc<PcgScalars>(ctx, 1);
+    if (hipHostMalloc((void **)&ctx->S_host, sizeof(PcgScalars), hipHostMallocDefault) != hipSuccess || !ctx->S) {
+        jfm_destroy(ctx);
+        return JFM_ENOMEM;
+    }
+    std::memset(ctx->S_host, 0, sizeof(PcgScalars));
+    *out = ctx;
+    return JFM_OK;
+}
+
+extern "C" void jfm_destroy(jfm_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (void *p : ctx->allocs) (void)hipFree(p);
+    ctx->allocs.clear();
+    if (ctx->S_host) (void)hipHostFree(ctx->S_host);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+extern "C" const char *jfm_last_error(const jfm_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+extern "C" int jfm_get_info(jfm_ctx *ctx, jfm_info *info) {
+    if (!ctx || !info) return JFM_EINVAL;
+    info->n_nodes = ctx->N;
+    info->n_el = ctx->n_el;
+    info->n_dofs = ctx->ndof;
+    info->nnzb = ctx->nnzb;
+    info->nnz = 9 * ctx->nnzb;
+    info->n_faces = ctx->n_faces;
+    info->npe = ctx->npe;
+    info->affine = ctx->affine ? 1 : 0;
+    info->device_bytes = ctx->bytes;
+    return JFM_OK;
+}
+
+// ============================================================ mesh upload ==
+static void release_mesh(jfm_ctx *ctx) {
+    free_ptr(ctx, ctx->X); free_ptr(ctx, ctx->conn); free_ptr(ctx, ctx->E); free_ptr(ctx, ctx->nu);
+    free_ptr(ctx, ctx->gf64); free_ptr(ctx, ctx->gf32); free_ptr(ctx, ctx->inc_ptr); free_ptr(ctx, ctx->inc);
+    free_ptr(ctx, ctx->brow); free_ptr(ctx, ctx->bcol); free_ptr(ctx, ctx->bdiag); free_ptr(ctx, ctx->blk_row);
+    free_ptr(ctx, ctx->cptr); free_ptr(ctx, ctx->contrib); free_ptr(ctx, ctx->vals64); free_ptr(ctx, ctx->vals32);
+    free_ptr(ctx, ctx->Ke); free_ptr(ctx, ctx->ye64); free_ptr(ctx, ctx->ye32);
+    free_ptr(ctx, ctx->vx64); free_ptr(ctx, ctx->vy64); free_ptr(ctx, ctx->vx32); free_ptr(ctx, ctx->vy32);
+    free_ptr(ctx, ctx->fixed); free_ptr(ctx, ctx->g); free_ptr(ctx, ctx->face_conn); free_ptr(ctx, ctx->face_t);
+    free_ptr(ctx, ctx->finc_ptr); free_ptr(ctx, ctx->finc); free_ptr(ctx, ctx->fe); free_ptr(ctx, ctx->f_ext);
+    free_ptr(ctx, ctx->load); free_ptr(ctx, ctx->u64); free_ptr(ctx, ctx->rr64);
+    free_ptr(ctx, ctx->partials); free_ptr(ctx, ctx->counter);
+    for (Workspace *w : {&ctx->ws64, &ctx->ws32}) {
+        free_ptr(ctx, w->x); free_ptr(ctx, w->r); free_ptr(ctx, w->z); free_ptr(ctx, w->p);
+        free_ptr(ctx, w->q); free_ptr(ctx, w->b); free_ptr(ctx, w->Minv);
+        w->ok = false;
+    }
+    free_ptr(ctx, ctx->Mlearn64); free_ptr(ctx, ctx->Mlearn32); free_ptr(ctx, ctx->feat);
+    ctx->has_mesh = ctx->has_bcs = ctx->assembled = ctx->has_load = false;
+    ctx->learned_ready = false;
+    ctx->h_conn.clear(); ctx->h_brow.clear(); ctx->h_bcol.clear();
+}
+
+// Tet10 edge mid nodes at their edge midpoints (and Hex8 parallelepipeds) make
+// J constant over the element; one factor set per element then suffices.
+static bool detect_affine(const std::vector<int32_t> &conn, const double *X, int64_t n_el, int npe) {
+    if (npe == 4) return true;
+    std::atomic<bool> ok{true};
+    parallel_for(n_el, [&](int64_t a, int64_t b) {
+        for (int64_t e = a; e < b && ok.load(std::memory_order_relaxed); ++e) {
+            const int32_t *c = conn.data() + e * npe;
+            auto P = [&](int i, int k) { return X[3 * (int64_t)c[i] + k]; };
+            double h = 0;
+            for (int k = 0; k < 3; ++k) h = std::max(h, std::fabs(P(1, k) - P(0, k)));
+            const double tol = 1e-12 * std::max(h, 1e-300);
+            if (npe == 10) {
+                static const int ed[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+                for (int m = 0; m < 6; ++m)
+                    for (int k = 0; k < 3; ++k)
+                        if (std::fabs(P(4 + m, k) - 0.5 * (P(ed[m][0], k) + P(ed[m][1], k))) > tol) ok = false;
+            } else {  // Hex8: x_i = x0 + sum of the three edge vectors it spans
+                static const int s[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                                            {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+                for (int i = 0; i < 8; ++i)
+                    for (int k = 0; k < 3; ++k) {
+                        const double v = P(0, k) + s[i][0] * (P(1, k) - P(0, k)) + s[i][1] * (P(3, k) - P(0, k)) +
+                                         s[i][2] * (P(4, k) - P(0, k));
+                        if (std::fabs(P(i, k) - v) > tol * 8) ok = false;
+                    }
+            }
+        }
+    });
+    return ok.load();
+}
+
+extern "C" int jfm_upload_mesh(jfm_ctx *ctx, int64_t n_nodes, const double *X, int64_t n_el, int32_t npe,
+                               const int64_t *conn, const double *E, const double *nu, int32_t index_base) {
+    if (!ctx) return JFM_EINVAL;
+    JFM_HIP(ctx, hipSetDevice(ctx->device));
+    if (npe != 10 && npe != 4 && npe != 8)
+        return jfm_fail(ctx, JFM_EUNSUPPORTED, "unsupported element: nodes_per_element must be 10 (Tet10), 4 (Tet4) or 8 (Hex8)");
+    if (n_nodes <= 0 || n_el < 0 || !X || (n_el > 0 && (!conn || !E || !nu)))
+        return jfm_fail(ctx, JFM_EINVAL, "invalid mesh arguments");
+    if (index_base != 0 && index_base != 1) return jfm_fail(ctx, JFM_EINVAL, "index_base must be 0 or 1");
+    if (n_nodes > (int64_t)INT32_MAX / 3 || n_el * (int64_t)npe * npe > (int64_t)INT32_MAX)
+        return jfm_fail(ctx, JFM_EINVAL, "mesh too large for 32-bit indices");
+    release_mesh(ctx);
+    ctx->N = n_nodes;
+    ctx->n_el = n_el;
+    ctx->ndof = 3 * n_nodes;
+    ctx->npe = npe;
+    ctx->ngp = npe == 10 ? 4 : (npe == 4 ? 1 : 8);
+    ctx->npair = npe * (npe + 1) / 2;
+
+    // connectivity -> 0-based int32, range checked
+    ctx->h_conn.resize((size_t)n_el * npe);
+    std::atomic<bool> bad{false};
+    parallel_for(n_el * npe, [&](int64_t a, int64_t b) {
+        for (int64_t k = a; k < b; ++k) {
+            const int64_t v = conn[k] - index_base;
+            if (v < 0 || v >= n_nodes) bad = true;
+            ctx->h_conn[k] = (int32_t)v;
+        }
+    });
+    if (bad) return jfm_fail(ctx, JFM_EINVAL, "connectivity references a node outside [base, base+n_nodes)");
+    for (int64_t e = 0; e < n_el; ++e)
+        if (!(E[e] > 0.0) || !(nu[e] > -1.0 && nu[e] < 0.5))
+            return jfm_fail(ctx, JFM_EINVAL, "material out of range (E > 0, -1 < nu < 0.5 required)");
+
+    const std::vector<int32_t> &hc = ctx->h_conn;
+    // node -> incidences (e*npe + i), ascending e  (ext/JuliaFEMCUDAExt.jl:179-193)
+    std::vector<int32_t> inc_ptr(n_nodes + 1, 0), inc((size_t)n_el * npe);
+    for (int64_t k = 0; k < n_el * npe; ++k) inc_ptr[hc[k] + 1]++;
+    for (int64_t a = 0; a < n_nodes; ++a) inc_ptr[a + 1] += inc_ptr[a];
+    {
+        std::vector<int32_t> pos(inc_ptr.begin(), inc_ptr.end() - 1);
+        for (int64_t e = 0; e < n_el; ++e)
+            for (int i = 0; i < npe; ++i) inc[pos[hc[e * npe + i]]++] = (int32_t)(e * npe + i);
+    }
+
+    // 3x3-block pattern = node adjacency, columns sorted (parallel over rows)
+    const int nt = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
+    std::vector<int32_t> rowlen(n_nodes);
+    std::vector<std::vector<int32_t>> parts(nt);
+    {
+        const int64_t chunk = (n_nodes + nt - 1) / nt;
+        std::vector<std::thread> th;
+        for (int t = 0; t < nt; ++t) {
+            th.emplace_back([&, t] {
+                const int64_t a0 = t * chunk, a1 = std::min<int64_t>(n_nodes, a0 + chunk);
+                std::vector<int32_t> tmp;
+                auto &out = parts[t];
+                for (int64_t a = a0; a < a1; ++a) {
+                    tmp.clear();
+                    for (int k = inc_ptr[a]; k < inc_ptr[a + 1]; ++k) {
+                        const int e = inc[k] / npe;
+                        for (int j = 0; j < npe; ++j) tmp.push_back(hc[(int64_t)e * npe + j]);
+                    }
+                    std::sort(tmp.begin(), tmp.end());
+                    tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+                    rowlen[a] = (int32_t)tmp.size();
+                    out.insert(out.end(), tmp.begin(), tmp.end());
+                }
+            });
+        }
+        for (auto &x : th) x.join();
+    }
+    ctx->h_brow.assign(n_nodes + 1, 0);
+    for (int64_t a = 0; a < n_nodes; ++a) {
+        if ((int64_t)ctx->h_brow[a] + rowlen[a] > INT32_MAX)
+            return jfm_fail(ctx, JFM_EINVAL, "too many nonzero blocks for 32-bit offsets");
+        ctx->h_brow[a + 1] = ctx->h_brow[a] + rowlen[a];
+    }
+    ctx->nnzb = ctx->h_brow[n_nodes];
+    ctx->h_bcol.resize(ctx->nnzb);
+    {
+        size_t off = 0;
+        for (int t = 0; t < nt; ++t) {
+            std::copy(parts[t].begin(), parts[t].end(), ctx->h_bcol.begin() + off);
+            off += parts[t].size();
+            std::vector<int32_t>().swap(parts[t]);
+        }
+    }
+    const std::vector<int32_t> &brow = ctx->h_brow, &bcol = ctx->h_bcol;
+    const int64_t nnzb = ctx->nnzb;
+
+    // per-block contribution lists (e, i, j), ascending e
+    std::vector<int32_t> cptr(nnzb + 1, 0);
+    auto block_of = [&](int64_t a, int32_t b) -> int64_t {
+        const int32_t *lo = bcol.data() + brow[a], *hi = bcol.data() + brow[a + 1];
+        return (std::lower_bound(lo, hi, b) - bcol.data());
+    };
+    parallel_for(n_nodes, [&](int64_t a0, int64_t a1) {
+        for (int64_t a = a0; a < a1; ++a)
+            for (int k = inc_ptr[a]; k < inc_ptr[a + 1]; ++k) {
+                const int e = inc[k] / npe;
+                for (int j = 0; j < npe; ++j) cptr[block_of(a, hc[(int64_t)e * npe + j]) + 1]++;
+            }
+    });
+    for (int64_t s = 0; s < nnzb; ++s) cptr[s + 1] += cptr[s];
+    std::vector<int32_t> contrib(cptr[nnzb]), blk_row(nnzb), bdiag(n_nodes);
+    parallel_for(n_nodes, [&](int64_t a0, int64_t a1) {
+        std::vector<int32_t> cur;
+        for (int64_t a = a0; a < a1; ++a) {
+            cur.assign(cptr.begin() + brow[a], cptr.begin() + brow[a + 1]);
+            for (int k = inc_ptr[a]; k < inc_ptr[a + 1]; ++k) {
+                const int e = inc[k] / npe, i = inc[k] - e * npe;
+                for (int j = 0; j < npe; ++j) {
+                    const int64_t s = block_of(a, hc[(int64_t)e * npe + j]);
+                    contrib[cur[s - brow[a]]++] = (int32_t)(((int64_t)e * npe + i) * npe + j);
+                }
+            }
+            for (int32_t s = brow[a]; s < brow[a + 1]; ++s) blk_row[s] = (int32_t)a;
+            bdiag[a] = (int32_t)block_of(a, (int32_t)a);
+        }
+    });
+
+    ctx->affine = detect_affine(hc, X, n_el, npe);
+    ctx->ngf = ctx->affine ? 1 : ctx->ngp;
+
+    // device copies
+    ALLOC(ctx, ctx->X, double, 3 * n_nodes);
+    ALLOC(ctx, ctx->conn, int32_t, n_el * npe);
+    ALLOC(ctx, ctx->E, double, n_el);
+    ALLOC(ctx, ctx->nu, double, n_el);
+    ALLOC(ctx, ctx->gf64, double, (int64_t)GF * ctx->ngf * n_el);
+    ALLOC(ctx, ctx->gf32, float, (int64_t)GF * ctx->ngf * n_el);
+    ALLOC(ctx, ctx->inc_ptr, int32_t, n_nodes + 1);
+    ALLOC(ctx, ctx->inc, int32_t, n_el * npe);
+    ALLOC(ctx, ctx->brow, int32_t, n_nodes + 1);
+    ALLOC(ctx, ctx->bcol, int32_t, nnzb);
+    ALLOC(ctx, ctx->bdiag, int32_t, n_nodes);
+    ALLOC(ctx, ctx->blk_row, int32_t, nnzb);
+    ALLOC(ctx, ctx->cptr, int32_t, nnzb + 1);
+    ALLOC(ctx, ctx->contrib, int32_t, (int64_t)contrib.size());
+    ALLOC(ctx, ctx->ye64, double, n_el * npe * 3);
+    ALLOC(ctx, ctx->ye32, float, n_el * npe * 3);
+    ALLOC(ctx, ctx->fixed, uint8_t, 3 * n_nodes);
+    ALLOC(ctx, ctx->g, double, 3 * n_nodes);
+    ALLOC(ctx, ctx->f_ext, double, 3 * n_nodes);
+    ALLOC(ctx, ctx->load, double, 3 * n_nodes);
+    ALLOC(ctx, ctx->finc_ptr, int32_t, n_nodes + 1);
+    ALLOC(ctx, ctx->vx64, double, 3 * n_nodes);
+    ALLOC(ctx, ctx->vy64, double, 3 * n_nodes);
+    ALLOC(ctx, ctx->vx32, float, 3 * n_nodes);
+    ALLOC(ctx, ctx->vy32, float, 3 * n_nodes);
+    // fused-reduction scratch: the widest grids are the 8-lanes-per-row SpMV
+    // and the per-DOF kernels
+    ctx->max_blocks = (int)std::max<int64_t>((n_nodes * 8 + kBlock - 1) / kBlock, (3 * n_nodes + kBlock - 1) / kBlock) + 1;
+    ALLOC(ctx, ctx->partials, double, 2 * (int64_t)ctx->max_blocks + 64);
+    ALLOC(ctx, ctx->counter, unsigned, 64);
+
+    std::vector<int32_t> conn_soa((size_t)n_el * npe);
+    parallel_for(n_el, [&](int64_t a, int64_t b) {
+        for (int64_t e = a; e < b; ++e)
+            for (int i = 0; i < npe; ++i) conn_soa[(size_t)i * n_el + e] = hc[(size_t)e * npe + i];
+    });
+    hipStream_t s = ctx->stream;
+    JFM_HIP(ctx, hipMemcpyAsync(ctx->X, X, sizeof(double) * 3 * n_nodes, hipMemcpyHostToDevice, s));
+    JFM_HIP(ctx, hipMemcpyAsync(ctx->conn, conn_soa.data(), sizeof(int32_t) * n_el * npe, hipMemcpyHostToDevice, s));
+    JFM_HIP(ctx, hipMemcpyAsync(ctx->E, E, sizeof(double) * n_el, hipMemcpyHostToDevice, s));
+    JFM_HIP(ctx, hipMemcpyAsync(ctx->nu, nu, sizeof(double) * n_el, hipMemcpyHostToDevice, s));
+    JFM_HIP(ctx, hipMemcpyAsync(ctx->inc_ptr, inc_ptr.data(), sizeof(int32_t) * (n_nodes + 1), hipMemcpyHostToDevice, s));
+    JFM_HIP(ctx, hipMemcpyAsync(ctx->inc, inc.data(), sizeof(int32_t) * n_el * npe, hipMemcpyHostToDevice, s));
+    JFM_HIP(ctx, hipMemcpyAsync(ctx->brow, brow.data(), sizeof(int32_t) * (n_nodes + 1), hipMemcpyHostToDevice, s));
+    JFM_HIP(ctx, hipMemcpyAsync(ctx->bcol, bcol.data(), sizeof(int32_t) * nnzb, hipMemcpyHostToDevice, s));
+    JFM_HIP(ctx, hipMemcpyAsync(ctx->bdiag, bdiag.data(), sizeof(int32_t) * n_nodes, hipMemcpyHostToDevice, s));
+    JFM_HIP(ctx, hipMemcpyAsync(ctx->blk_row, blk_row.data(), sizeof(int32_t) * nnzb, hipMemcpyHostToDevice, s));
+    JFM_HIP(ctx, hipMemcpyAsync(ctx->cptr, cptr.data(), sizeof(int32_t) * (nnzb + 1), hipMemcpyHostToDevice, s));
+    JFM_HIP(ctx, hipMemcpyAsync(ctx->contrib, contrib.data(), sizeof(int32_t) * contrib.size(), hipMemcpyHostToDevice, s));
+    JFM_HIP(ctx, hipMemsetAsync(ctx->fixed, 0, 3 * n_nodes, s));
+    JFM_HIP(ctx, hipMemsetAsync(ctx->g, 0, sizeof(double) * 3 * n_nodes, s));
+    JFM_HIP(ctx, hipMemsetAsync(ctx->f_ext, 0, sizeof(double) * 3 * n_nodes, s));
+    JFM_HIP(ctx, hipMemsetAsync(ctx->load, 0, sizeof(double) * 3 * n_nodes, s));
+    JFM_HIP(ctx, hipMemsetAsync(ctx->finc_ptr, 0, sizeof(int32_t) * (n_nodes + 1), s));
+    CHECK_LAUNCH(ctx, launch_zero_u32(ctx, ctx->counter, 64));
+    int *d_bad = (int *)(ctx->counter + 32);
+    CHECK_LAUNCH(ctx, launch_geom_factors(ctx, d_bad));
+    int h_bad = 0;
+    JFM_HIP(ctx, hipMemcpyAsync(&h_bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, s));
+    if (int rc = sync(ctx)) return rc;
+    // synchronous copies above read pageable host buffers that go out of scope here
+    if (h_bad) return jfm_fail(ctx, JFM_ENOTSPD, "element with detJ <= 0 at a Gauss point (inverted or degenerate)");
+    ctx->has_mesh = true;
+    return JFM_OK;
+}
+
+// ============================================================ BCs, loads ===
+extern "C" int jfm_upload_bcs(jfm_ctx *ctx, const uint8_t *fixed, const double *u_prescribed, int64_t n_faces,
+                              int32_t nodes_per_face, const int64_t *face_conn, const double *traction,
+                              int32_t index_base) {
+    if (!ctx) return JFM_EINVAL;
+    if (!ctx->has_mesh) return jfm_fail(ctx, JFM_ESTATE, "upload the mesh before boundary conditions");
+    JFM_HIP(ctx, hipSetDevice(ctx->device));
+    if (n_faces < 0 || (n_faces > 0 && (!face_conn || !traction)))
+        return jfm_fail(ctx, JFM_EINVAL, "invalid face arguments");
+    if (n_faces > 0 && nodes_per_face != 6 && nodes_per_face != 3 && nodes_per_face != 4)
+        return jfm_fail(ctx, JFM_EUNSUPPORTED, "nodes_per_face must be 6 (Tri6), 3 (Tri3) or 4 (Quad4)");
+    const int64_t N = ctx->N, n = ctx->ndof;
+    hipStream_t s = ctx->stream;
+    if (fixed) JFM_HIP(ctx, hipMemcpyAsync(ctx->fixed, fixed, n, hipMemcpyHostToDevice, s));
+    else JFM_HIP(ctx, hipMemsetAsync(ctx->fixed, 0, n, s));
+    ctx->g_nonzero = false;
+    if (u_prescribed) {
+        for (int64_t i = 0; i < n; ++i)
+            if (u_prescribed[i] != 0.0 && fixed && fixed[i]) { ctx->g_nonzero = true; break; }
+    }
+    if (ctx->g_nonzero) {
+        std::vector<double> gh(n, 0.0);
+        for (int64_t i = 0; i < n; ++i) gh[i] = fixed[i] ? u_prescribed[i] : 0.0;
+        JFM_HIP(ctx, hipMemcpyAsync(ctx->g, gh.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+        if (int rc = sync(ctx)) return rc;
+    } else {
+        JFM_HIP(ctx, hipMemsetAsync(ctx->g, 0, sizeof(double) * n, s));
+    }
+    free_ptr(ctx, ctx->face_conn); free_ptr(ctx, ctx->face_t); free_ptr(ctx, ctx->finc); free_ptr(ctx, ctx->fe);
+    ctx->n_faces = n_faces;
+    ctx->nfn = nodes_per_face;
+    std::vector<int32_t> fptr(N + 1, 0), finc, fsoa;
+    std::vector<double> ft;
+    if (n_faces > 0) {
+        const int nfn = nodes_per_face;
+        fsoa.resize((size_t)n_faces * nfn);
+        for (int64_t f = 0; f < n_faces; ++f)
+            for (int i = 0; i < nfn; ++i) {
+                const int64_t v = face_conn[f * nfn + i] - index_base;
+                if (v < 0 || v >= N) return jfm_fail(ctx, JFM_EINVAL, "face connectivity out of range");
+                fsoa[(size_t)i * n_faces + f] = (int32_t)v;
+                fptr[v + 1]++;
+            }
+        for (int64_t a = 0; a < N; ++a) fptr[a + 1] += fptr[a];
+        finc.resize((size_t)n_faces * nfn);
+        std::vector<int32_t> pos(fptr.begin(), fptr.end() - 1);
+        for (int64_t f = 0; f < n_faces; ++f)
+            for (int i = 0; i < nfn; ++i) finc[pos[fsoa[(size_t)i * n_faces + f]]++] = (int32_t)(f * nfn + i);
+        ft.assign(traction, traction + 3 * n_faces);
+        ALLOC(ctx, ctx->face_conn, int32_t, n_faces * nfn);
+        ALLOC(ctx, ctx->face_t, double, 3 * n_faces);
+        ALLOC(ctx, ctx->finc, int32_t, n_faces * nfn);
+        ALLOC(ctx, ctx->fe, double, n_faces * nfn * 3);
+        JFM_HIP(ctx, hipMemcpyAsync(ctx->face_conn, fsoa.data(), sizeof(int32_t) * fsoa.size(), hipMemcpyHostToDevice, s));
+        JFM_HIP(ctx, hipMemcpyAsync(ctx->face_t, ft.data(), sizeof(double) * ft.size(), hipMemcpyHostToDevice, s));
+        JFM_HIP(ctx, hipMemcpyAsync(ctx->finc, finc.data(), sizeof(int32_t) * finc.size(), hipMemcpyHostToDevice, s));
+    }
+    JFM_HIP(ctx, hipMemcpyAsync(ctx->finc_ptr, fptr.data(), sizeof(int32_t) * (N + 1), hipMemcpyHostToDevice, s));
+    CHECK_LAUNCH(ctx, launch_face_loads(ctx));
+    // the load of an unassembled problem is the external load
+    JFM_HIP(ctx, hipMemcpyAsync(ctx->load, ctx->f_ext, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    if (int rc = sync(ctx)) return rc;
+    ctx->has_bcs = true;
+    ctx->has_load = true;
+    ctx->learned_ready = false;
+    for (Workspace *w : {&ctx->ws64, &ctx->ws32}) w->ok = false;  // preconditioners depend on the fixed set
+    return JFM_OK;
+}
+
+// ================================================================ pattern ==
+extern "C" int jfm_csr_pattern(jfm_ctx *ctx, int64_t *nnz_out, int64_t *rowptr, int64_t *colind, int32_t base) {
+    if (!ctx) return JFM_EINVAL;
+    if (!ctx->has_mesh) return jfm_fail(ctx, JFM_ESTATE, "no mesh");
+    if (nnz_out) *nnz_out = 9 * ctx->nnzb;
+    if (!rowptr && !colind) return JFM_OK;
+    const std::vector<int32_t> &br = ctx->h_brow, &bc = ctx->h_bcol;
+    parallel_for(ctx->N, [&](int64_t a0, int64_t a1) {
+        for (int64_t a = a0; a < a1; ++a) {
+            const int64_t r0 = br[a], nb = br[a + 1] - r0;
+            for (int c = 0; c < 3; ++c) {
+                const int64_t start = 9 * r0 + c * 3 * nb;
+                if (rowptr) rowptr[3 * a + c] = start + base;
+                if (colind)
+                    for (int64_t k = 0; k < nb; ++k)
+                        for (int d = 0; d < 3; ++d) colind[start + 3 * k + d] = 3 * (int64_t)bc[r0 + k] + d + base;
+            }
+        }
+    });
+    if (rowptr) rowptr[ctx->ndof] = 9 * ctx->nnzb + base;
+    return JFM_OK;
+}
+
+// =============================================================== operator ==
+template <typename T> static T *vals_of(jfm_ctx *ctx);
+template <> double *vals_of<double>(jfm_ctx *ctx) { return ctx->vals64; }
+template <> float *vals_of<float>(jfm_ctx *ctx) { return ctx->vals32; }
+
+static int ensure_element_matrices(jfm_ctx *ctx) {
+    if (ctx->Ke) return JFM_OK;
+    ALLOC(ctx, ctx->Ke, double, ctx->n_el * ctx->npair * 9);
+    CHECK_LAUNCH(ctx, launch_element_matrices(ctx));
+    return JFM_OK;
+}
+
+static int ensure_assembled(jfm_ctx *ctx) {
+    if (ctx->assembled) return JFM_OK;
+    if (int rc = ensure_element_matrices(ctx)) return rc;
+    if (!ctx->vals64) ALLOC(ctx, ctx->vals64, double, 9 * ctx->nnzb);
+    if (!ctx->vals32) ALLOC(ctx, ctx->vals32, float, 9 * ctx->nnzb);
+    CHECK_LAUNCH(ctx, launch_assemble_values(ctx));
+    ctx->assembled = true;
+    return JFM_OK;
+}
+
+static int prepare_op(jfm_ctx *ctx, int op) {
+    if (op == JFM_OP_CSR) return ensure_assembled(ctx);
+    if (op == JFM_OP_EBE) return ensure_element_matrices(ctx);
+    if (op == JFM_OP_MATFREE) return JFM_OK;
+    return jfm_fail(ctx, JFM_EINVAL, "unknown operator variant");
+}
+
+// y = K x on free rows (fixed rows zeroed); fused dot(y, dotv) with `fin`.
+template <typename T>
+static hipError_t op_apply(jfm_ctx *ctx, int op, const T *x, T *y, const T *dotv, int fin, const int *done) {
+    if (op == JFM_OP_CSR) return launch_bsr_spmv<T>(ctx, vals_of<T>(ctx), x, y, dotv, fin, done);
+    return launch_two_phase<T>(ctx, op, x, y, dotv, fin, done);
+}
+
+// Same without the row mask (all rows), for f_int and reactions.
+static hipError_t op_apply_all_rows(jfm_ctx *ctx, int op, const double *x, double *y) {
+    uint8_t *saved = ctx->fixed;
+    ctx->fixed = nullptr;
+    hipError_t e = op_apply<double>(ctx, op, x, y, nullptr, FIN_NONE, nullptr);
+    ctx->fixed = saved;
+    return e;
+}
+
+extern "C" int jfm_apply(jfm_ctx *ctx, int32_t op, int32_t precision, const double *x, double *y) {
+    if (!ctx || !x || !y) return JFM_EINVAL;
+    if (!ctx->has_mesh) return jfm_fail(ctx, JFM_ESTATE, "no mesh");
+    if (precision != JFM_FP64 && precision != JFM_FP32) return jfm_fail(ctx, JFM_EINVAL, "precision must be FP64 or FP32");
+    JFM_HIP(ctx, hipSetDevice(ctx->device));
+    if (int rc = prepare_op(ctx, op)) return rc;
+    const int64_t n = ctx->ndof;
+    hipStream_t s = ctx->stream;
+    JFM_HIP(ctx, hipMemcpyAsync(ctx->vx64, x, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    if (precision == JFM_FP64) {
+        CHECK_LAUNCH(ctx, op_apply<double>(ctx, op, ctx->vx64, ctx->vy64, nullptr, FIN_NONE, nullptr));
+    } else {
+        CHECK_LAUNCH(ctx, (launch_convert<double, float>(ctx, ctx->vx64, ctx->vx32, n, 1.0)));
+        CHECK_LAUNCH(ctx, op_apply<float>(ctx, op, ctx->vx32, ctx->vy32, nullptr, FIN_NONE, nullptr));
+        CHECK_LAUNCH(ctx, (launch_convert<float, double>(ctx, ctx->vy32, ctx->vy64, n, 1.0)));
+    }
+    JFM_HIP(ctx, hipMemcpyAsync(y, ctx->vy64, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    return sync(ctx);
+}
+
+extern "C" int jfm_time_apply(jfm_ctx *ctx, int32_t op, int32_t precision, int32_t reps, double *avg_ms) {
+    if (!ctx || !avg_ms || reps <= 0) return JFM_EINVAL;
+    if (!ctx->has_mesh) return jfm_fail(ctx, JFM_ESTATE, "no mesh");
+    if (precision != JFM_FP64 && precision != JFM_FP32) return jfm_fail(ctx, JFM_EINVAL, "precision must be FP64 or FP32");
+    JFM_HIP(ctx, hipSetDevice(ctx->device));
+    if (int rc = prepare_op(ctx, op)) return rc;
+    std::vector<hipEvent_t> ev(3 * (size_t)reps);
+    for (auto &e : ev) JFM_HIP(ctx, hipEventCreate(&e));
+    hipEvent_t t0, t1;
+    JFM_HIP(ctx, hipEventCreate(&t0));
+    JFM_HIP(ctx, hipEventCreate(&t1));
+    const bool two = op != JFM_OP_CSR;
+    auto run = [&](int r, bool timed) -> hipError_t {
+        hipError_t e = hipSuccess;
+        if (timed) (void)hipEventRecord(ev[3 * r], ctx->stream);
+        if (!two) {
+            e = precision == JFM_FP64 ? launch_bsr_spmv<double>(ctx, ctx->vals64, ctx->vx64, ctx->vy64, nullptr, FIN_NONE, nullptr)
+                                      : launch_bsr_spmv<float>(ctx, ctx->vals32, ctx->vx32, ctx->vy32, nullptr, FIN_NONE, nullptr);
+            if (timed) (void)hipEventRecord(ev[3 * r + 1], ctx->stream);
+        } else {
+            e = precision == JFM_FP64 ? launch_elem_only<double>(ctx, op, ctx->vx64)
+                                      : launch_elem_only<float>(ctx, op, ctx->vx32);
+            if (e != hipSuccess) return e;
+            if (timed) (void)hipEventRecord(ev[3 * r + 1], ctx->stream);
+            e = precision == JFM_FP64
+                    ? launch_gather<double>(ctx, ctx->inc_ptr, ctx->inc, ctx->N, ctx->ye64, ctx->fixed, ctx->vy64, nullptr, FIN_NONE, nullptr)
+                    : launch_gather<float>(ctx, ctx->inc_ptr, ctx->inc, ctx->N, ctx->ye32, ctx->fixed, ctx->vy32, nullptr, FIN_NONE, nullptr);
+        }
+        if (timed) (void)hipEventRecord(ev[3 * r + 2], ctx->stream);
+        return e;
+    };
+    CHECK_LAUNCH(ctx, run(0, false));  // warm
+    JFM_HIP(ctx, hipEventRecord(t0, ctx->stream));
+    for (int r = 0; r < reps; ++r) CHECK_LAUNCH(ctx, run(r, true));
+    JFM_HIP(ctx, hipEventRecord(t1, ctx->stream));
+    JFM_HIP(ctx, hipEventSynchronize(t1));
+    double p1 = 0, p2 = 0;
+    for (int r = 0; r < reps; ++r) {
+        float a = 0.f, b = 0.f;
+        JFM_HIP(ctx, hipEventElapsedTime(&a, ev[3 * r], ev[3 * r + 1]));
+        JFM_HIP(ctx, hipEventElapsedTime(&b, ev[3 * r + 1], ev[3 * r + 2]));
+        p1 += a;
+        p2 += b;
+    }
+    float tot = 0.f;
+    JFM_HIP(ctx, hipEventElapsedTime(&tot, t0, t1));
+    avg_ms[0] = tot / reps;
+    avg_ms[1] = p1 / reps;
+    avg_ms[2] = two ? p2 / reps : 0.0;
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    return JFM_OK;
+}
+
+// =============================================================== assembly ==
+extern "C" int jfm_assemble(jfm_ctx *ctx, const double *u, double *vals, double *f) {
+    if (!ctx) return JFM_EINVAL;
+    if (!ctx->has_mesh) return jfm_fail(ctx, JFM_ESTATE, "no mesh");
+    JFM_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    ctx->assembled = false;
+    free_ptr(ctx, ctx->Ke);
+    if (int rc = ensure_assembled(ctx)) return rc;
+    const int64_t n = ctx->ndof;
+    // f = f_ext - f_int(u)  (src/problems_elasticity.jl:432-433)
+    if (u) {
+        JFM_HIP(ctx, hipMemcpyAsync(ctx->vx64, u, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        CHECK_LAUNCH(ctx, op_apply_all_rows(ctx, JFM_OP_MATFREE, ctx->vx64, ctx->vy64));
+        CHECK_LAUNCH(ctx, (launch_convert<double, double>(ctx, ctx->vy64, ctx->vy64, n, -1.0)));
+        std::vector<double> fi(n), fe(n);
+        JFM_HIP(ctx, hipMemcpyAsync(fi.data(), ctx->vy64, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        JFM_HIP(ctx, hipMemcpyAsync(fe.data(), ctx->f_ext, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        if (int rc = sync(ctx)) return rc;
+        for (int64_t i = 0; i < n; ++i) fe[i] += fi[i];
+        JFM_HIP(ctx, hipMemcpyAsync(ctx->load, fe.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+        if (int rc = sync(ctx)) return rc;
+    } else {
+        JFM_HIP(ctx, hipMemcpyAsync(ctx->load, ctx->f_ext, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    }
+    ctx->has_load = true;
+    if (vals) JFM_HIP(ctx, hipMemcpyAsync(vals, ctx->vals64, sizeof(double) * 9 * ctx->nnzb, hipMemcpyDeviceToHost, s));
+    if (f) JFM_HIP(ctx, hipMemcpyAsync(f, ctx->load, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    return sync(ctx);
+}
+
+// ================================================================ solvers ==
+template <typename T> static Workspace &ws_of(jfm_ctx *ctx);
+template <> Workspace &ws_of<double>(jfm_ctx *ctx) { return ctx->ws64; }
+template <> Workspace &ws_of<float>(jfm_ctx *ctx) { return ctx->ws32; }
+
+static int ensure_workspaces(jfm_ctx *ctx) {
+    const int64_t n = ctx->ndof, N = ctx->N;
+    if (!ctx->ws64.x) {
+        Workspace &w = ctx->ws64;
+        ALLOC(ctx, w.x, double, n); ALLOC(ctx, w.r, double, n); ALLOC(ctx, w.z, double, n);
+        ALLOC(ctx, w.p, double, n); ALLOC(ctx, w.q, double, n); ALLOC(ctx, w.b, double, n);
+        ALLOC(ctx, w.Minv, double, 9 * N);
+    }
+    if (!ctx->ws32.x) {
+        Workspace &w = ctx->ws32;
+        ALLOC(ctx, w.x, float, n); ALLOC(ctx, w.r, float, n); ALLOC(ctx, w.z, float, n);
+        ALLOC(ctx, w.p, float, n); ALLOC(ctx, w.q, float, n); ALLOC(ctx, w.b, float, n);
+        ALLOC(ctx, w.Minv, float, 9 * N);
+    }
+    if (!ctx->u64) ALLOC(ctx, ctx->u64, double, n);
+    if (!ctx->rr64) ALLOC(ctx, ctx->rr64, double, n);
+    if (!ctx->ws64.ok) {
+        CHECK_LAUNCH(ctx, launch_block_jacobi(ctx, (double *)ctx->ws64.Minv, (float *)ctx->ws32.Minv));
+        ctx->ws64.ok = ctx->ws32.ok = true;
+    }
+    return JFM_OK;
+}
+
+static int read_scalars(jfm_ctx *ctx) {
+    JFM_HIP(ctx, hipMemcpyAsync(ctx->S_host, ctx->S, sizeof(PcgScalars), hipMemcpyDeviceToHost, ctx->stream));
+    return sync(ctx);
+}
+
+static int write_scalars(jfm_ctx *ctx) {
+    JFM_HIP(ctx, hipMemcpyAsync(ctx->S, ctx->S_host, sizeof(PcgScalars), hipMemcpyHostToDevice, ctx->stream));
+    return sync(ctx);
+}
+
+// PCG on the workspace of precision T: solves K x = b (free DOFs) from x = 0.
+// The iteration runs in device batches of `check_every` with the scalars on
+// the device; the host reads them once per batch.
+template <typename T>
+static int pcg_run(jfm_ctx *ctx, int op, const T *Minv, double rtol, int maxit, int check_every, int *iters,
+                   int *status) {
+    Workspace &w = ws_of<T>(ctx);
+    T *x = (T *)w.x, *r = (T *)w.r, *z = (T *)w.z, *p = (T *)w.p, *q = (T *)w.q, *b = (T *)w.b;
+    const int64_t n = ctx->ndof;
+    hipStream_t s = ctx->stream;
+    std::memset(ctx->S_host, 0, sizeof(PcgScalars));
+    ctx->S_host->rtol2 = rtol * rtol;
+    ctx->S_host->maxit = maxit;
+    if (int rc = write_scalars(ctx)) return rc;
+    JFM_HIP(ctx, hipMemsetAsync(x, 0, sizeof(T) * n, s));
+    JFM_HIP(ctx, hipMemcpyAsync(r, b, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
+    // b.b -> tol2 ; then z = M r, p = z, (r.z, r.r)
+    {
+        // dot in the precision of the vectors, accumulated in double
+        if (sizeof(T) == 8) {
+            CHECK_LAUNCH(ctx, launch_dot(ctx, (const double *)b, (const double *)b, n, FIN_BB));
+        } else {
+            CHECK_LAUNCH(ctx, (launch_convert<float, double>(ctx, (const float *)b, ctx->rr64, n, 1.0)));
+            CHECK_LAUNCH(ctx, launch_dot(ctx, ctx->rr64, ctx->rr64, n, FIN_BB));
+        }
+    }
+    CHECK_LAUNCH(ctx, launch_pcg_init<T>(ctx, ctx->N, r, z, p, Minv));
+    const int *done = &ctx->S->done;
+    int it = 0;
+    if (int rc = read_scalars(ctx)) return rc;
+    while (ctx->S_host->done == DONE_RUNNING) {
+        for (int k = 0; k < check_every; ++k) {
+            CHECK_LAUNCH(ctx, op_apply<T>(ctx, op, p, q, p, FIN_ALPHA, done));
+            CHECK_LAUNCH(ctx, launch_pcg_update<T>(ctx, ctx->N, x, r, z, p, q, Minv));
+            CHECK_LAUNCH(ctx, launch_pcg_direction<T>(ctx, ctx->N, p, z));
+        }
+        if (int rc = read_scalars(ctx)) return rc;
+    }
+    it = ctx->S_host->iter;
+    *iters = it;
+    *status = ctx->S_host->done;
+    return JFM_OK;
+}
+
+// FP64 true residual r = f_eff - K u over free DOFs; returns |r| via *rnorm.
+static int true_residual(jfm_ctx *ctx, int op, const double *b, const double *u, double *r, double *rnorm) {
+    CHECK_LAUNCH(ctx, op_apply<double>(ctx, op, u, ctx->vy64, nullptr, FIN_NONE, nullptr));
+    CHECK_LAUNCH(ctx, launch_residual(ctx, b, ctx->vy64, ctx->fixed, r, ctx->ndof));
+    if (int rc = read_scalars(ctx)) return rc;
+    *rnorm = std::sqrt(ctx->S_host->tmp0);
+    return JFM_OK;
+}
+
+extern "C" int jfm_solve(jfm_ctx *ctx, const jfm_solve_opts *opts, const double *f, double *u, double *la,
+                         jfm_solve_stats *stats) {
+    if (!ctx || !opts || !u) return JFM_EINVAL;
+    if (!ctx->has_mesh) return jfm_fail(ctx, JFM_ESTATE, "no mesh");
+    if (!f && !ctx->has_load) return jfm_fail(ctx, JFM_ESTATE, "no load vector: upload BCs or pass f");
+    if (!(opts->rtol > 0.0) || opts->maxit <= 0) return jfm_fail(ctx, JFM_EINVAL, "rtol > 0 and maxit > 0 required");
+    if (opts->method != JFM_SOLVE_PCG && opts->method != JFM_SOLVE_MIXED_IR)
+        return jfm_fail(ctx, JFM_EINVAL, "unknown solve method");
+    JFM_HIP(ctx, hipSetDevice(ctx->device));
+    const int op = opts->op;
+    const int op_outer = opts->method == JFM_SOLVE_MIXED_IR ? opts->op_outer : op;
+    if (int rc = prepare_op(ctx, op)) return rc;
+    if (int rc = prepare_op(ctx, op_outer)) return rc;
+    if (int rc = ensure_workspaces(ctx)) return rc;
+    const int64_t n = ctx->ndof;
+    hipStream_t s = ctx->stream;
+    const int check = std::max(1, opts->check_every > 0 ? opts->check_every : 16);
+    jfm_solve_stats st{};
+    hipEvent_t e0, e1;
+    JFM_HIP(ctx, hipEventCreate(&e0));
+    JFM_HIP(ctx, hipEventCreate(&e1));
+
+    // f_eff = f - K g on free rows (elimination of prescribed DOFs, src/solvers.jl:206-210)
+    double *feff = ctx->rr64;  // reused below only after feff is consumed into ws64.b
+    if (f) JFM_HIP(ctx, hipMemcpyAsync(ctx->vx64, f, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    else JFM_HIP(ctx, hipMemcpyAsync(ctx->vx64, ctx->load, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    double *bfull = (double *)ctx->ws64.b;
+    JFM_HIP(ctx, hipEventRecord(e0, s));
+    if (ctx->g_nonzero) {
+        CHECK_LAUNCH(ctx, op_apply<double>(ctx, op_outer, ctx->g, ctx->vy64, nullptr, FIN_NONE, nullptr));
+        CHECK_LAUNCH(ctx, launch_residual(ctx, ctx->vx64, ctx->vy64, ctx->fixed, bfull, n));
+    } else {
+        JFM_HIP(ctx, hipMemsetAsync(ctx->vy64, 0, sizeof(double) * n, s));
+        CHECK_LAUNCH(ctx, launch_residual(ctx, ctx->vx64, ctx->vy64, ctx->fixed, bfull, n));
+    }
+    (void)feff;
+    if (int rc = read_scalars(ctx)) return rc;
+    const double bnorm = std::sqrt(ctx->S_host->tmp0);
+    const double *Minv64 = (const double *)ctx->ws64.Minv;
+    const float *Minv32 = (const float *)ctx->ws32.Minv;
+    if (opts->precond == JFM_PREC_LEARNED) {
+        if (int rc = jfem_learned_prepare(ctx, &st.setup_ms)) return rc;
+        Minv64 = ctx->Mlearn64;
+        Minv32 = ctx->Mlearn32;
+    }
+
+    if (bnorm == 0.0) {
+        JFM_HIP(ctx, hipMemcpyAsync(ctx->u64, ctx->g, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        st.converged = 1;
+    } else if (opts->method == JFM_SOLVE_PCG) {
+        int it = 0, status = 0;
+        if (int rc = pcg_run<double>(ctx, op, Minv64, opts->rtol, opts->maxit, check, &it, &status)) return rc;
+        st.iterations = it;
+        st.outer = 1;
+        // u = x + g
+        JFM_HIP(ctx, hipMemcpyAsync(ctx->u64, ctx->ws64.x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        if (ctx->g_nonzero) {
+            std::vector<double> a(n), gg(n);
+            JFM_HIP(ctx, hipMemcpyAsync(a.data(), ctx->u64, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+            JFM_HIP(ctx, hipMemcpyAsync(gg.data(), ctx->g, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+            if (int rc = sync(ctx)) return rc;
+            for (int64_t i = 0; i < n; ++i) a[i] += gg[i];
+            JFM_HIP(ctx, hipMemcpyAsync(ctx->u64, a.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+            if (int rc = sync(ctx)) return rc;
+        }
+    } else {
+        // FP64 iterative refinement around FP32 PCG: r = f - K u in FP64,
+        // K d = r / |r| in FP32 to inner_rtol, u += |r| d.
+        JFM_HIP(ctx, hipMemcpyAsync(ctx->u64, ctx->g, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        int total = 0, outer = 0;
+        const double inner = opts->inner_rtol > 0 ? opts->inner_rtol : 1e-4;
+        const int max_outer = opts->max_outer > 0 ? opts->max_outer : 20;
+        while (outer < max_outer) {
+            double rn = 0.0;
+            if (int rc = true_residual(ctx, op_outer, ctx->vx64, ctx->u64, ctx->rr64, &rn)) return rc;
+            if (rn <= opts->rtol * bnorm || total >= opts->maxit) break;
+            CHECK_LAUNCH(ctx, (launch_convert<double, float>(ctx, ctx->rr64, (float *)ctx->ws32.b, n, 1.0 / rn)));
+            int it = 0, status = 0;
+            if (int rc = pcg_run<float>(ctx, op, Minv32, inner, opts->maxit - total, check, &it, &status)) return rc;
+            total += it;
+            ++outer;
+            CHECK_LAUNCH(ctx, launch_axpy_scaled(ctx, ctx->u64, (const float *)ctx->ws32.x, rn, n));
+            if (status == DONE_BREAKDOWN) break;
+        }
+        st.iterations = total;
+        st.outer = outer;
+    }
+    // final FP64 true residual with the FP64 operator
+    double rn = 0.0;
+    if (int rc = true_residual(ctx, op_outer, ctx->vx64, ctx->u64, ctx->rr64, &rn)) return rc;
+    JFM_HIP(ctx, hipEventRecord(e1, s));
+    JFM_HIP(ctx, hipEventSynchronize(e1));
+    float ms = 0.f;
+    JFM_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    st.relres = bnorm > 0 ? rn / bnorm : 0.0;
+    st.converged = st.relres <= opts->rtol ? 1 : 0;
+    st.solve_ms = ms;
+    JFM_HIP(ctx, hipMemcpyAsync(u, ctx->u64, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    if (la) {
+        // la = f - K u on fixed DOFs (src/solvers.jl:212-213), 0 elsewhere
+        CHECK_LAUNCH(ctx, op_apply_all_rows(ctx, op_outer, ctx->u64, ctx->vy64));
+        std::vector<double> ku(n), ff(n), fx(0);
+        std::vector<uint8_t> fxd(n);
+        JFM_HIP(ctx, hipMemcpyAsync(ku.data(), ctx->vy64, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        JFM_HIP(ctx, hipMemcpyAsync(ff.data(), ctx->vx64, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        JFM_HIP(ctx, hipMemcpyAsync(fxd.data(), ctx->fixed, n, hipMemcpyDeviceToHost, s));
+        if (int rc = sync(ctx)) return rc;
+        for (int64_t i = 0; i < n; ++i) la[i] = fxd[i] ? ff[i] - ku[i] : 0.0;
+    }
+    if (int rc = sync(ctx)) return rc;
+    if (stats) *stats = st;
+    if (!st.converged) return jfm_fail(ctx, JFM_ENOCONV, "solver did not reach rtol (relres " + std::to_string(st.relres) + ")");
+    return JFM_OK;
+}

@@ -1,0 +1,643 @@
+// fem_kernels.hip -- element-level kernels of the elasticity path (gfx950).
+//
+// Reference algorithm (paths relative to the reference repository root):
+//   element stiffness      src/problems_elasticity.jl:203-451
+//                          (Km += w BL' D BL, :369-375; w = ip.weight*detJ, :248)
+//   Jacobian and gradients src/basis/math.jl:185-212 (J = sum dN_i (x) X_i,
+//                          grad_i = invJ . dN_i, signed detJ)
+//   surface traction       src/problems_elasticity.jl:454-502
+//   nodal-assembly order   ext/JuliaFEMCUDAExt.jl:179-193, 293-361
+//
+// For the isotropic D of src/problems_elasticity.jl:310-317 the 3x3 block
+// (i,j) of BL' D BL is  lambda g_i (x) g_j + mu (g_j (x) g_i + (g_i . g_j) I)
+// (g = grad N), which is what the kernels evaluate instead of the 6x30
+// products; the oracle keeps the reference's BL form and the tests compare.
+//
+// Every nodal sum runs over the node's incidences in ascending element id;
+// there are no floating-point atomics.
+#include "jfem_device.h"
+
+namespace jfem {
+
+// ---------------------------------------------------------- constant tables --
+template <int NPE> struct DNTable { double v[Elem<NPE>::NGP][NPE][3]; };
+template <int NPE> constexpr DNTable<NPE> make_dn() {
+    DNTable<NPE> t{};
+    for (int q = 0; q < Elem<NPE>::NGP; ++q)
+        for (int i = 0; i < NPE; ++i)
+            for (int c = 0; c < 3; ++c) t.v[q][i][c] = Elem<NPE>::dN(q, i, c);
+    return t;
+}
+__constant__ DNTable<10> c_dn10 = make_dn<10>();
+__constant__ DNTable<4> c_dn4 = make_dn<4>();
+__constant__ DNTable<8> c_dn8 = make_dn<8>();
+
+template <int NPE> __device__ __forceinline__ double dn_rt(int q, int i, int c);
+template <> __device__ __forceinline__ double dn_rt<10>(int q, int i, int c) { return c_dn10.v[q][i][c]; }
+template <> __device__ __forceinline__ double dn_rt<4>(int q, int i, int c) { return c_dn4.v[q][i][c]; }
+template <> __device__ __forceinline__ double dn_rt<8>(int q, int i, int c) { return c_dn8.v[q][i][c]; }
+
+template <int NPE> struct PairTable { int i[Elem<NPE>::NPAIR], j[Elem<NPE>::NPAIR]; };
+template <int NPE> constexpr PairTable<NPE> make_pairs() {
+    PairTable<NPE> t{};
+    int p = 0;
+    for (int i = 0; i < NPE; ++i)
+        for (int j = i; j < NPE; ++j) { t.i[p] = i; t.j[p] = j; ++p; }
+    return t;
+}
+template <int NPE> __host__ __device__ constexpr int pair_index(int i, int j) {
+    return i * NPE - (i * (i - 1)) / 2 + (j - i);
+}
+
+// ------------------------------------------------------- geometric factors --
+// invJ, lambda*w*detJ, mu*w*detJ per element (affine) or per Gauss point.
+template <int NPE>
+__global__ __launch_bounds__(256) void k_geom(int n_el, int ngf, const int32_t *__restrict__ conn,
+                                             const double *__restrict__ X, const double *__restrict__ E,
+                                             const double *__restrict__ nu, double *__restrict__ gf64,
+                                             float *__restrict__ gf32, int *__restrict__ bad) {
+    using EL = Elem<NPE>;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_el) return;
+    double Xe[NPE][3];
+#pragma unroll
+    for (int i = 0; i < NPE; ++i) {
+        const int nd = conn[(size_t)i * n_el + e];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Xe[i][c] = X[3 * (size_t)nd + c];
+    }
+    const double Ev = E[e], nv = nu[e];
+    const double la = Ev * nv / ((1.0 + nv) * (1.0 - 2.0 * nv));
+    const double mu = Ev / (2.0 * (1.0 + nv));
+#pragma unroll
+    for (int q = 0; q < EL::NGP; ++q) {
+        if (q >= ngf) break;
+        double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+        for (int i = 0; i < NPE; ++i)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const double d = EL::dN(q, i, a);
+                if (d != 0.0)
+#pragma unroll
+                    for (int b = 0; b < 3; ++b) J[a][b] += d * Xe[i][b];
+            }
+        const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+        const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+        const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+        const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+        if (!(det > 0.0)) atomicMax(bad, 1);
+        const double id = 1.0 / det;
+        double inv[9];
+        inv[0] = c00 * id;
+        inv[1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+        inv[2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+        inv[3] = c01 * id;
+        inv[4] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+        inv[5] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+        inv[6] = c02 * id;
+        inv[7] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+        inv[8] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+        const double wd = EL::w(q) * det;
+        double *g = gf64 + (size_t)q * GF * n_el + e;
+        float *h = gf32 + (size_t)q * GF * n_el + e;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) { g[(size_t)k * n_el] = inv[k]; h[(size_t)k * n_el] = (float)inv[k]; }
+        g[(size_t)9 * n_el] = la * wd;
+        g[(size_t)10 * n_el] = mu * wd;
+        h[(size_t)9 * n_el] = (float)(la * wd);
+        h[(size_t)10 * n_el] = (float)(mu * wd);
+    }
+}
+
+// ---------------------------------------------------- matrix-free element --
+// Phase 1 of the two-phase operator: y_e = K_e x_e from the geometric factors.
+// H_q = sum_i x_i (x) dN_i(xi_q) (reference gradient), grad x = H_q invJ',
+// sigma_w = w detJ (lambda tr(eps) I + 2 mu eps), P_q = invJ' sigma_w,
+// y_i = sum_q P_q' dN_i(xi_q).
+template <int NPE, typename T, bool AFF>
+__global__ __launch_bounds__(256) void k_mf_elem(int n_el, const int32_t *__restrict__ conn,
+                                                const T *__restrict__ gf, const T *__restrict__ x,
+                                                T *__restrict__ ye, const int *__restrict__ done) {
+    using EL = Elem<NPE>;
+    constexpr int NG = EL::NGP;
+    if (done && *done) return;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_el) return;
+    T H[NG][9];
+#pragma unroll
+    for (int q = 0; q < NG; ++q)
+#pragma unroll
+        for (int k = 0; k < 9; ++k) H[q][k] = T(0);
+#pragma unroll
+    for (int i = 0; i < NPE; ++i) {
+        const int nd = conn[(size_t)i * n_el + e];
+        const T u0 = x[3 * (size_t)nd], u1 = x[3 * (size_t)nd + 1], u2 = x[3 * (size_t)nd + 2];
+#pragma unroll
+        for (int q = 0; q < NG; ++q)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double d = EL::dN(q, i, c);
+                if (d != 0.0) {
+                    H[q][c] += u0 * T(d);
+                    H[q][3 + c] += u1 * T(d);
+                    H[q][6 + c] += u2 * T(d);
+                }
+            }
+    }
+    T Ji[9], cl, cm;
+    auto load = [&](int q) {
+        const T *g = gf + (size_t)q * GF * n_el + e;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) Ji[k] = g[(size_t)k * n_el];
+        cl = g[(size_t)9 * n_el];
+        cm = g[(size_t)10 * n_el];
+    };
+    if (AFF) load(0);
+#pragma unroll
+    for (int q = 0; q < NG; ++q) {
+        if (!AFF) load(q);
+        T G[9];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+                G[a * 3 + b] = H[q][a * 3] * Ji[b * 3] + H[q][a * 3 + 1] * Ji[b * 3 + 1] + H[q][a * 3 + 2] * Ji[b * 3 + 2];
+        const T tr = cl * (G[0] + G[4] + G[8]);
+        T s[9];
+        s[0] = tr + T(2) * cm * G[0];
+        s[4] = tr + T(2) * cm * G[4];
+        s[8] = tr + T(2) * cm * G[8];
+        s[1] = s[3] = cm * (G[1] + G[3]);
+        s[2] = s[6] = cm * (G[2] + G[6]);
+        s[5] = s[7] = cm * (G[5] + G[7]);
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                H[q][d * 3 + c] = Ji[d] * s[c] + Ji[3 + d] * s[3 + c] + Ji[6 + d] * s[6 + c];
+    }
+    T *out = ye + (size_t)e * NPE * 3;
+#pragma unroll
+    for (int i = 0; i < NPE; ++i) {
+        T f0 = T(0), f1 = T(0), f2 = T(0);
+#pragma unroll
+        for (int q = 0; q < NG; ++q)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const double dd = EL::dN(q, i, d);
+                if (dd != 0.0) {
+                    f0 += T(dd) * H[q][d * 3];
+                    f1 += T(dd) * H[q][d * 3 + 1];
+                    f2 += T(dd) * H[q][d * 3 + 2];
+                }
+            }
+        out[3 * i] = f0;
+        out[3 * i + 1] = f1;
+        out[3 * i + 2] = f2;
+    }
+}
+
+// ------------------------------------------------------------- EBE element --
+// Phase 1 with stored element matrices: y_e = K_e x_e (upper blocks stored).
+template <int NPE, typename T>
+__global__ __launch_bounds__(256) void k_ebe_elem(int n_el, const int32_t *__restrict__ conn,
+                                                 const double *__restrict__ Ke, const T *__restrict__ x,
+                                                 T *__restrict__ ye, const int *__restrict__ done) {
+    using EL = Elem<NPE>;
+    constexpr PairTable<NPE> PT = make_pairs<NPE>();
+    if (done && *done) return;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_el) return;
+    T u[NPE][3], f[NPE][3];
+#pragma unroll
+    for (int i = 0; i < NPE; ++i) {
+        const int nd = conn[(size_t)i * n_el + e];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) { u[i][c] = x[3 * (size_t)nd + c]; f[i][c] = T(0); }
+    }
+    const double *K = Ke + (size_t)e * EL::NPAIR * 9;
+#pragma unroll
+    for (int p = 0; p < EL::NPAIR; ++p) {
+        const int i = PT.i[p], j = PT.j[p];
+        T B[9];
+#pragma unroll
+        for (int m = 0; m < 9; ++m) B[m] = T(K[p * 9 + m]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) f[i][c] += B[c * 3] * u[j][0] + B[c * 3 + 1] * u[j][1] + B[c * 3 + 2] * u[j][2];
+        if (i != j) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) f[j][c] += B[c] * u[i][0] + B[3 + c] * u[i][1] + B[6 + c] * u[i][2];
+        }
+    }
+    T *out = ye + (size_t)e * NPE * 3;
+#pragma unroll
+    for (int i = 0; i < NPE; ++i)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) out[3 * i + c] = f[i][c];
+}
+
+// ------------------------------------------------------------ nodal gather --
+// Phase 2: y[a] = sum over incidences (e, i) of node a, ascending e, of
+// ye[e][i]; fixed DOFs zeroed; optional fused dot(y, dotv) with finalize.
+template <typename T>
+__global__ __launch_bounds__(256) void k_gather(int nrows, const int32_t *__restrict__ ptr,
+                                               const int32_t *__restrict__ inc, const T *__restrict__ ye,
+                                               const uint8_t *__restrict__ fixed, T *__restrict__ y,
+                                               const T *__restrict__ dotv, double *partials,
+                                               unsigned *counter, PcgScalars *S, int fin,
+                                               const int *__restrict__ done) {
+    if (done && *done) return;
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    double dv = 0.0;
+    if (a < nrows) {
+        T s0 = T(0), s1 = T(0), s2 = T(0);
+        const int k1 = ptr[a + 1];
+        for (int k = ptr[a]; k < k1; ++k) {
+            const size_t idx = 3 * (size_t)inc[k];
+            s0 += ye[idx];
+            s1 += ye[idx + 1];
+            s2 += ye[idx + 2];
+        }
+        if (fixed) {
+            if (fixed[3 * (size_t)a]) s0 = T(0);
+            if (fixed[3 * (size_t)a + 1]) s1 = T(0);
+            if (fixed[3 * (size_t)a + 2]) s2 = T(0);
+        }
+        y[3 * (size_t)a] = s0;
+        y[3 * (size_t)a + 1] = s1;
+        y[3 * (size_t)a + 2] = s2;
+        if (dotv)
+            dv = (double)s0 * (double)dotv[3 * (size_t)a] + (double)s1 * (double)dotv[3 * (size_t)a + 1] +
+                 (double)s2 * (double)dotv[3 * (size_t)a + 2];
+    }
+    if (fin != FIN_NONE) {
+        double v[1] = {dv};
+        reduce_finalize<1>(v, partials, counter, S, fin);
+    }
+}
+
+// --------------------------------------------------------- element matrices --
+// One 64-lane block per element: lanes (q, i) form grad N_i at Gauss point q
+// in LDS, lane p then accumulates block pair p = (i <= j) over the points.
+template <int NPE>
+__global__ __launch_bounds__(64) void k_elem_mat(int n_el, int ngf, const double *__restrict__ gf,
+                                                double *__restrict__ Ke) {
+    using EL = Elem<NPE>;
+    constexpr int NG = EL::NGP, NP = EL::NPAIR;
+    constexpr PairTable<NPE> PT = make_pairs<NPE>();
+    __shared__ double g[NG][NPE][3];
+    __shared__ double scl[NG], scm[NG];
+    const int e = blockIdx.x, t = threadIdx.x;
+    if (t < NG * NPE) {
+        const int q = t / NPE, i = t % NPE, qf = (ngf == 1) ? 0 : q;
+        const double *G = gf + (size_t)qf * GF * n_el + e;
+        double Ji[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) Ji[k] = G[(size_t)k * n_el];
+        const double d0 = dn_rt<NPE>(q, i, 0), d1 = dn_rt<NPE>(q, i, 1), d2 = dn_rt<NPE>(q, i, 2);
+#pragma unroll
+        for (int b = 0; b < 3; ++b) g[q][i][b] = Ji[b * 3] * d0 + Ji[b * 3 + 1] * d1 + Ji[b * 3 + 2] * d2;
+        if (i == 0) {
+            scl[q] = G[(size_t)9 * n_el];
+            scm[q] = G[(size_t)10 * n_el];
+        }
+    }
+    __syncthreads();
+    if (t < NP) {
+        const int i = PT.i[t], j = PT.j[t];
+        double K[9];
+#pragma unroll
+        for (int m = 0; m < 9; ++m) K[m] = 0.0;
+#pragma unroll
+        for (int q = 0; q < NG; ++q) {
+            const double gi0 = g[q][i][0], gi1 = g[q][i][1], gi2 = g[q][i][2];
+            const double gj0 = g[q][j][0], gj1 = g[q][j][1], gj2 = g[q][j][2];
+            const double cl = scl[q], cm = scm[q];
+            const double gi[3] = {gi0, gi1, gi2}, gj[3] = {gj0, gj1, gj2};
+            const double dot = gi0 * gj0 + gi1 * gj1 + gi2 * gj2;
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b)
+                    K[a * 3 + b] += cl * gi[a] * gj[b] + cm * (gi[b] * gj[a] + (a == b ? dot : 0.0));
+        }
+        double *out = Ke + ((size_t)e * NP + t) * 9;
+#pragma unroll
+        for (int m = 0; m < 9; ++m) out[m] = K[m];
+    }
+}
+
+// ------------------------------------------------- CSR values (phase 2) ----
+// One thread per 3x3 block: sums its contributions (ascending element id)
+// from the element matrices and writes the scalar-CSR-ordered values.
+template <int NPE>
+__global__ __launch_bounds__(256) void k_asm_vals(int nnzb, const int32_t *__restrict__ cptr,
+                                                 const int32_t *__restrict__ contrib,
+                                                 const int32_t *__restrict__ blk_row,
+                                                 const int32_t *__restrict__ brow, const double *__restrict__ Ke,
+                                                 double *__restrict__ vals, float *__restrict__ vals32) {
+    using EL = Elem<NPE>;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nnzb) return;
+    double acc[9];
+#pragma unroll
+    for (int m = 0; m < 9; ++m) acc[m] = 0.0;
+    const int k1 = cptr[s + 1];
+    for (int k = cptr[s]; k < k1; ++k) {
+        const int c = contrib[k];
+        const int e = c / (NPE * NPE), rem = c - e * NPE * NPE;
+        const int i = rem / NPE, j = rem - i * NPE;
+        if (i <= j) {
+            const double *B = Ke + ((size_t)e * EL::NPAIR + pair_index<NPE>(i, j)) * 9;
+#pragma unroll
+            for (int m = 0; m < 9; ++m) acc[m] += B[m];
+        } else {
+            const double *B = Ke + ((size_t)e * EL::NPAIR + pair_index<NPE>(j, i)) * 9;
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b) acc[a * 3 + b] += B[b * 3 + a];
+        }
+    }
+    const int a = blk_row[s];
+    const int r0 = brow[a], nb = brow[a + 1] - r0;
+    const size_t base = 9 * (size_t)r0 + 3 * (size_t)(s - r0);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const size_t o = base + (size_t)c * 3 * nb + d;
+            vals[o] = acc[c * 3 + d];
+            vals32[o] = (float)acc[c * 3 + d];
+        }
+}
+
+// ------------------------------------------------------------ face loads ----
+// Consistent traction load per face (src/problems_elasticity.jl:454-502):
+// f_i += w |dX/du x dX/dv| t N_i.  Tri6: 3-pt rule (gauss_points.jl:100-106),
+// Tri3: 1-pt (:93-97), Quad4: 2x2 (:133-142).
+__device__ void surf_basis(int nfn, double u, double v, double *N, double (*dN)[2]) {
+    if (nfn == 6) {
+        N[0] = 1 - 3.0 * u - 3.0 * v + 2.0 * u * u + 4.0 * u * v + 2.0 * v * v;
+        N[1] = -u + 2.0 * u * u;
+        N[2] = -v + 2.0 * v * v;
+        N[3] = 4.0 * u - 4.0 * u * u - 4.0 * u * v;
+        N[4] = 4.0 * u * v;
+        N[5] = 4.0 * v - 4.0 * u * v - 4.0 * v * v;
+        dN[0][0] = -3.0 + 4.0 * u + 4.0 * v; dN[0][1] = -3.0 + 4.0 * u + 4.0 * v;
+        dN[1][0] = -1.0 + 4.0 * u; dN[1][1] = 0.0;
+        dN[2][0] = 0.0; dN[2][1] = -1.0 + 4.0 * v;
+        dN[3][0] = 4.0 - 8.0 * u - 4.0 * v; dN[3][1] = -4.0 * u;
+        dN[4][0] = 4.0 * v; dN[4][1] = 4.0 * u;
+        dN[5][0] = -4.0 * v; dN[5][1] = 4.0 - 4.0 * u - 8.0 * v;
+    } else if (nfn == 3) {
+        N[0] = 1 - u - v; N[1] = u; N[2] = v;
+        dN[0][0] = -1; dN[0][1] = -1; dN[1][0] = 1; dN[1][1] = 0; dN[2][0] = 0; dN[2][1] = 1;
+    } else {
+        const double r[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
+        for (int i = 0; i < 4; ++i) {
+            const double a = 1 + r[i][0] * u, b = 1 + r[i][1] * v;
+            N[i] = 0.25 * a * b;
+            dN[i][0] = 0.25 * r[i][0] * b;
+            dN[i][1] = 0.25 * a * r[i][1];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_face(int nf, int nfn, const int32_t *__restrict__ fconn,
+                                             const double *__restrict__ X, const double *__restrict__ ft,
+                                             double *__restrict__ fe) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf) return;
+    double Xf[6][3];
+    for (int i = 0; i < nfn; ++i) {
+        const int nd = fconn[(size_t)i * nf + f];
+        for (int c = 0; c < 3; ++c) Xf[i][c] = X[3 * (size_t)nd + c];
+    }
+    double w[4], pu[4], pv[4];
+    int ng;
+    if (nfn == 6) {
+        ng = 3;
+        w[0] = w[1] = w[2] = 1.0 / 6;
+        pu[0] = 1.0 / 6; pv[0] = 1.0 / 6;
+        pu[1] = 2.0 / 3; pv[1] = 1.0 / 6;
+        pu[2] = 1.0 / 6; pv[2] = 2.0 / 3;
+    } else if (nfn == 3) {
+        ng = 1;
+        w[0] = 0.5; pu[0] = pv[0] = 1.0 / 3;
+    } else {
+        ng = 4;
+        const double a = 1.0 / sqrt(3.0);
+        w[0] = w[1] = w[2] = w[3] = 1.0;
+        pu[0] = -a; pv[0] = -a; pu[1] = a; pv[1] = -a; pu[2] = -a; pv[2] = a; pu[3] = a; pv[3] = a;
+    }
+    double out[6][3];
+    for (int i = 0; i < nfn; ++i) out[i][0] = out[i][1] = out[i][2] = 0.0;
+    const double t0 = ft[3 * (size_t)f], t1 = ft[3 * (size_t)f + 1], t2 = ft[3 * (size_t)f + 2];
+    for (int q = 0; q < ng; ++q) {
+        double N[6], dN[6][2];
+        surf_basis(nfn, pu[q], pv[q], N, dN);
+        double J[2][3] = {{0, 0, 0}, {0, 0, 0}};
+        for (int i = 0; i < nfn; ++i)
+            for (int a = 0; a < 2; ++a)
+                for (int b = 0; b < 3; ++b) J[a][b] += dN[i][a] * Xf[i][b];
+        const double cx = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+        const double cy = J[0][2] * J[1][0] - J[0][0] * J[1][2];
+        const double cz = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+        const double wq = w[q] * sqrt(cx * cx + cy * cy + cz * cz);
+        for (int i = 0; i < nfn; ++i) {
+            out[i][0] += wq * t0 * N[i];
+            out[i][1] += wq * t1 * N[i];
+            out[i][2] += wq * t2 * N[i];
+        }
+    }
+    for (int i = 0; i < nfn; ++i)
+        for (int c = 0; c < 3; ++c) fe[((size_t)f * nfn + i) * 3 + c] = out[i][c];
+}
+
+// ----------------------------------------------------------- block Jacobi --
+// Diagonal 3x3 blocks sum_e K_e(i,i) over the node's incidences (ascending
+// element id), fixed rows/columns replaced by identity before inversion and
+// zeroed after, so the preconditioner never touches fixed DOFs.
+template <int NPE>
+__global__ __launch_bounds__(256) void k_block_jacobi(int N, const int32_t *__restrict__ ptr,
+                                                     const int32_t *__restrict__ inc, int n_el, int ngf,
+                                                     const double *__restrict__ gf,
+                                                     const uint8_t *__restrict__ fixed,
+                                                     double *__restrict__ M64, float *__restrict__ M32) {
+    using EL = Elem<NPE>;
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= N) return;
+    double B[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const int k1 = ptr[a + 1];
+    for (int k = ptr[a]; k < k1; ++k) {
+        const int idx = inc[k];
+        const int e = idx / NPE, i = idx - e * NPE;
+        for (int q = 0; q < EL::NGP; ++q) {
+            const double *G = gf + (size_t)(ngf == 1 ? 0 : q) * GF * n_el + e;
+            double Ji[9];
+            for (int m = 0; m < 9; ++m) Ji[m] = G[(size_t)m * n_el];
+            const double cl = G[(size_t)9 * n_el], cm = G[(size_t)10 * n_el];
+            const double d0 = dn_rt<NPE>(q, i, 0), d1 = dn_rt<NPE>(q, i, 1), d2 = dn_rt<NPE>(q, i, 2);
+            double g[3];
+            for (int b = 0; b < 3; ++b) g[b] = Ji[b * 3] * d0 + Ji[b * 3 + 1] * d1 + Ji[b * 3 + 2] * d2;
+            const double dot = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) B[r * 3 + c] += cl * g[r] * g[c] + cm * (g[c] * g[r] + (r == c ? dot : 0.0));
+        }
+    }
+    const uint8_t f0 = fixed ? fixed[3 * (size_t)a] : 0, f1 = fixed ? fixed[3 * (size_t)a + 1] : 0,
+                  f2 = fixed ? fixed[3 * (size_t)a + 2] : 0;
+    const uint8_t fx[3] = {f0, f1, f2};
+    for (int c = 0; c < 3; ++c)
+        if (fx[c])
+            for (int d = 0; d < 3; ++d) { B[c * 3 + d] = (c == d); B[d * 3 + c] = (c == d); }
+    const double c00 = B[4] * B[8] - B[5] * B[7];
+    const double c01 = B[5] * B[6] - B[3] * B[8];
+    const double c02 = B[3] * B[7] - B[4] * B[6];
+    const double id = 1.0 / (B[0] * c00 + B[1] * c01 + B[2] * c02);
+    double M[9];
+    M[0] = c00 * id;
+    M[1] = (B[2] * B[7] - B[1] * B[8]) * id;
+    M[2] = (B[1] * B[5] - B[2] * B[4]) * id;
+    M[3] = c01 * id;
+    M[4] = (B[0] * B[8] - B[2] * B[6]) * id;
+    M[5] = (B[2] * B[3] - B[0] * B[5]) * id;
+    M[6] = c02 * id;
+    M[7] = (B[1] * B[6] - B[0] * B[7]) * id;
+    M[8] = (B[0] * B[4] - B[1] * B[3]) * id;
+    for (int c = 0; c < 3; ++c)
+        if (fx[c])
+            for (int d = 0; d < 3; ++d) { M[c * 3 + d] = 0.0; M[d * 3 + c] = 0.0; }
+    for (int m = 0; m < 9; ++m) {
+        M64[9 * (size_t)a + m] = M[m];
+        M32[9 * (size_t)a + m] = (float)M[m];
+    }
+}
+
+// ------------------------------------------------------------- launchers ----
+static inline unsigned nblk(int64_t n, int b = kBlock) { return (unsigned)((n + b - 1) / b); }
+
+hipError_t launch_geom_factors(jfm_ctx *ctx, int *d_bad) {
+    const int n = (int)ctx->n_el;
+    if (n == 0) return hipSuccess;
+    switch (ctx->npe) {
+    case 10: k_geom<10><<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->ngf, ctx->conn, ctx->X, ctx->E, ctx->nu, ctx->gf64, ctx->gf32, d_bad); break;
+    case 4: k_geom<4><<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->ngf, ctx->conn, ctx->X, ctx->E, ctx->nu, ctx->gf64, ctx->gf32, d_bad); break;
+    case 8: k_geom<8><<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->ngf, ctx->conn, ctx->X, ctx->E, ctx->nu, ctx->gf64, ctx->gf32, d_bad); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_element_matrices(jfm_ctx *ctx) {
+    const int n = (int)ctx->n_el;
+    if (n == 0) return hipSuccess;
+    switch (ctx->npe) {
+    case 10: k_elem_mat<10><<<n, 64, 0, ctx->stream>>>(n, ctx->ngf, ctx->gf64, ctx->Ke); break;
+    case 4: k_elem_mat<4><<<n, 64, 0, ctx->stream>>>(n, ctx->ngf, ctx->gf64, ctx->Ke); break;
+    case 8: k_elem_mat<8><<<n, 64, 0, ctx->stream>>>(n, ctx->ngf, ctx->gf64, ctx->Ke); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_assemble_values(jfm_ctx *ctx) {
+    const int n = (int)ctx->nnzb;
+    if (n == 0) return hipSuccess;
+    switch (ctx->npe) {
+    case 10: k_asm_vals<10><<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->cptr, ctx->contrib, ctx->blk_row, ctx->brow, ctx->Ke, ctx->vals64, ctx->vals32); break;
+    case 4: k_asm_vals<4><<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->cptr, ctx->contrib, ctx->blk_row, ctx->brow, ctx->Ke, ctx->vals64, ctx->vals32); break;
+    case 8: k_asm_vals<8><<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->cptr, ctx->contrib, ctx->blk_row, ctx->brow, ctx->Ke, ctx->vals64, ctx->vals32); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_face_loads(jfm_ctx *ctx) {
+    const int nf = (int)ctx->n_faces;
+    if (nf > 0) {
+        k_face<<<nblk(nf), kBlock, 0, ctx->stream>>>(nf, ctx->nfn, ctx->face_conn, ctx->X, ctx->face_t, ctx->fe);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return launch_gather<double>(ctx, ctx->finc_ptr, ctx->finc, ctx->N, ctx->fe, nullptr, ctx->f_ext,
+                                 nullptr, FIN_NONE, nullptr);
+}
+
+hipError_t launch_block_jacobi(jfm_ctx *ctx, double *Minv64, float *Minv32) {
+    const int N = (int)ctx->N;
+    switch (ctx->npe) {
+    case 10: k_block_jacobi<10><<<nblk(N), kBlock, 0, ctx->stream>>>(N, ctx->inc_ptr, ctx->inc, (int)ctx->n_el, ctx->ngf, ctx->gf64, ctx->fixed, Minv64, Minv32); break;
+    case 4: k_block_jacobi<4><<<nblk(N), kBlock, 0, ctx->stream>>>(N, ctx->inc_ptr, ctx->inc, (int)ctx->n_el, ctx->ngf, ctx->gf64, ctx->fixed, Minv64, Minv32); break;
+    case 8: k_block_jacobi<8><<<nblk(N), kBlock, 0, ctx->stream>>>(N, ctx->inc_ptr, ctx->inc, (int)ctx->n_el, ctx->ngf, ctx->gf64, ctx->fixed, Minv64, Minv32); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_gather(jfm_ctx *ctx, const int32_t *ptr, const int32_t *inc, int64_t nrows, const T *ye,
+                         const uint8_t *fixed, T *y, const T *dotv, int fin, const int *done) {
+    if (nrows == 0) return hipSuccess;
+    k_gather<T><<<nblk(nrows), kBlock, 0, ctx->stream>>>((int)nrows, ptr, inc, ye, fixed, y, dotv,
+                                                        ctx->partials, ctx->counter, ctx->S, fin, done);
+    return hipGetLastError();
+}
+
+template <typename T> static T *ye_of(jfm_ctx *ctx);
+template <> double *ye_of<double>(jfm_ctx *ctx) { return ctx->ye64; }
+template <> float *ye_of<float>(jfm_ctx *ctx) { return ctx->ye32; }
+template <typename T> static const T *gf_of(jfm_ctx *ctx);
+template <> const double *gf_of<double>(jfm_ctx *ctx) { return ctx->gf64; }
+template <> const float *gf_of<float>(jfm_ctx *ctx) { return ctx->gf32; }
+
+template <int NPE, typename T>
+static void launch_elem_phase(jfm_ctx *ctx, int op, const T *x, T *ye, const int *done) {
+    const int n = (int)ctx->n_el;
+    if (op == JFM_OP_EBE) {
+        k_ebe_elem<NPE, T><<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->conn, ctx->Ke, x, ye, done);
+    } else if (ctx->affine) {
+        k_mf_elem<NPE, T, true><<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->conn, gf_of<T>(ctx), x, ye, done);
+    } else {
+        k_mf_elem<NPE, T, false><<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->conn, gf_of<T>(ctx), x, ye, done);
+    }
+}
+
+template <typename T>
+hipError_t launch_two_phase(jfm_ctx *ctx, int op, const T *x, T *y, const T *dotv, int fin, const int *done) {
+    T *ye = ye_of<T>(ctx);
+    if (ctx->n_el > 0) {
+        switch (ctx->npe) {
+        case 10: launch_elem_phase<10, T>(ctx, op, x, ye, done); break;
+        case 4: launch_elem_phase<4, T>(ctx, op, x, ye, done); break;
+        case 8: launch_elem_phase<8, T>(ctx, op, x, ye, done); break;
+        default: return hipErrorInvalidValue;
+        }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return launch_gather<T>(ctx, ctx->inc_ptr, ctx->inc, ctx->N, ye, ctx->fixed, y, dotv, fin, done);
+}
+
+template <typename T>
+hipError_t launch_elem_only(jfm_ctx *ctx, int op, const T *x) {
+    if (ctx->n_el == 0) return hipSuccess;
+    switch (ctx->npe) {
+    case 10: launch_elem_phase<10, T>(ctx, op, x, ye_of<T>(ctx), nullptr); break;
+    case 4: launch_elem_phase<4, T>(ctx, op, x, ye_of<T>(ctx), nullptr); break;
+    case 8: launch_elem_phase<8, T>(ctx, op, x, ye_of<T>(ctx), nullptr); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template hipError_t launch_elem_only<double>(jfm_ctx *, int, const double *);
+template hipError_t launch_elem_only<float>(jfm_ctx *, int, const float *);
+template hipError_t launch_two_phase<double>(jfm_ctx *, int, const double *, double *, const double *, int, const int *);
+template hipError_t launch_two_phase<float>(jfm_ctx *, int, const float *, float *, const float *, int, const int *);
+template hipError_t launch_gather<double>(jfm_ctx *, const int32_t *, const int32_t *, int64_t, const double *, const uint8_t *, double *, const double *, int, const int *);
+template hipError_t launch_gather<float>(jfm_ctx *, const int32_t *, const int32_t *, int64_t, const float *, const uint8_t *, float *, const float *, int, const int *);
+
+}  // namespace jfem

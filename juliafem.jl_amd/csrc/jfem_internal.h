@@ -1,0 +1,243 @@
+// jfem_internal.h -- shared declarations of libjfem_hip.so (not installed).
+//
+// Data layout in HBM (n = 3N DOFs, DOF 3a+c):
+//   X        N x 3 f64            node coordinates
+//   conn     npe x n_el i32 (SoA) conn[i*n_el + e] = node of local i
+//   gf       GF x n_el (SoA)      per element (affine) or per Gauss point:
+//                                 invJ (9, row-major), lambda*w*detJ, mu*w*detJ
+//   inc      node -> (e*npe + i)  incidences in ascending element id
+//   brow/bcol                     3x3-block pattern (node adjacency), rows sorted
+//   vals     9*nnzb               CSR values in the scalar CSR order: the block
+//                                 row a occupies vals[9*brow[a] ...]; scalar row
+//                                 3a+c starts at 9*brow[a] + c*3*nb, entry
+//                                 (3a+c, 3*bcol[s]+d) at + 3*(s-brow[a]) + d
+//   Ke       n_el x NPAIR x 9     element matrix blocks (i <= j), row-major 3x3
+//   ye       n_el x npe x 3       element results of the two-phase operators
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/jfem.h"
+
+namespace jfem {
+
+// ------------------------------------------------------------------ traits --
+// Reference elements, Gauss rules and dN/dxi at the Gauss points, as
+// constexpr tables so that fully unrolled loops fold them into immediates.
+//   Tet10: src/basis/lagrange_generated.jl:267-282, 4-pt rule with the
+//          15-digit literals of src/integration/gauss_points.jl:173-182
+//   Tet4:  :229-256, 1-pt rule (gauss_points.jl:166-170)
+//   Hex8:  :285-310, 2x2x2 rule (gauss_points.jl:207-219)
+template <int NPE> struct Elem;
+
+template <> struct Elem<10> {
+    static constexpr int NPE = 10, NGP = 4, NPAIR = 55;
+    __host__ __device__ static constexpr double w(int) { return 1.0 / 24.0; }
+    __host__ __device__ static constexpr double xi(int q, int c) {
+        return (q == 3) ? 0.138196601125011 : ((q == c) ? 0.585410196624968 : 0.138196601125011);
+    }
+    __host__ __device__ static constexpr double dN(int q, int i, int c) {
+        const double u = xi(q, 0), v = xi(q, 1), s = xi(q, 2);
+        switch (i) {
+        case 0: return -3.0 + 4.0 * u + 4.0 * v + 4.0 * s;
+        case 1: return c == 0 ? -1.0 + 4.0 * u : 0.0;
+        case 2: return c == 1 ? -1.0 + 4.0 * v : 0.0;
+        case 3: return c == 2 ? -1.0 + 4.0 * s : 0.0;
+        case 4: return c == 0 ? 4.0 - 8.0 * u - 4.0 * v - 4.0 * s : -4.0 * u;
+        case 5: return c == 0 ? 4.0 * v : (c == 1 ? 4.0 * u : 0.0);
+        case 6: return c == 1 ? 4.0 - 8.0 * v - 4.0 * u - 4.0 * s : -4.0 * v;
+        case 7: return c == 2 ? 4.0 - 8.0 * s - 4.0 * u - 4.0 * v : -4.0 * s;
+        case 8: return c == 0 ? 4.0 * s : (c == 2 ? 4.0 * u : 0.0);
+        default: return c == 1 ? 4.0 * s : (c == 2 ? 4.0 * v : 0.0);
+        }
+    }
+};
+
+template <> struct Elem<4> {
+    static constexpr int NPE = 4, NGP = 1, NPAIR = 10;
+    __host__ __device__ static constexpr double w(int) { return 1.0 / 6.0; }
+    __host__ __device__ static constexpr double dN(int, int i, int c) {
+        return i == 0 ? -1.0 : ((i - 1) == c ? 1.0 : 0.0);
+    }
+};
+
+template <> struct Elem<8> {
+    static constexpr int NPE = 8, NGP = 8, NPAIR = 36;
+    // 1/sqrt(3) to double precision
+    static constexpr double A = 0.57735026918962576451;
+    __host__ __device__ static constexpr double w(int) { return 1.0; }
+    __host__ __device__ static constexpr double ref(int i, int c) {
+        // corners (-1,-1,-1),(1,-1,-1),(1,1,-1),(-1,1,-1),(-1,-1,1),(1,-1,1),(1,1,1),(-1,1,1)
+        return c == 0 ? ((i == 1 || i == 2 || i == 5 || i == 6) ? 1.0 : -1.0)
+             : c == 1 ? ((i == 2 || i == 3 || i == 6 || i == 7) ? 1.0 : -1.0)
+                      : (i >= 4 ? 1.0 : -1.0);
+    }
+    // Gauss point q: x fastest (gauss_points.jl:207-219 ordering)
+    __host__ __device__ static constexpr double xi(int q, int c) {
+        return ((q >> c) & 1) ? A : -A;
+    }
+    __host__ __device__ static constexpr double dN(int q, int i, int c) {
+        const double a = 1.0 + ref(i, 0) * xi(q, 0), b = 1.0 + ref(i, 1) * xi(q, 1),
+                     d = 1.0 + ref(i, 2) * xi(q, 2);
+        return c == 0 ? 0.125 * ref(i, 0) * b * d : (c == 1 ? 0.125 * a * ref(i, 1) * d : 0.125 * a * b * ref(i, 2));
+    }
+};
+
+constexpr int GF = 11;  // invJ(9), lambda*w*detJ, mu*w*detJ
+
+// --------------------------------------------------------- PCG scalars ----
+struct PcgScalars {
+    double rz;     // r.z of the current iterate
+    double pq;     // p.q
+    double alpha, beta;
+    double rr;     // r.r
+    double bb;     // b.b over free DOFs
+    double tol2;   // rtol^2 * bb
+    double rtol2;  // set by the host before FIN_BB
+    double tmp0, tmp1;
+    int iter, done, maxit, pad;
+};
+// done codes
+enum { DONE_RUNNING = 0, DONE_CONVERGED = 1, DONE_BREAKDOWN = 2, DONE_MAXIT = 3 };
+// finalize modes of the fused reductions
+enum { FIN_NONE = 0, FIN_ALPHA = 1, FIN_UPDATE = 2, FIN_INIT = 3, FIN_DOT = 4, FIN_DOT2 = 5, FIN_BB = 6 };
+
+struct Reduce {           // scratch of the fused last-block reductions
+    double *partials;     // >= 2 * max blocks
+    unsigned *counter;    // one counter, returns to 0 after every reduction
+    PcgScalars *S;        // device scalars
+};
+
+// --------------------------------------------------------------- context --
+struct Workspace {        // PCG vectors of one precision
+    void *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *b = nullptr;
+    void *Minv = nullptr; // N x 9 preconditioner blocks
+    bool ok = false;
+};
+
+}  // namespace jfem
+
+struct jfm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int64_t bytes = 0;
+    std::vector<void *> allocs;
+
+    // mesh
+    int64_t N = 0, n_el = 0, ndof = 0;
+    int npe = 0, ngp = 0, npair = 0;
+    bool affine = false, has_mesh = false;
+    std::vector<int32_t> h_conn;  // n_el x npe (element-major), 0-based
+    double *X = nullptr;
+    int32_t *conn = nullptr;      // SoA
+    double *E = nullptr, *nu = nullptr;
+    double *gf64 = nullptr;
+    float *gf32 = nullptr;
+    int ngf = 0;                  // factor sets per element (1 if affine else ngp)
+    int32_t *inc_ptr = nullptr, *inc = nullptr;
+
+    // pattern and assembly
+    int64_t nnzb = 0;
+    std::vector<int32_t> h_brow, h_bcol;
+    int32_t *brow = nullptr, *bcol = nullptr, *bdiag = nullptr, *blk_row = nullptr;
+    int32_t *cptr = nullptr, *contrib = nullptr;
+    double *vals64 = nullptr;
+    float *vals32 = nullptr;
+    double *Ke = nullptr;
+    bool assembled = false;
+
+    // boundary conditions and loads
+    bool has_bcs = false;
+    uint8_t *fixed = nullptr;
+    double *g = nullptr;          // prescribed values (n)
+    bool g_nonzero = false;
+    int64_t n_faces = 0;
+    int nfn = 0;
+    int32_t *face_conn = nullptr;  // SoA nfn x n_faces
+    double *face_t = nullptr;      // n_faces x 3
+    int32_t *finc_ptr = nullptr, *finc = nullptr;
+    double *fe = nullptr;          // n_faces x nfn x 3
+    double *f_ext = nullptr;
+    double *load = nullptr;        // f = f_ext - K u from the last assembly
+    bool has_load = false;
+
+    // operator scratch
+    double *ye64 = nullptr;
+    float *ye32 = nullptr;
+    double *vx64 = nullptr, *vy64 = nullptr;  // apply/time buffers
+    float *vx32 = nullptr, *vy32 = nullptr;
+
+    // solver
+    jfem::Workspace ws64, ws32;
+    double *u64 = nullptr;         // outer solution (mixed IR)
+    double *rr64 = nullptr;        // outer residual (mixed IR)
+    jfem::PcgScalars *S = nullptr;
+    jfem::PcgScalars *S_host = nullptr;  // pinned
+    double *partials = nullptr;
+    unsigned *counter = nullptr;
+    int max_blocks = 0;
+
+    // learned preconditioner
+    void *mlp_w = nullptr;         // weight blob on the device
+    std::vector<unsigned char> mlp_host;
+    bool has_mlp = false;
+    bool learned_ready = false;    // Mlearn matches the current mesh/BCs/weights
+    double *Mlearn64 = nullptr;    // N x 9 learned SPD blocks
+    float *Mlearn32 = nullptr;
+    float *feat = nullptr;         // N x 24 node features
+};
+
+// -------------------------------------------------------------- helpers ----
+int jfm_fail(jfm_ctx *ctx, int code, const std::string &msg);
+int jfm_hip_fail(jfm_ctx *ctx, hipError_t e, const char *what);
+#define JFM_HIP(ctx, call)                                   \
+    do {                                                     \
+        hipError_t e_ = (call);                              \
+        if (e_ != hipSuccess) return jfm_hip_fail(ctx, e_, #call); \
+    } while (0)
+
+template <typename T> T *jfm_alloc(jfm_ctx *ctx, int64_t count);
+void jfm_free(jfm_ctx *ctx, void *p);
+
+// ------------------------------------------------------- kernel launchers --
+// fem_kernels.hip
+namespace jfem {
+hipError_t launch_geom_factors(jfm_ctx *ctx, int *d_bad);
+hipError_t launch_element_matrices(jfm_ctx *ctx);
+hipError_t launch_assemble_values(jfm_ctx *ctx);
+hipError_t launch_face_loads(jfm_ctx *ctx);
+hipError_t launch_block_jacobi(jfm_ctx *ctx, double *Minv64, float *Minv32);
+// y = K x (masked) with the two-phase operators; T = double or float
+template <typename T>
+hipError_t launch_two_phase(jfm_ctx *ctx, int op, const T *x, T *y, const T *dotv, int fin,
+                            const int *done);
+template <typename T>
+hipError_t launch_elem_only(jfm_ctx *ctx, int op, const T *x);   // phase 1 into ye
+template <typename T>
+hipError_t launch_gather(jfm_ctx *ctx, const int32_t *ptr, const int32_t *inc, int64_t nrows,
+                         const T *ye, const uint8_t *fixed, T *y, const T *dotv, int fin,
+                         const int *done);
+// linalg_kernels.hip
+template <typename T>
+hipError_t launch_bsr_spmv(jfm_ctx *ctx, const T *vals, const T *x, T *y, const T *dotv, int fin,
+                           const int *done);
+template <typename T>
+hipError_t launch_pcg_init(jfm_ctx *ctx, int64_t N, T *r, T *z, T *p, const T *Minv);
+template <typename T>
+hipError_t launch_pcg_update(jfm_ctx *ctx, int64_t N, T *x, T *r, T *z, const T *p, const T *q,
+                             const T *Minv);
+template <typename T>
+hipError_t launch_pcg_direction(jfm_ctx *ctx, int64_t N, T *p, const T *z);
+hipError_t launch_zero_u32(jfm_ctx *ctx, unsigned *p, int64_t n);
+template <typename TI, typename TO>
+hipError_t launch_convert(jfm_ctx *ctx, const TI *in, TO *out, int64_t n, double scale);
+hipError_t launch_residual(jfm_ctx *ctx, const double *b, const double *Ku, const uint8_t *fixed,
+                           double *r, int64_t n);
+hipError_t launch_axpy_scaled(jfm_ctx *ctx, double *u, const float *d, double scale, int64_t n);
+hipError_t launch_dot(jfm_ctx *ctx, const double *a, const double *b, int64_t n, int fin);
+}  // namespace jfem

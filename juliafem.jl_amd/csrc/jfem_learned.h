@@ -1,0 +1,35 @@
+// jfem_learned.h -- learned block preconditioner: weight blob and hooks.
+//
+// Per-node network 24 -> 64 -> 64 -> 6 with GELU, BF16 weights and FP32
+// accumulation.  Its 6 outputs fill a lower-triangular L (diagonal through
+// softplus); the node's preconditioner block is
+//     M_a = C_a (L L' + eps I) C_a',   C_a C_a' = inv(D_a),
+// with D_a the node's 3x3 diagonal block of K, so M_a is SPD for any weights
+// and L = I reproduces block Jacobi up to the factor (1 + eps).
+#pragma once
+
+#include <cstdint>
+
+#include "jfem_internal.h"
+
+namespace jfem {
+
+constexpr int kFeat = 24, kHid = 64, kOut = 6;
+constexpr uint32_t kBlobMagic = 0x4C4D464Au;  // "JFML"
+
+// Blob layout (little endian, no padding): every field in this order.
+struct LearnedBlob {
+    uint32_t magic, version, n_in, n_h1, n_h2, n_out;
+    uint16_t W1[kHid * kFeat];   // bf16, row-major [out][in]
+    uint16_t W2[kHid * kHid];
+    uint16_t W3[kOut * kHid];
+    float b1[kHid], b2[kHid], b3[kOut];
+    float feat_shift[kFeat], feat_scale[kFeat];  // x_norm = (x - shift) * scale
+    float eps;
+};
+
+}  // namespace jfem
+
+// Builds features, runs the network and forms Mlearn64/Mlearn32 for the
+// current mesh and boundary conditions.  *ms receives the device time.
+int jfem_learned_prepare(jfm_ctx *ctx, double *ms);

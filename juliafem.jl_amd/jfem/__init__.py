@@ -1,0 +1,7 @@
+"""jfem: MI355X elasticity path of JuliaFEM (host side of libjfem_hip.so)."""
+from . import mesh
+from ._lib import Context, JfemError, load
+from .problems import (Assembly, Dirichlet, Elasticity, LinearSolver, Problem, assemble, run, solve)
+
+__all__ = ["mesh", "Context", "JfemError", "load", "Assembly", "Dirichlet", "Elasticity", "LinearSolver",
+           "Problem", "assemble", "run", "solve"]
