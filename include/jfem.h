@@ -23,6 +23,9 @@
  *   - Results are deterministic: no floating-point atomics anywhere; every
  *     sum has a fixed order (ascending element id for nodal sums, fixed
  *     reduction trees for dot products), so repeated calls give identical bits.
+ *   - PCG iterations run in batches captured as HIP graphs; setting the
+ *     environment variable JFEM_GRAPHS=0 launches the same kernels one by
+ *     one instead (for tools that do not follow graph launches).
  */
 #ifndef JFEM_H
 #define JFEM_H
@@ -77,6 +80,8 @@ typedef struct {
     int32_t max_outer;    /* refinement-step cap (mixed only) */
     int32_t check_every;  /* iterations per device batch between host checks */
     int32_t learned_probe;/* iterations of the learned-vs-block-Jacobi probe; 0 = none */
+    int32_t inner_maxit;  /* cap on the iterations of one inner solve (mixed); 0 = none */
+    double first_rtol;    /* mixed only: FP64 PCG to this relative residual first; 0 = none */
 } jfm_solve_opts;
 
 typedef struct {
@@ -154,6 +159,11 @@ int jfm_time_apply(jfm_ctx *ctx, int32_t op, int32_t precision, int32_t reps, do
  * f - K u on fixed DOFs (the reference's Lagrange multipliers la[B]). */
 int jfm_solve(jfm_ctx *ctx, const jfm_solve_opts *opts, const double *f, double *u, double *la,
               jfm_solve_stats *stats);
+
+/* Convergence history of the last jfm_solve: per refinement step (or PCG
+ * restart) the cumulative iteration count and the FP64 true relative
+ * residual after it.  Returns the number of entries (<= max_entries). */
+int jfm_solve_history(jfm_ctx *ctx, int32_t *iterations, double *relres, int32_t max_entries);
 
 /* ---- learned preconditioner (no reference counterpart) ---------------
  * Per-node MLP 24 -> 64 -> 64 -> 6 (GELU), BF16 weights, FP32 accumulation.

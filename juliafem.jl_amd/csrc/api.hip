@@ -4,6 +4,8 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <thread>
 
@@ -78,6 +80,8 @@ template <typename F> static void parallel_for(int64_t n, F f) {
     for (auto &x : th) x.join();
 }
 
+static void drop_graphs(jfm_ctx *ctx);
+
 static int sync(jfm_ctx *ctx) {
     JFM_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return JFM_OK;
@@ -111,6 +115,7 @@ extern "C" void jfm_destroy(jfm_ctx *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    drop_graphs(ctx);
     for (void *p : ctx->allocs) (void)hipFree(p);
     ctx->allocs.clear();
     if (ctx->S_host) (void)hipHostFree(ctx->S_host);
@@ -135,6 +140,12 @@ extern "C" int jfm_get_info(jfm_ctx *ctx, jfm_info *info) {
 }
 
 // ============================================================ mesh upload ==
+static void drop_graphs(jfm_ctx *ctx) {
+    for (auto &g : ctx->graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    ctx->graphs.clear();
+}
+
 static void release_mesh(jfm_ctx *ctx) {
     free_ptr(ctx, ctx->X); free_ptr(ctx, ctx->conn); free_ptr(ctx, ctx->E); free_ptr(ctx, ctx->nu);
     free_ptr(ctx, ctx->gf64); free_ptr(ctx, ctx->gf32); free_ptr(ctx, ctx->inc_ptr); free_ptr(ctx, ctx->inc);
@@ -145,7 +156,8 @@ static void release_mesh(jfm_ctx *ctx) {
     free_ptr(ctx, ctx->fixed); free_ptr(ctx, ctx->g); free_ptr(ctx, ctx->face_conn); free_ptr(ctx, ctx->face_t);
     free_ptr(ctx, ctx->finc_ptr); free_ptr(ctx, ctx->finc); free_ptr(ctx, ctx->fe); free_ptr(ctx, ctx->f_ext);
     free_ptr(ctx, ctx->load); free_ptr(ctx, ctx->u64); free_ptr(ctx, ctx->rr64);
-    free_ptr(ctx, ctx->partials); free_ptr(ctx, ctx->counter);
+    free_ptr(ctx, ctx->partials); free_ptr(ctx, ctx->counter); free_ptr(ctx, ctx->inc_slot);
+    drop_graphs(ctx);
     for (Workspace *w : {&ctx->ws64, &ctx->ws32}) {
         free_ptr(ctx, w->x); free_ptr(ctx, w->r); free_ptr(ctx, w->z); free_ptr(ctx, w->p);
         free_ptr(ctx, w->q); free_ptr(ctx, w->b); free_ptr(ctx, w->Minv);
@@ -340,11 +352,18 @@ extern "C" int jfm_upload_mesh(jfm_ctx *ctx, int64_t n_nodes, const double *X, i
     ALLOC(ctx, ctx->vy64, double, 3 * n_nodes);
     ALLOC(ctx, ctx->vx32, float, 3 * n_nodes);
     ALLOC(ctx, ctx->vy32, float, 3 * n_nodes);
-    // fused-reduction scratch: the widest grids are the 8-lanes-per-row SpMV
-    // and the per-DOF kernels
-    ctx->max_blocks = (int)std::max<int64_t>((n_nodes * 8 + kBlock - 1) / kBlock, (3 * n_nodes + kBlock - 1) / kBlock) + 1;
-    ALLOC(ctx, ctx->partials, double, 2 * (int64_t)ctx->max_blocks + 64);
+    ALLOC(ctx, ctx->partials, double, kPartTotal);
     ALLOC(ctx, ctx->counter, unsigned, 64);
+    ALLOC(ctx, ctx->inc_slot, int32_t, n_el * npe);
+    // slot of incidence (e, i) in the node-sorted list, SoA by local index
+    std::vector<int32_t> slot_soa((size_t)n_el * npe);
+    parallel_for(n_nodes, [&](int64_t a0, int64_t a1) {
+        for (int64_t a = a0; a < a1; ++a)
+            for (int k = inc_ptr[a]; k < inc_ptr[a + 1]; ++k) {
+                const int e = inc[k] / npe, i = inc[k] - e * npe;
+                slot_soa[(size_t)i * n_el + e] = k;
+            }
+    });
 
     std::vector<int32_t> conn_soa((size_t)n_el * npe);
     parallel_for(n_el, [&](int64_t a, int64_t b) {
@@ -364,6 +383,7 @@ extern "C" int jfm_upload_mesh(jfm_ctx *ctx, int64_t n_nodes, const double *X, i
     JFM_HIP(ctx, hipMemcpyAsync(ctx->blk_row, blk_row.data(), sizeof(int32_t) * nnzb, hipMemcpyHostToDevice, s));
     JFM_HIP(ctx, hipMemcpyAsync(ctx->cptr, cptr.data(), sizeof(int32_t) * (nnzb + 1), hipMemcpyHostToDevice, s));
     JFM_HIP(ctx, hipMemcpyAsync(ctx->contrib, contrib.data(), sizeof(int32_t) * contrib.size(), hipMemcpyHostToDevice, s));
+    JFM_HIP(ctx, hipMemcpyAsync(ctx->inc_slot, slot_soa.data(), sizeof(int32_t) * slot_soa.size(), hipMemcpyHostToDevice, s));
     JFM_HIP(ctx, hipMemsetAsync(ctx->fixed, 0, 3 * n_nodes, s));
     JFM_HIP(ctx, hipMemsetAsync(ctx->g, 0, sizeof(double) * 3 * n_nodes, s));
     JFM_HIP(ctx, hipMemsetAsync(ctx->f_ext, 0, sizeof(double) * 3 * n_nodes, s));
@@ -502,18 +522,20 @@ static int prepare_op(jfm_ctx *ctx, int op) {
     return jfm_fail(ctx, JFM_EINVAL, "unknown operator variant");
 }
 
-// y = K x on free rows (fixed rows zeroed); fused dot(y, dotv) with `fin`.
+// y = K x on free rows (fixed rows zeroed); optional partials of dot(y, dotv)
+// into `part` (one per block of the last kernel, count in *npart).
 template <typename T>
-static hipError_t op_apply(jfm_ctx *ctx, int op, const T *x, T *y, const T *dotv, int fin, const int *done) {
-    if (op == JFM_OP_CSR) return launch_bsr_spmv<T>(ctx, vals_of<T>(ctx), x, y, dotv, fin, done);
-    return launch_two_phase<T>(ctx, op, x, y, dotv, fin, done);
+static hipError_t op_apply(jfm_ctx *ctx, int op, const T *x, T *y, const T *dotv, double *part, const int *done,
+                           int *npart) {
+    if (op == JFM_OP_CSR) return launch_bsr_spmv<T>(ctx, vals_of<T>(ctx), x, y, dotv, part, done, npart);
+    return launch_two_phase<T>(ctx, op, x, y, dotv, part, done, npart);
 }
 
 // Same without the row mask (all rows), for f_int and reactions.
 static hipError_t op_apply_all_rows(jfm_ctx *ctx, int op, const double *x, double *y) {
     uint8_t *saved = ctx->fixed;
     ctx->fixed = nullptr;
-    hipError_t e = op_apply<double>(ctx, op, x, y, nullptr, FIN_NONE, nullptr);
+    hipError_t e = op_apply<double>(ctx, op, x, y, nullptr, nullptr, nullptr, nullptr);
     ctx->fixed = saved;
     return e;
 }
@@ -528,10 +550,10 @@ extern "C" int jfm_apply(jfm_ctx *ctx, int32_t op, int32_t precision, const doub
     hipStream_t s = ctx->stream;
     JFM_HIP(ctx, hipMemcpyAsync(ctx->vx64, x, sizeof(double) * n, hipMemcpyHostToDevice, s));
     if (precision == JFM_FP64) {
-        CHECK_LAUNCH(ctx, op_apply<double>(ctx, op, ctx->vx64, ctx->vy64, nullptr, FIN_NONE, nullptr));
+        CHECK_LAUNCH(ctx, op_apply<double>(ctx, op, ctx->vx64, ctx->vy64, nullptr, nullptr, nullptr, nullptr));
     } else {
         CHECK_LAUNCH(ctx, (launch_convert<double, float>(ctx, ctx->vx64, ctx->vx32, n, 1.0)));
-        CHECK_LAUNCH(ctx, op_apply<float>(ctx, op, ctx->vx32, ctx->vy32, nullptr, FIN_NONE, nullptr));
+        CHECK_LAUNCH(ctx, op_apply<float>(ctx, op, ctx->vx32, ctx->vy32, nullptr, nullptr, nullptr, nullptr));
         CHECK_LAUNCH(ctx, (launch_convert<float, double>(ctx, ctx->vy32, ctx->vy64, n, 1.0)));
     }
     JFM_HIP(ctx, hipMemcpyAsync(y, ctx->vy64, sizeof(double) * n, hipMemcpyDeviceToHost, s));
@@ -550,21 +572,20 @@ extern "C" int jfm_time_apply(jfm_ctx *ctx, int32_t op, int32_t precision, int32
     JFM_HIP(ctx, hipEventCreate(&t0));
     JFM_HIP(ctx, hipEventCreate(&t1));
     const bool two = op != JFM_OP_CSR;
+    const bool f64 = precision == JFM_FP64;
     auto run = [&](int r, bool timed) -> hipError_t {
         hipError_t e = hipSuccess;
         if (timed) (void)hipEventRecord(ev[3 * r], ctx->stream);
         if (!two) {
-            e = precision == JFM_FP64 ? launch_bsr_spmv<double>(ctx, ctx->vals64, ctx->vx64, ctx->vy64, nullptr, FIN_NONE, nullptr)
-                                      : launch_bsr_spmv<float>(ctx, ctx->vals32, ctx->vx32, ctx->vy32, nullptr, FIN_NONE, nullptr);
+            e = f64 ? launch_bsr_spmv<double>(ctx, ctx->vals64, ctx->vx64, ctx->vy64, nullptr, nullptr, nullptr, nullptr)
+                    : launch_bsr_spmv<float>(ctx, ctx->vals32, ctx->vx32, ctx->vy32, nullptr, nullptr, nullptr, nullptr);
             if (timed) (void)hipEventRecord(ev[3 * r + 1], ctx->stream);
         } else {
-            e = precision == JFM_FP64 ? launch_elem_only<double>(ctx, op, ctx->vx64)
-                                      : launch_elem_only<float>(ctx, op, ctx->vx32);
+            e = f64 ? launch_elem_only<double>(ctx, op, ctx->vx64, nullptr) : launch_elem_only<float>(ctx, op, ctx->vx32, nullptr);
             if (e != hipSuccess) return e;
             if (timed) (void)hipEventRecord(ev[3 * r + 1], ctx->stream);
-            e = precision == JFM_FP64
-                    ? launch_gather<double>(ctx, ctx->inc_ptr, ctx->inc, ctx->N, ctx->ye64, ctx->fixed, ctx->vy64, nullptr, FIN_NONE, nullptr)
-                    : launch_gather<float>(ctx, ctx->inc_ptr, ctx->inc, ctx->N, ctx->ye32, ctx->fixed, ctx->vy32, nullptr, FIN_NONE, nullptr);
+            e = f64 ? launch_gather<double>(ctx, ctx->ye64, ctx->fixed, ctx->vy64, nullptr, nullptr, nullptr, nullptr)
+                    : launch_gather<float>(ctx, ctx->ye32, ctx->fixed, ctx->vy32, nullptr, nullptr, nullptr, nullptr);
         }
         if (timed) (void)hipEventRecord(ev[3 * r + 2], ctx->stream);
         return e;
@@ -662,12 +683,77 @@ static int write_scalars(jfm_ctx *ctx) {
     return sync(ctx);
 }
 
-// PCG on the workspace of precision T: solves K x = b (free DOFs) from x = 0.
-// The iteration runs in device batches of `check_every` with the scalars on
-// the device; the host reads them once per batch.
+// One PCG iteration of parity `slot` (three or four kernels, see
+// linalg_kernels.hip); every kernel returns at once when done[slot] is set.
+template <typename T>
+static hipError_t pcg_iteration(jfm_ctx *ctx, int op, int slot, const T *Minv) {
+    Workspace &w = ws_of<T>(ctx);
+    T *x = (T *)w.x, *r = (T *)w.r, *z = (T *)w.z, *p = (T *)w.p, *q = (T *)w.q;
+    int np_pq = 0, np_rz = 0;
+    hipError_t e = op_apply<T>(ctx, op, p, q, p, ctx->partials + part_pq(slot), &ctx->S->done[slot], &np_pq);
+    if (e != hipSuccess) return e;
+    e = launch_pcg_update<T>(ctx, slot, np_pq, x, r, z, p, q, Minv, &np_rz);
+    if (e != hipSuccess) return e;
+    return launch_pcg_direction<T>(ctx, slot, np_rz, p, z);
+}
+
+// A batch of `batch` (even) iterations captured once as a HIP graph and
+// replayed; kernels past convergence return immediately.
+template <typename T>
+static int pcg_batch(jfm_ctx *ctx, int op, const T *Minv, int batch) {
+    const int prec = sizeof(T) == 8 ? JFM_FP64 : JFM_FP32;
+    for (auto &g : ctx->graphs)
+        if (g.op == op && g.prec == prec && g.minv == (const void *)Minv && g.batch == batch) {
+            JFM_HIP(ctx, hipGraphLaunch(g.exec, ctx->stream));
+            return JFM_OK;
+        }
+    static const bool use_graphs = [] {
+        const char *v = std::getenv("JFEM_GRAPHS");
+        return !(v && v[0] == '0');
+    }();
+    if (!use_graphs) {
+        for (int it = 0; it < batch; ++it) CHECK_LAUNCH(ctx, pcg_iteration<T>(ctx, op, it & 1, Minv));
+        return JFM_OK;
+    }
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool ok = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    hipError_t launch_err = hipSuccess;
+    if (ok) {
+        for (int it = 0; it < batch && launch_err == hipSuccess; ++it) launch_err = pcg_iteration<T>(ctx, op, it & 1, Minv);
+        ok = hipStreamEndCapture(ctx->stream, &graph) == hipSuccess && launch_err == hipSuccess;
+    }
+    if (ok) ok = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipGetLastError();
+    if (!ok) {  // capture unavailable: launch the same kernels directly
+        if (exec) (void)hipGraphExecDestroy(exec);
+        for (int it = 0; it < batch; ++it) CHECK_LAUNCH(ctx, pcg_iteration<T>(ctx, op, it & 1, Minv));
+        return JFM_OK;
+    }
+    jfm_ctx::Graph g;
+    g.op = op;
+    g.prec = prec;
+    g.minv = Minv;
+    g.batch = batch;
+    g.exec = exec;
+    ctx->graphs.push_back(g);
+    JFM_HIP(ctx, hipGraphLaunch(exec, ctx->stream));
+    return JFM_OK;
+}
+
+// sum of the misc partials -> scalars (one-block kernel)
+static int finalize_misc(jfm_ctx *ctx, int npart, int nv, int fin) {
+    CHECK_LAUNCH(ctx, launch_finalize(ctx, ctx->partials + kPartMisc, npart, nv, fin));
+    return JFM_OK;
+}
+
+// PCG on the workspace of precision T: solves K x = b (free DOFs) from x = 0,
+// or from the current x with its true residual (warm, FP64 only).  The host
+// reads the scalars once per batch of `check_every` iterations.
 template <typename T>
 static int pcg_run(jfm_ctx *ctx, int op, const T *Minv, double rtol, int maxit, int check_every, int *iters,
-                   int *status) {
+                   int *status, bool warm = false) {
     Workspace &w = ws_of<T>(ctx);
     T *x = (T *)w.x, *r = (T *)w.r, *z = (T *)w.z, *p = (T *)w.p, *q = (T *)w.q, *b = (T *)w.b;
     const int64_t n = ctx->ndof;
@@ -676,40 +762,36 @@ static int pcg_run(jfm_ctx *ctx, int op, const T *Minv, double rtol, int maxit, 
     ctx->S_host->rtol2 = rtol * rtol;
     ctx->S_host->maxit = maxit;
     if (int rc = write_scalars(ctx)) return rc;
-    JFM_HIP(ctx, hipMemsetAsync(x, 0, sizeof(T) * n, s));
-    JFM_HIP(ctx, hipMemcpyAsync(r, b, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
-    // b.b -> tol2 ; then z = M r, p = z, (r.z, r.r)
-    {
-        // dot in the precision of the vectors, accumulated in double
-        if (sizeof(T) == 8) {
-            CHECK_LAUNCH(ctx, launch_dot(ctx, (const double *)b, (const double *)b, n, FIN_BB));
-        } else {
-            CHECK_LAUNCH(ctx, (launch_convert<float, double>(ctx, (const float *)b, ctx->rr64, n, 1.0)));
-            CHECK_LAUNCH(ctx, launch_dot(ctx, ctx->rr64, ctx->rr64, n, FIN_BB));
-        }
+    int np = 0;
+    if (warm && sizeof(T) == 8) {
+        // restart from the current x with the true residual r = b - K x
+        CHECK_LAUNCH(ctx, op_apply<T>(ctx, op, x, q, nullptr, nullptr, nullptr, nullptr));
+        CHECK_LAUNCH(ctx, launch_residual(ctx, (const double *)b, (const double *)q, ctx->fixed, (double *)r, n, &np));
+    } else {
+        JFM_HIP(ctx, hipMemsetAsync(x, 0, sizeof(T) * n, s));
+        JFM_HIP(ctx, hipMemcpyAsync(r, b, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
     }
-    CHECK_LAUNCH(ctx, launch_pcg_init<T>(ctx, ctx->N, r, z, p, Minv));
-    const int *done = &ctx->S->done;
-    int it = 0;
+    CHECK_LAUNCH(ctx, launch_dot<T>(ctx, b, b, n, &np));
+    if (int rc = finalize_misc(ctx, np, 1, FIN_BB)) return rc;
+    CHECK_LAUNCH(ctx, launch_pcg_init<T>(ctx, r, z, p, Minv, &np));
+    if (int rc = finalize_misc(ctx, np, 2, FIN_INIT)) return rc;
     if (int rc = read_scalars(ctx)) return rc;
-    while (ctx->S_host->done == DONE_RUNNING) {
-        for (int k = 0; k < check_every; ++k) {
-            CHECK_LAUNCH(ctx, op_apply<T>(ctx, op, p, q, p, FIN_ALPHA, done));
-            CHECK_LAUNCH(ctx, launch_pcg_update<T>(ctx, ctx->N, x, r, z, p, q, Minv));
-            CHECK_LAUNCH(ctx, launch_pcg_direction<T>(ctx, ctx->N, p, z));
-        }
+    const int batch = std::max(2, (check_every + 1) & ~1);
+    while (ctx->S_host->done[0] == DONE_RUNNING) {
+        if (int rc = pcg_batch<T>(ctx, op, Minv, batch)) return rc;
         if (int rc = read_scalars(ctx)) return rc;
     }
-    it = ctx->S_host->iter;
-    *iters = it;
-    *status = ctx->S_host->done;
+    *iters = ctx->S_host->iter;
+    *status = ctx->S_host->done[0];
     return JFM_OK;
 }
 
-// FP64 true residual r = f_eff - K u over free DOFs; returns |r| via *rnorm.
+// FP64 true residual r = b - K u over free DOFs; *rnorm = |r|.
 static int true_residual(jfm_ctx *ctx, int op, const double *b, const double *u, double *r, double *rnorm) {
-    CHECK_LAUNCH(ctx, op_apply<double>(ctx, op, u, ctx->vy64, nullptr, FIN_NONE, nullptr));
-    CHECK_LAUNCH(ctx, launch_residual(ctx, b, ctx->vy64, ctx->fixed, r, ctx->ndof));
+    int np = 0;
+    CHECK_LAUNCH(ctx, op_apply<double>(ctx, op, u, ctx->vy64, nullptr, nullptr, nullptr, nullptr));
+    CHECK_LAUNCH(ctx, launch_residual(ctx, b, ctx->vy64, ctx->fixed, r, ctx->ndof, &np));
+    if (int rc = finalize_misc(ctx, np, 1, FIN_DOT)) return rc;
     if (int rc = read_scalars(ctx)) return rc;
     *rnorm = std::sqrt(ctx->S_host->tmp0);
     return JFM_OK;
@@ -723,6 +805,8 @@ extern "C" int jfm_solve(jfm_ctx *ctx, const jfm_solve_opts *opts, const double 
     if (!(opts->rtol > 0.0) || opts->maxit <= 0) return jfm_fail(ctx, JFM_EINVAL, "rtol > 0 and maxit > 0 required");
     if (opts->method != JFM_SOLVE_PCG && opts->method != JFM_SOLVE_MIXED_IR)
         return jfm_fail(ctx, JFM_EINVAL, "unknown solve method");
+    if (opts->precond != JFM_PREC_BLOCK_JACOBI && opts->precond != JFM_PREC_LEARNED)
+        return jfm_fail(ctx, JFM_EINVAL, "unknown preconditioner");
     JFM_HIP(ctx, hipSetDevice(ctx->device));
     const int op = opts->op;
     const int op_outer = opts->method == JFM_SOLVE_MIXED_IR ? opts->op_outer : op;
@@ -731,28 +815,22 @@ extern "C" int jfm_solve(jfm_ctx *ctx, const jfm_solve_opts *opts, const double 
     if (int rc = ensure_workspaces(ctx)) return rc;
     const int64_t n = ctx->ndof;
     hipStream_t s = ctx->stream;
-    const int check = std::max(1, opts->check_every > 0 ? opts->check_every : 16);
+    const int check = std::max(2, opts->check_every > 0 ? opts->check_every : 32);
     jfm_solve_stats st{};
     hipEvent_t e0, e1;
     JFM_HIP(ctx, hipEventCreate(&e0));
     JFM_HIP(ctx, hipEventCreate(&e1));
 
-    // f_eff = f - K g on free rows (elimination of prescribed DOFs, src/solvers.jl:206-210)
-    double *feff = ctx->rr64;  // reused below only after feff is consumed into ws64.b
+    // right-hand side b = f - K g on free rows (elimination of prescribed
+    // DOFs, src/solvers.jl:206-210); f stays in vx64 for the reactions
     if (f) JFM_HIP(ctx, hipMemcpyAsync(ctx->vx64, f, sizeof(double) * n, hipMemcpyHostToDevice, s));
     else JFM_HIP(ctx, hipMemcpyAsync(ctx->vx64, ctx->load, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-    double *bfull = (double *)ctx->ws64.b;
+    double *b = (double *)ctx->ws64.b;
     JFM_HIP(ctx, hipEventRecord(e0, s));
-    if (ctx->g_nonzero) {
-        CHECK_LAUNCH(ctx, op_apply<double>(ctx, op_outer, ctx->g, ctx->vy64, nullptr, FIN_NONE, nullptr));
-        CHECK_LAUNCH(ctx, launch_residual(ctx, ctx->vx64, ctx->vy64, ctx->fixed, bfull, n));
-    } else {
-        JFM_HIP(ctx, hipMemsetAsync(ctx->vy64, 0, sizeof(double) * n, s));
-        CHECK_LAUNCH(ctx, launch_residual(ctx, ctx->vx64, ctx->vy64, ctx->fixed, bfull, n));
-    }
-    (void)feff;
-    if (int rc = read_scalars(ctx)) return rc;
-    const double bnorm = std::sqrt(ctx->S_host->tmp0);
+    double bnorm = 0.0;
+    if (int rc = true_residual(ctx, op_outer, ctx->vx64, ctx->g, b, &bnorm)) return rc;
+    ctx->hist_it.clear();
+    ctx->hist_res.clear();
     const double *Minv64 = (const double *)ctx->ws64.Minv;
     const float *Minv32 = (const float *)ctx->ws32.Minv;
     if (opts->precond == JFM_PREC_LEARNED) {
@@ -763,23 +841,28 @@ extern "C" int jfm_solve(jfm_ctx *ctx, const jfm_solve_opts *opts, const double 
 
     if (bnorm == 0.0) {
         JFM_HIP(ctx, hipMemcpyAsync(ctx->u64, ctx->g, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-        st.converged = 1;
     } else if (opts->method == JFM_SOLVE_PCG) {
-        int it = 0, status = 0;
+        int it = 0, status = 0, total = 0, restarts = 0;
         if (int rc = pcg_run<double>(ctx, op, Minv64, opts->rtol, opts->maxit, check, &it, &status)) return rc;
-        st.iterations = it;
-        st.outer = 1;
-        // u = x + g
-        JFM_HIP(ctx, hipMemcpyAsync(ctx->u64, ctx->ws64.x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-        if (ctx->g_nonzero) {
-            std::vector<double> a(n), gg(n);
-            JFM_HIP(ctx, hipMemcpyAsync(a.data(), ctx->u64, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-            JFM_HIP(ctx, hipMemcpyAsync(gg.data(), ctx->g, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-            if (int rc = sync(ctx)) return rc;
-            for (int64_t i = 0; i < n; ++i) a[i] += gg[i];
-            JFM_HIP(ctx, hipMemcpyAsync(ctx->u64, a.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
-            if (int rc = sync(ctx)) return rc;
+        total = it;
+        // The recursive residual drifts from the true one near the FP64 floor:
+        // restart from x with the true residual until it meets rtol (residual
+        // replacement), a bounded number of times.
+        while (status == DONE_CONVERGED && restarts < 8 && total < opts->maxit) {
+            double rn = 0.0;
+            if (int rc = true_residual(ctx, op, b, (const double *)ctx->ws64.x, ctx->rr64, &rn)) return rc;
+            ctx->hist_it.push_back(total);
+            ctx->hist_res.push_back(rn / bnorm);
+            if (rn <= opts->rtol * bnorm) break;
+            if (int rc = pcg_run<double>(ctx, op, Minv64, opts->rtol, opts->maxit - total, check, &it, &status, true))
+                return rc;
+            total += it;
+            ++restarts;
         }
+        st.iterations = total;
+        st.outer = 1 + restarts;
+        // u = x + g
+        CHECK_LAUNCH(ctx, launch_add(ctx, ctx->g, (const double *)ctx->ws64.x, ctx->u64, n));
     } else {
         // FP64 iterative refinement around FP32 PCG: r = f - K u in FP64,
         // K d = r / |r| in FP32 to inner_rtol, u += |r| d.
@@ -787,13 +870,25 @@ extern "C" int jfm_solve(jfm_ctx *ctx, const jfm_solve_opts *opts, const double 
         int total = 0, outer = 0;
         const double inner = opts->inner_rtol > 0 ? opts->inner_rtol : 1e-4;
         const int max_outer = opts->max_outer > 0 ? opts->max_outer : 20;
+        if (opts->first_rtol > 0.0) {
+            // FP64 phase for the smooth low modes that FP32 cannot resolve
+            int it = 0, status = 0;
+            if (int rc = pcg_run<double>(ctx, op_outer, Minv64, opts->first_rtol, opts->maxit, check, &it, &status))
+                return rc;
+            total += it;
+            CHECK_LAUNCH(ctx, launch_add(ctx, ctx->g, (const double *)ctx->ws64.x, ctx->u64, n));
+        }
         while (outer < max_outer) {
             double rn = 0.0;
             if (int rc = true_residual(ctx, op_outer, ctx->vx64, ctx->u64, ctx->rr64, &rn)) return rc;
+            ctx->hist_it.push_back(total);
+            ctx->hist_res.push_back(rn / bnorm);
             if (rn <= opts->rtol * bnorm || total >= opts->maxit) break;
             CHECK_LAUNCH(ctx, (launch_convert<double, float>(ctx, ctx->rr64, (float *)ctx->ws32.b, n, 1.0 / rn)));
             int it = 0, status = 0;
-            if (int rc = pcg_run<float>(ctx, op, Minv32, inner, opts->maxit - total, check, &it, &status)) return rc;
+            int cap = opts->maxit - total;
+            if (opts->inner_maxit > 0) cap = std::min(cap, opts->inner_maxit);
+            if (int rc = pcg_run<float>(ctx, op, Minv32, inner, cap, check, &it, &status)) return rc;
             total += it;
             ++outer;
             CHECK_LAUNCH(ctx, launch_axpy_scaled(ctx, ctx->u64, (const float *)ctx->ws32.x, rn, n));
@@ -818,7 +913,7 @@ extern "C" int jfm_solve(jfm_ctx *ctx, const jfm_solve_opts *opts, const double 
     if (la) {
         // la = f - K u on fixed DOFs (src/solvers.jl:212-213), 0 elsewhere
         CHECK_LAUNCH(ctx, op_apply_all_rows(ctx, op_outer, ctx->u64, ctx->vy64));
-        std::vector<double> ku(n), ff(n), fx(0);
+        std::vector<double> ku(n), ff(n);
         std::vector<uint8_t> fxd(n);
         JFM_HIP(ctx, hipMemcpyAsync(ku.data(), ctx->vy64, sizeof(double) * n, hipMemcpyDeviceToHost, s));
         JFM_HIP(ctx, hipMemcpyAsync(ff.data(), ctx->vx64, sizeof(double) * n, hipMemcpyDeviceToHost, s));
@@ -828,6 +923,21 @@ extern "C" int jfm_solve(jfm_ctx *ctx, const jfm_solve_opts *opts, const double 
     }
     if (int rc = sync(ctx)) return rc;
     if (stats) *stats = st;
-    if (!st.converged) return jfm_fail(ctx, JFM_ENOCONV, "solver did not reach rtol (relres " + std::to_string(st.relres) + ")");
+    if (!st.converged) {
+        char msg[160];
+        std::snprintf(msg, sizeof msg, "solver did not reach rtol %.3e: relres %.3e after %d iterations", opts->rtol,
+                      st.relres, st.iterations);
+        return jfm_fail(ctx, JFM_ENOCONV, msg);
+    }
     return JFM_OK;
+}
+
+extern "C" int jfm_solve_history(jfm_ctx *ctx, int32_t *iterations, double *relres, int32_t max_entries) {
+    if (!ctx || max_entries < 0) return JFM_EINVAL;
+    const int n = std::min<int>(max_entries, (int)ctx->hist_it.size());
+    for (int k = 0; k < n; ++k) {
+        if (iterations) iterations[k] = ctx->hist_it[k];
+        if (relres) relres[k] = ctx->hist_res[k];
+    }
+    return n;
 }

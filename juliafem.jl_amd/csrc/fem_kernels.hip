@@ -15,6 +15,8 @@
 //
 // Every nodal sum runs over the node's incidences in ascending element id;
 // there are no floating-point atomics.
+#include <algorithm>
+
 #include "jfem_device.h"
 
 namespace jfem {
@@ -112,13 +114,16 @@ __global__ __launch_bounds__(256) void k_geom(int n_el, int ngf, const int32_t *
 
 // ---------------------------------------------------- matrix-free element --
 // Phase 1 of the two-phase operator: y_e = K_e x_e from the geometric factors.
-// H_q = sum_i x_i (x) dN_i(xi_q) (reference gradient), grad x = H_q invJ',
+// H_q = sum_i (x_i - x_0) (x) dN_i(xi_q) (reference gradient; sum_i dN_i = 0,
+// so subtracting node 0's value removes the element's translation, which in
+// FP32 would otherwise cancel away the strain), grad x = H_q invJ',
 // sigma_w = w detJ (lambda tr(eps) I + 2 mu eps), P_q = invJ' sigma_w,
 // y_i = sum_q P_q' dN_i(xi_q).
 template <int NPE, typename T, bool AFF>
 __global__ __launch_bounds__(256) void k_mf_elem(int n_el, const int32_t *__restrict__ conn,
-                                                const T *__restrict__ gf, const T *__restrict__ x,
-                                                T *__restrict__ ye, const int *__restrict__ done) {
+                                                const int32_t *__restrict__ slot, const T *__restrict__ gf,
+                                                const T *__restrict__ x, T *__restrict__ ye,
+                                                const int *__restrict__ done) {
     using EL = Elem<NPE>;
     constexpr int NG = EL::NGP;
     if (done && *done) return;
@@ -129,10 +134,12 @@ __global__ __launch_bounds__(256) void k_mf_elem(int n_el, const int32_t *__rest
     for (int q = 0; q < NG; ++q)
 #pragma unroll
         for (int k = 0; k < 9; ++k) H[q][k] = T(0);
+    const int n0 = conn[e];
+    const T t0 = x[3 * (size_t)n0], t1 = x[3 * (size_t)n0 + 1], t2 = x[3 * (size_t)n0 + 2];
 #pragma unroll
-    for (int i = 0; i < NPE; ++i) {
+    for (int i = 1; i < NPE; ++i) {
         const int nd = conn[(size_t)i * n_el + e];
-        const T u0 = x[3 * (size_t)nd], u1 = x[3 * (size_t)nd + 1], u2 = x[3 * (size_t)nd + 2];
+        const T u0 = x[3 * (size_t)nd] - t0, u1 = x[3 * (size_t)nd + 1] - t1, u2 = x[3 * (size_t)nd + 2] - t2;
 #pragma unroll
         for (int q = 0; q < NG; ++q)
 #pragma unroll
@@ -177,9 +184,9 @@ __global__ __launch_bounds__(256) void k_mf_elem(int n_el, const int32_t *__rest
             for (int c = 0; c < 3; ++c)
                 H[q][d * 3 + c] = Ji[d] * s[c] + Ji[3 + d] * s[3 + c] + Ji[6 + d] * s[6 + c];
     }
-    T *out = ye + (size_t)e * NPE * 3;
 #pragma unroll
     for (int i = 0; i < NPE; ++i) {
+        T *out = ye + 3 * (size_t)slot[(size_t)i * n_el + e];
         T f0 = T(0), f1 = T(0), f2 = T(0);
 #pragma unroll
         for (int q = 0; q < NG; ++q)
@@ -192,9 +199,9 @@ __global__ __launch_bounds__(256) void k_mf_elem(int n_el, const int32_t *__rest
                     f2 += T(dd) * H[q][d * 3 + 2];
                 }
             }
-        out[3 * i] = f0;
-        out[3 * i + 1] = f1;
-        out[3 * i + 2] = f2;
+        out[0] = f0;
+        out[1] = f1;
+        out[2] = f2;
     }
 }
 
@@ -202,19 +209,23 @@ __global__ __launch_bounds__(256) void k_mf_elem(int n_el, const int32_t *__rest
 // Phase 1 with stored element matrices: y_e = K_e x_e (upper blocks stored).
 template <int NPE, typename T>
 __global__ __launch_bounds__(256) void k_ebe_elem(int n_el, const int32_t *__restrict__ conn,
-                                                 const double *__restrict__ Ke, const T *__restrict__ x,
-                                                 T *__restrict__ ye, const int *__restrict__ done) {
+                                                 const int32_t *__restrict__ slot, const double *__restrict__ Ke,
+                                                 const T *__restrict__ x, T *__restrict__ ye,
+                                                 const int *__restrict__ done) {
     using EL = Elem<NPE>;
     constexpr PairTable<NPE> PT = make_pairs<NPE>();
     if (done && *done) return;
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n_el) return;
-    T u[NPE][3], f[NPE][3];
+    // K_e annihilates translations: subtracting node 0's value keeps FP32 accurate
+    T u[NPE][3], f[NPE][3], t[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) t[c] = x[3 * (size_t)conn[e] + c];
 #pragma unroll
     for (int i = 0; i < NPE; ++i) {
         const int nd = conn[(size_t)i * n_el + e];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) { u[i][c] = x[3 * (size_t)nd + c]; f[i][c] = T(0); }
+        for (int c = 0; c < 3; ++c) { u[i][c] = x[3 * (size_t)nd + c] - t[c]; f[i][c] = T(0); }
     }
     const double *K = Ke + (size_t)e * EL::NPAIR * 9;
 #pragma unroll
@@ -230,51 +241,70 @@ __global__ __launch_bounds__(256) void k_ebe_elem(int n_el, const int32_t *__res
             for (int c = 0; c < 3; ++c) f[j][c] += B[c] * u[i][0] + B[3 + c] * u[i][1] + B[6 + c] * u[i][2];
         }
     }
-    T *out = ye + (size_t)e * NPE * 3;
 #pragma unroll
-    for (int i = 0; i < NPE; ++i)
+    for (int i = 0; i < NPE; ++i) {
+        T *out = ye + 3 * (size_t)slot[(size_t)i * n_el + e];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) out[3 * i + c] = f[i][c];
+        for (int c = 0; c < 3; ++c) out[c] = f[i][c];
+    }
 }
 
 // ------------------------------------------------------------ nodal gather --
-// Phase 2: y[a] = sum over incidences (e, i) of node a, ascending e, of
-// ye[e][i]; fixed DOFs zeroed; optional fused dot(y, dotv) with finalize.
+// Phase 2: y[a] = sum over the incidences (e, i) of node a, in ascending
+// element id, of the element results.  Phase 1 wrote the result of incidence
+// (e, i) into its slot k of the node-sorted incidence list (ptr/inc order),
+// so a node's terms are contiguous and the gather is a streaming segmented
+// sum.  Fixed DOFs are zeroed; optional partials of dot(y, dotv).
 template <typename T>
-__global__ __launch_bounds__(256) void k_gather(int nrows, const int32_t *__restrict__ ptr,
-                                               const int32_t *__restrict__ inc, const T *__restrict__ ye,
+__global__ __launch_bounds__(256) void k_gather(int N, const int32_t *__restrict__ ptr, const T *__restrict__ yinc,
                                                const uint8_t *__restrict__ fixed, T *__restrict__ y,
-                                               const T *__restrict__ dotv, double *partials,
-                                               unsigned *counter, PcgScalars *S, int fin,
+                                               const T *__restrict__ dotv, double *__restrict__ part,
                                                const int *__restrict__ done) {
     if (done && *done) return;
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
     double dv = 0.0;
-    if (a < nrows) {
+    for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < N; a += gridDim.x * blockDim.x) {
+        const int k0 = ptr[a], k1 = ptr[a + 1];
         T s0 = T(0), s1 = T(0), s2 = T(0);
-        const int k1 = ptr[a + 1];
-        for (int k = ptr[a]; k < k1; ++k) {
-            const size_t idx = 3 * (size_t)inc[k];
-            s0 += ye[idx];
-            s1 += ye[idx + 1];
-            s2 += ye[idx + 2];
+        const T *v = yinc + 3 * (size_t)k0;
+#pragma unroll 4
+        for (int k = 0; k < k1 - k0; ++k) {
+            s0 += v[3 * k];
+            s1 += v[3 * k + 1];
+            s2 += v[3 * k + 2];
         }
+        const size_t o = 3 * (size_t)a;
         if (fixed) {
-            if (fixed[3 * (size_t)a]) s0 = T(0);
-            if (fixed[3 * (size_t)a + 1]) s1 = T(0);
-            if (fixed[3 * (size_t)a + 2]) s2 = T(0);
+            if (fixed[o]) s0 = T(0);
+            if (fixed[o + 1]) s1 = T(0);
+            if (fixed[o + 2]) s2 = T(0);
         }
-        y[3 * (size_t)a] = s0;
-        y[3 * (size_t)a + 1] = s1;
-        y[3 * (size_t)a + 2] = s2;
-        if (dotv)
-            dv = (double)s0 * (double)dotv[3 * (size_t)a] + (double)s1 * (double)dotv[3 * (size_t)a + 1] +
-                 (double)s2 * (double)dotv[3 * (size_t)a + 2];
+        y[o] = s0;
+        y[o + 1] = s1;
+        y[o + 2] = s2;
+        if (dotv) dv += (double)s0 * (double)dotv[o] + (double)s1 * (double)dotv[o + 1] + (double)s2 * (double)dotv[o + 2];
     }
-    if (fin != FIN_NONE) {
-        double v[1] = {dv};
-        reduce_finalize<1>(v, partials, counter, S, fin);
+    if (part) {
+        double v1[1] = {dv};
+        write_partials<1>(v1, part);
     }
+}
+
+// Thread-per-node gather (face loads; not on the iteration path).
+__global__ __launch_bounds__(256) void k_gather_simple(int nrows, const int32_t *__restrict__ ptr,
+                                                      const int32_t *__restrict__ inc, const double *__restrict__ ye,
+                                                      double *__restrict__ y) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nrows) return;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int k = ptr[a]; k < ptr[a + 1]; ++k) {
+        const size_t idx = 3 * (size_t)inc[k];
+        s0 += ye[idx];
+        s1 += ye[idx + 1];
+        s2 += ye[idx + 2];
+    }
+    y[3 * (size_t)a] = s0;
+    y[3 * (size_t)a + 1] = s1;
+    y[3 * (size_t)a + 2] = s2;
 }
 
 // --------------------------------------------------------- element matrices --
@@ -562,8 +592,13 @@ hipError_t launch_face_loads(jfm_ctx *ctx) {
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    return launch_gather<double>(ctx, ctx->finc_ptr, ctx->finc, ctx->N, ctx->fe, nullptr, ctx->f_ext,
-                                 nullptr, FIN_NONE, nullptr);
+    return launch_face_gather(ctx);
+}
+
+hipError_t launch_face_gather(jfm_ctx *ctx) {
+    const int N = (int)ctx->N;
+    k_gather_simple<<<nblk(N), kBlock, 0, ctx->stream>>>(N, ctx->finc_ptr, ctx->finc, ctx->fe, ctx->f_ext);
+    return hipGetLastError();
 }
 
 hipError_t launch_block_jacobi(jfm_ctx *ctx, double *Minv64, float *Minv32) {
@@ -578,11 +613,11 @@ hipError_t launch_block_jacobi(jfm_ctx *ctx, double *Minv64, float *Minv32) {
 }
 
 template <typename T>
-hipError_t launch_gather(jfm_ctx *ctx, const int32_t *ptr, const int32_t *inc, int64_t nrows, const T *ye,
-                         const uint8_t *fixed, T *y, const T *dotv, int fin, const int *done) {
-    if (nrows == 0) return hipSuccess;
-    k_gather<T><<<nblk(nrows), kBlock, 0, ctx->stream>>>((int)nrows, ptr, inc, ye, fixed, y, dotv,
-                                                        ctx->partials, ctx->counter, ctx->S, fin, done);
+hipError_t launch_gather(jfm_ctx *ctx, const T *ye, const uint8_t *fixed, T *y, const T *dotv, double *part,
+                         const int *done, int *npart) {
+    const unsigned g = capped_grid(ctx->N, kBlock);
+    if (npart) *npart = (int)g;
+    k_gather<T><<<g, kBlock, 0, ctx->stream>>>((int)ctx->N, ctx->inc_ptr, ye, fixed, y, dotv, part, done);
     return hipGetLastError();
 }
 
@@ -597,47 +632,39 @@ template <int NPE, typename T>
 static void launch_elem_phase(jfm_ctx *ctx, int op, const T *x, T *ye, const int *done) {
     const int n = (int)ctx->n_el;
     if (op == JFM_OP_EBE) {
-        k_ebe_elem<NPE, T><<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->conn, ctx->Ke, x, ye, done);
+        k_ebe_elem<NPE, T><<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->conn, ctx->inc_slot, ctx->Ke, x, ye, done);
     } else if (ctx->affine) {
-        k_mf_elem<NPE, T, true><<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->conn, gf_of<T>(ctx), x, ye, done);
+        k_mf_elem<NPE, T, true><<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->conn, ctx->inc_slot, gf_of<T>(ctx), x, ye, done);
     } else {
-        k_mf_elem<NPE, T, false><<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->conn, gf_of<T>(ctx), x, ye, done);
+        k_mf_elem<NPE, T, false><<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->conn, ctx->inc_slot, gf_of<T>(ctx), x, ye, done);
     }
 }
 
 template <typename T>
-hipError_t launch_two_phase(jfm_ctx *ctx, int op, const T *x, T *y, const T *dotv, int fin, const int *done) {
-    T *ye = ye_of<T>(ctx);
-    if (ctx->n_el > 0) {
-        switch (ctx->npe) {
-        case 10: launch_elem_phase<10, T>(ctx, op, x, ye, done); break;
-        case 4: launch_elem_phase<4, T>(ctx, op, x, ye, done); break;
-        case 8: launch_elem_phase<8, T>(ctx, op, x, ye, done); break;
-        default: return hipErrorInvalidValue;
-        }
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-    }
-    return launch_gather<T>(ctx, ctx->inc_ptr, ctx->inc, ctx->N, ye, ctx->fixed, y, dotv, fin, done);
-}
-
-template <typename T>
-hipError_t launch_elem_only(jfm_ctx *ctx, int op, const T *x) {
+hipError_t launch_elem_only(jfm_ctx *ctx, int op, const T *x, const int *done) {
     if (ctx->n_el == 0) return hipSuccess;
     switch (ctx->npe) {
-    case 10: launch_elem_phase<10, T>(ctx, op, x, ye_of<T>(ctx), nullptr); break;
-    case 4: launch_elem_phase<4, T>(ctx, op, x, ye_of<T>(ctx), nullptr); break;
-    case 8: launch_elem_phase<8, T>(ctx, op, x, ye_of<T>(ctx), nullptr); break;
+    case 10: launch_elem_phase<10, T>(ctx, op, x, ye_of<T>(ctx), done); break;
+    case 4: launch_elem_phase<4, T>(ctx, op, x, ye_of<T>(ctx), done); break;
+    case 8: launch_elem_phase<8, T>(ctx, op, x, ye_of<T>(ctx), done); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-template hipError_t launch_elem_only<double>(jfm_ctx *, int, const double *);
-template hipError_t launch_elem_only<float>(jfm_ctx *, int, const float *);
-template hipError_t launch_two_phase<double>(jfm_ctx *, int, const double *, double *, const double *, int, const int *);
-template hipError_t launch_two_phase<float>(jfm_ctx *, int, const float *, float *, const float *, int, const int *);
-template hipError_t launch_gather<double>(jfm_ctx *, const int32_t *, const int32_t *, int64_t, const double *, const uint8_t *, double *, const double *, int, const int *);
-template hipError_t launch_gather<float>(jfm_ctx *, const int32_t *, const int32_t *, int64_t, const float *, const uint8_t *, float *, const float *, int, const int *);
+template <typename T>
+hipError_t launch_two_phase(jfm_ctx *ctx, int op, const T *x, T *y, const T *dotv, double *part, const int *done,
+                            int *npart) {
+    hipError_t e = launch_elem_only<T>(ctx, op, x, done);
+    if (e != hipSuccess) return e;
+    return launch_gather<T>(ctx, ye_of<T>(ctx), ctx->fixed, y, dotv, part, done, npart);
+}
+
+template hipError_t launch_elem_only<double>(jfm_ctx *, int, const double *, const int *);
+template hipError_t launch_elem_only<float>(jfm_ctx *, int, const float *, const int *);
+template hipError_t launch_two_phase<double>(jfm_ctx *, int, const double *, double *, const double *, double *, const int *, int *);
+template hipError_t launch_two_phase<float>(jfm_ctx *, int, const float *, float *, const float *, double *, const int *, int *);
+template hipError_t launch_gather<double>(jfm_ctx *, const double *, const uint8_t *, double *, const double *, double *, const int *, int *);
+template hipError_t launch_gather<float>(jfm_ctx *, const float *, const uint8_t *, float *, const float *, double *, const int *, int *);
 
 }  // namespace jfem

@@ -1,10 +1,10 @@
 // jfem_device.h -- device helpers shared by the kernel files.
 //
-// Reductions are deterministic: a fixed xor-butterfly inside each wave, the
-// waves of a block in index order, and the per-block partials summed by the
-// last block to finish (found with an integer atomic counter) in a fixed
-// strided order.  Which block happens to finish last does not change the
-// order of any floating-point addition.
+// Reductions are deterministic and free of atomics: a producer kernel writes
+// one partial per block (fixed xor-butterfly inside each wave, waves summed in
+// index order), and every block of the consuming kernel sums the same
+// partials in the same fixed order, so all blocks obtain bit-identical
+// scalars without a grid-wide synchronisation or a memory fence.
 #pragma once
 
 #include "jfem_internal.h"
@@ -19,9 +19,9 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Sum over the block; the result is valid in thread 0 only.
-template <int NV>
-__device__ __forceinline__ void block_sum(double (&v)[NV], double (*sh)[16]) {
+// Sum over the block, returned to every thread.
+template <int NV> __device__ __forceinline__ void block_sum_all(double (&v)[NV]) {
+    __shared__ double sh[NV][16];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int nw = (blockDim.x + 63) >> 6;
 #pragma unroll
@@ -31,89 +31,45 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double (*sh)[16]) {
         for (int k = 0; k < NV; ++k) sh[k][wid] = v[k];
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
 #pragma unroll
-        for (int k = 0; k < NV; ++k) {
-            double s = 0.0;
-            for (int w = 0; w < nw; ++w) s += sh[k][w];
-            v[k] = s;
-        }
+    for (int k = 0; k < NV; ++k) {
+        double s = 0.0;
+        for (int w = 0; w < nw; ++w) s += sh[k][w];
+        v[k] = s;
     }
     __syncthreads();
+}
+
+// Block sums of v[] -> part[k*kMaxPart + blockIdx.x].  Called by all threads.
+template <int NV> __device__ __forceinline__ void write_partials(double (&v)[NV], double *part) {
+    block_sum_all<NV>(v);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) part[k * kMaxPart + blockIdx.x] = v[k];
+    }
+}
+
+// Sum of npart partials per value, fixed order, returned to every thread.
+template <int NV>
+__device__ __forceinline__ void sum_partials(const double *__restrict__ part, int npart, double (&out)[NV]) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double s = 0.0;
+        for (int i = threadIdx.x; i < npart; i += blockDim.x) s += part[k * kMaxPart + i];
+        out[k] = s;
+    }
+    block_sum_all<NV>(out);
 }
 
 __device__ __forceinline__ bool finite(double x) { return x - x == 0.0; }
 
-__device__ inline void finalize(PcgScalars *S, int fin, const double *t) {
-    switch (fin) {
-    case FIN_ALPHA:
-        S->pq = t[0];
-        if (t[0] > 0.0 && finite(t[0])) {
-            S->alpha = S->rz / t[0];
-        } else {
-            S->alpha = 0.0;
-            S->done = DONE_BREAKDOWN;
-        }
-        break;
-    case FIN_UPDATE: {
-        const double rzn = t[0];
-        S->rr = t[1];
-        S->iter += 1;
-        S->beta = rzn / S->rz;
-        S->rz = rzn;
-        if (S->rr <= S->tol2) S->done = DONE_CONVERGED;
-        else if (!(rzn > 0.0) || !finite(rzn)) S->done = DONE_BREAKDOWN;
-        else if (S->iter >= S->maxit) S->done = DONE_MAXIT;
-        break;
-    }
-    case FIN_INIT:
-        S->rz = t[0];
-        S->rr = t[1];
-        S->iter = 0;
-        S->beta = 0.0;
-        S->done = (S->rr <= S->tol2) ? DONE_CONVERGED : DONE_RUNNING;
-        break;
-    case FIN_DOT: S->tmp0 = t[0]; break;
-    case FIN_DOT2: S->tmp0 = t[0]; S->tmp1 = t[1]; break;
-    case FIN_BB: S->bb = t[0]; S->tol2 = S->rtol2 * t[0]; break;
-    default: break;
-    }
+// Grid size for a grid-stride producer kernel: enough blocks for `work`
+// items at `per_block` items per block, capped at kMaxPart.
+inline unsigned capped_grid(int64_t work, int per_block) {
+    int64_t g = (work + per_block - 1) / per_block;
+    if (g < 1) g = 1;
+    if (g > kMaxPart) g = kMaxPart;
+    return (unsigned)g;
 }
-
-// Block sums of v[] go to partials[k*gridDim.x + blockIdx.x]; the last block
-// to arrive sums them and applies `fin` to the scalars.
-template <int NV>
-__device__ __forceinline__ void reduce_finalize(double (&v)[NV], double *partials, unsigned *counter,
-                                                PcgScalars *S, int fin) {
-    __shared__ double sh[NV][16];
-    __shared__ int s_last;
-    block_sum<NV>(v, sh);
-    const int nb = gridDim.x;
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; ++k) partials[k * nb + blockIdx.x] = v[k];
-        __threadfence();
-        const unsigned prev = atomicAdd(counter, 1u);
-        s_last = (prev == (unsigned)(nb - 1));
-    }
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-    double t[NV];
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        double s = 0.0;
-        for (int i = threadIdx.x; i < nb; i += blockDim.x)
-            s += __hip_atomic_load(partials + k * nb + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        t[k] = s;
-    }
-    block_sum<NV>(t, sh);
-    if (threadIdx.x == 0) {
-        *counter = 0u;
-        finalize(S, fin, t);
-    }
-}
-
-template <typename T> struct Vec3 { T x, y, z; };
 
 }  // namespace jfem

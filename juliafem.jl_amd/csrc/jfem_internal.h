@@ -12,7 +12,9 @@
 //                                 3a+c starts at 9*brow[a] + c*3*nb, entry
 //                                 (3a+c, 3*bcol[s]+d) at + 3*(s-brow[a]) + d
 //   Ke       n_el x NPAIR x 9     element matrix blocks (i <= j), row-major 3x3
-//   ye       n_el x npe x 3       element results of the two-phase operators
+//   ye       (n_el*npe) x 3       element results of the two-phase operators,
+//                                 in node-sorted incidence order (slot k of
+//                                 inc), so each node's terms are contiguous
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -90,27 +92,29 @@ template <> struct Elem<8> {
 constexpr int GF = 11;  // invJ(9), lambda*w*detJ, mu*w*detJ
 
 // --------------------------------------------------------- PCG scalars ----
+// Iteration-parity slots: kernels of iteration i read slot i&1 and the last
+// kernel of the iteration writes slot (i+1)&1, so no kernel reads a value that
+// another block of the same kernel may be writing.
 struct PcgScalars {
-    double rz;     // r.z of the current iterate
-    double pq;     // p.q
-    double alpha, beta;
-    double rr;     // r.r
-    double bb;     // b.b over free DOFs
-    double tol2;   // rtol^2 * bb
-    double rtol2;  // set by the host before FIN_BB
-    double tmp0, tmp1;
-    int iter, done, maxit, pad;
+    double rz[2];       // r.z entering the iteration of that parity
+    double alpha, beta, pq, rr;   // last values (diagnostics)
+    double bb;          // b.b over free DOFs
+    double tol2;        // rtol^2 * bb
+    double rtol2;       // set by the host
+    double tmp0, tmp1;  // results of one-off reductions
+    int done[2];        // 0 running, DONE_* otherwise, by parity
+    int iter, maxit;
 };
 // done codes
 enum { DONE_RUNNING = 0, DONE_CONVERGED = 1, DONE_BREAKDOWN = 2, DONE_MAXIT = 3 };
-// finalize modes of the fused reductions
-enum { FIN_NONE = 0, FIN_ALPHA = 1, FIN_UPDATE = 2, FIN_INIT = 3, FIN_DOT = 4, FIN_DOT2 = 5, FIN_BB = 6 };
-
-struct Reduce {           // scratch of the fused last-block reductions
-    double *partials;     // >= 2 * max blocks
-    unsigned *counter;    // one counter, returns to 0 after every reduction
-    PcgScalars *S;        // device scalars
-};
+// what a one-block finalize kernel does with the summed partials
+enum { FIN_NONE = 0, FIN_INIT = 3, FIN_DOT = 4, FIN_DOT2 = 5, FIN_BB = 6 };
+// partials layout (doubles): [PQ slot0 | PQ slot1 | RZ slot0 (2) | RZ slot1 (2) | misc (2)]
+constexpr int kMaxPart = 1024;      // producer grids are capped at this many blocks
+inline constexpr int part_pq(int slot) { return slot * kMaxPart; }
+inline constexpr int part_rz(int slot) { return 2 * kMaxPart + slot * 2 * kMaxPart; }
+constexpr int kPartMisc = 6 * kMaxPart;
+constexpr int kPartTotal = 8 * kMaxPart;
 
 // --------------------------------------------------------------- context --
 struct Workspace {        // PCG vectors of one precision
@@ -140,6 +144,7 @@ struct jfm_ctx {
     float *gf32 = nullptr;
     int ngf = 0;                  // factor sets per element (1 if affine else ngp)
     int32_t *inc_ptr = nullptr, *inc = nullptr;
+    int32_t *inc_slot = nullptr;  // SoA npe x n_el: slot in inc of incidence (e, i)
 
     // pattern and assembly
     int64_t nnzb = 0;
@@ -178,9 +183,13 @@ struct jfm_ctx {
     double *rr64 = nullptr;        // outer residual (mixed IR)
     jfem::PcgScalars *S = nullptr;
     jfem::PcgScalars *S_host = nullptr;  // pinned
-    double *partials = nullptr;
-    unsigned *counter = nullptr;
-    int max_blocks = 0;
+    double *partials = nullptr;          // kPartTotal doubles
+    unsigned *counter = nullptr;         // small integer scratch (element checks)
+    // captured PCG iteration batches, keyed by (op, precision, preconditioner)
+    struct Graph { int op = -1, prec = -1, batch = 0; const void *minv = nullptr; hipGraphExec_t exec = nullptr; };
+    std::vector<Graph> graphs;
+    std::vector<int32_t> hist_it;          // convergence history of the last solve
+    std::vector<double> hist_res;
 
     // learned preconditioner
     void *mlp_w = nullptr;         // weight blob on the device
@@ -212,32 +221,37 @@ hipError_t launch_element_matrices(jfm_ctx *ctx);
 hipError_t launch_assemble_values(jfm_ctx *ctx);
 hipError_t launch_face_loads(jfm_ctx *ctx);
 hipError_t launch_block_jacobi(jfm_ctx *ctx, double *Minv64, float *Minv32);
-// y = K x (masked) with the two-phase operators; T = double or float
+// Two-phase operators; `part` (may be null) receives one partial of
+// dot(y, dotv) per block of the gather; returns the partial count in *npart.
 template <typename T>
-hipError_t launch_two_phase(jfm_ctx *ctx, int op, const T *x, T *y, const T *dotv, int fin,
-                            const int *done);
+hipError_t launch_elem_only(jfm_ctx *ctx, int op, const T *x, const int *done);
 template <typename T>
-hipError_t launch_elem_only(jfm_ctx *ctx, int op, const T *x);   // phase 1 into ye
+hipError_t launch_gather(jfm_ctx *ctx, const T *ye, const uint8_t *fixed, T *y, const T *dotv, double *part,
+                         const int *done, int *npart);
+hipError_t launch_face_gather(jfm_ctx *ctx);
 template <typename T>
-hipError_t launch_gather(jfm_ctx *ctx, const int32_t *ptr, const int32_t *inc, int64_t nrows,
-                         const T *ye, const uint8_t *fixed, T *y, const T *dotv, int fin,
-                         const int *done);
+hipError_t launch_two_phase(jfm_ctx *ctx, int op, const T *x, T *y, const T *dotv, double *part,
+                            const int *done, int *npart);
 // linalg_kernels.hip
 template <typename T>
-hipError_t launch_bsr_spmv(jfm_ctx *ctx, const T *vals, const T *x, T *y, const T *dotv, int fin,
-                           const int *done);
+hipError_t launch_bsr_spmv(jfm_ctx *ctx, const T *vals, const T *x, T *y, const T *dotv, double *part,
+                           const int *done, int *npart);
 template <typename T>
-hipError_t launch_pcg_init(jfm_ctx *ctx, int64_t N, T *r, T *z, T *p, const T *Minv);
+hipError_t launch_pcg_init(jfm_ctx *ctx, T *r, T *z, T *p, const T *Minv, int *npart);
 template <typename T>
-hipError_t launch_pcg_update(jfm_ctx *ctx, int64_t N, T *x, T *r, T *z, const T *p, const T *q,
-                             const T *Minv);
+hipError_t launch_pcg_update(jfm_ctx *ctx, int slot, int npart_pq, T *x, T *r, T *z, const T *p, const T *q,
+                             const T *Minv, int *npart);
 template <typename T>
-hipError_t launch_pcg_direction(jfm_ctx *ctx, int64_t N, T *p, const T *z);
+hipError_t launch_pcg_direction(jfm_ctx *ctx, int slot, int npart_rz, T *p, const T *z);
+hipError_t launch_finalize(jfm_ctx *ctx, const double *part, int npart, int nv, int fin);
 hipError_t launch_zero_u32(jfm_ctx *ctx, unsigned *p, int64_t n);
 template <typename TI, typename TO>
 hipError_t launch_convert(jfm_ctx *ctx, const TI *in, TO *out, int64_t n, double scale);
-hipError_t launch_residual(jfm_ctx *ctx, const double *b, const double *Ku, const uint8_t *fixed,
-                           double *r, int64_t n);
+// r = b - Ku (fixed rows 0) and one partial of r.r per block -> partials misc
+hipError_t launch_residual(jfm_ctx *ctx, const double *b, const double *Ku, const uint8_t *fixed, double *r,
+                           int64_t n, int *npart);
 hipError_t launch_axpy_scaled(jfm_ctx *ctx, double *u, const float *d, double scale, int64_t n);
-hipError_t launch_dot(jfm_ctx *ctx, const double *a, const double *b, int64_t n, int fin);
+hipError_t launch_add(jfm_ctx *ctx, const double *a, const double *b, double *out, int64_t n);  // out = a + b
+template <typename T>
+hipError_t launch_dot(jfm_ctx *ctx, const T *a, const T *b, int64_t n, int *npart);
 }  // namespace jfem

@@ -6,12 +6,17 @@
 // order, so the scalar CSR handed to the caller is the values array itself.
 // y = K x reads 8 (or 4) bytes per value plus 4 bytes per 9 values of index.
 //
-// PCG (preconditioned conjugate gradients with a 3x3 block-diagonal
-// preconditioner) keeps every scalar on the device: the dot products are
-// fused into the producing kernels and finished by the last block (see
-// jfem_device.h), so an iteration needs no host round trip.  The
-// reference's CG (ext/JuliaFEMCUDAExt.jl:531-577) copies two dot products to
-// the host per iteration and has no preconditioner.
+// PCG (conjugate gradients with a 3x3 block-diagonal preconditioner) keeps
+// every scalar on the device.  Each iteration is three kernels --
+//   q = K p (+ partials of p.q)            [operator, slot s = i & 1]
+//   x += a p, r -= a q, z = M r (+ r.z, r.r)  a = rz[s] / sum(p.q partials)
+//   p = z + b p                            b = sum(r.z partials) / rz[s],
+//                                          block 0 writes rz[s^1], done[s^1]
+// -- with no host round trip and no atomics.  The reference's CG
+// (ext/JuliaFEMCUDAExt.jl:531-577) has no preconditioner and copies two dot
+// products to the host per iteration.
+#include <algorithm>
+
 #include "jfem_device.h"
 
 namespace jfem {
@@ -23,107 +28,176 @@ template <typename T, int G>
 __global__ __launch_bounds__(256) void k_bsr_spmv(int N, const int32_t *__restrict__ brow,
                                                  const int32_t *__restrict__ bcol, const T *__restrict__ vals,
                                                  const T *__restrict__ x, const uint8_t *__restrict__ fixed,
-                                                 T *__restrict__ y, const T *__restrict__ dotv, double *partials,
-                                                 unsigned *counter, PcgScalars *S, int fin,
-                                                 const int *__restrict__ done) {
+                                                 T *__restrict__ y, const T *__restrict__ dotv,
+                                                 double *__restrict__ part, const int *__restrict__ done) {
     if (done && *done) return;
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int a = gid / G, lane = gid % G;
-    T acc0 = T(0), acc1 = T(0), acc2 = T(0);
-    if (a < N) {
-        const int r0 = brow[a], nb = brow[a + 1] - r0;
-        const T *v = vals + 9 * (size_t)r0;
-        const size_t rs = 3 * (size_t)nb;
-        for (int k = lane; k < nb; k += G) {
-            const size_t col = 3 * (size_t)bcol[r0 + k];
-            const T x0 = x[col], x1 = x[col + 1], x2 = x[col + 2];
-            const T *v0 = v + 3 * (size_t)k;
-            const T *v1 = v0 + rs;
-            const T *v2 = v1 + rs;
-            acc0 += v0[0] * x0 + v0[1] * x1 + v0[2] * x2;
-            acc1 += v1[0] * x0 + v1[1] * x1 + v1[2] * x2;
-            acc2 += v2[0] * x0 + v2[1] * x1 + v2[2] * x2;
+    constexpr int RPB = kBlock / G;  // rows per block sweep
+    const int lane = threadIdx.x % G;
+    double dv = 0.0;
+    for (int a0 = blockIdx.x * RPB; a0 < N; a0 += gridDim.x * RPB) {
+        const int a = a0 + threadIdx.x / G;
+        T acc0 = T(0), acc1 = T(0), acc2 = T(0);
+        if (a < N) {
+            const int r0 = brow[a], nb = brow[a + 1] - r0;
+            const T *v = vals + 9 * (size_t)r0;
+            const size_t rs = 3 * (size_t)nb;
+            for (int k = lane; k < nb; k += G) {
+                const size_t col = 3 * (size_t)bcol[r0 + k];
+                const T x0 = x[col], x1 = x[col + 1], x2 = x[col + 2];
+                const T *v0 = v + 3 * (size_t)k;
+                const T *v1 = v0 + rs;
+                const T *v2 = v1 + rs;
+                acc0 += v0[0] * x0 + v0[1] * x1 + v0[2] * x2;
+                acc1 += v1[0] * x0 + v1[1] * x1 + v1[2] * x2;
+                acc2 += v2[0] * x0 + v2[1] * x1 + v2[2] * x2;
+            }
+        }
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1) {
+            acc0 += __shfl_xor(acc0, off, G);
+            acc1 += __shfl_xor(acc1, off, G);
+            acc2 += __shfl_xor(acc2, off, G);
+        }
+        if (a < N && lane < 3) {
+            const size_t row = 3 * (size_t)a + lane;
+            T val = lane == 0 ? acc0 : (lane == 1 ? acc1 : acc2);
+            if (fixed && fixed[row]) val = T(0);
+            y[row] = val;
+            if (dotv) dv += (double)val * (double)dotv[row];
         }
     }
-#pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1) {
-        acc0 += __shfl_xor(acc0, off, G);
-        acc1 += __shfl_xor(acc1, off, G);
-        acc2 += __shfl_xor(acc2, off, G);
-    }
-    double dv = 0.0;
-    if (a < N && lane < 3) {
-        const size_t row = 3 * (size_t)a + lane;
-        T val = lane == 0 ? acc0 : (lane == 1 ? acc1 : acc2);
-        if (fixed && fixed[row]) val = T(0);
-        y[row] = val;
-        if (dotv) dv = (double)val * (double)dotv[row];
-    }
-    if (fin != FIN_NONE) {
+    if (part) {
         double vv[1] = {dv};
-        reduce_finalize<1>(vv, partials, counter, S, fin);
+        write_partials<1>(vv, part);
     }
 }
 
-// z = Minv r, p = z, (r.z, r.r) -> FIN_INIT
+// z = Minv r, p = z; partials (r.z, r.r) -> misc
 template <typename T>
 __global__ __launch_bounds__(256) void k_pcg_init(int N, const T *__restrict__ r, T *__restrict__ z,
                                                  T *__restrict__ p, const T *__restrict__ Minv,
-                                                 double *partials, unsigned *counter, PcgScalars *S) {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+                                                 double *__restrict__ part) {
     double v[2] = {0.0, 0.0};
-    if (a < N) {
-        const T r0 = r[3 * (size_t)a], r1 = r[3 * (size_t)a + 1], r2 = r[3 * (size_t)a + 2];
+    for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < N; a += gridDim.x * blockDim.x) {
+        const size_t o = 3 * (size_t)a;
+        const T r0 = r[o], r1 = r[o + 1], r2 = r[o + 2];
         const T *M = Minv + 9 * (size_t)a;
         const T z0 = M[0] * r0 + M[1] * r1 + M[2] * r2;
         const T z1 = M[3] * r0 + M[4] * r1 + M[5] * r2;
         const T z2 = M[6] * r0 + M[7] * r1 + M[8] * r2;
-        z[3 * (size_t)a] = z0; z[3 * (size_t)a + 1] = z1; z[3 * (size_t)a + 2] = z2;
-        p[3 * (size_t)a] = z0; p[3 * (size_t)a + 1] = z1; p[3 * (size_t)a + 2] = z2;
-        v[0] = (double)r0 * z0 + (double)r1 * z1 + (double)r2 * z2;
-        v[1] = (double)r0 * r0 + (double)r1 * r1 + (double)r2 * r2;
+        z[o] = z0; z[o + 1] = z1; z[o + 2] = z2;
+        p[o] = z0; p[o + 1] = z1; p[o + 2] = z2;
+        v[0] += (double)r0 * z0 + (double)r1 * z1 + (double)r2 * z2;
+        v[1] += (double)r0 * r0 + (double)r1 * r1 + (double)r2 * r2;
     }
-    reduce_finalize<2>(v, partials, counter, S, FIN_INIT);
+    write_partials<2>(v, part);
 }
 
-// x += alpha p, r -= alpha q, z = Minv r, (r.z, r.r) -> FIN_UPDATE
+// x += alpha p, r -= alpha q, z = Minv r; partials (r.z, r.r).  The first
+// node's operands are loaded before the alpha reduction so that their
+// latency overlaps it.
 template <typename T>
-__global__ __launch_bounds__(256) void k_pcg_update(int N, T *__restrict__ x, T *__restrict__ r,
-                                                   T *__restrict__ z, const T *__restrict__ p,
+__global__ __launch_bounds__(256) void k_pcg_update(int N, int slot, int npart_pq, T *__restrict__ x,
+                                                   T *__restrict__ r, T *__restrict__ z, const T *__restrict__ p,
                                                    const T *__restrict__ q, const T *__restrict__ Minv,
-                                                   double *partials, unsigned *counter, PcgScalars *S) {
-    if (S->done) return;
-    const T alpha = (T)S->alpha;
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+                                                   double *__restrict__ partials, PcgScalars *S) {
+    if (S->done[slot]) return;
+    const int stride = gridDim.x * blockDim.x;
+    int a = blockIdx.x * blockDim.x + threadIdx.x;
+    T xv[3], pv[3], rv[3], qv[3], M[9];
+    auto load = [&](int aa) {
+        const size_t o = 3 * (size_t)aa;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) { xv[c] = x[o + c]; pv[c] = p[o + c]; rv[c] = r[o + c]; qv[c] = q[o + c]; }
+#pragma unroll
+        for (int m = 0; m < 9; ++m) M[m] = Minv[9 * (size_t)aa + m];
+    };
+    if (a < N) load(a);
+    double pq[1];
+    sum_partials<1>(partials + part_pq(slot), npart_pq, pq);
+    const double alpha_d = (pq[0] > 0.0 && finite(pq[0])) ? S->rz[slot] / pq[0] : 0.0;
+    const T alpha = (T)alpha_d;
     double v[2] = {0.0, 0.0};
-    if (a < N) {
+    while (a < N) {
         const size_t o = 3 * (size_t)a;
-        T rv[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            x[o + c] += alpha * p[o + c];
-            rv[c] = r[o + c] - alpha * q[o + c];
+            x[o + c] = xv[c] + alpha * pv[c];
+            rv[c] = rv[c] - alpha * qv[c];
             r[o + c] = rv[c];
         }
-        const T *M = Minv + 9 * (size_t)a;
         const T z0 = M[0] * rv[0] + M[1] * rv[1] + M[2] * rv[2];
         const T z1 = M[3] * rv[0] + M[4] * rv[1] + M[5] * rv[2];
         const T z2 = M[6] * rv[0] + M[7] * rv[1] + M[8] * rv[2];
         z[o] = z0; z[o + 1] = z1; z[o + 2] = z2;
-        v[0] = (double)rv[0] * z0 + (double)rv[1] * z1 + (double)rv[2] * z2;
-        v[1] = (double)rv[0] * rv[0] + (double)rv[1] * rv[1] + (double)rv[2] * rv[2];
+        v[0] += (double)rv[0] * z0 + (double)rv[1] * z1 + (double)rv[2] * z2;
+        v[1] += (double)rv[0] * rv[0] + (double)rv[1] * rv[1] + (double)rv[2] * rv[2];
+        a += stride;
+        if (a < N) load(a);
     }
-    reduce_finalize<2>(v, partials, counter, S, FIN_UPDATE);
+    write_partials<2>(v, partials + part_rz(slot));
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        S->alpha = alpha_d;
+        S->pq = pq[0];
+    }
 }
 
-// p = z + beta p
+// p = z + beta p; block 0 records the next iteration's scalars and status
 template <typename T>
-__global__ __launch_bounds__(256) void k_pcg_dir(int n, T *__restrict__ p, const T *__restrict__ z,
-                                                const PcgScalars *__restrict__ S) {
-    if (S->done) return;
-    const T beta = (T)S->beta;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = z[i] + beta * p[i];
+__global__ __launch_bounds__(256) void k_pcg_dir(int n3, int slot, int npart_rz, T *__restrict__ p,
+                                                const T *__restrict__ z, const double *__restrict__ partials,
+                                                PcgScalars *S) {
+    const int d = S->done[slot];
+    if (d) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) S->done[slot ^ 1] = d;
+        return;
+    }
+    const int stride = gridDim.x * blockDim.x;
+    const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
+    // first operands in flight before the beta reduction
+    T z0 = i0 < n3 ? z[i0] : T(0), p0 = i0 < n3 ? p[i0] : T(0);
+    double t[2];
+    sum_partials<2>(partials + part_rz(slot), npart_rz, t);
+    const double beta_d = t[0] / S->rz[slot];
+    const T beta = (T)beta_d;
+    if (i0 < n3) p[i0] = z0 + beta * p0;
+    for (int i = i0 + stride; i < n3; i += stride) p[i] = z[i] + beta * p[i];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const int it = S->iter + 1;
+        S->iter = it;
+        S->rz[slot ^ 1] = t[0];
+        S->rr = t[1];
+        S->beta = beta_d;
+        int nd = DONE_RUNNING;
+        if (t[1] <= S->tol2) nd = DONE_CONVERGED;
+        else if (!(S->pq > 0.0) || !(t[0] > 0.0) || !finite(t[0])) nd = DONE_BREAKDOWN;
+        else if (it >= S->maxit) nd = DONE_MAXIT;
+        S->done[slot ^ 1] = nd;
+    }
+}
+
+// one block: sum partials and apply `fin`
+__global__ __launch_bounds__(256) void k_finalize(const double *__restrict__ part, int npart, int nv, int fin,
+                                                 PcgScalars *S) {
+    double t[2];
+    sum_partials<2>(part, npart, t);
+    if (threadIdx.x != 0) return;
+    if (nv < 2) t[1] = 0.0;
+    switch (fin) {
+    case FIN_INIT: {
+        S->rz[0] = t[0];
+        S->rr = t[1];
+        S->iter = 0;
+        const int d = (t[1] <= S->tol2) ? DONE_CONVERGED : DONE_RUNNING;
+        S->done[0] = d;
+        S->done[1] = d;
+        break;
+    }
+    case FIN_DOT: S->tmp0 = t[0]; break;
+    case FIN_DOT2: S->tmp0 = t[0]; S->tmp1 = t[1]; break;
+    case FIN_BB: S->bb = t[0]; S->tol2 = S->rtol2 * t[0]; break;
+    default: break;
+    }
 }
 
 __global__ void k_zero_u32(unsigned *p, int n) {
@@ -138,18 +212,17 @@ __global__ __launch_bounds__(256) void k_convert(const TI *__restrict__ in, TO *
     if (i < n) out[i] = (TO)((double)in[i] * scale);
 }
 
-// r = b - Ku on free DOFs, 0 on fixed; (r.r) -> FIN_DOT
+// r = b - Ku on free DOFs, 0 on fixed; partials of r.r
 __global__ __launch_bounds__(256) void k_residual(const double *__restrict__ b, const double *__restrict__ Ku,
                                                  const uint8_t *__restrict__ fixed, double *__restrict__ r,
-                                                 int n, double *partials, unsigned *counter, PcgScalars *S) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+                                                 int n, double *__restrict__ part) {
     double v[1] = {0.0};
-    if (i < n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const double ri = (fixed && fixed[i]) ? 0.0 : b[i] - Ku[i];
         r[i] = ri;
-        v[0] = ri * ri;
+        v[0] += ri * ri;
     }
-    reduce_finalize<1>(v, partials, counter, S, FIN_DOT);
+    write_partials<1>(v, part);
 }
 
 __global__ __launch_bounds__(256) void k_axpy_scaled(double *__restrict__ u, const float *__restrict__ d,
@@ -158,41 +231,65 @@ __global__ __launch_bounds__(256) void k_axpy_scaled(double *__restrict__ u, con
     if (i < n) u[i] += scale * (double)d[i];
 }
 
-__global__ __launch_bounds__(256) void k_dot(const double *__restrict__ a, const double *__restrict__ b, int n,
-                                            double *partials, unsigned *counter, PcgScalars *S, int fin) {
+__global__ __launch_bounds__(256) void k_add(const double *__restrict__ a, const double *__restrict__ b,
+                                            double *__restrict__ out, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double v[1] = {i < n ? a[i] * b[i] : 0.0};
-    reduce_finalize<1>(v, partials, counter, S, fin);
+    if (i < n) out[i] = a[i] + b[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_dot(const T *__restrict__ a, const T *__restrict__ b, int n,
+                                            double *__restrict__ part) {
+    double v[1] = {0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        v[0] += (double)a[i] * (double)b[i];
+    write_partials<1>(v, part);
 }
 
 // ------------------------------------------------------------- launchers ----
 constexpr int kSpmvLanes = 8;
 
 template <typename T>
-hipError_t launch_bsr_spmv(jfm_ctx *ctx, const T *vals, const T *x, T *y, const T *dotv, int fin,
-                           const int *done) {
+hipError_t launch_bsr_spmv(jfm_ctx *ctx, const T *vals, const T *x, T *y, const T *dotv, double *part,
+                           const int *done, int *npart) {
     const int N = (int)ctx->N;
-    if (N == 0) return hipSuccess;
-    k_bsr_spmv<T, kSpmvLanes><<<nblk((int64_t)N * kSpmvLanes), kBlock, 0, ctx->stream>>>(
-        N, ctx->brow, ctx->bcol, vals, x, ctx->fixed, y, dotv, ctx->partials, ctx->counter, ctx->S, fin, done);
+    const unsigned g = capped_grid((int64_t)N * kSpmvLanes, kBlock);
+    if (npart) *npart = (int)g;
+    k_bsr_spmv<T, kSpmvLanes><<<g, kBlock, 0, ctx->stream>>>(N, ctx->brow, ctx->bcol, vals, x, ctx->fixed, y,
+                                                            dotv, part, done);
     return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_pcg_init(jfm_ctx *ctx, int64_t N, T *r, T *z, T *p, const T *Minv) {
-    k_pcg_init<T><<<nblk(N), kBlock, 0, ctx->stream>>>((int)N, r, z, p, Minv, ctx->partials, ctx->counter, ctx->S);
+hipError_t launch_pcg_init(jfm_ctx *ctx, T *r, T *z, T *p, const T *Minv, int *npart) {
+    const unsigned g = capped_grid(ctx->N, kBlock);
+    *npart = (int)g;
+    k_pcg_init<T><<<g, kBlock, 0, ctx->stream>>>((int)ctx->N, r, z, p, Minv, ctx->partials + kPartMisc);
+    return hipGetLastError();
+}
+
+constexpr int kVecBlocks = 512;  // grid of the PCG vector kernels
+
+template <typename T>
+hipError_t launch_pcg_update(jfm_ctx *ctx, int slot, int npart_pq, T *x, T *r, T *z, const T *p, const T *q,
+                             const T *Minv, int *npart) {
+    const unsigned g = std::min<unsigned>(capped_grid(ctx->N, kBlock), kVecBlocks);
+    *npart = (int)g;
+    k_pcg_update<T><<<g, kBlock, 0, ctx->stream>>>((int)ctx->N, slot, npart_pq, x, r, z, p, q, Minv,
+                                                  ctx->partials, ctx->S);
     return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_pcg_update(jfm_ctx *ctx, int64_t N, T *x, T *r, T *z, const T *p, const T *q, const T *Minv) {
-    k_pcg_update<T><<<nblk(N), kBlock, 0, ctx->stream>>>((int)N, x, r, z, p, q, Minv, ctx->partials, ctx->counter, ctx->S);
+hipError_t launch_pcg_direction(jfm_ctx *ctx, int slot, int npart_rz, T *p, const T *z) {
+    const int64_t n3 = 3 * ctx->N;
+    k_pcg_dir<T><<<std::min<unsigned>(capped_grid(n3, kBlock), 2 * kVecBlocks), kBlock, 0, ctx->stream>>>((int)n3, slot, npart_rz, p, z, ctx->partials,
+                                                                      ctx->S);
     return hipGetLastError();
 }
 
-template <typename T>
-hipError_t launch_pcg_direction(jfm_ctx *ctx, int64_t N, T *p, const T *z) {
-    k_pcg_dir<T><<<nblk(3 * N), kBlock, 0, ctx->stream>>>((int)(3 * N), p, z, ctx->S);
+hipError_t launch_finalize(jfm_ctx *ctx, const double *part, int npart, int nv, int fin) {
+    k_finalize<<<1, kBlock, 0, ctx->stream>>>(part, npart, nv, fin, ctx->S);
     return hipGetLastError();
 }
 
@@ -209,8 +306,10 @@ hipError_t launch_convert(jfm_ctx *ctx, const TI *in, TO *out, int64_t n, double
 }
 
 hipError_t launch_residual(jfm_ctx *ctx, const double *b, const double *Ku, const uint8_t *fixed, double *r,
-                           int64_t n) {
-    k_residual<<<nblk(n), kBlock, 0, ctx->stream>>>(b, Ku, fixed, r, (int)n, ctx->partials, ctx->counter, ctx->S);
+                           int64_t n, int *npart) {
+    const unsigned g = capped_grid(n, kBlock);
+    *npart = (int)g;
+    k_residual<<<g, kBlock, 0, ctx->stream>>>(b, Ku, fixed, r, (int)n, ctx->partials + kPartMisc);
     return hipGetLastError();
 }
 
@@ -219,21 +318,31 @@ hipError_t launch_axpy_scaled(jfm_ctx *ctx, double *u, const float *d, double sc
     return hipGetLastError();
 }
 
-hipError_t launch_dot(jfm_ctx *ctx, const double *a, const double *b, int64_t n, int fin) {
-    k_dot<<<nblk(n), kBlock, 0, ctx->stream>>>(a, b, (int)n, ctx->partials, ctx->counter, ctx->S, fin);
+hipError_t launch_add(jfm_ctx *ctx, const double *a, const double *b, double *out, int64_t n) {
+    k_add<<<nblk(n), kBlock, 0, ctx->stream>>>(a, b, out, (int)n);
     return hipGetLastError();
 }
 
-template hipError_t launch_bsr_spmv<double>(jfm_ctx *, const double *, const double *, double *, const double *, int, const int *);
-template hipError_t launch_bsr_spmv<float>(jfm_ctx *, const float *, const float *, float *, const float *, int, const int *);
-template hipError_t launch_pcg_init<double>(jfm_ctx *, int64_t, double *, double *, double *, const double *);
-template hipError_t launch_pcg_init<float>(jfm_ctx *, int64_t, float *, float *, float *, const float *);
-template hipError_t launch_pcg_update<double>(jfm_ctx *, int64_t, double *, double *, double *, const double *, const double *, const double *);
-template hipError_t launch_pcg_update<float>(jfm_ctx *, int64_t, float *, float *, float *, const float *, const float *, const float *);
-template hipError_t launch_pcg_direction<double>(jfm_ctx *, int64_t, double *, const double *);
-template hipError_t launch_pcg_direction<float>(jfm_ctx *, int64_t, float *, const float *);
+template <typename T>
+hipError_t launch_dot(jfm_ctx *ctx, const T *a, const T *b, int64_t n, int *npart) {
+    const unsigned g = capped_grid(n, kBlock);
+    *npart = (int)g;
+    k_dot<T><<<g, kBlock, 0, ctx->stream>>>(a, b, (int)n, ctx->partials + kPartMisc);
+    return hipGetLastError();
+}
+
+template hipError_t launch_bsr_spmv<double>(jfm_ctx *, const double *, const double *, double *, const double *, double *, const int *, int *);
+template hipError_t launch_bsr_spmv<float>(jfm_ctx *, const float *, const float *, float *, const float *, double *, const int *, int *);
+template hipError_t launch_pcg_init<double>(jfm_ctx *, double *, double *, double *, const double *, int *);
+template hipError_t launch_pcg_init<float>(jfm_ctx *, float *, float *, float *, const float *, int *);
+template hipError_t launch_pcg_update<double>(jfm_ctx *, int, int, double *, double *, double *, const double *, const double *, const double *, int *);
+template hipError_t launch_pcg_update<float>(jfm_ctx *, int, int, float *, float *, float *, const float *, const float *, const float *, int *);
+template hipError_t launch_pcg_direction<double>(jfm_ctx *, int, int, double *, const double *);
+template hipError_t launch_pcg_direction<float>(jfm_ctx *, int, int, float *, const float *);
 template hipError_t launch_convert<double, float>(jfm_ctx *, const double *, float *, int64_t, double);
 template hipError_t launch_convert<float, double>(jfm_ctx *, const float *, double *, int64_t, double);
 template hipError_t launch_convert<double, double>(jfm_ctx *, const double *, double *, int64_t, double);
+template hipError_t launch_dot<double>(jfm_ctx *, const double *, const double *, int64_t, int *);
+template hipError_t launch_dot<float>(jfm_ctx *, const float *, const float *, int64_t, int *);
 
 }  // namespace jfem

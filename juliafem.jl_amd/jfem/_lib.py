@@ -25,7 +25,7 @@ ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ENOTSPD", -4: "ENOCONV", -5: "ESTATE", 
 # Exported symbols (must match include/jfem.h)
 SYMBOLS = ("jfm_create", "jfm_destroy", "jfm_last_error", "jfm_get_info", "jfm_upload_mesh",
            "jfm_upload_bcs", "jfm_csr_pattern", "jfm_assemble", "jfm_apply", "jfm_time_apply",
-           "jfm_solve", "jfm_learned_blob_bytes", "jfm_learned_set_weights", "jfm_learned_get_weights")
+           "jfm_solve", "jfm_solve_history", "jfm_learned_blob_bytes", "jfm_learned_set_weights", "jfm_learned_get_weights")
 
 
 class JfemError(RuntimeError):
@@ -37,7 +37,8 @@ class JfemError(RuntimeError):
 class SolveOpts(C.Structure):
     _fields_ = [("method", C.c_int32), ("op", C.c_int32), ("op_outer", C.c_int32), ("precond", C.c_int32),
                 ("rtol", C.c_double), ("inner_rtol", C.c_double), ("maxit", C.c_int32),
-                ("max_outer", C.c_int32), ("check_every", C.c_int32), ("learned_probe", C.c_int32)]
+                ("max_outer", C.c_int32), ("check_every", C.c_int32), ("learned_probe", C.c_int32),
+                ("inner_maxit", C.c_int32), ("first_rtol", C.c_double)]
 
 
 class SolveStats(C.Structure):
@@ -81,6 +82,7 @@ def load():
     lib.jfm_apply.argtypes = [_P, C.c_int32, C.c_int32, _P, _P]
     lib.jfm_time_apply.argtypes = [_P, C.c_int32, C.c_int32, C.c_int32, _P]
     lib.jfm_solve.argtypes = [_P, C.POINTER(SolveOpts), _P, _P, _P, C.POINTER(SolveStats)]
+    lib.jfm_solve_history.argtypes = [_P, _P, _P, C.c_int32]
     lib.jfm_learned_blob_bytes.argtypes = []
     lib.jfm_learned_blob_bytes.restype = C.c_size_t
     lib.jfm_learned_set_weights.argtypes = [_P, _P, C.c_size_t]
@@ -188,10 +190,10 @@ class Context:
 
     def solve(self, f=None, method=SOLVE_PCG, op="matfree", op_outer="matfree", precond=PREC_BLOCK_JACOBI,
               rtol=1e-8, inner_rtol=1e-4, maxit=100000, max_outer=20, check_every=16, learned_probe=0,
-              want_la=False, raise_on_noconv=True):
+              inner_maxit=0, first_rtol=0.0, want_la=False, raise_on_noconv=True):
         n = self.info()["n_dofs"]
         opts = SolveOpts(method, OPS[op], OPS[op_outer], precond, rtol, inner_rtol, maxit, max_outer,
-                         check_every, learned_probe)
+                         check_every, learned_probe, inner_maxit, first_rtol)
         st = SolveStats()
         u = np.empty(n)
         la = np.empty(n) if want_la else None
@@ -200,6 +202,15 @@ class Context:
         if rc != 0 and not (rc == -4 and not raise_on_noconv):
             self._check(rc)
         return u, la, st.as_dict()
+
+    def history(self, max_entries: int = 64):
+        """[(cumulative iterations, FP64 relative residual)] of the last solve."""
+        it = np.zeros(max_entries, np.int32)
+        rr = np.zeros(max_entries)
+        n = self._lib.jfm_solve_history(self._h, _p(it), _p(rr), max_entries)
+        if n < 0:
+            self._check(n)
+        return list(zip(it[:n].tolist(), rr[:n].tolist()))
 
     def set_learned_weights(self, blob: bytes):
         buf = np.frombuffer(blob, dtype=np.uint8).copy()

@@ -202,7 +202,8 @@ def test_host_mirror_run(jf, oracle, direct0):
 def test_inverted_element_rejected(jf):
     m = M.box_tet10(2, 1, 1)
     conn = m.conn.copy()
-    conn[0, [1, 2]] = conn[0, [2, 1]]     # swap two corners: detJ < 0
+    # mirror element 0: swap corners 2,3 and the mid nodes of the swapped edges
+    conn[0] = conn[0, [0, 2, 1, 3, 6, 5, 4, 7, 9, 8]]
     ctx = jf.Context(0)
     with pytest.raises(jf.JfemError, match="ENOTSPD"):
         ctx.upload_mesh(m.X, conn, "Tet10", m.E, m.nu)
