@@ -66,7 +66,8 @@ enum {
 /* preconditioners */
 enum {
     JFM_PREC_BLOCK_JACOBI = 0, /* inverse 3x3 node diagonal blocks */
-    JFM_PREC_LEARNED = 1       /* network-predicted SPD blocks M = C (L L' + eps I) C' */
+    JFM_PREC_LEARNED = 1,      /* network-predicted SPD blocks M = C (L L' + eps I) C' */
+    JFM_PREC_USER = 2          /* blocks given with jfm_precond_set_blocks */
 };
 
 typedef struct {
@@ -79,13 +80,16 @@ typedef struct {
     int32_t maxit;        /* iteration cap (inner iterations summed for mixed) */
     int32_t max_outer;    /* refinement-step cap (mixed only) */
     int32_t check_every;  /* iterations per device batch between host checks */
-    int32_t learned_probe;/* iterations of the learned-vs-block-Jacobi probe; 0 = none */
+    int32_t learned_probe;/* learned only: PCG iterations run with each preconditioner
+                           * before the solve; the learned blocks are used only if
+                           * they capture more energy b.x_k (else block Jacobi,
+                           * stats.fallback = 1); 0 = no probe */
     int32_t inner_maxit;  /* cap on the iterations of one inner solve (mixed); 0 = none */
     double first_rtol;    /* mixed only: FP64 PCG to this relative residual first; 0 = none */
 } jfm_solve_opts;
 
 typedef struct {
-    int32_t iterations;   /* PCG iterations (inner iterations summed for mixed) */
+    int32_t iterations;   /* PCG iterations (inner iterations summed for mixed, probe included) */
     int32_t outer;        /* refinement steps (mixed), 1 for plain PCG */
     int32_t converged;    /* 1 when the FP64 true residual met rtol */
     int32_t fallback;     /* 1 when the learned preconditioner was rejected */
@@ -160,6 +164,11 @@ int jfm_time_apply(jfm_ctx *ctx, int32_t op, int32_t precision, int32_t reps, do
 int jfm_solve(jfm_ctx *ctx, const jfm_solve_opts *opts, const double *f, double *u, double *la,
               jfm_solve_stats *stats);
 
+/* User 3x3 preconditioner blocks (n_nodes x 9, row-major, each SPD on the
+ * free DOFs; rows/columns of fixed DOFs are zeroed by the library), used by
+ * jfm_solve with precond = JFM_PREC_USER until the mesh or BCs change. */
+int jfm_precond_set_blocks(jfm_ctx *ctx, const double *M);
+
 /* Convergence history of the last jfm_solve: per refinement step (or PCG
  * restart) the cumulative iteration count and the FP64 true relative
  * residual after it.  Returns the number of entries (<= max_entries). */
@@ -170,8 +179,20 @@ int jfm_solve_history(jfm_ctx *ctx, int32_t *iterations, double *relres, int32_t
  * The weight blob is the packed layout described in jfem_learned.h of the
  * package sources; jfm_learned_blob_bytes() gives its size. */
 size_t jfm_learned_blob_bytes(void);
+int32_t jfm_learned_n_params(void);   /* trainable parameters, gradient order W1 b1 W2 b2 W3 b3 */
 int jfm_learned_set_weights(jfm_ctx *ctx, const void *blob, size_t bytes);
 int jfm_learned_get_weights(jfm_ctx *ctx, void *blob, size_t bytes);
+/* The 24 node features (n_nodes x 24, FP32) for the uploaded mesh and BCs. */
+int jfm_learned_features(jfm_ctx *ctx, float *feat);
+/* Network outputs (n_nodes x 6) and preconditioner blocks (n_nodes x 9,
+ * row-major 3x3; may be NULL) for the current weights. */
+int jfm_learned_outputs(jfm_ctx *ctx, float *out, double *M);
+/* Training step pieces: mean over n_pairs (residual r, error e) pairs
+ * (each n_dofs, row p at r + p*n_dofs, zero on fixed DOFs, e = K^-1 r) of
+ *     loss = 1 - (z.r)^2 / ((z.Kz)(e.r)),  z = M r,
+ * and its gradient with respect to the parameters (jfm_learned_n_params()
+ * floats), computed by the network's forward and backward kernels. */
+int jfm_learned_grad(jfm_ctx *ctx, int32_t n_pairs, const double *r, const double *e, double *loss, float *grad);
 
 #ifdef __cplusplus
 }

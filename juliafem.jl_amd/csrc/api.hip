@@ -163,7 +163,9 @@ static void release_mesh(jfm_ctx *ctx) {
         free_ptr(ctx, w->q); free_ptr(ctx, w->b); free_ptr(ctx, w->Minv);
         w->ok = false;
     }
-    free_ptr(ctx, ctx->Mlearn64); free_ptr(ctx, ctx->Mlearn32); free_ptr(ctx, ctx->feat);
+    free_ptr(ctx, ctx->Mlearn64); free_ptr(ctx, ctx->Mlearn32); free_ptr(ctx, ctx->feat); free_ptr(ctx, ctx->lout);
+    free_ptr(ctx, ctx->Muser64); free_ptr(ctx, ctx->Muser32);
+    ctx->has_user = false;
     ctx->has_mesh = ctx->has_bcs = ctx->assembled = ctx->has_load = false;
     ctx->learned_ready = false;
     ctx->h_conn.clear(); ctx->h_brow.clear(); ctx->h_bcol.clear();
@@ -466,6 +468,7 @@ extern "C" int jfm_upload_bcs(jfm_ctx *ctx, const uint8_t *fixed, const double *
     ctx->has_bcs = true;
     ctx->has_load = true;
     ctx->learned_ready = false;
+    ctx->has_user = false;
     for (Workspace *w : {&ctx->ws64, &ctx->ws32}) w->ok = false;  // preconditioners depend on the fixed set
     return JFM_OK;
 }
@@ -673,6 +676,26 @@ static int ensure_workspaces(jfm_ctx *ctx) {
     return JFM_OK;
 }
 
+int jfem_ensure_workspaces(jfm_ctx *ctx) { return ensure_workspaces(ctx); }
+
+int jfem_apply(jfm_ctx *ctx, int op, const double *x, double *y) {
+    if (int rc = prepare_op(ctx, op)) return rc;
+    CHECK_LAUNCH(ctx, op_apply<double>(ctx, op, x, y, nullptr, nullptr, nullptr, nullptr));
+    return JFM_OK;
+}
+
+static int read_scalars(jfm_ctx *ctx);
+static int finalize_misc(jfm_ctx *ctx, int npart, int nv, int fin);
+
+int jfem_dot(jfm_ctx *ctx, const double *a, const double *b, int64_t n, double *out) {
+    int np = 0;
+    CHECK_LAUNCH(ctx, launch_dot<double>(ctx, a, b, n, &np));
+    if (int rc = finalize_misc(ctx, np, 1, FIN_DOT)) return rc;
+    if (int rc = read_scalars(ctx)) return rc;
+    *out = ctx->S_host->tmp0;
+    return JFM_OK;
+}
+
 static int read_scalars(jfm_ctx *ctx) {
     JFM_HIP(ctx, hipMemcpyAsync(ctx->S_host, ctx->S, sizeof(PcgScalars), hipMemcpyDeviceToHost, ctx->stream));
     return sync(ctx);
@@ -805,8 +828,10 @@ extern "C" int jfm_solve(jfm_ctx *ctx, const jfm_solve_opts *opts, const double 
     if (!(opts->rtol > 0.0) || opts->maxit <= 0) return jfm_fail(ctx, JFM_EINVAL, "rtol > 0 and maxit > 0 required");
     if (opts->method != JFM_SOLVE_PCG && opts->method != JFM_SOLVE_MIXED_IR)
         return jfm_fail(ctx, JFM_EINVAL, "unknown solve method");
-    if (opts->precond != JFM_PREC_BLOCK_JACOBI && opts->precond != JFM_PREC_LEARNED)
+    if (opts->precond != JFM_PREC_BLOCK_JACOBI && opts->precond != JFM_PREC_LEARNED && opts->precond != JFM_PREC_USER)
         return jfm_fail(ctx, JFM_EINVAL, "unknown preconditioner");
+    if (opts->precond == JFM_PREC_USER && !ctx->has_user)
+        return jfm_fail(ctx, JFM_ESTATE, "no user preconditioner blocks set");
     JFM_HIP(ctx, hipSetDevice(ctx->device));
     const int op = opts->op;
     const int op_outer = opts->method == JFM_SOLVE_MIXED_IR ? opts->op_outer : op;
@@ -833,10 +858,35 @@ extern "C" int jfm_solve(jfm_ctx *ctx, const jfm_solve_opts *opts, const double 
     ctx->hist_res.clear();
     const double *Minv64 = (const double *)ctx->ws64.Minv;
     const float *Minv32 = (const float *)ctx->ws32.Minv;
-    if (opts->precond == JFM_PREC_LEARNED) {
+    if (opts->precond == JFM_PREC_USER) {
+        Minv64 = ctx->Muser64;
+        Minv32 = ctx->Muser32;
+    }
+    if (opts->precond == JFM_PREC_LEARNED && bnorm > 0.0) {
         if (int rc = jfem_learned_prepare(ctx, &st.setup_ms)) return rc;
-        Minv64 = ctx->Mlearn64;
-        Minv32 = ctx->Mlearn32;
+        bool use = true;
+        if (opts->learned_probe > 0) {
+            // Runtime check: after the same number of PCG iterations from 0 the
+            // iterate with the larger energy b.x_k (CG decreases
+            // 1/2 x'Kx - b.x = -1/2 b.x_k monotonically) has the smaller
+            // energy-norm error; keep the learned blocks only if they win.
+            double phi[2] = {0.0, 0.0};
+            const double *cand[2] = {(const double *)ctx->ws64.Minv, ctx->Mlearn64};
+            for (int c = 0; c < 2; ++c) {
+                int it = 0, status = 0;
+                if (int rc = pcg_run<double>(ctx, op_outer, cand[c], 1e-30, opts->learned_probe, check, &it, &status))
+                    return rc;
+                if (int rc = jfem_dot(ctx, b, (const double *)ctx->ws64.x, n, &phi[c])) return rc;
+                st.iterations += it;
+            }
+            use = phi[1] > phi[0];
+        }
+        if (use) {
+            Minv64 = ctx->Mlearn64;
+            Minv32 = ctx->Mlearn32;
+        } else {
+            st.fallback = 1;
+        }
     }
 
     if (bnorm == 0.0) {
@@ -859,7 +909,7 @@ extern "C" int jfm_solve(jfm_ctx *ctx, const jfm_solve_opts *opts, const double 
             total += it;
             ++restarts;
         }
-        st.iterations = total;
+        st.iterations += total;
         st.outer = 1 + restarts;
         // u = x + g
         CHECK_LAUNCH(ctx, launch_add(ctx, ctx->g, (const double *)ctx->ws64.x, ctx->u64, n));
@@ -894,7 +944,7 @@ extern "C" int jfm_solve(jfm_ctx *ctx, const jfm_solve_opts *opts, const double 
             CHECK_LAUNCH(ctx, launch_axpy_scaled(ctx, ctx->u64, (const float *)ctx->ws32.x, rn, n));
             if (status == DONE_BREAKDOWN) break;
         }
-        st.iterations = total;
+        st.iterations += total;
         st.outer = outer;
     }
     // final FP64 true residual with the FP64 operator
@@ -940,4 +990,31 @@ extern "C" int jfm_solve_history(jfm_ctx *ctx, int32_t *iterations, double *relr
         if (relres) relres[k] = ctx->hist_res[k];
     }
     return n;
+}
+
+extern "C" int jfm_precond_set_blocks(jfm_ctx *ctx, const double *M) {
+    if (!ctx || !M) return JFM_EINVAL;
+    if (!ctx->has_mesh) return jfm_fail(ctx, JFM_ESTATE, "no mesh");
+    JFM_HIP(ctx, hipSetDevice(ctx->device));
+    const int64_t N = ctx->N;
+    std::vector<double> m(M, M + 9 * N);
+    std::vector<uint8_t> fx(3 * N);
+    JFM_HIP(ctx, hipMemcpyAsync(fx.data(), ctx->fixed, 3 * N, hipMemcpyDeviceToHost, ctx->stream));
+    if (int rc = sync(ctx)) return rc;
+    std::vector<float> m32(9 * N);
+    for (int64_t a = 0; a < N; ++a)
+        for (int c = 0; c < 3; ++c) {
+            for (int d = 0; d < 3; ++d) {
+                if (fx[3 * a + c] || fx[3 * a + d]) m[9 * a + 3 * c + d] = 0.0;
+                if (!std::isfinite(m[9 * a + 3 * c + d])) return jfm_fail(ctx, JFM_EINVAL, "non-finite preconditioner block");
+            }
+        }
+    for (int64_t k = 0; k < 9 * N; ++k) m32[k] = (float)m[k];
+    if (!ctx->Muser64) ALLOC(ctx, ctx->Muser64, double, 9 * N);
+    if (!ctx->Muser32) ALLOC(ctx, ctx->Muser32, float, 9 * N);
+    JFM_HIP(ctx, hipMemcpyAsync(ctx->Muser64, m.data(), sizeof(double) * 9 * N, hipMemcpyHostToDevice, ctx->stream));
+    JFM_HIP(ctx, hipMemcpyAsync(ctx->Muser32, m32.data(), sizeof(float) * 9 * N, hipMemcpyHostToDevice, ctx->stream));
+    if (int rc = sync(ctx)) return rc;
+    ctx->has_user = true;
+    return JFM_OK;
 }

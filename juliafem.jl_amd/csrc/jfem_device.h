@@ -63,6 +63,25 @@ __device__ __forceinline__ void sum_partials(const double *__restrict__ part, in
 
 __device__ __forceinline__ bool finite(double x) { return x - x == 0.0; }
 
+// dN/dxi at the Gauss points as constant tables for runtime (q, i) indexing
+// (compile-time indexing uses Elem<NPE>::dN directly).
+template <int NPE> struct DNTable { double v[Elem<NPE>::NGP][NPE][3]; };
+template <int NPE> constexpr DNTable<NPE> make_dn() {
+    DNTable<NPE> t{};
+    for (int q = 0; q < Elem<NPE>::NGP; ++q)
+        for (int i = 0; i < NPE; ++i)
+            for (int c = 0; c < 3; ++c) t.v[q][i][c] = Elem<NPE>::dN(q, i, c);
+    return t;
+}
+static __constant__ DNTable<10> c_dn10 = make_dn<10>();
+static __constant__ DNTable<4> c_dn4 = make_dn<4>();
+static __constant__ DNTable<8> c_dn8 = make_dn<8>();
+
+template <int NPE> __device__ __forceinline__ double dn_rt(int q, int i, int c);
+template <> __device__ __forceinline__ double dn_rt<10>(int q, int i, int c) { return c_dn10.v[q][i][c]; }
+template <> __device__ __forceinline__ double dn_rt<4>(int q, int i, int c) { return c_dn4.v[q][i][c]; }
+template <> __device__ __forceinline__ double dn_rt<8>(int q, int i, int c) { return c_dn8.v[q][i][c]; }
+
 // Grid size for a grid-stride producer kernel: enough blocks for `work`
 // items at `per_block` items per block, capped at kMaxPart.
 inline unsigned capped_grid(int64_t work, int per_block) {

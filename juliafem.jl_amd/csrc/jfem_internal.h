@@ -199,7 +199,16 @@ struct jfm_ctx {
     double *Mlearn64 = nullptr;    // N x 9 learned SPD blocks
     float *Mlearn32 = nullptr;
     float *feat = nullptr;         // N x 24 node features
+    float *lout = nullptr;         // N x 6 network outputs
+    double *Muser64 = nullptr;     // N x 9 user blocks (JFM_PREC_USER)
+    float *Muser32 = nullptr;
+    bool has_user = false;
 };
+
+// api.hip helpers used by learned.hip
+int jfem_ensure_workspaces(jfm_ctx *ctx);                                   // vectors + block Jacobi
+int jfem_apply(jfm_ctx *ctx, int op, const double *x, double *y);          // masked FP64 K.x (device)
+int jfem_dot(jfm_ctx *ctx, const double *a, const double *b, int64_t n, double *out);
 
 // -------------------------------------------------------------- helpers ----
 int jfm_fail(jfm_ctx *ctx, int code, const std::string &msg);

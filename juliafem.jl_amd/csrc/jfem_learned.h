@@ -1,11 +1,13 @@
 // jfem_learned.h -- learned block preconditioner: weight blob and hooks.
 //
 // Per-node network 24 -> 64 -> 64 -> 6 with GELU, BF16 weights and FP32
-// accumulation.  Its 6 outputs fill a lower-triangular L (diagonal through
-// softplus); the node's preconditioner block is
+// accumulation.  Its 6 outputs o fill a lower-triangular L,
+//     L = [[e^o0, 0, 0], [o1, e^o2, 0], [o3, o4, e^o5]]   (exponents clamped to +-5),
+// and the node's preconditioner block is
 //     M_a = C_a (L L' + eps I) C_a',   C_a C_a' = inv(D_a),
 // with D_a the node's 3x3 diagonal block of K, so M_a is SPD for any weights
-// and L = I reproduces block Jacobi up to the factor (1 + eps).
+// and L = I reproduces block Jacobi up to the factor (1 + eps).  Nodes with a
+// fixed DOF keep the block-Jacobi block.
 #pragma once
 
 #include <cstdint>
@@ -16,6 +18,9 @@ namespace jfem {
 
 constexpr int kFeat = 24, kHid = 64, kOut = 6;
 constexpr uint32_t kBlobMagic = 0x4C4D464Au;  // "JFML"
+// trainable parameters in gradient order: W1, b1, W2, b2, W3, b3
+constexpr int kOffW1 = 0, kOffB1 = kOffW1 + kHid * kFeat, kOffW2 = kOffB1 + kHid, kOffB2 = kOffW2 + kHid * kHid,
+              kOffW3 = kOffB2 + kHid, kOffB3 = kOffW3 + kOut * kHid, kNParam = kOffB3 + kOut;
 
 // Blob layout (little endian, no padding): every field in this order.
 struct LearnedBlob {
@@ -28,6 +33,11 @@ struct LearnedBlob {
     float eps;
 };
 
+// device launchers (learned.hip)
+hipError_t launch_features(jfm_ctx *ctx, float *feat);
+hipError_t launch_mlp(jfm_ctx *ctx, const float *feat, const void *blob, float *out, const double *G,
+                      const double *Minv, float *gpart, int *nwaves);
+hipError_t launch_mblocks(jfm_ctx *ctx, const float *out, const double *Minv, float eps, double *M64, float *M32);
 }  // namespace jfem
 
 // Builds features, runs the network and forms Mlearn64/Mlearn32 for the

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(_PKG, "libjfem_hip.so")
 OP_CSR, OP_EBE, OP_MATFREE = 0, 1, 2
 FP64, FP32 = 0, 1
 SOLVE_PCG, SOLVE_MIXED_IR = 0, 1
-PREC_BLOCK_JACOBI, PREC_LEARNED = 0, 1
+PREC_BLOCK_JACOBI, PREC_LEARNED, PREC_USER = 0, 1, 2
 NPE = {"Tet10": 10, "Tet4": 4, "Hex8": 8}
 NFN = {"Tri6": 6, "Tri3": 3, "Quad4": 4}
 OPS = {"csr": OP_CSR, "ebe": OP_EBE, "matfree": OP_MATFREE}
@@ -25,7 +25,9 @@ ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ENOTSPD", -4: "ENOCONV", -5: "ESTATE", 
 # Exported symbols (must match include/jfem.h)
 SYMBOLS = ("jfm_create", "jfm_destroy", "jfm_last_error", "jfm_get_info", "jfm_upload_mesh",
            "jfm_upload_bcs", "jfm_csr_pattern", "jfm_assemble", "jfm_apply", "jfm_time_apply",
-           "jfm_solve", "jfm_solve_history", "jfm_learned_blob_bytes", "jfm_learned_set_weights", "jfm_learned_get_weights")
+           "jfm_solve", "jfm_solve_history", "jfm_precond_set_blocks", "jfm_learned_blob_bytes", "jfm_learned_n_params",
+           "jfm_learned_set_weights", "jfm_learned_get_weights", "jfm_learned_features", "jfm_learned_outputs",
+           "jfm_learned_grad")
 
 
 class JfemError(RuntimeError):
@@ -83,12 +85,18 @@ def load():
     lib.jfm_time_apply.argtypes = [_P, C.c_int32, C.c_int32, C.c_int32, _P]
     lib.jfm_solve.argtypes = [_P, C.POINTER(SolveOpts), _P, _P, _P, C.POINTER(SolveStats)]
     lib.jfm_solve_history.argtypes = [_P, _P, _P, C.c_int32]
+    lib.jfm_precond_set_blocks.argtypes = [_P, _P]
     lib.jfm_learned_blob_bytes.argtypes = []
     lib.jfm_learned_blob_bytes.restype = C.c_size_t
+    lib.jfm_learned_n_params.argtypes = []
+    lib.jfm_learned_n_params.restype = C.c_int32
+    lib.jfm_learned_features.argtypes = [_P, _P]
+    lib.jfm_learned_outputs.argtypes = [_P, _P, _P]
+    lib.jfm_learned_grad.argtypes = [_P, C.c_int32, _P, _P, C.POINTER(C.c_double), _P]
     lib.jfm_learned_set_weights.argtypes = [_P, _P, C.c_size_t]
     lib.jfm_learned_get_weights.argtypes = [_P, _P, C.c_size_t]
     for name in SYMBOLS:
-        if name not in ("jfm_destroy", "jfm_last_error", "jfm_learned_blob_bytes"):
+        if name not in ("jfm_destroy", "jfm_last_error", "jfm_learned_blob_bytes", "jfm_learned_n_params"):
             getattr(lib, name).restype = C.c_int
     _lib = lib
     return lib
@@ -212,6 +220,11 @@ class Context:
             self._check(n)
         return list(zip(it[:n].tolist(), rr[:n].tolist()))
 
+    def set_preconditioner_blocks(self, M):
+        """User 3x3 blocks (n_nodes x 9) for precond=PREC_USER."""
+        M = np.ascontiguousarray(M, dtype=np.float64)
+        self._check(self._lib.jfm_precond_set_blocks(self._h, _p(M)))
+
     def set_learned_weights(self, blob: bytes):
         buf = np.frombuffer(blob, dtype=np.uint8).copy()
         self._check(self._lib.jfm_learned_set_weights(self._h, _p(buf), buf.size))
@@ -221,3 +234,26 @@ class Context:
         buf = np.empty(n, np.uint8)
         self._check(self._lib.jfm_learned_get_weights(self._h, _p(buf), n))
         return buf.tobytes()
+
+    def learned_features(self):
+        N = self.info()["n_nodes"]
+        f = np.empty((N, 24), np.float32)
+        self._check(self._lib.jfm_learned_features(self._h, _p(f)))
+        return f
+
+    def learned_outputs(self):
+        """(network outputs N x 6, preconditioner blocks N x 9) for the current weights."""
+        N = self.info()["n_nodes"]
+        o = np.empty((N, 6), np.float32)
+        M = np.empty((N, 9))
+        self._check(self._lib.jfm_learned_outputs(self._h, _p(o), _p(M)))
+        return o, M
+
+    def learned_grad(self, R, E):
+        """(mean loss, gradient) over residual/error pairs (rows of R and E)."""
+        R = np.ascontiguousarray(R, dtype=np.float64)
+        E = np.ascontiguousarray(E, dtype=np.float64)
+        g = np.empty(self._lib.jfm_learned_n_params(), np.float32)
+        loss = C.c_double(0)
+        self._check(self._lib.jfm_learned_grad(self._h, R.shape[0], _p(R), _p(E), C.byref(loss), _p(g)))
+        return loss.value, g

@@ -724,3 +724,294 @@ void orc_basis(int type, int surface, const double *xi, double *N, double *dN) {
 }
 
 int orc_surface_rule(int ftype, double *w, double *xi) { return surface_rule(ftype, w, (double (*)[2])xi); }
+
+/* ---------------------------------------------- learned preconditioner ---- */
+/* CPU restatement of the learned 3x3 block preconditioner (no reference
+ * counterpart; the library's learned.hip).  Features are built from the
+ * oracle's own BL'DBL element matrices; the network runs in FP64 with the
+ * library's BF16 rounding points (network inputs and hidden activations on
+ * the forward pass, output and hidden gradients on the backward pass). */
+
+#define LF 24
+#define LH 64
+#define LO 6
+
+static int g_emulate_bf16 = 1;  /* round where the library rounds to BF16 */
+
+static double bf16_round(double v) {
+    if (!g_emulate_bf16) return v;
+    float f = (float)v;
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    u &= 0xFFFF0000u;
+    memcpy(&f, &u, 4);
+    return (double)f;
+}
+static double gelu_d(double x) { return 0.5 * x * (1.0 + erf(x / sqrt(2.0))); }
+static double gelu_grad_d(double x) { return 0.5 * (1.0 + erf(x / sqrt(2.0))) + x * exp(-0.5 * x * x) * 0.39894228040143268; }
+
+/* 24 node features (feat N x 24).  Nodes are visited with their elements in
+ * ascending id (the incidence order of the nodal assembly). */
+void orc_learned_features(int64_t N, const double *X, int64_t n_el, int etype, const int64_t *conn, const double *E,
+                          const double *nu, const uint8_t *fixed, const uint8_t *onface, double *feat) {
+    const int npe = etype, nd = 3 * npe;
+    const int ncorner = etype == ET_TET10 ? 4 : npe;
+    int64_t *ptr = (int64_t *)malloc(((size_t)N + 1) * sizeof(int64_t));
+    int64_t *data = (int64_t *)malloc((size_t)n_el * npe * sizeof(int64_t));
+    orc_node_to_elem(N, n_el, npe, conn, ptr, data);
+    /* per element: K_e, quality q_e, has-fixed flag */
+    double *Kd = (double *)malloc((size_t)n_el * npe * 9 * sizeof(double)); /* diagonal blocks */
+    double *qe = (double *)malloc((size_t)n_el * sizeof(double));
+    uint8_t *fe = (uint8_t *)malloc((size_t)n_el);
+    for (int64_t e = 0; e < n_el; ++e) {
+        double Xe[30], Ke[900], w[8], xi[8][3], dN[10][3], g[10][3];
+        const int64_t *cn = conn + e * npe;
+        gather_X(X, cn, npe, Xe);
+        orc_element_stiffness(etype, Xe, E[e], nu[e], Ke);
+        for (int i = 0; i < npe; ++i)
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) Kd[((size_t)e * npe + i) * 9 + r * 3 + c] = Ke[(3 * i + r) * nd + 3 * i + c];
+        orc_volume_rule(etype, w, xi);
+        volume_basis(etype, xi[0], NULL, dN);
+        double J[3][3] = {{0}};
+        for (int i = 0; i < npe; ++i)
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) J[a][b] += dN[i][a] * Xe[3 * i + b];
+        const double det = eval_basis(npe, (const double (*)[3])dN, Xe, g);
+        /* |invJ|_F from the adjugate */
+        double fro = 0;
+        const double c[9] = {J[1][1] * J[2][2] - J[1][2] * J[2][1], J[0][2] * J[2][1] - J[0][1] * J[2][2],
+                             J[0][1] * J[1][2] - J[0][2] * J[1][1], J[1][2] * J[2][0] - J[1][0] * J[2][2],
+                             J[0][0] * J[2][2] - J[0][2] * J[2][0], J[0][2] * J[1][0] - J[0][0] * J[1][2],
+                             J[1][0] * J[2][1] - J[1][1] * J[2][0], J[0][1] * J[2][0] - J[0][0] * J[2][1],
+                             J[0][0] * J[1][1] - J[0][1] * J[1][0]};
+        for (int m = 0; m < 9; ++m) fro += (c[m] / det) * (c[m] / det);
+        qe[e] = sqrt(fro) * cbrt(det);
+        uint8_t f = 0;
+        for (int i = 0; i < npe; ++i) f |= fixed[3 * cn[i]] | fixed[3 * cn[i] + 1] | fixed[3 * cn[i] + 2];
+        fe[e] = f;
+    }
+    for (int64_t a = 0; a < N; ++a) {
+        double D[9] = {0}, tmin = 1e300, tmax = 0, tsum = 0, tsq = 0, emin = 1e300, emax = 0, esum = 0, lesum = 0,
+               tle = 0, qsum = 0, qmin = 1e300, qmax = 0;
+        int nc = 0, nfx = 0;
+        const int64_t n = ptr[a + 1] - ptr[a];
+        double *f = feat + a * LF;
+        if (n == 0) { for (int k = 0; k < LF; ++k) f[k] = 0.0; continue; }
+        for (int64_t k = ptr[a]; k < ptr[a + 1]; ++k) {
+            const int64_t e = data[k];
+            int i = 0;
+            while (conn[e * npe + i] != a) ++i;
+            const double *B = Kd + ((size_t)e * npe + i) * 9;
+            for (int m = 0; m < 9; ++m) D[m] += B[m];
+            const double te = B[0] + B[4] + B[8];
+            tmin = fmin(tmin, te); tmax = fmax(tmax, te); tsum += te; tsq += te * te;
+            emin = fmin(emin, E[e]); emax = fmax(emax, E[e]); esum += E[e]; lesum += log(E[e]); tle += te * log(E[e]);
+            qsum += qe[e]; qmin = fmin(qmin, qe[e]); qmax = fmax(qmax, qe[e]);
+            if (i < ncorner) ++nc;
+            if (fe[e]) ++nfx;
+        }
+        const double emean = esum / n;
+        int nab = 0;
+        for (int64_t k = ptr[a]; k < ptr[a + 1]; ++k) nab += E[data[k]] > emean;
+        const double T = D[0] + D[4] + D[8], s = 3.0 / T;
+        const double d00 = D[0] * s, d11 = D[4] * s, d22 = D[8] * s, d01 = D[1] * s, d12 = D[5] * s, d02 = D[2] * s;
+        const double tmean = tsum / n;
+        f[0] = d00; f[1] = d11; f[2] = d22; f[3] = d01; f[4] = d12; f[5] = d02;
+        f[6] = log((double)n);
+        f[7] = tmax / tsum;
+        f[8] = n * tmin / tsum;
+        f[9] = sqrt(fmax(tsq / n - tmean * tmean, 0.0)) / tmean;
+        f[10] = log(emax / emin);
+        f[11] = log(emean) - lesum / n;
+        f[12] = (double)nc / n;
+        f[13] = (fixed[3 * a] + fixed[3 * a + 1] + fixed[3 * a + 2]) / 3.0;
+        f[14] = (double)nfx / n;
+        f[15] = onface[a] ? 1.0 : 0.0;
+        f[16] = d00 * (d11 * d22 - d12 * d12) - d01 * (d01 * d22 - d12 * d02) + d02 * (d01 * d12 - d11 * d02);
+        f[17] = sqrt(2.0 * (d01 * d01 + d12 * d12 + d02 * d02));
+        f[18] = qsum / n; f[19] = qmin; f[20] = qmax;
+        f[21] = tle / tsum - log(emax);
+        f[22] = (tmax - tmin) / tsum;
+        f[23] = (double)nab / n;
+    }
+    free(ptr); free(data); free(Kd); free(qe); free(fe);
+}
+
+typedef struct {
+    const double *W1, *W2, *W3;     /* [out][in] (BF16 values when emulating the library) */
+    const double *b1, *b2, *b3, *shift, *scale;
+} orc_net;
+
+/* forward for one node; keeps pre-activations for the backward pass */
+static void net_forward(const orc_net *nt, const double *feat, double *x, double *z1, double *h1, double *z2,
+                        double *h2, double *o) {
+    for (int k = 0; k < LF; ++k)
+        x[k] = bf16_round(g_emulate_bf16 ? (double)(((float)feat[k] - (float)nt->shift[k]) * (float)nt->scale[k])
+                                         : (feat[k] - nt->shift[k]) * nt->scale[k]);
+    for (int j = 0; j < LH; ++j) {
+        double s = 0;
+        for (int k = 0; k < LF; ++k) s += nt->W1[j * LF + k] * x[k];
+        z1[j] = s + nt->b1[j];
+        h1[j] = bf16_round(gelu_d(z1[j]));
+    }
+    for (int j = 0; j < LH; ++j) {
+        double s = 0;
+        for (int k = 0; k < LH; ++k) s += nt->W2[j * LH + k] * h1[k];
+        z2[j] = s + nt->b2[j];
+        h2[j] = bf16_round(gelu_d(z2[j]));
+    }
+    for (int j = 0; j < LO; ++j) {
+        double s = 0;
+        for (int k = 0; k < LH; ++k) s += nt->W3[j * LH + k] * h2[k];
+        o[j] = s + nt->b3[j];
+    }
+}
+
+static double clampo(double v) { return v < -5 ? -5 : (v > 5 ? 5 : v); }
+
+static void chol3(const double *S, double *C) {
+    const double c00 = sqrt(S[0]), c10 = S[3] / c00, c20 = S[6] / c00;
+    const double c11 = sqrt(S[4] - c10 * c10), c21 = (S[7] - c20 * c10) / c11;
+    const double c22 = sqrt(S[8] - c20 * c20 - c21 * c21);
+    const double t[9] = {c00, 0, 0, c10, c11, 0, c20, c21, c22};
+    memcpy(C, t, sizeof t);
+}
+
+static void lower_of(const double *o, double *L) {
+    const double t[9] = {exp(clampo(o[0])), 0, 0, o[1], exp(clampo(o[2])), 0, o[3], o[4], exp(clampo(o[5]))};
+    memcpy(L, t, sizeof t);
+}
+
+/* network outputs (N x 6) and blocks M = C (L L' + eps I) C' (N x 9); nodes
+ * with a fixed DOF or a zero block keep Minv */
+void orc_learned_blocks(int64_t N, const double *feat, const double *W1, const double *b1, const double *W2,
+                        const double *b2, const double *W3, const double *b3, const double *shift, const double *scale,
+                        double eps, const double *Minv, const uint8_t *fixed, double *out, double *M, int emulate_bf16) {
+    g_emulate_bf16 = emulate_bf16;
+    orc_net nt = {W1, W2, W3, b1, b2, b3, shift, scale};
+    for (int64_t a = 0; a < N; ++a) {
+        double x[LF], z1[LH], h1[LH], z2[LH], h2[LH], o[LO];
+        net_forward(&nt, feat + a * LF, x, z1, h1, z2, h2, o);
+        for (int j = 0; j < LO; ++j) out[a * LO + j] = o[j];
+        const double *Mi = Minv + 9 * a;
+        double *Ma = M + 9 * a;
+        if (fixed[3 * a] || fixed[3 * a + 1] || fixed[3 * a + 2] || !(Mi[0] > 0)) {
+            memcpy(Ma, Mi, 9 * sizeof(double));
+            continue;
+        }
+        double C[9], L[9], A[9];
+        chol3(Mi, C);
+        lower_of(o, L);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                double s = (i == j) ? eps : 0.0;
+                for (int k = 0; k < 3; ++k) s += L[i * 3 + k] * L[j * 3 + k];
+                A[i * 3 + j] = s;
+            }
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                double s = 0;
+                for (int k = 0; k < 3; ++k)
+                    for (int m = 0; m < 3; ++m) s += C[i * 3 + k] * A[k * 3 + m] * C[j * 3 + m];
+                Ma[i * 3 + j] = s;
+            }
+    }
+}
+
+/* Loss over the pairs and its gradient (6150 doubles, order W1 b1 W2 b2 W3 b3).
+ * K is the CSR of the whole system; rows/columns of fixed DOFs are masked. */
+double orc_learned_loss_grad(int64_t N, const int64_t *rowptr, const int64_t *colind, const double *vals,
+                             const uint8_t *fixed, const double *Minv, const double *feat, const double *W1,
+                             const double *b1, const double *W2, const double *b2, const double *W3, const double *b3,
+                             const double *shift, const double *scale, double eps, int n_pairs, const double *R,
+                             const double *Er, double *grad, int emulate_bf16) {
+    const int64_t n = 3 * N;
+    double *out = (double *)malloc((size_t)N * LO * sizeof(double));
+    double *M = (double *)malloc((size_t)N * 9 * sizeof(double));
+    double *z = (double *)malloc((size_t)n * sizeof(double));
+    double *w = (double *)malloc((size_t)n * sizeof(double));
+    double *G = (double *)calloc((size_t)N * 9, sizeof(double));
+    orc_learned_blocks(N, feat, W1, b1, W2, b2, W3, b3, shift, scale, eps, Minv, fixed, out, M, emulate_bf16);
+    double loss = 0;
+    for (int p = 0; p < n_pairs; ++p) {
+        const double *r = R + (size_t)p * n, *e = Er + (size_t)p * n;
+        for (int64_t a = 0; a < N; ++a)
+            for (int i = 0; i < 3; ++i) {
+                double s = 0;
+                for (int j = 0; j < 3; ++j) s += M[9 * a + 3 * i + j] * r[3 * a + j];
+                z[3 * a + i] = s;
+            }
+        orc_spmv(n, rowptr, colind, vals, z, w, 1);
+        double zr = 0, zw = 0, er = 0;
+        for (int64_t i = 0; i < n; ++i) {
+            if (fixed[i]) w[i] = 0;
+            zr += z[i] * r[i]; zw += z[i] * w[i]; er += e[i] * r[i];
+        }
+        const double c = zr / zw, coef = -2.0 * c / (er * n_pairs);
+        loss += 1.0 - zr * c / er;
+        for (int64_t a = 0; a < N; ++a)
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) G[9 * a + 3 * i + j] += coef * (r[3 * a + i] - c * w[3 * a + i]) * r[3 * a + j];
+    }
+    loss /= n_pairs;
+    orc_net nt = {W1, W2, W3, b1, b2, b3, shift, scale};
+    memset(grad, 0, 6150 * sizeof(double));
+    double *gW1 = grad, *gb1 = grad + 1536, *gW2 = grad + 1600, *gb2 = grad + 5696, *gW3 = grad + 5760, *gb3 = grad + 6144;
+    for (int64_t a = 0; a < N; ++a) {
+        const double *Mi = Minv + 9 * a;
+        if (fixed[3 * a] || fixed[3 * a + 1] || fixed[3 * a + 2] || !(Mi[0] > 0)) continue;
+        double x[LF], z1[LH], h1[LH], z2[LH], h2[LH], o[LO];
+        net_forward(&nt, feat + a * LF, x, z1, h1, z2, h2, o);
+        double C[9], L[9], Gs[9], Q[9], dL[9];
+        chol3(Mi, C);
+        lower_of(o, L);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) Gs[i * 3 + j] = G[9 * a + i * 3 + j] + G[9 * a + j * 3 + i];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                double s = 0;
+                for (int k = 0; k < 3; ++k)
+                    for (int m = 0; m < 3; ++m) s += C[k * 3 + i] * Gs[k * 3 + m] * C[m * 3 + j];
+                Q[i * 3 + j] = s;
+            }
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                double s = 0;
+                for (int k = 0; k < 3; ++k) s += Q[i * 3 + k] * L[k * 3 + j];
+                dL[i * 3 + j] = s;
+            }
+        double dO[LO];
+        dO[0] = (o[0] > -5 && o[0] < 5) ? dL[0] * L[0] : 0.0;
+        dO[1] = dL[3];
+        dO[2] = (o[2] > -5 && o[2] < 5) ? dL[4] * L[4] : 0.0;
+        dO[3] = dL[6];
+        dO[4] = dL[7];
+        dO[5] = (o[5] > -5 && o[5] < 5) ? dL[8] * L[8] : 0.0;
+        double dOb[LO], dz2[LH], dz2b[LH], dz1[LH], dz1b[LH];
+        for (int j = 0; j < LO; ++j) { dOb[j] = bf16_round(dO[j]); gb3[j] += dO[j]; }
+        for (int j = 0; j < LO; ++j)
+            for (int k = 0; k < LH; ++k) gW3[j * LH + k] += dOb[j] * h2[k];
+        for (int k = 0; k < LH; ++k) {
+            double s = 0;
+            for (int j = 0; j < LO; ++j) s += dOb[j] * W3[j * LH + k];
+            dz2[k] = s * gelu_grad_d(z2[k]);
+            dz2b[k] = bf16_round(dz2[k]);
+            gb2[k] += dz2[k];
+        }
+        for (int j = 0; j < LH; ++j)
+            for (int k = 0; k < LH; ++k) gW2[j * LH + k] += dz2b[j] * h1[k];
+        for (int k = 0; k < LH; ++k) {
+            double s = 0;
+            for (int j = 0; j < LH; ++j) s += dz2b[j] * W2[j * LH + k];
+            dz1[k] = s * gelu_grad_d(z1[k]);
+            dz1b[k] = bf16_round(dz1[k]);
+            gb1[k] += dz1[k];
+        }
+        for (int j = 0; j < LH; ++j)
+            for (int k = 0; k < LF; ++k) gW1[j * LF + k] += dz1b[j] * x[k];
+    }
+    free(out); free(M); free(z); free(w); free(G);
+    return loss;
+}

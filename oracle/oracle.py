@@ -211,3 +211,96 @@ def surface_rule(ftype: str):
     lib().orc_surface_rule.argtypes = [C.c_int, _P, _P]
     n = lib().orc_surface_rule(_FTYPE[ftype], _p(w), _p(xi))
     return w[:n].copy(), xi[:n].copy()
+
+
+# ------------------------------------------------- learned preconditioner --
+_U16 = np.uint16
+
+
+def _net_args(net):
+    """Network parameters as FP64 arrays.  `net` is a packed weight blob (the
+    BF16 values the library uses) or a dict of FP64 arrays W1 b1 W2 b2 W3 b3
+    shift scale eps (for finite-difference checks without rounding)."""
+    if isinstance(net, (bytes, bytearray)):
+        from jfem.learned import from_bf16, unpack_blob
+        w = unpack_blob(net)
+        w = {**w, **{k: from_bf16(w[k]).astype(np.float64) for k in ("W1", "W2", "W3")}}
+    else:
+        w = net
+    keep = [_f64(np.asarray(w[k], dtype=np.float64).ravel()) for k in
+            ("W1", "b1", "W2", "b2", "W3", "b3", "shift", "scale")]
+    return keep, float(w["eps"])
+
+
+def block_jacobi(csr, fixed):
+    """Inverse 3x3 diagonal blocks with fixed rows/columns removed (N x 9)."""
+    rowptr, colind, vals = csr
+    n = rowptr.size - 1
+    N = n // 3
+    fixed = np.asarray(fixed, dtype=np.uint8)
+    Minv = np.zeros((N, 9))
+    for a in range(N):
+        B = np.zeros((3, 3))
+        for c in range(3):
+            row = 3 * a + c
+            s, e = rowptr[row], rowptr[row + 1]
+            cols = colind[s:e]
+            m = (cols >= 3 * a) & (cols < 3 * a + 3)
+            B[c, cols[m] - 3 * a] = vals[s:e][m]
+        if not np.any(B):
+            continue
+        for c in range(3):
+            if fixed[3 * a + c]:
+                B[c, :] = 0
+                B[:, c] = 0
+                B[c, c] = 1
+        Mi = np.linalg.inv(B)
+        for c in range(3):
+            if fixed[3 * a + c]:
+                Mi[c, :] = 0
+                Mi[:, c] = 0
+        Minv[a] = Mi.ravel()
+    return Minv
+
+
+def learned_features(mesh, fixed, faces=None):
+    """Node features; `faces` are the loaded surface elements (default: the mesh's)."""
+    lib().orc_learned_features.argtypes = [_I64, _P, _I64, C.c_int, _P, _P, _P, _P, _P, _P]
+    lib().orc_learned_features.restype = None
+    feat = np.zeros((mesh.n_nodes, 24))
+    fx = np.ascontiguousarray(fixed, dtype=np.uint8)
+    faces = mesh.faces if faces is None else faces
+    onface = np.zeros(mesh.n_nodes, np.uint8)
+    onface[np.asarray(faces, dtype=np.int64).ravel()] = 1
+    lib().orc_learned_features(mesh.n_nodes, _p(_f64(mesh.X)), mesh.n_el, _ETYPE[mesh.etype], _p(_i64(mesh.conn)),
+                               _p(_f64(mesh.E)), _p(_f64(mesh.nu)), _p(fx), _p(onface), _p(feat))
+    return feat
+
+
+def learned_blocks(feat, net, Minv, fixed, emulate_bf16=True):
+    keep, eps = _net_args(net)
+    N = feat.shape[0]
+    out = np.zeros((N, 6))
+    M = np.zeros((N, 9))
+    f = lib().orc_learned_blocks
+    f.argtypes = [_I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.c_double, _P, _P, _P, _P, C.c_int]
+    f.restype = None
+    f(N, _p(_f64(feat)), *[_p(k) for k in keep], eps, _p(_f64(Minv)),
+      _p(np.ascontiguousarray(fixed, dtype=np.uint8)), _p(out), _p(M), int(emulate_bf16))
+    return out, M
+
+
+def learned_loss_grad(csr, fixed, Minv, feat, net, R, E, emulate_bf16=True):
+    keep, eps = _net_args(net)
+    rowptr, colind, vals = csr
+    N = feat.shape[0]
+    R = _f64(R)
+    E = _f64(E)
+    grad = np.zeros(6150)
+    f = lib().orc_learned_loss_grad
+    f.argtypes = [_I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.c_double, C.c_int, _P, _P, _P,
+                  C.c_int]
+    f.restype = C.c_double
+    loss = f(N, _p(rowptr), _p(colind), _p(vals), _p(np.ascontiguousarray(fixed, dtype=np.uint8)), _p(_f64(Minv)),
+             _p(_f64(feat)), *[_p(k) for k in keep], eps, R.shape[0], _p(R), _p(E), _p(grad), int(emulate_bf16))
+    return loss, grad

@@ -1,0 +1,120 @@
+"""Learned preconditioner on the GPU against the oracle.
+
+Bars: features within FP32 rounding; network outputs and SPD blocks within
+BF16 operand rounding (both sides round at the same points, FP32 vs FP64
+accumulation remains); the weight gradient within 1e-2 relative in norm;
+training lowers the loss; the runtime probe falls back to block Jacobi for
+a harmful network and the solve still reaches 1e-8.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from jfem import mesh as M
+from jfem.learned import LearnedPreconditioner, harvest_pairs, train
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def setup(oracle):
+    import jfem
+    m = M.box_tet10(8, 2, 2, seed=21, jitter=0.1, E_sigma=0.7)
+    ctx = jfem.Context(0)
+    ctx.upload(m)
+    csr = oracle.assemble_csr(m)
+    f = oracle.load_vector(m)
+    fx = m.fixed_mask()
+    return m, ctx, csr, f, fx
+
+
+def trained_like_net(seed):
+    lp = LearnedPreconditioner(seed=seed)
+    rng = np.random.default_rng(seed + 100)
+    lp.params["W3"] = (rng.standard_normal((6, 64)) * 0.05).astype(np.float32)
+    lp.params["b3"] = (rng.standard_normal(6) * 0.05).astype(np.float32)
+    return lp
+
+
+def test_features_match_oracle(setup, oracle):
+    m, ctx, csr, f, fx = setup
+    say("features")
+    g = ctx.learned_features()
+    o = oracle.learned_features(m, fx)
+    assert np.abs(g - o).max() <= 2e-5 * max(1.0, np.abs(o).max())
+
+
+def test_outputs_and_blocks_match_oracle(setup, oracle):
+    m, ctx, csr, f, fx = setup
+    lp = trained_like_net(1)
+    lp.fit_normalisation([oracle.learned_features(m, fx)])
+    ctx.set_learned_weights(lp.blob())
+    say("network outputs")
+    out, Mg = ctx.learned_outputs()
+    feat = ctx.learned_features().astype(np.float64)
+    Minv = oracle.block_jacobi(csr, fx)
+    out_o, Mo = oracle.learned_blocks(feat, lp.blob(), Minv, fx, emulate_bf16=True)
+    err = np.abs(out - out_o)
+    assert np.median(err) <= 1e-5 and err.max() <= 2e-2 * max(1.0, np.abs(out_o).max())
+    rel = np.abs(Mg - Mo).max(axis=1) / np.abs(Mo).max(axis=1).clip(1e-300)
+    assert np.median(rel) <= 1e-5 and rel.max() <= 5e-2
+
+
+def test_gradient_matches_oracle(setup, oracle):
+    m, ctx, csr, f, fx = setup
+    lp = trained_like_net(2)
+    feat = ctx.learned_features().astype(np.float64)
+    lp.fit_normalisation([feat])
+    ctx.set_learned_weights(lp.blob())
+    say("harvest pairs")
+    R, E = harvest_pairs(ctx, m, n_pairs=4)
+    say("gradient")
+    loss_g, grad_g = ctx.learned_grad(R, E)
+    Minv = oracle.block_jacobi(csr, fx)
+    loss_o, grad_o = oracle.learned_loss_grad(csr, fx, Minv, feat, lp.blob(), R, E, emulate_bf16=True)
+    assert loss_g == pytest.approx(loss_o, rel=1e-6)
+    assert np.linalg.norm(grad_g - grad_o) <= 1e-2 * np.linalg.norm(grad_o)
+    for lo, hi in ((0, 1536), (1600, 5696), (5760, 6144)):      # W1, W2, W3 blocks
+        assert np.linalg.norm(grad_g[lo:hi] - grad_o[lo:hi]) <= 2e-2 * np.linalg.norm(grad_o[lo:hi])
+
+
+def test_training_lowers_loss_and_solves(setup):
+    m, ctx, csr, f, fx = setup
+    import jfem
+    meshes = [M.box_tet10(8, 2, 2, seed=s, jitter=0.1, E_sigma=0.7) for s in (31, 32)]
+    ctxs, pairs, feats = [], [], []
+    for mm in meshes:
+        c = jfem.Context(0)
+        c.upload(mm)
+        ctxs.append(c)
+        feats.append(c.learned_features())
+        pairs.append(harvest_pairs(c, mm, n_pairs=6))
+    lp = LearnedPreconditioner(seed=3)
+    lp.fit_normalisation(feats)
+    say("train 40 steps")
+    hist = train(ctxs, pairs, lp, steps=40, lr=3e-3)
+    assert hist[-1] < hist[0]
+    ctx.set_learned_weights(lp.blob())
+    u, _, st = ctx.solve(method=0, op="matfree", precond=1, rtol=1e-8)
+    assert st["converged"] == 1 and st["fallback"] == 0
+    ub, _, sb = ctx.solve(method=0, op="matfree", precond=0, rtol=1e-8)
+    say(f"held-out: learned {st['iterations']} iterations, block Jacobi {sb['iterations']}")
+    assert np.linalg.norm(u - ub) / np.linalg.norm(ub) <= 1e-6
+
+
+def test_harmful_network_falls_back(setup):
+    m, ctx, csr, f, fx = setup
+    lp = LearnedPreconditioner(seed=4)
+    rng = np.random.default_rng(9)
+    lp.params["W3"] = (rng.standard_normal((6, 64)) * 2.0).astype(np.float32)   # wild blocks
+    ctx.set_learned_weights(lp.blob())
+    u, _, st = ctx.solve(method=0, op="matfree", precond=1, learned_probe=20, rtol=1e-8)
+    assert st["fallback"] == 1 and st["converged"] == 1
+    u2, _, st2 = ctx.solve(method=1, op="matfree", op_outer="matfree", precond=1, learned_probe=20, rtol=1e-8)
+    assert st2["fallback"] == 1 and st2["converged"] == 1
