@@ -54,7 +54,9 @@ enum {
 enum {
     JFM_OP_CSR = 0,      /* assembled K (3x3-block compressed CSR) */
     JFM_OP_EBE = 1,      /* stored element matrices, deterministic nodal gather */
-    JFM_OP_MATFREE = 2   /* element geometric factors, recompute, nodal gather */
+    JFM_OP_MATFREE = 2,  /* element geometric factors, recompute, nodal gather */
+    JFM_OP_TANGENT = 3   /* neo-Hookean tangent K_T(u) at the state set by the
+                            nonlinear solver (matrix-free, nodal gather) */
 };
 enum { JFM_FP64 = 0, JFM_FP32 = 1 };
 
@@ -173,6 +175,46 @@ int jfm_precond_set_blocks(jfm_ctx *ctx, const double *M);
  * restart) the cumulative iteration count and the FP64 true relative
  * residual after it.  Returns the number of entries (<= max_entries). */
 int jfm_solve_history(jfm_ctx *ctx, int32_t *iterations, double *relres, int32_t max_entries);
+
+/* ---- nonlinear path (run!(solver::Solver{Nonlinear}), src/solvers.jl:575-620;
+ * solve_newton_krylov_gpu!, ext/JuliaFEMCUDAExt.jl:685-854) -----------------
+ * Compressible neo-Hookean, total Lagrangian (src/materials/neo_hookean.jl:
+ * 129-231; src/problems_elasticity.jl:255-289, 378-404), Lame parameters from
+ * the uploaded E and nu.  Load is applied in equal steps of the uploaded
+ * traction; each step runs Newton to ||lf f_ext - f_int(u)|| <= rtol ||lf f_ext||
+ * on the free DOFs; each tangent solve is PCG (FP64, or FP32 inside FP64
+ * refinement) to an Eisenstat-Walker tolerance. */
+typedef struct {
+    int32_t load_steps;   /* equal load increments (>= 1) */
+    int32_t max_newton;   /* Newton iterations per load step */
+    double rtol;          /* Newton tolerance on the relative residual */
+    int32_t method;       /* JFM_SOLVE_PCG or JFM_SOLVE_MIXED_IR for the tangent solves */
+    int32_t precond;      /* JFM_PREC_BLOCK_JACOBI or JFM_PREC_LEARNED (diagonal of K_T) */
+    double eta_max;       /* first and largest linear tolerance of Eisenstat-Walker
+                             choice 2 (0: 0.1) */
+    int32_t maxit_linear; /* PCG iteration cap per tangent solve */
+    int32_t learned_probe;/* as in jfm_solve_opts */
+    double inner_rtol;    /* JFM_SOLVE_MIXED_IR: FP32 inner tolerance (0: 1e-2) */
+    int32_t inner_maxit;  /* JFM_SOLVE_MIXED_IR: FP32 iterations per inner solve (0: no cap) */
+    int32_t check_every;  /* PCG iterations between host checks (0: 32) */
+} jfm_newton_opts;
+
+typedef struct {
+    int32_t newton_iterations; /* summed over load steps */
+    int32_t linear_iterations; /* PCG iterations summed over all tangent solves */
+    int32_t converged;         /* 1 when every load step met rtol */
+    int32_t fallbacks;         /* tangent solves where the learned blocks were rejected */
+    double relres;             /* final relative residual of the last load step */
+    double solve_ms;           /* device time */
+} jfm_newton_stats;
+
+/* R = lf f_ext - f_int(u) on the free DOFs (0 on fixed ones) for the
+ * displacement u (n_dofs); *rnorm = |R|.  R may be NULL. */
+int jfm_nl_residual(jfm_ctx *ctx, const double *u, double load_factor, double *R, double *rnorm);
+
+/* Load-stepped Newton from u = u_prescribed; u (n_dofs) receives the final
+ * displacement. */
+int jfm_solve_nonlinear(jfm_ctx *ctx, const jfm_newton_opts *opts, double *u, jfm_newton_stats *stats);
 
 /* ---- learned preconditioner (no reference counterpart) ---------------
  * Per-node MLP 24 -> 64 -> 64 -> 6 (GELU), BF16 weights, FP32 accumulation.

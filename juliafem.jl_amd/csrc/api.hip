@@ -165,6 +165,9 @@ static void release_mesh(jfm_ctx *ctx) {
     }
     free_ptr(ctx, ctx->Mlearn64); free_ptr(ctx, ctx->Mlearn32); free_ptr(ctx, ctx->feat); free_ptr(ctx, ctx->lout);
     free_ptr(ctx, ctx->Muser64); free_ptr(ctx, ctx->Muser32);
+    free_ptr(ctx, ctx->ustate64); free_ptr(ctx, ctx->ustate32); free_ptr(ctx, ctx->Dtan);
+    free_ptr(ctx, ctx->nl_u); free_ptr(ctx, ctx->nl_r); free_ptr(ctx, ctx->nl_du); free_ptr(ctx, ctx->nl_zero);
+    ctx->has_state = ctx->minv_tangent = false;
     ctx->has_user = false;
     ctx->has_mesh = ctx->has_bcs = ctx->assembled = ctx->has_load = false;
     ctx->learned_ready = false;
@@ -522,6 +525,9 @@ static int prepare_op(jfm_ctx *ctx, int op) {
     if (op == JFM_OP_CSR) return ensure_assembled(ctx);
     if (op == JFM_OP_EBE) return ensure_element_matrices(ctx);
     if (op == JFM_OP_MATFREE) return JFM_OK;
+    if (op == JFM_OP_TANGENT)
+        return ctx->has_state ? JFM_OK
+                              : jfm_fail(ctx, JFM_ESTATE, "tangent operator needs a state: call jfm_nl_residual first");
     return jfm_fail(ctx, JFM_EINVAL, "unknown operator variant");
 }
 
@@ -820,6 +826,29 @@ static int true_residual(jfm_ctx *ctx, int op, const double *b, const double *u,
     return JFM_OK;
 }
 
+// Learned blocks against block Jacobi for the system in ws64.b: builds the
+// learned blocks from the block Jacobi now in ws*.Minv, then, with probe > 0,
+// runs `probe` PCG iterations from 0 with each.  After the same number of
+// iterations the iterate with the larger energy b.x_k (CG decreases
+// 1/2 x'Kx - b.x = -1/2 b.x_k monotonically) has the smaller energy-norm
+// error; the learned blocks are kept only if they win.
+static int learned_select(jfm_ctx *ctx, int op, int probe, int check, bool *use, int *iters, double *setup_ms) {
+    if (int rc = jfem_learned_prepare(ctx, setup_ms)) return rc;
+    *use = true;
+    if (probe <= 0) return JFM_OK;
+    double phi[2] = {0.0, 0.0};
+    const double *cand[2] = {(const double *)ctx->ws64.Minv, ctx->Mlearn64};
+    for (int c = 0; c < 2; ++c) {
+        int it = 0, status = 0;
+        if (int rc = pcg_run<double>(ctx, op, cand[c], 1e-30, probe, check, &it, &status)) return rc;
+        if (int rc = jfem_dot(ctx, (const double *)ctx->ws64.b, (const double *)ctx->ws64.x, ctx->ndof, &phi[c]))
+            return rc;
+        *iters += it;
+    }
+    *use = phi[1] > phi[0];
+    return JFM_OK;
+}
+
 extern "C" int jfm_solve(jfm_ctx *ctx, const jfm_solve_opts *opts, const double *f, double *u, double *la,
                          jfm_solve_stats *stats) {
     if (!ctx || !opts || !u) return JFM_EINVAL;
@@ -863,24 +892,10 @@ extern "C" int jfm_solve(jfm_ctx *ctx, const jfm_solve_opts *opts, const double 
         Minv32 = ctx->Muser32;
     }
     if (opts->precond == JFM_PREC_LEARNED && bnorm > 0.0) {
-        if (int rc = jfem_learned_prepare(ctx, &st.setup_ms)) return rc;
         bool use = true;
-        if (opts->learned_probe > 0) {
-            // Runtime check: after the same number of PCG iterations from 0 the
-            // iterate with the larger energy b.x_k (CG decreases
-            // 1/2 x'Kx - b.x = -1/2 b.x_k monotonically) has the smaller
-            // energy-norm error; keep the learned blocks only if they win.
-            double phi[2] = {0.0, 0.0};
-            const double *cand[2] = {(const double *)ctx->ws64.Minv, ctx->Mlearn64};
-            for (int c = 0; c < 2; ++c) {
-                int it = 0, status = 0;
-                if (int rc = pcg_run<double>(ctx, op_outer, cand[c], 1e-30, opts->learned_probe, check, &it, &status))
-                    return rc;
-                if (int rc = jfem_dot(ctx, b, (const double *)ctx->ws64.x, n, &phi[c])) return rc;
-                st.iterations += it;
-            }
-            use = phi[1] > phi[0];
-        }
+        int its = 0;
+        if (int rc = learned_select(ctx, op_outer, opts->learned_probe, check, &use, &its, &st.setup_ms)) return rc;
+        st.iterations += its;
         if (use) {
             Minv64 = ctx->Mlearn64;
             Minv32 = ctx->Mlearn32;
@@ -1016,5 +1031,236 @@ extern "C" int jfm_precond_set_blocks(jfm_ctx *ctx, const double *M) {
     JFM_HIP(ctx, hipMemcpyAsync(ctx->Muser32, m32.data(), sizeof(float) * 9 * N, hipMemcpyHostToDevice, ctx->stream));
     if (int rc = sync(ctx)) return rc;
     ctx->has_user = true;
+    return JFM_OK;
+}
+
+// ============================================================== nonlinear ==
+static int nl_buffers(jfm_ctx *ctx) {
+    const int64_t n = ctx->ndof, N = ctx->N;
+    if (!ctx->ustate64) ALLOC(ctx, ctx->ustate64, double, n);
+    if (!ctx->ustate32) ALLOC(ctx, ctx->ustate32, float, n);
+    if (!ctx->Dtan) ALLOC(ctx, ctx->Dtan, double, 9 * N);
+    if (!ctx->nl_u) ALLOC(ctx, ctx->nl_u, double, n);
+    if (!ctx->nl_r) ALLOC(ctx, ctx->nl_r, double, n);
+    if (!ctx->nl_du) ALLOC(ctx, ctx->nl_du, double, n);
+    if (!ctx->nl_zero) {
+        ALLOC(ctx, ctx->nl_zero, double, n);
+        JFM_HIP(ctx, hipMemsetAsync(ctx->nl_zero, 0, sizeof(double) * n, ctx->stream));
+    }
+    return JFM_OK;
+}
+
+// Makes u (device, n_dofs) the state of the tangent operator and computes
+// R = lf f_ext - f_int(u) on the free DOFs (0 on fixed ones), *rnorm = |R|
+// (NAN when some Gauss point has det F <= 0).  Uses vx64/vy64 as scratch.
+static int nl_residual(jfm_ctx *ctx, const double *u, double lf, double *R, double *rnorm) {
+    const int64_t n = ctx->ndof;
+    hipStream_t s = ctx->stream;
+    if (u != ctx->ustate64) JFM_HIP(ctx, hipMemcpyAsync(ctx->ustate64, u, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    CHECK_LAUNCH(ctx, (launch_convert<double, float>(ctx, ctx->ustate64, ctx->ustate32, n, 1.0)));
+    ctx->has_state = true;
+    int np = 0;
+    CHECK_LAUNCH(ctx, launch_nh_elem<double>(ctx, false, ctx->gf64, ctx->ustate64, nullptr, ctx->ye64, nullptr));
+    CHECK_LAUNCH(ctx, launch_gather<double>(ctx, ctx->ye64, ctx->fixed, ctx->vy64, nullptr, nullptr, nullptr, &np));
+    CHECK_LAUNCH(ctx, (launch_convert<double, double>(ctx, ctx->f_ext, ctx->vx64, n, lf)));
+    CHECK_LAUNCH(ctx, launch_residual(ctx, ctx->vx64, ctx->vy64, ctx->fixed, R, n, &np));
+    if (int rc = finalize_misc(ctx, np, 1, FIN_DOT)) return rc;
+    if (int rc = read_scalars(ctx)) return rc;
+    *rnorm = std::sqrt(ctx->S_host->tmp0);
+    return JFM_OK;
+}
+
+extern "C" int jfm_nl_residual(jfm_ctx *ctx, const double *u, double load_factor, double *R, double *rnorm) {
+    if (!ctx || !u) return JFM_EINVAL;
+    if (!ctx->has_mesh || !ctx->has_bcs) return jfm_fail(ctx, JFM_ESTATE, "upload the mesh and boundary conditions first");
+    JFM_HIP(ctx, hipSetDevice(ctx->device));
+    if (int rc = nl_buffers(ctx)) return rc;
+    const int64_t n = ctx->ndof;
+    JFM_HIP(ctx, hipMemcpyAsync(ctx->nl_u, u, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+    double rn = 0.0;
+    if (int rc = nl_residual(ctx, ctx->nl_u, load_factor, ctx->nl_r, &rn)) return rc;
+    if (R) JFM_HIP(ctx, hipMemcpyAsync(R, ctx->nl_r, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+    if (int rc = sync(ctx)) return rc;
+    if (rnorm) *rnorm = rn;
+    return JFM_OK;
+}
+
+// K_T du = r (r in nl_r, free DOFs) to |r - K_T du| <= eta |r|; du -> nl_du.
+static int tangent_solve(jfm_ctx *ctx, const jfm_newton_opts *o, const double *M64, const float *M32, double eta,
+                         double rnorm, int check, int *iters) {
+    const int64_t n = ctx->ndof;
+    hipStream_t s = ctx->stream;
+    *iters = 0;
+    if (o->method == JFM_SOLVE_PCG) {
+        JFM_HIP(ctx, hipMemcpyAsync(ctx->ws64.b, ctx->nl_r, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        int it = 0, status = 0;
+        if (int rc = pcg_run<double>(ctx, JFM_OP_TANGENT, M64, eta, o->maxit_linear, check, &it, &status)) return rc;
+        *iters = it;
+        JFM_HIP(ctx, hipMemcpyAsync(ctx->nl_du, ctx->ws64.x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        return JFM_OK;
+    }
+    // FP32 PCG inside FP64 refinement on the tangent
+    JFM_HIP(ctx, hipMemsetAsync(ctx->nl_du, 0, sizeof(double) * n, s));
+    const double inner = o->inner_rtol > 0 ? o->inner_rtol : 1e-2;
+    double rl = rnorm;
+    for (int outer = 0; outer < 50; ++outer) {
+        if (outer > 0)
+            if (int rc = true_residual(ctx, JFM_OP_TANGENT, ctx->nl_r, ctx->nl_du, ctx->rr64, &rl)) return rc;
+        if (rl <= eta * rnorm || *iters >= o->maxit_linear) break;
+        const double *src = outer == 0 ? ctx->nl_r : ctx->rr64;
+        CHECK_LAUNCH(ctx, (launch_convert<double, float>(ctx, src, (float *)ctx->ws32.b, n, 1.0 / rl)));
+        int it = 0, status = 0;
+        int cap = o->maxit_linear - *iters;
+        if (o->inner_maxit > 0) cap = std::min(cap, o->inner_maxit);
+        // no need to solve the correction beyond what the outer tolerance asks
+        const double tol = std::max(inner, 0.5 * eta * rnorm / rl);
+        if (int rc = pcg_run<float>(ctx, JFM_OP_TANGENT, M32, tol, cap, check, &it, &status)) return rc;
+        *iters += it;
+        CHECK_LAUNCH(ctx, launch_axpy_scaled(ctx, ctx->nl_du, (const float *)ctx->ws32.x, rl, n));
+        if (status == DONE_BREAKDOWN) break;
+    }
+    return JFM_OK;
+}
+
+extern "C" int jfm_solve_nonlinear(jfm_ctx *ctx, const jfm_newton_opts *opts, double *u, jfm_newton_stats *stats) {
+    if (!ctx || !opts || !u) return JFM_EINVAL;
+    if (!ctx->has_mesh || !ctx->has_bcs) return jfm_fail(ctx, JFM_ESTATE, "upload the mesh and boundary conditions first");
+    if (opts->load_steps < 1 || opts->max_newton < 1 || !(opts->rtol > 0.0) || opts->maxit_linear < 1)
+        return jfm_fail(ctx, JFM_EINVAL, "load_steps, max_newton, maxit_linear >= 1 and rtol > 0 required");
+    if (opts->method != JFM_SOLVE_PCG && opts->method != JFM_SOLVE_MIXED_IR)
+        return jfm_fail(ctx, JFM_EINVAL, "unknown solve method");
+    if (opts->precond != JFM_PREC_BLOCK_JACOBI && opts->precond != JFM_PREC_LEARNED)
+        return jfm_fail(ctx, JFM_EINVAL, "the tangent solves take JFM_PREC_BLOCK_JACOBI or JFM_PREC_LEARNED");
+    if (opts->precond == JFM_PREC_LEARNED && !ctx->has_mlp)
+        return jfm_fail(ctx, JFM_ESTATE, "learned preconditioner requested but no weights are loaded");
+    JFM_HIP(ctx, hipSetDevice(ctx->device));
+    if (int rc = ensure_workspaces(ctx)) return rc;
+    if (int rc = nl_buffers(ctx)) return rc;
+    const int64_t n = ctx->ndof;
+    hipStream_t s = ctx->stream;
+    const int check = std::max(2, opts->check_every > 0 ? opts->check_every : 32);
+    const double eta_max = opts->eta_max > 0.0 && opts->eta_max < 1.0 ? opts->eta_max : 0.1;
+    jfm_newton_stats st{};
+    ctx->hist_it.clear();
+    ctx->hist_res.clear();
+    hipEvent_t e0, e1;
+    JFM_HIP(ctx, hipEventCreate(&e0));
+    JFM_HIP(ctx, hipEventCreate(&e1));
+    JFM_HIP(ctx, hipEventRecord(e0, s));
+    // |f_ext| on the free DOFs
+    double fnorm = 0.0;
+    {
+        int np = 0;
+        CHECK_LAUNCH(ctx, launch_residual(ctx, ctx->f_ext, ctx->nl_zero, ctx->fixed, ctx->rr64, n, &np));
+        if (int rc = finalize_misc(ctx, np, 1, FIN_DOT)) return rc;
+        if (int rc = read_scalars(ctx)) return rc;
+        fnorm = std::sqrt(ctx->S_host->tmp0);
+    }
+    JFM_HIP(ctx, hipMemsetAsync(ctx->nl_u, 0, sizeof(double) * n, s));
+    bool all_ok = true;
+    double relres = 0.0;
+    struct Restore {  // the workspaces end with the tangent block Jacobi: rebuild the linear one on next use
+        jfm_ctx *c;
+        ~Restore() {
+            c->ws64.ok = c->ws32.ok = false;
+            c->minv_tangent = false;
+            c->learned_ready = false;
+        }
+    } restore{ctx};
+    for (int step = 1; step <= opts->load_steps && all_ok; ++step) {
+        const double lf = (double)step / opts->load_steps, lf0 = (double)(step - 1) / opts->load_steps;
+        // prescribed displacements follow the load factor (g is zero on free DOFs)
+        if (ctx->g_nonzero) {
+            CHECK_LAUNCH(ctx, (launch_convert<double, double>(ctx, ctx->g, ctx->rr64, n, lf - lf0)));
+            CHECK_LAUNCH(ctx, launch_add(ctx, ctx->nl_u, ctx->rr64, ctx->nl_u, n));
+        }
+        double rn = 0.0;
+        if (int rc = nl_residual(ctx, ctx->nl_u, lf, ctx->nl_r, &rn)) return rc;
+        if (!std::isfinite(rn))
+            return jfm_fail(ctx, JFM_ENOCONV, "inadmissible state (det F <= 0) at the start of a load step");
+        const double ref = lf * fnorm > 0.0 ? lf * fnorm : (rn > 0.0 ? rn : 1.0);
+        bool ok = false;
+        double rn_prev = rn, eta_prev = eta_max;
+        for (int it = 0;; ++it) {
+            relres = rn / ref;
+            ctx->hist_it.push_back(st.linear_iterations);
+            ctx->hist_res.push_back(relres);
+            if (rn <= opts->rtol * ref) { ok = true; break; }
+            if (it >= opts->max_newton) break;
+            // tangent block Jacobi at the current state (features use its diagonal)
+            CHECK_LAUNCH(ctx, launch_nh_block_jacobi(ctx, ctx->ustate64, (double *)ctx->ws64.Minv, (float *)ctx->ws32.Minv,
+                                                     ctx->Dtan));
+            ctx->ws64.ok = ctx->ws32.ok = true;
+            ctx->minv_tangent = true;
+            const double *M64 = (const double *)ctx->ws64.Minv;
+            const float *M32 = (const float *)ctx->ws32.Minv;
+            if (opts->precond == JFM_PREC_LEARNED) {
+                ctx->learned_ready = false;
+                bool use = true;
+                int its = 0;
+                JFM_HIP(ctx, hipMemcpyAsync(ctx->ws64.b, ctx->nl_r, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+                if (int rc = learned_select(ctx, JFM_OP_TANGENT, opts->learned_probe, check, &use, &its, nullptr))
+                    return rc;
+                st.linear_iterations += its;
+                if (use) {
+                    M64 = ctx->Mlearn64;
+                    M32 = ctx->Mlearn32;
+                } else {
+                    ++st.fallbacks;
+                }
+            }
+            // Forcing term.  The reference uses min(0.9, |R|^0.5) on the absolute
+            // residual (ext/JuliaFEMCUDAExt.jl:785-787), which is unit dependent
+            // and near 0.9 for a relative one; here Eisenstat-Walker choice 2,
+            // eta = 0.9 (|R_k| / |R_k-1|)^2 with their safeguard, starting each
+            // load step at eta_max, capped by eta_max and never tighter than the
+            // Newton tolerance needs.
+            double eta = eta_max;
+            if (it > 0) {
+                const double ratio = rn / rn_prev;
+                eta = 0.9 * ratio * ratio;
+                const double safe = 0.9 * eta_prev * eta_prev;
+                if (safe > 0.1) eta = std::max(eta, safe);
+                eta = std::min(eta, eta_max);
+            }
+            eta = std::max(eta, 0.5 * opts->rtol * ref / rn);
+            eta_prev = eta;
+            rn_prev = rn;
+            int lits = 0;
+            if (int rc = tangent_solve(ctx, opts, M64, M32, eta, rn, check, &lits)) return rc;
+            st.linear_iterations += lits;
+            ++st.newton_iterations;
+            // u += du; halve the step while some Gauss point leaves det F > 0
+            JFM_HIP(ctx, hipMemcpyAsync(ctx->rr64, ctx->nl_u, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+            double rnew = 0.0;
+            for (int bt = 0;; ++bt) {
+                CHECK_LAUNCH(ctx, launch_add(ctx, ctx->rr64, ctx->nl_du, ctx->nl_u, n));
+                if (int rc = nl_residual(ctx, ctx->nl_u, lf, ctx->nl_r, &rnew)) return rc;
+                if (std::isfinite(rnew) || bt >= 12) break;
+                CHECK_LAUNCH(ctx, (launch_convert<double, double>(ctx, ctx->nl_du, ctx->nl_du, n, 0.5)));
+            }
+            if (!std::isfinite(rnew)) break;
+            rn = rnew;
+        }
+        all_ok = ok;
+    }
+    JFM_HIP(ctx, hipEventRecord(e1, s));
+    JFM_HIP(ctx, hipEventSynchronize(e1));
+    float ms = 0.f;
+    JFM_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    JFM_HIP(ctx, hipMemcpyAsync(u, ctx->nl_u, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    if (int rc = sync(ctx)) return rc;
+    st.converged = all_ok ? 1 : 0;
+    st.relres = relres;
+    st.solve_ms = ms;
+    if (stats) *stats = st;
+    if (!all_ok) {
+        char msg[160];
+        std::snprintf(msg, sizeof msg, "Newton did not reach rtol %.3e: relres %.3e after %d iterations", opts->rtol,
+                      relres, st.newton_iterations);
+        return jfm_fail(ctx, JFM_ENOCONV, msg);
+    }
     return JFM_OK;
 }

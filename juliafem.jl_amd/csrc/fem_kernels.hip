@@ -625,6 +625,10 @@ static void launch_elem_phase(jfm_ctx *ctx, int op, const T *x, T *ye, const int
 template <typename T>
 hipError_t launch_elem_only(jfm_ctx *ctx, int op, const T *x, const int *done) {
     if (ctx->n_el == 0) return hipSuccess;
+    if (op == JFM_OP_TANGENT) {  // neo-Hookean K_T(u) x at the stored state
+        const T *u = sizeof(T) == 8 ? (const T *)ctx->ustate64 : (const T *)ctx->ustate32;
+        return launch_nh_elem<T>(ctx, true, gf_of<T>(ctx), u, x, ye_of<T>(ctx), done);
+    }
     switch (ctx->npe) {
     case 10: launch_elem_phase<10, T>(ctx, op, x, ye_of<T>(ctx), done); break;
     case 4: launch_elem_phase<4, T>(ctx, op, x, ye_of<T>(ctx), done); break;

@@ -200,6 +200,13 @@ struct jfm_ctx {
     float *Mlearn32 = nullptr;
     float *feat = nullptr;         // N x 24 node features
     float *lout = nullptr;         // N x 6 network outputs
+    // nonlinear state (tangent operator)
+    double *ustate64 = nullptr;    // displacement the tangent is taken at
+    float *ustate32 = nullptr;
+    double *Dtan = nullptr;        // N x 9 tangent diagonal blocks (features)
+    bool has_state = false;
+    bool minv_tangent = false;     // ws*.Minv hold the tangent block Jacobi
+    double *nl_u = nullptr, *nl_r = nullptr, *nl_du = nullptr, *nl_zero = nullptr;
     double *Muser64 = nullptr;     // N x 9 user blocks (JFM_PREC_USER)
     float *Muser32 = nullptr;
     bool has_user = false;
@@ -241,6 +248,11 @@ hipError_t launch_face_gather(jfm_ctx *ctx);
 template <typename T>
 hipError_t launch_two_phase(jfm_ctx *ctx, int op, const T *x, T *y, const T *dotv, double *part,
                             const int *done, int *npart);
+// nonlinear.hip (neo-Hookean): element phase of f_int(u) (tangent = false)
+// or K_T(u) v (tangent = true) into the incidence slots
+template <typename T>
+hipError_t launch_nh_elem(jfm_ctx *ctx, bool tangent, const T *gf, const T *u, const T *v, T *ye, const int *done);
+hipError_t launch_nh_block_jacobi(jfm_ctx *ctx, const double *u, double *M64, float *M32, double *D);
 // linalg_kernels.hip
 template <typename T>
 hipError_t launch_bsr_spmv(jfm_ctx *ctx, const T *vals, const T *x, T *y, const T *dotv, double *part,

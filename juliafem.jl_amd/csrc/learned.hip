@@ -49,7 +49,8 @@ __global__ __launch_bounds__(256) void k_features(int N, int n_el, int ngf, cons
                                                  const int32_t *__restrict__ inc, const int32_t *__restrict__ conn,
                                                  const double *__restrict__ gf, const double *__restrict__ E,
                                                  const double *__restrict__ nu, const uint8_t *__restrict__ fixed,
-                                                 const int32_t *__restrict__ fptr, float *__restrict__ feat) {
+                                                 const int32_t *__restrict__ fptr, const double *__restrict__ Dover,
+                                                 float *__restrict__ feat) {
     using EL = Elem<NPE>;
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= N) return;
@@ -97,6 +98,8 @@ __global__ __launch_bounds__(256) void k_features(int N, int n_el, int ngf, cons
         }
         if (fx) ++nfixel;
     }
+    if (Dover)  // diagonal block of the tangent in place of the linear one
+        for (int m = 0; m < 9; ++m) D[m] = Dover[9 * (size_t)a + m];
     const double emean = esum / n;
     int nabove = 0;
     for (int k = k0; k < k1; ++k) nabove += (E[inc[k] / NPE] > emean) ? 1 : 0;
@@ -481,10 +484,11 @@ constexpr int kTrainBlocks = 256;
 
 hipError_t launch_features(jfm_ctx *ctx, float *feat) {
     const int N = (int)ctx->N;
+    const double *Dover = ctx->minv_tangent ? ctx->Dtan : nullptr;
     switch (ctx->npe) {
-    case 10: k_features<10><<<nblk(N), kBlock, 0, ctx->stream>>>(N, (int)ctx->n_el, ctx->ngf, ctx->inc_ptr, ctx->inc, ctx->conn, ctx->gf64, ctx->E, ctx->nu, ctx->fixed, ctx->finc_ptr, feat); break;
-    case 4: k_features<4><<<nblk(N), kBlock, 0, ctx->stream>>>(N, (int)ctx->n_el, ctx->ngf, ctx->inc_ptr, ctx->inc, ctx->conn, ctx->gf64, ctx->E, ctx->nu, ctx->fixed, ctx->finc_ptr, feat); break;
-    case 8: k_features<8><<<nblk(N), kBlock, 0, ctx->stream>>>(N, (int)ctx->n_el, ctx->ngf, ctx->inc_ptr, ctx->inc, ctx->conn, ctx->gf64, ctx->E, ctx->nu, ctx->fixed, ctx->finc_ptr, feat); break;
+    case 10: k_features<10><<<nblk(N), kBlock, 0, ctx->stream>>>(N, (int)ctx->n_el, ctx->ngf, ctx->inc_ptr, ctx->inc, ctx->conn, ctx->gf64, ctx->E, ctx->nu, ctx->fixed, ctx->finc_ptr, Dover, feat); break;
+    case 4: k_features<4><<<nblk(N), kBlock, 0, ctx->stream>>>(N, (int)ctx->n_el, ctx->ngf, ctx->inc_ptr, ctx->inc, ctx->conn, ctx->gf64, ctx->E, ctx->nu, ctx->fixed, ctx->finc_ptr, Dover, feat); break;
+    case 8: k_features<8><<<nblk(N), kBlock, 0, ctx->stream>>>(N, (int)ctx->n_el, ctx->ngf, ctx->inc_ptr, ctx->inc, ctx->conn, ctx->gf64, ctx->E, ctx->nu, ctx->fixed, ctx->finc_ptr, Dover, feat); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

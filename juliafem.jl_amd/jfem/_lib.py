@@ -13,13 +13,13 @@ import numpy as np
 _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(_PKG, "libjfem_hip.so")
 
-OP_CSR, OP_EBE, OP_MATFREE = 0, 1, 2
+OP_CSR, OP_EBE, OP_MATFREE, OP_TANGENT = 0, 1, 2, 3
 FP64, FP32 = 0, 1
 SOLVE_PCG, SOLVE_MIXED_IR = 0, 1
 PREC_BLOCK_JACOBI, PREC_LEARNED, PREC_USER = 0, 1, 2
 NPE = {"Tet10": 10, "Tet4": 4, "Hex8": 8}
 NFN = {"Tri6": 6, "Tri3": 3, "Quad4": 4}
-OPS = {"csr": OP_CSR, "ebe": OP_EBE, "matfree": OP_MATFREE}
+OPS = {"csr": OP_CSR, "ebe": OP_EBE, "matfree": OP_MATFREE, "tangent": OP_TANGENT}
 ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ENOTSPD", -4: "ENOCONV", -5: "ESTATE", -6: "ENOMEM", -7: "EUNSUPPORTED"}
 
 # Exported symbols (must match include/jfem.h)
@@ -27,7 +27,7 @@ SYMBOLS = ("jfm_create", "jfm_destroy", "jfm_last_error", "jfm_get_info", "jfm_u
            "jfm_upload_bcs", "jfm_csr_pattern", "jfm_assemble", "jfm_apply", "jfm_time_apply",
            "jfm_solve", "jfm_solve_history", "jfm_precond_set_blocks", "jfm_learned_blob_bytes", "jfm_learned_n_params",
            "jfm_learned_set_weights", "jfm_learned_get_weights", "jfm_learned_features", "jfm_learned_outputs",
-           "jfm_learned_grad")
+           "jfm_learned_grad", "jfm_nl_residual", "jfm_solve_nonlinear")
 
 
 class JfemError(RuntimeError):
@@ -47,6 +47,21 @@ class SolveStats(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("outer", C.c_int32), ("converged", C.c_int32),
                 ("fallback", C.c_int32), ("relres", C.c_double), ("solve_ms", C.c_double),
                 ("setup_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class NewtonOpts(C.Structure):
+    _fields_ = [("load_steps", C.c_int32), ("max_newton", C.c_int32), ("rtol", C.c_double), ("method", C.c_int32),
+                ("precond", C.c_int32), ("eta_max", C.c_double), ("maxit_linear", C.c_int32),
+                ("learned_probe", C.c_int32), ("inner_rtol", C.c_double), ("inner_maxit", C.c_int32),
+                ("check_every", C.c_int32)]
+
+
+class NewtonStats(C.Structure):
+    _fields_ = [("newton_iterations", C.c_int32), ("linear_iterations", C.c_int32), ("converged", C.c_int32),
+                ("fallbacks", C.c_int32), ("relres", C.c_double), ("solve_ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -94,6 +109,8 @@ def load():
     lib.jfm_learned_outputs.argtypes = [_P, _P, _P]
     lib.jfm_learned_grad.argtypes = [_P, C.c_int32, _P, _P, C.POINTER(C.c_double), _P]
     lib.jfm_learned_set_weights.argtypes = [_P, _P, C.c_size_t]
+    lib.jfm_nl_residual.argtypes = [_P, _P, C.c_double, _P, C.POINTER(C.c_double)]
+    lib.jfm_solve_nonlinear.argtypes = [_P, C.POINTER(NewtonOpts), _P, C.POINTER(NewtonStats)]
     lib.jfm_learned_get_weights.argtypes = [_P, _P, C.c_size_t]
     for name in SYMBOLS:
         if name not in ("jfm_destroy", "jfm_last_error", "jfm_learned_blob_bytes", "jfm_learned_n_params"):
@@ -211,8 +228,32 @@ class Context:
             self._check(rc)
         return u, la, st.as_dict()
 
+    def nl_residual(self, u, load_factor=1.0):
+        """(R = lf f_ext - f_int(u) on free DOFs, |R|) for the neo-Hookean model;
+        u also becomes the state of op="tangent"."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        R = np.empty_like(u)
+        rn = C.c_double(0.0)
+        self._check(self._lib.jfm_nl_residual(self._h, _p(u), load_factor, _p(R), C.byref(rn)))
+        return R, rn.value
+
+    def solve_nonlinear(self, load_steps=5, max_newton=20, rtol=1e-8, method=SOLVE_PCG, precond=PREC_BLOCK_JACOBI,
+                        eta_max=0.1, maxit_linear=20000, learned_probe=0, inner_rtol=1e-2, inner_maxit=0,
+                        check_every=32, raise_on_noconv=True):
+        """Load-stepped Newton for the compressible neo-Hookean model; (u, stats)."""
+        n = self.info()["n_dofs"]
+        opts = NewtonOpts(load_steps, max_newton, rtol, method, precond, eta_max, maxit_linear, learned_probe,
+                          inner_rtol, inner_maxit, check_every)
+        st = NewtonStats()
+        u = np.empty(n)
+        rc = self._lib.jfm_solve_nonlinear(self._h, C.byref(opts), _p(u), C.byref(st))
+        if rc != 0 and not (rc == -4 and not raise_on_noconv):
+            self._check(rc)
+        return u, st.as_dict()
+
     def history(self, max_entries: int = 64):
-        """[(cumulative iterations, FP64 relative residual)] of the last solve."""
+        """[(cumulative iterations, FP64 relative residual)] of the last solve
+        (one entry per Newton iteration after solve_nonlinear)."""
         it = np.zeros(max_entries, np.int32)
         rr = np.zeros(max_entries)
         n = self._lib.jfm_solve_history(self._h, _p(it), _p(rr), max_entries)

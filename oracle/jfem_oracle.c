@@ -24,12 +24,19 @@
  *   Dirichlet + solve    src/problems_dirichlet.jl:60-89, src/solvers.jl:192-216
  *   node->element map    ext/JuliaFEMCUDAExt.jl:179-193
  *   two-phase K.u        ext/JuliaFEMCUDAExt.jl:222-361, 442-478
+ *   neo-Hookean TL       src/materials/neo_hookean.jl:129-137, 205-231;
+ *                        src/problems_elasticity.jl:246-289, 369-409
  *
  * Pinning (tests/test_oracle.py): closed forms (quadrature moments,
  * partition of unity, affine-field energy u'Ku = vol * eps:D:eps, flat-face
  * consistent loads), invariants (symmetry, rigid-body null space, equilibrium),
  * the reference demo's single-Tet4 answers (demos/nodal_assembly_cpu.jl), a
  * brute-force sparsity pattern and scipy's SuperLU on the same system.
+ * Neo-Hookean (tests/test_nonlinear_oracle.py): S and D against central
+ * differences of psi and S, D(C = I) = the linear D, the element tangent
+ * against differences of the element force and at u = 0 against the linear
+ * element stiffness, zero force under rigid translations and rotations, the
+ * patch test (interior nodes in equilibrium under homogeneous F).
  *
  * Deviations, each deliberate: Hex8 integrates with 2x2x2 Gauss points
  * (src/integration/integration.jl:121; the new-API default Gauss{1} at
@@ -724,6 +731,178 @@ void orc_basis(int type, int surface, const double *xi, double *N, double *dN) {
 }
 
 int orc_surface_rule(int ftype, double *w, double *xi) { return surface_rule(ftype, w, (double (*)[2])xi); }
+
+/* ------------------------------------- neo-Hookean total Lagrangian ---- */
+/* Compressible neo-Hookean (src/materials/neo_hookean.jl:129-137):
+ *   psi = mu/2 (tr C - 3) - mu ln J + lambda/2 ln^2 J,  J = sqrt(det C),
+ * evaluated like the reference's compute_stress(material, E) (:205-231) from
+ * the Green-Lagrange strain E (3x3), C = I + 2E.  The reference differentiates
+ * psi by AD; here S = dpsi/dE and D = d2psi/dE2 are written in closed form with
+ * A = C^-1:
+ *   S    = mu A (C - I) + lambda ln J A        (= mu (I - A) + lambda ln J A),
+ *   D_ijkl = lambda A_ij A_kl + (mu - lambda ln J)(A_ik A_jl + A_il A_jk),
+ * D in Voigt order 11,22,33,12,23,13 (engineering shear).  mu A (C - I) and
+ * ln J = log1p(det C - 1)/2 with det C - 1 = I1 + I2 + I3 of 2E keep full
+ * relative precision at small strain, where mu (I - A) cancels.  Returns psi,
+ * or NAN when det C <= 0.  S (3x3) and D (6x6) may be NULL. */
+static const int VI[6] = {0, 1, 2, 0, 1, 0}, VJ[6] = {0, 1, 2, 1, 2, 2};
+
+static double det3(const double *M) {
+    return M[0] * (M[4] * M[8] - M[5] * M[7]) - M[1] * (M[3] * M[8] - M[5] * M[6])
+         + M[2] * (M[3] * M[7] - M[4] * M[6]);
+}
+
+/* det(I + M) - 1 = I1(M) + I2(M) + I3(M) */
+static double det_ipm_minus1(const double *M) {
+    const double i1 = M[0] + M[4] + M[8];
+    const double i2 = M[0] * M[4] - M[1] * M[3] + M[4] * M[8] - M[5] * M[7] + M[0] * M[8] - M[2] * M[6];
+    return i1 + i2 + det3(M);
+}
+
+double orc_nh_material(const double *Eg, double mu, double la, double *S, double *D) {
+    double E2[9], C[9];
+    for (int k = 0; k < 9; ++k) {
+        E2[k] = 2.0 * Eg[k];
+        C[k] = E2[k] + (k % 4 == 0 ? 1.0 : 0.0);
+    }
+    const double dm1 = det_ipm_minus1(E2);
+    const double det = 1.0 + dm1;
+    if (!(det > 0.0)) return NAN;
+    const double lnJ = 0.5 * log1p(dm1);
+    double A[9];
+    A[0] = (C[4] * C[8] - C[5] * C[7]) / det;
+    A[1] = (C[2] * C[7] - C[1] * C[8]) / det;
+    A[2] = (C[1] * C[5] - C[2] * C[4]) / det;
+    A[3] = (C[5] * C[6] - C[3] * C[8]) / det;
+    A[4] = (C[0] * C[8] - C[2] * C[6]) / det;
+    A[5] = (C[2] * C[3] - C[0] * C[5]) / det;
+    A[6] = (C[3] * C[7] - C[4] * C[6]) / det;
+    A[7] = (C[1] * C[6] - C[0] * C[7]) / det;
+    A[8] = (C[0] * C[4] - C[1] * C[3]) / det;
+    if (S) {
+        double AE[9];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                AE[i * 3 + j] = A[i * 3] * E2[j] + A[i * 3 + 1] * E2[3 + j] + A[i * 3 + 2] * E2[6 + j];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                S[i * 3 + j] = 0.5 * mu * (AE[i * 3 + j] + AE[j * 3 + i]) + la * lnJ * A[i * 3 + j];
+    }
+    if (D)
+        for (int I = 0; I < 6; ++I)
+            for (int J = 0; J < 6; ++J) {
+                const int i = VI[I], j = VJ[I], k = VI[J], l = VJ[J];
+                D[I * 6 + J] = la * A[i * 3 + j] * A[k * 3 + l]
+                             + (mu - la * lnJ) * (A[i * 3 + k] * A[j * 3 + l] + A[i * 3 + l] * A[j * 3 + k]);
+            }
+    return 0.5 * mu * (E2[0] + E2[4] + E2[8]) - mu * lnJ + 0.5 * la * lnJ * lnJ;
+}
+
+/* Total-Lagrangian element (src/problems_elasticity.jl:246-289 for F, E and
+ * BL(F); :369-375 material stiffness; :378-404 geometric stiffness with
+ * BNL and S3 = blockdiag(S, S, S); :406-409 f_int), Lame parameters from E and
+ * nu as in :310-317:
+ *   f  = sum_gp w BL(F)' S_voigt,
+ *   Kt = sum_gp w (BL(F)' D BL(F) + BNL' S3 BNL),   w = ip.weight * detJ.
+ * ue is (3npe) in node-major order; Kt (row-major) may be NULL.  Returns the
+ * smallest det F seen (<= 0 or NAN: the state is not admissible). */
+double orc_nh_element(int etype, const double *Xe, const double *ue, double E, double nu, double *fe, double *Ke) {
+    const int npe = etype, nd = 3 * npe;
+    const double la = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+    const double mu = E / (2.0 * (1.0 + nu));
+    double w[8], xi[8][3], dN[10][3], g[10][3];
+    double BL[6 * 30], S[9], D[36];
+    const int ng = orc_volume_rule(etype, w, xi);
+    memset(fe, 0, nd * sizeof(double));
+    if (Ke) memset(Ke, 0, nd * nd * sizeof(double));
+    double min_detF = INFINITY;
+    for (int q = 0; q < ng; ++q) {
+        volume_basis(etype, xi[q], NULL, dN);
+        const double det = eval_basis(npe, (const double (*)[3])dN, Xe, g);
+        const double wq = w[q] * det;
+        double H[9] = {0};  /* H[a][b] = du_a / dX_b */
+        for (int i = 0; i < npe; ++i)
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) H[a * 3 + b] += ue[3 * i + a] * g[i][b];
+        /* F = I + H, E = (H + H' + H'H) / 2 (src/problems_elasticity.jl:258-260) */
+        double F[9], Eg[9];
+        for (int k = 0; k < 9; ++k) F[k] = H[k] + (k % 4 == 0 ? 1.0 : 0.0);
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                Eg[a * 3 + b] = 0.5 * (H[a * 3 + b] + H[b * 3 + a] + H[a] * H[b] + H[3 + a] * H[3 + b] + H[6 + a] * H[6 + b]);
+        const double detF = 1.0 + det_ipm_minus1(H);
+        if (!(detF >= min_detF)) min_detF = detF;
+        if (!(detF > 0.0)) continue;
+        orc_nh_material(Eg, mu, la, S, Ke ? D : NULL);
+        const double Sv[6] = {S[0], S[4], S[8], S[1], S[5], S[2]};
+        /* BL(F): row I of strain I, column 3i + a (F[a][b] = F[a * 3 + b]) */
+        memset(BL, 0, 6 * nd * sizeof(double));
+        for (int i = 0; i < npe; ++i)
+            for (int a = 0; a < 3; ++a) {
+                const int c = 3 * i + a;
+                BL[0 * nd + c] = F[a * 3 + 0] * g[i][0];
+                BL[1 * nd + c] = F[a * 3 + 1] * g[i][1];
+                BL[2 * nd + c] = F[a * 3 + 2] * g[i][2];
+                BL[3 * nd + c] = F[a * 3 + 0] * g[i][1] + F[a * 3 + 1] * g[i][0];
+                BL[4 * nd + c] = F[a * 3 + 1] * g[i][2] + F[a * 3 + 2] * g[i][1];
+                BL[5 * nd + c] = F[a * 3 + 2] * g[i][0] + F[a * 3 + 0] * g[i][2];
+            }
+        for (int c = 0; c < nd; ++c) {
+            double acc = 0.0;
+            for (int t = 0; t < 6; ++t) acc += BL[t * nd + c] * Sv[t];
+            fe[c] += wq * acc;
+        }
+        if (!Ke) continue;
+        for (int r = 0; r < nd; ++r) {
+            double BtD[6];
+            for (int s = 0; s < 6; ++s) {
+                double acc = 0.0;
+                for (int t = 0; t < 6; ++t) acc += BL[t * nd + r] * D[t * 6 + s];
+                BtD[s] = acc;
+            }
+            for (int c = 0; c < nd; ++c) {
+                double acc = 0.0;
+                for (int t = 0; t < 6; ++t) acc += BtD[t] * BL[t * nd + c];
+                Ke[r * nd + c] += wq * acc;
+            }
+        }
+        /* BNL' S3 BNL couples equal components: (3i+a, 3j+a) += g_i . S g_j */
+        for (int i = 0; i < npe; ++i)
+            for (int j = 0; j < npe; ++j) {
+                double acc = 0.0;
+                for (int b = 0; b < 3; ++b)
+                    for (int d = 0; d < 3; ++d) acc += g[i][b] * S[b * 3 + d] * g[j][d];
+                for (int a = 0; a < 3; ++a) Ke[(3 * i + a) * nd + 3 * j + a] += wq * acc;
+            }
+    }
+    return min_detF;
+}
+
+/* Global f_int(u) (n_dofs) and, when I is not NULL, the tangent as COO
+ * triplets in the order of orc_assemble_coo.  Returns the smallest det F. */
+double orc_nh_assemble(const double *X, int64_t n_el, int etype, const int64_t *conn, const double *E,
+                       const double *nu, const double *u, double *fint, int64_t n_dofs, int64_t *I, int64_t *Jc,
+                       double *V) {
+    const int npe = etype, nd = 3 * npe;
+    double min_detF = INFINITY;
+    memset(fint, 0, (size_t)n_dofs * sizeof(double));
+    for (int64_t e = 0; e < n_el; ++e) {   /* serial: f_int sums in element order */
+        double Xe[30], ue[30], fe[30], Ke[900];
+        int64_t gd[30];
+        const int64_t *cn = conn + e * npe;
+        gather_X(X, cn, npe, Xe);
+        for (int i = 0; i < npe; ++i)
+            for (int c = 0; c < 3; ++c) { gd[3 * i + c] = 3 * cn[i] + c; ue[3 * i + c] = u[3 * cn[i] + c]; }
+        const double d = orc_nh_element(etype, Xe, ue, E[e], nu[e], fe, I ? Ke : NULL);
+        if (!(d >= min_detF)) min_detF = d;
+        for (int c = 0; c < nd; ++c) fint[gd[c]] += fe[c];
+        if (!I) continue;
+        int64_t k = e * nd * nd;
+        for (int j = 0; j < nd; ++j)
+            for (int i = 0; i < nd; ++i, ++k) { I[k] = gd[i]; Jc[k] = gd[j]; V[k] = Ke[i * nd + j]; }
+    }
+    return min_detF;
+}
 
 /* ---------------------------------------------- learned preconditioner ---- */
 /* CPU restatement of the learned 3x3 block preconditioner (no reference

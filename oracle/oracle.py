@@ -56,6 +56,12 @@ def _load():
     lib.orc_node_to_elem.argtypes = [_I64, _I64, C.c_int, _P, _P, _P]
     lib.orc_matfree_apply.restype = None
     lib.orc_matfree_apply.argtypes = [_I64, _P, _I64, C.c_int, _P, _P, _P, _P, _P, _P]
+    lib.orc_nh_material.restype = C.c_double
+    lib.orc_nh_material.argtypes = [_P, C.c_double, C.c_double, _P, _P]
+    lib.orc_nh_element.restype = C.c_double
+    lib.orc_nh_element.argtypes = [C.c_int, _P, _P, C.c_double, C.c_double, _P, _P]
+    lib.orc_nh_assemble.restype = C.c_double
+    lib.orc_nh_assemble.argtypes = [_P, _I64, C.c_int, _P, _P, _P, _P, _P, _I64, _P, _P, _P]
     return lib
 
 
@@ -304,3 +310,78 @@ def learned_loss_grad(csr, fixed, Minv, feat, net, R, E, emulate_bf16=True):
     loss = f(N, _p(rowptr), _p(colind), _p(vals), _p(np.ascontiguousarray(fixed, dtype=np.uint8)), _p(_f64(Minv)),
              _p(_f64(feat)), *[_p(k) for k in keep], eps, R.shape[0], _p(R), _p(E), _p(grad), int(emulate_bf16))
     return loss, grad
+
+
+# ---------------------------------------------------------------- neo-Hookean --
+def nh_material(Eg, mu: float, la: float):
+    """(psi, S 3x3, D 6x6 Voigt) of the compressible neo-Hookean at the
+    Green-Lagrange strain E."""
+    Eg = _f64(np.asarray(Eg, np.float64).reshape(9))
+    S = np.empty(9)
+    D = np.empty(36)
+    psi = lib().orc_nh_material(_p(Eg), mu, la, _p(S), _p(D))
+    return psi, S.reshape(3, 3), D.reshape(6, 6)
+
+
+def nh_element(etype: str, Xe, ue, E: float, nu: float, tangent: bool = True):
+    """TL element force (3npe), tangent ((3npe)^2 or None) and min det F."""
+    npe = _ETYPE[etype]
+    fe = np.empty(3 * npe)
+    Ke = np.empty((3 * npe, 3 * npe)) if tangent else None
+    d = lib().orc_nh_element(npe, _p(_f64(Xe)), _p(_f64(ue)), E, nu, _p(fe), _p(Ke) if tangent else None)
+    return fe, Ke, d
+
+
+def nh_assemble(mesh, u, tangent: bool = True):
+    """(f_int(u), K_T(u) as CSR or None, min det F) over the whole mesh."""
+    et = _ETYPE[mesh.etype]
+    n = mesh.n_dofs
+    fint = np.empty(n)
+    args = [None, None, None]
+    if tangent:
+        m = int(lib().orc_coo_size(mesh.n_el, et))
+        I, J, V = np.empty(m, np.int64), np.empty(m, np.int64), np.empty(m)
+        args = [_p(I), _p(J), _p(V)]
+    d = lib().orc_nh_assemble(_p(_f64(mesh.X)), mesh.n_el, et, _p(_i64(mesh.conn)), _p(_f64(mesh.E)),
+                              _p(_f64(mesh.nu)), _p(_f64(u)), _p(fint), n, *args)
+    if not tangent:
+        return fint, None, d
+    rowptr = np.empty(n + 1, np.int64)
+    colind = np.empty(m, np.int64)
+    vals = np.empty(m)
+    perm = np.empty(m, np.int64)
+    nnz = int(lib().orc_coo_to_csr(n, m, _p(I), _p(J), _p(V), _p(rowptr), _p(colind), _p(vals), _p(perm)))
+    return fint, (rowptr, colind[:nnz].copy(), vals[:nnz].copy()), d
+
+
+def nh_newton(mesh, load_steps: int = 5, rtol: float = 1e-10, max_newton: int = 30):
+    """Load-stepped full Newton with direct tangent solves (scipy SuperLU on
+    the free DOFs): the CPU answer for the nonlinear path.  Returns
+    (u, per-step Newton iteration counts)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+    fixed = mesh.fixed_mask().astype(bool)
+    free = ~fixed
+    f = load_vector(mesh)
+    u = np.zeros(mesh.n_dofs)
+    counts = []
+    for k in range(1, load_steps + 1):
+        lf = k / load_steps
+        ref = np.linalg.norm(lf * f[free])
+        for it in range(max_newton + 1):
+            fint, csr, d = nh_assemble(mesh, u, tangent=True)
+            if not d > 0:
+                raise ValueError("inverted state in the oracle Newton")
+            R = lf * f - fint
+            rn = np.linalg.norm(R[free])
+            if rn <= rtol * ref:
+                break
+            if it == max_newton:
+                raise RuntimeError(f"oracle Newton did not converge at load step {k}: {rn / ref:.3e}")
+            K = sp.csr_matrix((csr[2], csr[1], csr[0]), shape=(mesh.n_dofs,) * 2)
+            Kff = K[free][:, free].tocsc()
+            du = np.zeros(mesh.n_dofs)
+            du[free] = sla.spsolve(Kff, R[free])
+            u += du
+        counts.append(it)
+    return u, counts
