@@ -26,6 +26,16 @@ FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 
 
+# headline candidates: (jfm_solve method, preconditioner, extra options)
+METHODS = {
+    "mixed-learned": (1, 1, dict(learned_probe=16, inner_rtol=1e-2, first_rtol=1e-3)),
+    "mixed-bj": (1, 0, dict(inner_rtol=1e-2, first_rtol=1e-3)),
+    "mixed-bj-plain": (1, 0, dict(inner_rtol=1e-4)),
+    "fp64-bj": (0, 0, {}),
+}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_matfree_fp32_tet10_160x16x16.json")
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -38,11 +48,25 @@ def kernel_bytes(inf, op, prec):
     if op == "csr":
         spmv = 9 * nnzb * T + 4 * nnzb + 4 * (N + 1) + 2 * 3 * N * T + 3 * N
         return {"spmv": spmv}
-    elem = 4 * npe * n_el + 11 * ngf * n_el * T + 3 * N * T + 3 * npe * n_el * T
+    # element phase: connectivity + incidence slots (int32, npe each), the
+    # geometric factors (11 per factor set) or the stored K_e blocks, x once,
+    # the element results (3 npe per element) written once
+    elem = 8 * npe * n_el + 11 * ngf * n_el * T + 3 * N * T + 3 * npe * n_el * T
     if op == "ebe":
-        elem = 4 * npe * n_el + 8 * 9 * (npe * (npe + 1) // 2) * n_el + 3 * N * T + 3 * npe * n_el * T
-    gather = 4 * (N + 1) + 4 * npe * n_el + 3 * npe * n_el * T + 3 * N * T + 3 * N
+        elem = 8 * npe * n_el + 8 * 9 * (npe * (npe + 1) // 2) * n_el + 3 * N * T + 3 * npe * n_el * T
+    # gather: node pointers, the element results read once, y and the fixed mask
+    gather = 4 * (N + 1) + 3 * npe * n_el * T + 3 * N * T + 3 * N
     return {"element": elem, "gather": gather}
+
+
+def survey_ku(inf, prec):
+    """Algorithmic work of one matrix-free K.u by the per-unit figures of the
+    hot-path measurement plan: bytes = 16n + 40 n_el + 24N + 8 n_el (FP64;
+    coordinates, connectivity, E, x in, y out; no scratch) and 1.6 kflop per
+    affine Tet10 element."""
+    n, N, n_el = inf["n_dofs"], inf["n_nodes"], inf["n_el"]
+    scale = 1.0 if prec == 0 else 0.5
+    return {"bytes": int(scale * (16 * n + 24 * N + 8 * n_el) + 40 * n_el), "flops": 1600 * n_el}
 
 
 def flops_matfree_elem(inf):
@@ -61,6 +85,10 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=20)
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="time the K.u kernels and a 1 GiB device copy only (for counter runs)")
+    ap.add_argument("--method", default="mixed-learned", choices=sorted(METHODS),
+                    help="headline solve method")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
@@ -110,11 +138,22 @@ def main():
             kernels[name] = ent
             log(f"K.u {name}: {json.dumps({k: (round(v, 4) if isinstance(v, float) else v) for k, v in ent.items()})}")
 
+    if args.kernels_only:
+        copy = copy_calibration(ctx, inf["n_dofs"])
+        print(json.dumps({"kernels": kernels, "copy": copy, "n_el": inf["n_el"], "n_dofs": inf["n_dofs"]}), flush=True)
+        return
+
     # ---- solves ----------------------------------------------------------------
-    def solve(method, op, op_outer="matfree"):
+    method, precond, extra = METHODS[args.method]
+    if precond == 1:
+        from jfem.learned import DEFAULT_WEIGHTS
+        with open(DEFAULT_WEIGHTS, "rb") as fh:
+            ctx.set_learned_weights(fh.read())
+
+    def solve(method, op, op_outer="matfree", precond=0, **kw):
         t = time.perf_counter()
-        u, _, st = ctx.solve(method=method, op=op, op_outer=op_outer, rtol=1e-8, inner_rtol=1e-4,
-                             check_every=32, raise_on_noconv=False)
+        u, _, st = ctx.solve(method=method, op=op, op_outer=op_outer, precond=precond, rtol=1e-8,
+                             check_every=32, raise_on_noconv=False, **{"inner_rtol": 1e-4, **kw})
         st["wall_s"] = time.perf_counter() - t
         return u, st
 
@@ -124,28 +163,36 @@ def main():
     log(f"FP64 BJ-PCG csr: {st64c}")
 
     for _ in range(args.warmup):
-        solve(1, "matfree")
+        solve(method, "matfree", precond=precond, **extra)
     times, walls, stats = [], [], None
-    u_mixed = None
+    u_head = None
     for _ in range(args.steps):
-        u_mixed, st = solve(1, "matfree")
-        times.append(st["solve_ms"] / 1e3)
+        u_head, st = solve(method, "matfree", precond=precond, **extra)
+        times.append((st["solve_ms"] + st["setup_ms"]) / 1e3)
         walls.append(st["wall_s"])
         stats = st
-        log(f"mixed step: {st}")
+        log(f"{args.method} step: {st}")
     value = statistics.median(times)
-    err_vs_fp64 = float(np.linalg.norm(u_mixed - u64) / np.linalg.norm(u64))
+    err_vs_fp64 = float(np.linalg.norm(u_head - u64) / np.linalg.norm(u64))
 
     # ---- roofline of the dominant kernel of the headline solve ---------------
-    mf32 = kernels["matfree_fp32"]
-    if mf32["element_ms"] >= mf32["gather_ms"]:
-        dom, dom_ms, dom_bytes = "k_mf_elem<10,float,true>", mf32["element_ms"], mf32["bytes"]["element"]
+    hk = kernels["matfree_fp32" if method == 1 else "matfree_fp64"]
+    tname = "float" if method == 1 else "double"
+    if hk["element_ms"] >= hk["gather_ms"]:
+        dom, dom_ms, dom_bytes = f"k_mf_elem<10,{tname},AFF>", hk["element_ms"], hk["bytes"]["element"]
     else:
-        dom, dom_ms, dom_bytes = "k_gather<float>", mf32["gather_ms"], mf32["bytes"]["gather"]
+        dom, dom_ms, dom_bytes = f"k_gather<{tname}>", hk["gather_ms"], hk["bytes"]["gather"]
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    sv = survey_ku(inf, 1 if method == 1 else 0)
+    ku_ms = hk["ms"]
+    peak_tf = FP32_PEAK_TFLOPS if method == 1 else FP64_PEAK_TFLOPS
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
-                "bytes_per_launch": dom_bytes, "avg_launch_ms": dom_ms}
+                "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom, inf), "kernel": dom,
+                "bytes_per_launch": dom_bytes, "avg_launch_ms": dom_ms,
+                "ku_ideal": {"bytes": sv["bytes"], "flops": sv["flops"], "ku_ms": ku_ms,
+                             "GBps": sv["bytes"] / (ku_ms * 1e-3) / 1e9,
+                             "TFLOPs": sv["flops"] / (ku_ms * 1e-3) / 1e12,
+                             "flop_frac": sv["flops"] / (ku_ms * 1e-3) / 1e12 / peak_tf}}
 
     # ---- CPU baseline (oracle, bounded sample) --------------------------------
     cpu = None
@@ -155,23 +202,49 @@ def main():
     out = {
         "metric": METRIC, "value": value, "unit": "s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": value * 1e3, "higher_is_better": False, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32-inner/f64-outer", "data": "synthetic (seeded jittered box mesh)",
+        "vs_baseline": None, "dtype": "f32-inner/f64-outer" if method == 1 else "f64",
+        "data": "synthetic (seeded jittered box mesh)",
         "config": {"workload": f"Tet10 cantilever {nx}x{ny}x{nz} hexes x6, {inf['n_dofs']} DOFs, "
                                f"log-normal E sigma 0.5, rtol 1e-8", "n_dofs": inf["n_dofs"],
-                   "nnz": inf["nnz"], "method": "FP32 matrix-free block-Jacobi PCG (inner rtol 1e-4) in FP64 "
-                                                 "iterative refinement", "parallelism": "single GPU"},
+                   "nnz": inf["nnz"], "method": args.method, "method_options": extra,
+                   "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "fp64_pcg_s": st64["solve_ms"] / 1e3, "fp64_pcg_iters": st64["iterations"],
         "fp64_pcg_csr_s": st64c["solve_ms"] / 1e3, "fp64_pcg_csr_iters": st64c["iterations"],
         "speedup_vs_fp64_matfree": st64["solve_ms"] / 1e3 / value,
         "speedup_vs_fp64_csr": st64c["solve_ms"] / 1e3 / value,
-        "mixed_iters": stats["iterations"], "mixed_outer": stats["outer"], "relres": stats["relres"],
+        "iters": stats["iterations"], "outer": stats["outer"], "relres": stats["relres"],
+        "learned_fallback": stats["fallback"], "setup_ms": stats["setup_ms"],
         "err_vs_fp64_l2": err_vs_fp64, "host_wall_s": statistics.median(walls),
         "dofs_per_s": inf["n_dofs"] / value, "upload_s": t_upload, "assembly_wall_s": t_asm,
         "kernels": kernels,
     }
     print(json.dumps(out), flush=True)
+
+
+def copy_calibration(ctx, n_dofs):
+    """Device-to-device copies of known size for scaling the FETCH_SIZE and
+    WRITE_SIZE counters (rocprofv3 --pmc) to bytes: a few PCG starts, each of
+    which copies FP64 vectors of n_dofs entries with hipMemcpyAsync (the
+    runtime's copyBuffer kernel)."""
+    for _ in range(3):
+        ctx.solve(method=0, op="matfree", rtol=1e-30, maxit=2, raise_on_noconv=False)
+    return {"kernel": "__amd_rocclr_copyBuffer", "bytes_read": 8 * n_dofs, "bytes_written": 8 * n_dofs}
+
+
+def measured_traffic(kernel, inf):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE, copy-kernel scaled), when it covers this mesh."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            t = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    if not k or t.get("n_el") != inf["n_el"]:
+        return None
+    return k["bytes_per_launch"]
 
 
 def cpu_baseline(mesh, iters_needed, sample_iters):

@@ -88,6 +88,10 @@ typedef struct {
                            * stats.fallback = 1); 0 = no probe */
     int32_t inner_maxit;  /* cap on the iterations of one inner solve (mixed); 0 = none */
     double first_rtol;    /* mixed only: FP64 PCG to this relative residual first; 0 = none */
+    int32_t coarse_nodes; /* > 0: add the two-level correction Q A_c^-1 Q' (rigid-body
+                           * modes on node aggregates of about this many nodes,
+                           * A_c = Q' K Q rebuilt per solve) to the block
+                           * preconditioner; 0 = single level */
 } jfm_solve_opts;
 
 typedef struct {
@@ -197,6 +201,7 @@ typedef struct {
     double inner_rtol;    /* JFM_SOLVE_MIXED_IR: FP32 inner tolerance (0: 1e-2) */
     int32_t inner_maxit;  /* JFM_SOLVE_MIXED_IR: FP32 iterations per inner solve (0: no cap) */
     int32_t check_every;  /* PCG iterations between host checks (0: 32) */
+    int32_t coarse_nodes; /* as in jfm_solve_opts; A_c = Q' K_T Q per Newton iteration */
 } jfm_newton_opts;
 
 typedef struct {

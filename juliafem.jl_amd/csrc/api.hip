@@ -116,6 +116,7 @@ extern "C" void jfm_destroy(jfm_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     drop_graphs(ctx);
+    jfem_coarse_destroy_handle(ctx);
     for (void *p : ctx->allocs) (void)hipFree(p);
     ctx->allocs.clear();
     if (ctx->S_host) (void)hipHostFree(ctx->S_host);
@@ -168,6 +169,8 @@ static void release_mesh(jfm_ctx *ctx) {
     free_ptr(ctx, ctx->ustate64); free_ptr(ctx, ctx->ustate32); free_ptr(ctx, ctx->Dtan);
     free_ptr(ctx, ctx->nl_u); free_ptr(ctx, ctx->nl_r); free_ptr(ctx, ctx->nl_du); free_ptr(ctx, ctx->nl_zero);
     ctx->has_state = ctx->minv_tangent = false;
+    jfem_coarse_release(ctx);
+    ctx->h_X.clear();
     ctx->has_user = false;
     ctx->has_mesh = ctx->has_bcs = ctx->assembled = ctx->has_load = false;
     ctx->learned_ready = false;
@@ -224,6 +227,7 @@ extern "C" int jfm_upload_mesh(jfm_ctx *ctx, int64_t n_nodes, const double *X, i
     ctx->npe = npe;
     ctx->ngp = npe == 10 ? 4 : (npe == 4 ? 1 : 8);
     ctx->npair = npe * (npe + 1) / 2;
+    ctx->h_X.assign(X, X + 3 * n_nodes);
 
     // connectivity -> 0-based int32, range checked
     ctx->h_conn.resize((size_t)n_el * npe);
@@ -473,6 +477,7 @@ extern "C" int jfm_upload_bcs(jfm_ctx *ctx, const uint8_t *fixed, const double *
     ctx->learned_ready = false;
     ctx->has_user = false;
     for (Workspace *w : {&ctx->ws64, &ctx->ws32}) w->ok = false;  // preconditioners depend on the fixed set
+    jfem_coarse_release(ctx);
     return JFM_OK;
 }
 
@@ -715,7 +720,7 @@ static int write_scalars(jfm_ctx *ctx) {
 // One PCG iteration of parity `slot` (three or four kernels, see
 // linalg_kernels.hip); every kernel returns at once when done[slot] is set.
 template <typename T>
-static hipError_t pcg_iteration(jfm_ctx *ctx, int op, int slot, const T *Minv) {
+static hipError_t pcg_iteration(jfm_ctx *ctx, int op, int slot, const T *Minv, bool coarse) {
     Workspace &w = ws_of<T>(ctx);
     T *x = (T *)w.x, *r = (T *)w.r, *z = (T *)w.z, *p = (T *)w.p, *q = (T *)w.q;
     int np_pq = 0, np_rz = 0;
@@ -723,16 +728,21 @@ static hipError_t pcg_iteration(jfm_ctx *ctx, int op, int slot, const T *Minv) {
     if (e != hipSuccess) return e;
     e = launch_pcg_update<T>(ctx, slot, np_pq, x, r, z, p, q, Minv, &np_rz);
     if (e != hipSuccess) return e;
+    if (coarse) {  // z += Q A_c^-1 Q' r; the (r.z, r.r) partials are rewritten
+        e = launch_coarse_correct<T>(ctx, r, z, (T *)nullptr, ctx->partials + part_rz(slot), &ctx->S->done[slot], &np_rz);
+        if (e != hipSuccess) return e;
+    }
     return launch_pcg_direction<T>(ctx, slot, np_rz, p, z);
 }
 
 // A batch of `batch` (even) iterations captured once as a HIP graph and
 // replayed; kernels past convergence return immediately.
 template <typename T>
-static int pcg_batch(jfm_ctx *ctx, int op, const T *Minv, int batch) {
+static int pcg_batch(jfm_ctx *ctx, int op, const T *Minv, int batch, bool coarse) {
     const int prec = sizeof(T) == 8 ? JFM_FP64 : JFM_FP32;
     for (auto &g : ctx->graphs)
-        if (g.op == op && g.prec == prec && g.minv == (const void *)Minv && g.batch == batch) {
+        if (g.op == op && g.prec == prec && g.minv == (const void *)Minv && g.batch == batch &&
+            g.coarse == (coarse ? ctx->coarse_gen : 0)) {
             JFM_HIP(ctx, hipGraphLaunch(g.exec, ctx->stream));
             return JFM_OK;
         }
@@ -741,7 +751,7 @@ static int pcg_batch(jfm_ctx *ctx, int op, const T *Minv, int batch) {
         return !(v && v[0] == '0');
     }();
     if (!use_graphs) {
-        for (int it = 0; it < batch; ++it) CHECK_LAUNCH(ctx, pcg_iteration<T>(ctx, op, it & 1, Minv));
+        for (int it = 0; it < batch; ++it) CHECK_LAUNCH(ctx, pcg_iteration<T>(ctx, op, it & 1, Minv, coarse));
         return JFM_OK;
     }
     hipGraph_t graph = nullptr;
@@ -749,7 +759,7 @@ static int pcg_batch(jfm_ctx *ctx, int op, const T *Minv, int batch) {
     bool ok = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
     hipError_t launch_err = hipSuccess;
     if (ok) {
-        for (int it = 0; it < batch && launch_err == hipSuccess; ++it) launch_err = pcg_iteration<T>(ctx, op, it & 1, Minv);
+        for (int it = 0; it < batch && launch_err == hipSuccess; ++it) launch_err = pcg_iteration<T>(ctx, op, it & 1, Minv, coarse);
         ok = hipStreamEndCapture(ctx->stream, &graph) == hipSuccess && launch_err == hipSuccess;
     }
     if (ok) ok = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
@@ -757,7 +767,7 @@ static int pcg_batch(jfm_ctx *ctx, int op, const T *Minv, int batch) {
     (void)hipGetLastError();
     if (!ok) {  // capture unavailable: launch the same kernels directly
         if (exec) (void)hipGraphExecDestroy(exec);
-        for (int it = 0; it < batch; ++it) CHECK_LAUNCH(ctx, pcg_iteration<T>(ctx, op, it & 1, Minv));
+        for (int it = 0; it < batch; ++it) CHECK_LAUNCH(ctx, pcg_iteration<T>(ctx, op, it & 1, Minv, coarse));
         return JFM_OK;
     }
     jfm_ctx::Graph g;
@@ -765,6 +775,7 @@ static int pcg_batch(jfm_ctx *ctx, int op, const T *Minv, int batch) {
     g.prec = prec;
     g.minv = Minv;
     g.batch = batch;
+    g.coarse = coarse ? ctx->coarse_gen : 0;
     g.exec = exec;
     ctx->graphs.push_back(g);
     JFM_HIP(ctx, hipGraphLaunch(exec, ctx->stream));
@@ -782,7 +793,7 @@ static int finalize_misc(jfm_ctx *ctx, int npart, int nv, int fin) {
 // reads the scalars once per batch of `check_every` iterations.
 template <typename T>
 static int pcg_run(jfm_ctx *ctx, int op, const T *Minv, double rtol, int maxit, int check_every, int *iters,
-                   int *status, bool warm = false) {
+                   int *status, bool warm = false, bool coarse = false) {
     Workspace &w = ws_of<T>(ctx);
     T *x = (T *)w.x, *r = (T *)w.r, *z = (T *)w.z, *p = (T *)w.p, *q = (T *)w.q, *b = (T *)w.b;
     const int64_t n = ctx->ndof;
@@ -803,11 +814,12 @@ static int pcg_run(jfm_ctx *ctx, int op, const T *Minv, double rtol, int maxit, 
     CHECK_LAUNCH(ctx, launch_dot<T>(ctx, b, b, n, &np));
     if (int rc = finalize_misc(ctx, np, 1, FIN_BB)) return rc;
     CHECK_LAUNCH(ctx, launch_pcg_init<T>(ctx, r, z, p, Minv, &np));
+    if (coarse) CHECK_LAUNCH(ctx, launch_coarse_correct<T>(ctx, r, z, p, ctx->partials + kPartMisc, nullptr, &np));
     if (int rc = finalize_misc(ctx, np, 2, FIN_INIT)) return rc;
     if (int rc = read_scalars(ctx)) return rc;
     const int batch = std::max(2, (check_every + 1) & ~1);
     while (ctx->S_host->done[0] == DONE_RUNNING) {
-        if (int rc = pcg_batch<T>(ctx, op, Minv, batch)) return rc;
+        if (int rc = pcg_batch<T>(ctx, op, Minv, batch, coarse)) return rc;
         if (int rc = read_scalars(ctx)) return rc;
     }
     *iters = ctx->S_host->iter;
@@ -832,15 +844,18 @@ static int true_residual(jfm_ctx *ctx, int op, const double *b, const double *u,
 // iterations the iterate with the larger energy b.x_k (CG decreases
 // 1/2 x'Kx - b.x = -1/2 b.x_k monotonically) has the smaller energy-norm
 // error; the learned blocks are kept only if they win.
-static int learned_select(jfm_ctx *ctx, int op, int probe, int check, bool *use, int *iters, double *setup_ms) {
-    if (int rc = jfem_learned_prepare(ctx, setup_ms)) return rc;
+static int learned_select(jfm_ctx *ctx, int op, int probe, int check, bool *use, int *iters, double *setup_ms,
+                          bool coarse) {
+    double lms = 0.0;
+    if (int rc = jfem_learned_prepare(ctx, &lms)) return rc;
+    if (setup_ms) *setup_ms += lms;
     *use = true;
     if (probe <= 0) return JFM_OK;
     double phi[2] = {0.0, 0.0};
     const double *cand[2] = {(const double *)ctx->ws64.Minv, ctx->Mlearn64};
     for (int c = 0; c < 2; ++c) {
         int it = 0, status = 0;
-        if (int rc = pcg_run<double>(ctx, op, cand[c], 1e-30, probe, check, &it, &status)) return rc;
+        if (int rc = pcg_run<double>(ctx, op, cand[c], 1e-30, probe, check, &it, &status, false, coarse)) return rc;
         if (int rc = jfem_dot(ctx, (const double *)ctx->ws64.b, (const double *)ctx->ws64.x, ctx->ndof, &phi[c]))
             return rc;
         *iters += it;
@@ -885,6 +900,9 @@ extern "C" int jfm_solve(jfm_ctx *ctx, const jfm_solve_opts *opts, const double 
     if (int rc = true_residual(ctx, op_outer, ctx->vx64, ctx->g, b, &bnorm)) return rc;
     ctx->hist_it.clear();
     ctx->hist_res.clear();
+    const bool coarse = opts->coarse_nodes > 0 && bnorm > 0.0;
+    if (coarse)
+        if (int rc = jfem_coarse_setup(ctx, op_outer, opts->coarse_nodes, &st.setup_ms)) return rc;
     const double *Minv64 = (const double *)ctx->ws64.Minv;
     const float *Minv32 = (const float *)ctx->ws32.Minv;
     if (opts->precond == JFM_PREC_USER) {
@@ -894,7 +912,8 @@ extern "C" int jfm_solve(jfm_ctx *ctx, const jfm_solve_opts *opts, const double 
     if (opts->precond == JFM_PREC_LEARNED && bnorm > 0.0) {
         bool use = true;
         int its = 0;
-        if (int rc = learned_select(ctx, op_outer, opts->learned_probe, check, &use, &its, &st.setup_ms)) return rc;
+        if (int rc = learned_select(ctx, op_outer, opts->learned_probe, check, &use, &its, &st.setup_ms, coarse))
+            return rc;
         st.iterations += its;
         if (use) {
             Minv64 = ctx->Mlearn64;
@@ -908,7 +927,8 @@ extern "C" int jfm_solve(jfm_ctx *ctx, const jfm_solve_opts *opts, const double 
         JFM_HIP(ctx, hipMemcpyAsync(ctx->u64, ctx->g, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     } else if (opts->method == JFM_SOLVE_PCG) {
         int it = 0, status = 0, total = 0, restarts = 0;
-        if (int rc = pcg_run<double>(ctx, op, Minv64, opts->rtol, opts->maxit, check, &it, &status)) return rc;
+        if (int rc = pcg_run<double>(ctx, op, Minv64, opts->rtol, opts->maxit, check, &it, &status, false, coarse))
+            return rc;
         total = it;
         // The recursive residual drifts from the true one near the FP64 floor:
         // restart from x with the true residual until it meets rtol (residual
@@ -919,7 +939,8 @@ extern "C" int jfm_solve(jfm_ctx *ctx, const jfm_solve_opts *opts, const double 
             ctx->hist_it.push_back(total);
             ctx->hist_res.push_back(rn / bnorm);
             if (rn <= opts->rtol * bnorm) break;
-            if (int rc = pcg_run<double>(ctx, op, Minv64, opts->rtol, opts->maxit - total, check, &it, &status, true))
+            if (int rc = pcg_run<double>(ctx, op, Minv64, opts->rtol, opts->maxit - total, check, &it, &status, true,
+                                         coarse))
                 return rc;
             total += it;
             ++restarts;
@@ -938,7 +959,8 @@ extern "C" int jfm_solve(jfm_ctx *ctx, const jfm_solve_opts *opts, const double 
         if (opts->first_rtol > 0.0) {
             // FP64 phase for the smooth low modes that FP32 cannot resolve
             int it = 0, status = 0;
-            if (int rc = pcg_run<double>(ctx, op_outer, Minv64, opts->first_rtol, opts->maxit, check, &it, &status))
+            if (int rc = pcg_run<double>(ctx, op_outer, Minv64, opts->first_rtol, opts->maxit, check, &it, &status, false,
+                                         coarse))
                 return rc;
             total += it;
             CHECK_LAUNCH(ctx, launch_add(ctx, ctx->g, (const double *)ctx->ws64.x, ctx->u64, n));
@@ -953,7 +975,7 @@ extern "C" int jfm_solve(jfm_ctx *ctx, const jfm_solve_opts *opts, const double 
             int it = 0, status = 0;
             int cap = opts->maxit - total;
             if (opts->inner_maxit > 0) cap = std::min(cap, opts->inner_maxit);
-            if (int rc = pcg_run<float>(ctx, op, Minv32, inner, cap, check, &it, &status)) return rc;
+            if (int rc = pcg_run<float>(ctx, op, Minv32, inner, cap, check, &it, &status, false, coarse)) return rc;
             total += it;
             ++outer;
             CHECK_LAUNCH(ctx, launch_axpy_scaled(ctx, ctx->u64, (const float *)ctx->ws32.x, rn, n));
@@ -1087,14 +1109,16 @@ extern "C" int jfm_nl_residual(jfm_ctx *ctx, const double *u, double load_factor
 
 // K_T du = r (r in nl_r, free DOFs) to |r - K_T du| <= eta |r|; du -> nl_du.
 static int tangent_solve(jfm_ctx *ctx, const jfm_newton_opts *o, const double *M64, const float *M32, double eta,
-                         double rnorm, int check, int *iters) {
+                         double rnorm, int check, bool coarse, int *iters) {
     const int64_t n = ctx->ndof;
     hipStream_t s = ctx->stream;
     *iters = 0;
     if (o->method == JFM_SOLVE_PCG) {
         JFM_HIP(ctx, hipMemcpyAsync(ctx->ws64.b, ctx->nl_r, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
         int it = 0, status = 0;
-        if (int rc = pcg_run<double>(ctx, JFM_OP_TANGENT, M64, eta, o->maxit_linear, check, &it, &status)) return rc;
+        if (int rc = pcg_run<double>(ctx, JFM_OP_TANGENT, M64, eta, o->maxit_linear, check, &it, &status, false,
+                                     coarse))
+            return rc;
         *iters = it;
         JFM_HIP(ctx, hipMemcpyAsync(ctx->nl_du, ctx->ws64.x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
         return JFM_OK;
@@ -1114,7 +1138,7 @@ static int tangent_solve(jfm_ctx *ctx, const jfm_newton_opts *o, const double *M
         if (o->inner_maxit > 0) cap = std::min(cap, o->inner_maxit);
         // no need to solve the correction beyond what the outer tolerance asks
         const double tol = std::max(inner, 0.5 * eta * rnorm / rl);
-        if (int rc = pcg_run<float>(ctx, JFM_OP_TANGENT, M32, tol, cap, check, &it, &status)) return rc;
+        if (int rc = pcg_run<float>(ctx, JFM_OP_TANGENT, M32, tol, cap, check, &it, &status, false, coarse)) return rc;
         *iters += it;
         CHECK_LAUNCH(ctx, launch_axpy_scaled(ctx, ctx->nl_du, (const float *)ctx->ws32.x, rl, n));
         if (status == DONE_BREAKDOWN) break;
@@ -1192,6 +1216,9 @@ extern "C" int jfm_solve_nonlinear(jfm_ctx *ctx, const jfm_newton_opts *opts, do
                                                      ctx->Dtan));
             ctx->ws64.ok = ctx->ws32.ok = true;
             ctx->minv_tangent = true;
+            const bool coarse = opts->coarse_nodes > 0;
+            if (coarse)  // coarse level of the tangent at this state
+                if (int rc = jfem_coarse_setup(ctx, JFM_OP_TANGENT, opts->coarse_nodes, nullptr)) return rc;
             const double *M64 = (const double *)ctx->ws64.Minv;
             const float *M32 = (const float *)ctx->ws32.Minv;
             if (opts->precond == JFM_PREC_LEARNED) {
@@ -1199,7 +1226,7 @@ extern "C" int jfm_solve_nonlinear(jfm_ctx *ctx, const jfm_newton_opts *opts, do
                 bool use = true;
                 int its = 0;
                 JFM_HIP(ctx, hipMemcpyAsync(ctx->ws64.b, ctx->nl_r, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-                if (int rc = learned_select(ctx, JFM_OP_TANGENT, opts->learned_probe, check, &use, &its, nullptr))
+                if (int rc = learned_select(ctx, JFM_OP_TANGENT, opts->learned_probe, check, &use, &its, nullptr, coarse))
                     return rc;
                 st.linear_iterations += its;
                 if (use) {
@@ -1227,7 +1254,7 @@ extern "C" int jfm_solve_nonlinear(jfm_ctx *ctx, const jfm_newton_opts *opts, do
             eta_prev = eta;
             rn_prev = rn;
             int lits = 0;
-            if (int rc = tangent_solve(ctx, opts, M64, M32, eta, rn, check, &lits)) return rc;
+            if (int rc = tangent_solve(ctx, opts, M64, M32, eta, rn, check, coarse, &lits)) return rc;
             st.linear_iterations += lits;
             ++st.newton_iterations;
             // u += du; halve the step while some Gauss point leaves det F > 0

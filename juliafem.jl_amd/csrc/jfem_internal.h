@@ -186,7 +186,7 @@ struct jfm_ctx {
     double *partials = nullptr;          // kPartTotal doubles
     unsigned *counter = nullptr;         // small integer scratch (element checks)
     // captured PCG iteration batches, keyed by (op, precision, preconditioner)
-    struct Graph { int op = -1, prec = -1, batch = 0; const void *minv = nullptr; hipGraphExec_t exec = nullptr; };
+    struct Graph { int op = -1, prec = -1, batch = 0, coarse = 0; const void *minv = nullptr; hipGraphExec_t exec = nullptr; };
     std::vector<Graph> graphs;
     std::vector<int32_t> hist_it;          // convergence history of the last solve
     std::vector<double> hist_res;
@@ -207,6 +207,25 @@ struct jfm_ctx {
     bool has_state = false;
     bool minv_tangent = false;     // ws*.Minv hold the tangent block Jacobi
     double *nl_u = nullptr, *nl_r = nullptr, *nl_du = nullptr, *nl_zero = nullptr;
+    // two-level coarse space (coarse.hip)
+    std::vector<double> h_X;       // host copy of the coordinates (aggregation)
+    int coarse_built_nodes = 0;    // coarse_nodes the aggregation was built for
+    int coarse_gen = 0;            // bumped per aggregation (keys captured graphs)
+    bool coarse_ready = false;     // Ahat matches the operator of the current solve
+    int64_t n_agg = 0, nc = 0;
+    int ncolor = 0;
+    int32_t *node_agg = nullptr, *agg_ptr = nullptr, *agg_nodes = nullptr, *agg_color = nullptr;
+    int32_t *partner = nullptr;    // ncolor x n_agg: the coupled aggregate of each colour
+    int32_t *dropped = nullptr;    // nc: mode dropped (dependent on its aggregate)
+    double *cd64 = nullptr;        // N x 3 offsets from the aggregate centroid
+    float *cd32 = nullptr;
+    double *Rinv = nullptr;        // n_agg x 36
+    double *Ac = nullptr;          // nc x nc (factorised in place)
+    double *Ahat64 = nullptr;      // nc x nc: R^-1 A_c^-1 R^-T
+    float *Ahat32 = nullptr;
+    double *wc64 = nullptr, *zc64 = nullptr;
+    float *wc32 = nullptr, *zc32 = nullptr;
+    void *blas = nullptr;          // rocblas_handle
     double *Muser64 = nullptr;     // N x 9 user blocks (JFM_PREC_USER)
     float *Muser32 = nullptr;
     bool has_user = false;
@@ -253,6 +272,12 @@ hipError_t launch_two_phase(jfm_ctx *ctx, int op, const T *x, T *y, const T *dot
 template <typename T>
 hipError_t launch_nh_elem(jfm_ctx *ctx, bool tangent, const T *gf, const T *u, const T *v, T *ye, const int *done);
 hipError_t launch_nh_block_jacobi(jfm_ctx *ctx, const double *u, double *M64, float *M32, double *D);
+// coarse.hip
+int jfem_coarse_setup(jfm_ctx *ctx, int op, int coarse_nodes, double *ms);
+void jfem_coarse_release(jfm_ctx *ctx);
+void jfem_coarse_destroy_handle(jfm_ctx *ctx);
+template <typename T>
+hipError_t launch_coarse_correct(jfm_ctx *ctx, const T *r, T *z, T *p, double *part, const int *done, int *npart);
 // linalg_kernels.hip
 template <typename T>
 hipError_t launch_bsr_spmv(jfm_ctx *ctx, const T *vals, const T *x, T *y, const T *dotv, double *part,

@@ -40,7 +40,7 @@ class SolveOpts(C.Structure):
     _fields_ = [("method", C.c_int32), ("op", C.c_int32), ("op_outer", C.c_int32), ("precond", C.c_int32),
                 ("rtol", C.c_double), ("inner_rtol", C.c_double), ("maxit", C.c_int32),
                 ("max_outer", C.c_int32), ("check_every", C.c_int32), ("learned_probe", C.c_int32),
-                ("inner_maxit", C.c_int32), ("first_rtol", C.c_double)]
+                ("inner_maxit", C.c_int32), ("first_rtol", C.c_double), ("coarse_nodes", C.c_int32)]
 
 
 class SolveStats(C.Structure):
@@ -215,10 +215,10 @@ class Context:
 
     def solve(self, f=None, method=SOLVE_PCG, op="matfree", op_outer="matfree", precond=PREC_BLOCK_JACOBI,
               rtol=1e-8, inner_rtol=1e-4, maxit=100000, max_outer=20, check_every=16, learned_probe=0,
-              inner_maxit=0, first_rtol=0.0, want_la=False, raise_on_noconv=True):
+              inner_maxit=0, first_rtol=0.0, coarse_nodes=0, want_la=False, raise_on_noconv=True):
         n = self.info()["n_dofs"]
         opts = SolveOpts(method, OPS[op], OPS[op_outer], precond, rtol, inner_rtol, maxit, max_outer,
-                         check_every, learned_probe, inner_maxit, first_rtol)
+                         check_every, learned_probe, inner_maxit, first_rtol, coarse_nodes)
         st = SolveStats()
         u = np.empty(n)
         la = np.empty(n) if want_la else None
@@ -239,11 +239,11 @@ class Context:
 
     def solve_nonlinear(self, load_steps=5, max_newton=20, rtol=1e-8, method=SOLVE_PCG, precond=PREC_BLOCK_JACOBI,
                         eta_max=0.1, maxit_linear=20000, learned_probe=0, inner_rtol=1e-2, inner_maxit=0,
-                        check_every=32, raise_on_noconv=True):
+                        check_every=32, coarse_nodes=0, raise_on_noconv=True):
         """Load-stepped Newton for the compressible neo-Hookean model; (u, stats)."""
         n = self.info()["n_dofs"]
         opts = NewtonOpts(load_steps, max_newton, rtol, method, precond, eta_max, maxit_linear, learned_probe,
-                          inner_rtol, inner_maxit, check_every)
+                          inner_rtol, inner_maxit, check_every, coarse_nodes)
         st = NewtonStats()
         u = np.empty(n)
         rc = self._lib.jfm_solve_nonlinear(self._h, C.byref(opts), _p(u), C.byref(st))

@@ -17,11 +17,26 @@ using SparseArrays
 
 const LIB = joinpath(@__DIR__, "..", "libjfem_hip.so")
 
+# field order and types mirror jfm_solve_opts / jfm_newton_opts (include/jfem.h);
+# isbits structs get the C layout, padding included
 struct SolveOpts
     method::Int32; op::Int32; op_outer::Int32; precond::Int32
     rtol::Float64; inner_rtol::Float64
     maxit::Int32; max_outer::Int32; check_every::Int32; learned_probe::Int32
+    inner_maxit::Int32; first_rtol::Float64; coarse_nodes::Int32
 end
+
+struct NewtonOpts
+    load_steps::Int32; max_newton::Int32; rtol::Float64; method::Int32; precond::Int32
+    eta_max::Float64; maxit_linear::Int32; learned_probe::Int32; inner_rtol::Float64
+    inner_maxit::Int32; check_every::Int32; coarse_nodes::Int32
+end
+
+mutable struct NewtonStats
+    newton_iterations::Int32; linear_iterations::Int32; converged::Int32; fallbacks::Int32
+    relres::Float64; solve_ms::Float64
+end
+NewtonStats() = NewtonStats(0, 0, 0, 0, 0.0, 0.0)
 
 mutable struct SolveStats
     iterations::Int32; outer::Int32; converged::Int32; fallback::Int32
@@ -118,7 +133,7 @@ function JuliaFEM.solve!(solver::JuliaFEM.Solver, K, C1, C2, D, f, g, u, la, ::T
     check(ccall((:jfm_upload_bcs, LIB), Cint,
           (Ptr{Cvoid}, Ptr{UInt8}, Ptr{Float64}, Int64, Int32, Ptr{Int64}, Ptr{Float64}, Int32),
           ctx(), fixed, gv, 0, 6, C_NULL, C_NULL, 1))
-    opts = Ref(SolveOpts(method, 2, 2, 0, rtol, 1e-4, 100000, 20, 32, 0))
+    opts = Ref(SolveOpts(method, 2, 2, 0, rtol, 1e-4, 100000, 20, 32, 0, 0, 0.0, 512))
     st = Ref(SolveStats())
     rc = ccall((:jfm_solve, LIB), Cint,
                (Ptr{Cvoid}, Ref{SolveOpts}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ref{SolveStats}),
@@ -126,6 +141,26 @@ function JuliaFEM.solve!(solver::JuliaFEM.Solver, K, C1, C2, D, f, g, u, la, ::T
     rc == 0 || return false
     @info("GPU solve: $(st[].iterations) iterations, relative residual $(st[].relres)")
     return true
+end
+
+# solve_backend!(data, physics; newton_tol, max_newton, ...) for the
+# neo-Hookean model (ext/JuliaFEMCUDAExt.jl:685-854, 886-916): the mesh and
+# boundary conditions are uploaded by the calls above; returns the reference's
+# (u, newton_iters, cg_iters, residual, history) tuple, history holding
+# (cumulative CG iterations, relative residual, 0.0) per Newton iterate.
+function solve_nonlinear_hip(n::Integer; load_steps=5, newton_tol=1e-8, max_newton=20, method=1, precond=0)
+    opts = Ref(NewtonOpts(load_steps, max_newton, newton_tol, method, precond, 0.1, 20000, 0, 1e-2, 0, 32, 4096))
+    st = Ref(NewtonStats())
+    u = Vector{Float64}(undef, n)
+    rc = ccall((:jfm_solve_nonlinear, LIB), Cint,
+               (Ptr{Cvoid}, Ref{NewtonOpts}, Ptr{Float64}, Ref{NewtonStats}), ctx(), opts, u, st)
+    rc == 0 || @warn("Newton did not converge: " * unsafe_string(ccall((:jfm_last_error, LIB), Cstring, (Ptr{Cvoid},), ctx())))
+    m = 1024
+    its = Vector{Int32}(undef, m)
+    res = Vector{Float64}(undef, m)
+    k = ccall((:jfm_solve_history, LIB), Cint, (Ptr{Cvoid}, Ptr{Int32}, Ptr{Float64}, Int32), ctx(), its, res, m)
+    history = [(Int(its[i]), res[i], 0.0) for i in 1:max(k, 0)]
+    return (u, Int(st[].newton_iterations), Int(st[].linear_iterations), st[].relres, history)
 end
 
 end # module

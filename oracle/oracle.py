@@ -385,3 +385,132 @@ def nh_newton(mesh, load_steps: int = 5, rtol: float = 1e-10, max_newton: int = 
             u += du
         counts.append(it)
     return u, counts
+
+
+# ------------------------------------------------------------- two-level PCG --
+def coarse_aggregates(mesh, coarse_nodes: int):
+    """Node aggregates as the library forms them: a coordinate grid with about
+    `coarse_nodes` nodes per cell, cells under max(8, coarse_nodes / 8) nodes
+    merged into their largest element-connected neighbour."""
+    X = np.asarray(mesh.X, np.float64)
+    N = X.shape[0]
+    lo, hi = X.min(axis=0), X.max(axis=0)
+    maxe = (hi - lo).max()
+    ext = np.maximum(hi - lo, 1e-12 * max(maxe, 1e-300))
+    cell = np.cbrt(ext.prod() / max(1.0, N / max(coarse_nodes, 1)))
+    nb = np.maximum(1, np.floor(ext / cell + 0.5).astype(np.int64))
+    b = np.floor((X - lo) / ext * nb).astype(np.int64)
+    b = np.minimum(np.maximum(b, 0), nb - 1)
+    key = (b[:, 0] * nb[1] + b[:, 1]) * nb[2] + b[:, 2]
+    _, agg = np.unique(key, return_inverse=True)
+    n_agg = int(agg.max()) + 1
+    conn = np.asarray(mesh.conn, np.int64)
+    # aggregate adjacency through elements
+    ea = agg[conn]
+    pairs = set()
+    for row in ea:
+        u = np.unique(row)
+        for i in u:
+            for j in u:
+                if i != j:
+                    pairs.add((int(i), int(j)))
+    adj = [[] for _ in range(n_agg)]
+    for i, j in sorted(pairs):
+        adj[i].append(j)
+    cnt = np.bincount(agg, minlength=n_agg)
+    small = max(8, coarse_nodes // 8)
+    into = np.arange(n_agg)
+    for g in range(n_agg):
+        if cnt[g] >= small:
+            continue
+        best = -1
+        for h in adj[g]:
+            if cnt[h] >= small and (best < 0 or cnt[h] > cnt[best]):
+                best = h
+        if best >= 0:
+            into[g] = best
+    keep = np.where(into == np.arange(n_agg))[0]
+    remap = -np.ones(n_agg, np.int64)
+    remap[keep] = np.arange(keep.size)
+    return remap[into[agg]]
+
+
+def twolevel_operator(mesh, csr, coarse_nodes: int):
+    """(P, A_c^-1, block Jacobi) of the additive two-level preconditioner
+    M^-1 r = B r + P A_c^-1 P' r with P the rigid-body modes of every
+    aggregate (masked, orthonormalised) and A_c = P' K P."""
+    import scipy.sparse as sp
+    agg = coarse_aggregates(mesh, coarse_nodes)
+    fixed = mesh.fixed_mask().astype(bool)
+    X = np.asarray(mesh.X, np.float64)
+    N = X.shape[0]
+    n_agg = int(agg.max()) + 1
+    cent = np.zeros((n_agg, 3))
+    np.add.at(cent, agg, X)
+    cent /= np.bincount(agg)[:, None]
+    d = X - cent[agg]
+    rows, cols, vals = [], [], []
+    ncol = 0
+    order = np.argsort(agg, kind="stable")
+    starts = np.searchsorted(agg[order], np.arange(n_agg + 1))
+    for g in range(n_agg):
+        nodes = order[starts[g]:starts[g + 1]]
+        B = np.zeros((3 * nodes.size, 6))
+        for c in range(3):
+            B[c::3, c] = 1.0
+        for k in range(3):
+            r = np.cross(np.eye(3)[k][None, :], d[nodes])
+            for c in range(3):
+                B[c::3, 3 + k] = r[:, c]
+        dofs = (3 * nodes[:, None] + np.arange(3)[None, :]).ravel()
+        B[fixed[dofs]] = 0.0
+        Q, R = np.linalg.qr(B)
+        dg = np.abs(np.diag(R))
+        good = dg > 1e-5 * max(dg.max(), 1e-300)
+        for j in np.where(good)[0]:
+            rows.append(dofs)
+            cols.append(np.full(dofs.size, ncol))
+            vals.append(Q[:, j])
+            ncol += 1
+    n = 3 * N
+    P = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, ncol))
+    K = sp.csr_matrix((csr[2], csr[1], csr[0]), shape=(n, n))
+    D = sp.diags((~fixed).astype(np.float64))
+    Ac = (P.T @ (D @ K @ D) @ P).toarray()
+    Ainv = np.linalg.inv(0.5 * (Ac + Ac.T))
+    Minv = block_jacobi(csr, mesh.fixed_mask()).reshape(N, 3, 3)
+    return P, Ainv, Minv
+
+
+def twolevel_pcg(mesh, csr, b, coarse_nodes: int, rtol: float = 1e-8, maxit: int = 20000):
+    """PCG with the two-level preconditioner (numpy); returns (x, iterations)."""
+    import scipy.sparse as sp
+    P, Ainv, Minv = twolevel_operator(mesh, csr, coarse_nodes)
+    fixed = mesh.fixed_mask().astype(bool)
+    n = b.size
+    K = sp.csr_matrix((csr[2], csr[1], csr[0]), shape=(n, n))
+    D = sp.diags((~fixed).astype(np.float64))
+    Km = D @ K @ D
+    b = np.where(fixed, 0.0, b)
+
+    def prec(r):
+        return np.einsum("aij,aj->ai", Minv, r.reshape(-1, 3)).ravel() + P @ (Ainv @ (P.T @ r))
+
+    x = np.zeros(n)
+    r = b.copy()
+    z = prec(r)
+    p = z.copy()
+    rz = r @ z
+    bn = np.linalg.norm(b)
+    for it in range(1, maxit + 1):
+        q = Km @ p
+        a = rz / (p @ q)
+        x += a * p
+        r -= a * q
+        if np.linalg.norm(r) <= rtol * bn:
+            return x, it
+        z = prec(r)
+        rz2 = r @ z
+        p = z + (rz2 / rz) * p
+        rz = rz2
+    return x, maxit

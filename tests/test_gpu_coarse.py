@@ -1,0 +1,120 @@
+"""Two-level (rigid-body modes on aggregates) preconditioned solves on the GPU.
+
+Bars: FP64 solves within 1e-8 relative L2 of the oracle's direct solve; the
+PCG iteration count within 3% (+2) of the oracle's numpy restatement of the
+same two-level preconditioner, which pins the aggregation, the mode bases
+and the coarse matrix (a wrong A_c still converges, but not in the same
+number of iterations); bitwise determinism; mixed precision and learned
+blocks with the coarse level; the Newton path with it.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from jfem import mesh as M
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def cfg0(oracle):
+    import jfem
+    m = M.box_tet10(40, 4, 4, seed=1, jitter=0.1, E_sigma=0.5)
+    ctx = jfem.Context(0)
+    ctx.upload(m)
+    csr = oracle.assemble_csr(m)
+    f = oracle.load_vector(m)
+    ud = oracle.solve_direct(csr, m.fixed_mask(), np.zeros(m.n_dofs), f)[0]
+    return m, ctx, csr, f, ud
+
+
+@pytest.mark.parametrize("cn", [64, 256])
+def test_two_level_matches_direct_and_oracle_iterations(cfg0, oracle, cn):
+    m, ctx, csr, f, ud = cfg0
+    say(f"two-level fp64 coarse_nodes={cn}")
+    u, _, st = ctx.solve(method=0, op="matfree", rtol=1e-9, coarse_nodes=cn)
+    assert st["converged"] == 1
+    assert np.linalg.norm(u - ud) / np.linalg.norm(ud) <= 1e-8
+    _, it_o = oracle.twolevel_pcg(m, csr, f, cn, rtol=1e-9)
+    say(f"  iterations gpu={st['iterations']} oracle={it_o} setup_ms={st['setup_ms']:.2f}")
+    assert abs(st["iterations"] - it_o) <= 2 + 0.03 * it_o
+    _, _, sb = ctx.solve(method=0, op="matfree", rtol=1e-9)
+    assert st["iterations"] * 5 < sb["iterations"]
+
+
+def test_two_level_csr_and_deterministic(cfg0):
+    m, ctx, csr, f, ud = cfg0
+    u1, _, s1 = ctx.solve(method=0, op="csr", op_outer="csr", rtol=1e-9, coarse_nodes=128)
+    u2, _, s2 = ctx.solve(method=0, op="csr", op_outer="csr", rtol=1e-9, coarse_nodes=128)
+    assert s1["iterations"] == s2["iterations"] and np.array_equal(u1, u2)
+    assert np.linalg.norm(u1 - ud) / np.linalg.norm(ud) <= 1e-8
+
+
+def test_two_level_mixed_and_learned(cfg0):
+    from jfem.learned import DEFAULT_WEIGHTS
+    m, ctx, csr, f, ud = cfg0
+    u, _, st = ctx.solve(method=1, op="matfree", op_outer="matfree", rtol=1e-9, inner_rtol=1e-3, coarse_nodes=128)
+    say(f"two-level mixed: {st}")
+    assert st["converged"] == 1
+    assert np.linalg.norm(u - ud) / np.linalg.norm(ud) <= 1e-8
+    with open(DEFAULT_WEIGHTS, "rb") as fh:
+        ctx.set_learned_weights(fh.read())
+    u, _, st = ctx.solve(method=1, op="matfree", op_outer="matfree", precond=1, learned_probe=8, rtol=1e-9,
+                         inner_rtol=1e-3, coarse_nodes=128)
+    say(f"two-level mixed learned: {st}")
+    assert st["converged"] == 1
+    assert np.linalg.norm(u - ud) / np.linalg.norm(ud) <= 1e-8
+
+
+@pytest.mark.parametrize("gen", [M.box_hex8, M.box_tet4])
+def test_two_level_other_elements(oracle, gen):
+    import jfem
+    m = gen(16, 4, 4, seed=2, jitter=0.1, E_sigma=0.3)
+    ctx = jfem.Context(0)
+    ctx.upload(m)
+    csr = oracle.assemble_csr(m)
+    f = oracle.load_vector(m)
+    ud = oracle.solve_direct(csr, m.fixed_mask(), np.zeros(m.n_dofs), f)[0]
+    u, _, st = ctx.solve(method=0, op="matfree", rtol=1e-10, coarse_nodes=64)
+    assert st["converged"] == 1
+    assert np.linalg.norm(u - ud) / np.linalg.norm(ud) <= 1e-8
+    _, it_o = oracle.twolevel_pcg(m, csr, f, 64, rtol=1e-10)
+    assert abs(st["iterations"] - it_o) <= 2 + 0.03 * it_o
+    ctx.close()
+
+
+def test_two_level_newton(oracle):
+    import jfem
+    m = M.box_tet10(10, 2, 2, seed=3, jitter=0.1, E_sigma=0.3, traction=(0.0, 0.0, -52.5))
+    uo, _ = oracle.nh_newton(m, load_steps=5, rtol=1e-10)
+    ctx = jfem.Context(0)
+    ctx.upload(m)
+    for method in (0, 1):
+        u, st = ctx.solve_nonlinear(load_steps=5, rtol=1e-10, method=method, coarse_nodes=32)
+        say(f"two-level Newton method={method}: {st}")
+        assert st["converged"] == 1
+        assert np.linalg.norm(u - uo) <= 1e-7 * np.linalg.norm(uo)
+    ctx.close()
+
+
+def test_two_level_config1():
+    import jfem
+    m = M.box_tet10(160, 16, 16, seed=1, jitter=0.1, E_sigma=0.5)
+    ctx = jfem.Context(0)
+    ctx.upload(m)
+    say("config1 fp64 block Jacobi")
+    u0, _, s0 = ctx.solve(method=0, op="matfree", rtol=1e-8)
+    for cn, method in ((512, 0), (512, 1)):
+        say(f"config1 two-level coarse_nodes={cn} method={method}")
+        u, _, st = ctx.solve(method=method, op="matfree", op_outer="matfree", rtol=1e-8, inner_rtol=1e-3,
+                             coarse_nodes=cn)
+        say(f"  {st}")
+        assert st["converged"] == 1
+        assert np.linalg.norm(u - u0) / np.linalg.norm(u0) <= 1e-6
+        assert st["iterations"] * 5 < s0["iterations"]
