@@ -28,6 +28,9 @@ FP32_PEAK_TFLOPS = 157.3
 
 # headline candidates: (jfm_solve method, preconditioner, extra options)
 METHODS = {
+    "mixed-learned-2l": (1, 1, dict(learned_probe=16, inner_rtol=1e-3, coarse_nodes=512)),
+    "mixed-bj-2l": (1, 0, dict(inner_rtol=1e-3, coarse_nodes=512)),
+    "fp64-bj-2l": (0, 0, dict(coarse_nodes=512)),
     "mixed-learned": (1, 1, dict(learned_probe=16, inner_rtol=1e-2, first_rtol=1e-3)),
     "mixed-bj": (1, 0, dict(inner_rtol=1e-2, first_rtol=1e-3)),
     "mixed-bj-plain": (1, 0, dict(inner_rtol=1e-4)),
@@ -87,7 +90,7 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--kernels-only", action="store_true",
                     help="time the K.u kernels and a 1 GiB device copy only (for counter runs)")
-    ap.add_argument("--method", default="mixed-learned", choices=sorted(METHODS),
+    ap.add_argument("--method", default="mixed-learned-2l", choices=sorted(METHODS),
                     help="headline solve method")
     args = ap.parse_args()
     if args.gpus != 1:

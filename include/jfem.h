@@ -236,10 +236,13 @@ int jfm_learned_features(jfm_ctx *ctx, float *feat);
 int jfm_learned_outputs(jfm_ctx *ctx, float *out, double *M);
 /* Training step pieces: mean over n_pairs (residual r, error e) pairs
  * (each n_dofs, row p at r + p*n_dofs, zero on fixed DOFs, e = K^-1 r) of
- *     loss = 1 - (z.r)^2 / ((z.Kz)(e.r)),  z = M r,
+ *     loss = 1 - (z.r)^2 / ((z.Kz)(e.r)),  z = M r (+ Q A_c^-1 Q' r),
  * and its gradient with respect to the parameters (jfm_learned_n_params()
- * floats), computed by the network's forward and backward kernels. */
-int jfm_learned_grad(jfm_ctx *ctx, int32_t n_pairs, const double *r, const double *e, double *loss, float *grad);
+ * floats), computed by the network's forward and backward kernels.  With
+ * coarse_nodes > 0 the blocks are trained as the fine level of the two-level
+ * preconditioner of jfm_solve_opts.coarse_nodes. */
+int jfm_learned_grad(jfm_ctx *ctx, int32_t n_pairs, const double *r, const double *e, int32_t coarse_nodes,
+                     double *loss, float *grad);
 
 #ifdef __cplusplus
 }

@@ -23,8 +23,9 @@
 //    aggregate g is Q_g' K Q_{p,j} for the single colour-c aggregate p coupled
 //    to g.  No atomics; the result is deterministic.
 //  * rocSOLVER potrf + potri invert A_c (dense, a few thousand rows); the
-//    inverse is folded with the R_g^-1 into Ahat = R^-1 A_c^-1 R^-T, so the
-//    iteration needs only the raw modes: Q A_c^-1 Q' r = Bm Ahat Bm' r.
+//    inverse is folded with the R_g^-1 into Ahat = R^-1 A_c^-1 R^-T (stored
+//    FP32), so the iteration needs only the raw modes: Q A_c^-1 Q' r =
+//    Bm Ahat Bm' r.
 //
 // Per PCG iteration (after the update kernel has written z = B r):
 //  k_coarse_restrict  w_g = sum_{a in g} [r_a ; d_a x r_a]   (one block per g)
@@ -131,8 +132,7 @@ __global__ __launch_bounds__(256) void k_coarse_fix(int nc, double *__restrict__
 // Ahat = R^-1 A^-1 R^-T from the lower triangle (column-major) that potri
 // leaves: entry (r, s) of A^-1 with r >= s is at L[s * nc + r].
 __global__ __launch_bounds__(256) void k_coarse_fold(int nc, const double *__restrict__ L, const double *__restrict__ Rinv,
-                                                    const int32_t *__restrict__ dropped, double *__restrict__ A64,
-                                                    float *__restrict__ A32) {
+                                                    const int32_t *__restrict__ dropped, float *__restrict__ A32) {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (int64_t)nc * nc) return;
     const int r = (int)(idx / nc), s = (int)(idx - (int64_t)r * nc);
@@ -152,48 +152,50 @@ __global__ __launch_bounds__(256) void k_coarse_fold(int nc, const double *__res
         }
         acc += rik * inner;
     }
-    A64[idx] = acc;
     A32[idx] = (float)acc;
 }
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_coarse_restrict(int n_agg, const int32_t *__restrict__ agg_ptr,
                                                         const int32_t *__restrict__ agg_nodes, const T *__restrict__ cd,
-                                                        const T *__restrict__ r, T *__restrict__ wc,
+                                                        const T *__restrict__ r, float *__restrict__ wc,
                                                         const int *__restrict__ done) {
     if (done && *done) return;
     const int g = blockIdx.x;
     double w[6];
     agg_sums<T>(g, agg_ptr, agg_nodes, cd, r, w);
-    if (threadIdx.x < 6) wc[6 * g + threadIdx.x] = (T)w[threadIdx.x];
+    if (threadIdx.x < 6) wc[6 * g + threadIdx.x] = (float)w[threadIdx.x];
 }
 
-// zc = Ahat w, one 64-lane wave per row, fixed-order lane sums
-template <typename T>
-__global__ __launch_bounds__(256) void k_coarse_gemv(int nc, const T *__restrict__ A, const T *__restrict__ w,
-                                                    T *__restrict__ zc, const int *__restrict__ done) {
+// zc = Ahat w, one 64-lane wave per row, fixed-order lane sums.  Ahat is
+// kept in FP32 for both precisions: it only scales the coarse correction of a
+// fixed SPD preconditioner, so its rounding changes the iteration count at
+// most, never the answer, and it halves the dominant read of the level.
+__global__ __launch_bounds__(256) void k_coarse_gemv(int nc, const float *__restrict__ A, const float *__restrict__ w,
+                                                    float *__restrict__ zc, const int *__restrict__ done) {
     if (done && *done) return;
     const int row = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
     const int lane = threadIdx.x & 63;
     if (row >= nc) return;  // whole wave leaves together
-    const T *Ar = A + (size_t)row * nc;
+    const float *Ar = A + (size_t)row * nc;
     double s = 0.0;
     for (int k = lane; k < nc; k += 64) s += (double)Ar[k] * (double)w[k];
     s = wave_sum(s);
-    if (lane == 0) zc[row] = (T)s;
+    if (lane == 0) zc[row] = (float)s;
 }
 
 // z_a += m_a o (t_g + omega_g x d_a); optionally p = z; partials (r.z, r.r)
 template <typename T>
 __global__ __launch_bounds__(256) void k_coarse_prolong(int N, const int32_t *__restrict__ node_agg, const T *__restrict__ cd,
-                                                       const uint8_t *__restrict__ fixed, const T *__restrict__ zc,
+                                                       const uint8_t *__restrict__ fixed, const float *__restrict__ zc,
                                                        const T *__restrict__ r, T *__restrict__ z, T *__restrict__ p,
                                                        double *__restrict__ part, const int *__restrict__ done) {
     if (done && *done) return;
     double v[2] = {0.0, 0.0};
     for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < N; a += gridDim.x * blockDim.x) {
         const size_t o = 3 * (size_t)a;
-        const T *y = zc + 6 * (size_t)node_agg[a];
+        const float *yf = zc + 6 * (size_t)node_agg[a];
+        const T y[6] = {(T)yf[0], (T)yf[1], (T)yf[2], (T)yf[3], (T)yf[4], (T)yf[5]};
         const T d0 = cd[o], d1 = cd[o + 1], d2 = cd[o + 2];
         const T c[3] = {y[0] + y[4] * d2 - y[5] * d1, y[1] + y[5] * d0 - y[3] * d2, y[2] + y[3] * d1 - y[4] * d0};
 #pragma unroll
@@ -418,10 +420,11 @@ template <typename T> static void free_dev(jfm_ctx *ctx, T *&p) {
 void jfem_coarse_release(jfm_ctx *ctx) {
     free_dev(ctx, ctx->node_agg); free_dev(ctx, ctx->agg_ptr); free_dev(ctx, ctx->agg_nodes);
     free_dev(ctx, ctx->agg_color); free_dev(ctx, ctx->partner); free_dev(ctx, ctx->cd64); free_dev(ctx, ctx->cd32);
-    free_dev(ctx, ctx->Rinv); free_dev(ctx, ctx->Ac); free_dev(ctx, ctx->Ahat64); free_dev(ctx, ctx->Ahat32);
-    free_dev(ctx, ctx->wc64); free_dev(ctx, ctx->zc64); free_dev(ctx, ctx->wc32); free_dev(ctx, ctx->zc32);
+    free_dev(ctx, ctx->Rinv); free_dev(ctx, ctx->Ac); free_dev(ctx, ctx->Ahat32);
+    free_dev(ctx, ctx->wc32); free_dev(ctx, ctx->zc32);
     free_dev(ctx, ctx->dropped);
     ctx->coarse_ready = false;
+    ctx->coarse_tag = -1;
     ctx->coarse_built_nodes = 0;
     ctx->n_agg = ctx->nc = 0;
 }
@@ -469,10 +472,7 @@ int jfem_coarse_setup(jfm_ctx *ctx, int op, int coarse_nodes, double *ms) {
         CALLOC(ctx->Rinv, double, 36 * (int64_t)A.n_agg);
         CALLOC(ctx->dropped, int32_t, nc);
         CALLOC(ctx->Ac, double, nc * nc);
-        CALLOC(ctx->Ahat64, double, nc * nc);
         CALLOC(ctx->Ahat32, float, nc * nc);
-        CALLOC(ctx->wc64, double, nc);
-        CALLOC(ctx->zc64, double, nc);
         CALLOC(ctx->wc32, float, nc);
         CALLOC(ctx->zc32, float, nc);
 #undef CALLOC
@@ -524,7 +524,7 @@ int jfem_coarse_setup(jfm_ctx *ctx, int op, int coarse_nodes, double *ms) {
     if (hinfo != 0) return jfm_fail(ctx, JFM_ENOTSPD, "coarse matrix is not positive definite");
     if (rocsolver_dpotri(h, rocblas_fill_lower, nc, ctx->Ac, nc, info) != rocblas_status_success)
         return jfm_fail(ctx, JFM_EHIP, "rocsolver_dpotri failed");
-    k_coarse_fold<<<nblk(nn), kBlock, 0, s>>>(nc, ctx->Ac, ctx->Rinv, ctx->dropped, ctx->Ahat64, ctx->Ahat32);
+    k_coarse_fold<<<nblk(nn), kBlock, 0, s>>>(nc, ctx->Ac, ctx->Rinv, ctx->dropped, ctx->Ahat32);
     JFM_HIP(ctx, hipGetLastError());
     JFM_HIP(ctx, hipEventRecord(e1, s));
     JFM_HIP(ctx, hipEventSynchronize(e1));
@@ -534,6 +534,7 @@ int jfem_coarse_setup(jfm_ctx *ctx, int op, int coarse_nodes, double *ms) {
     (void)hipEventDestroy(e1);
     if (ms) *ms += t;
     ctx->coarse_ready = true;
+    ctx->coarse_tag = op == JFM_OP_TANGENT ? 1 : 0;
     return JFM_OK;
 }
 
@@ -541,18 +542,14 @@ int jfem_coarse_setup(jfm_ctx *ctx, int op, int coarse_nodes, double *ms) {
 // init kernel with set_p): z += Q A_c^-1 Q' r, partials (r.z, r.r) to `part`
 template <typename T>
 hipError_t launch_coarse_correct(jfm_ctx *ctx, const T *r, T *z, T *p, double *part, const int *done, int *npart) {
-    const bool f64 = sizeof(T) == 8;
-    const T *cd = f64 ? (const T *)ctx->cd64 : (const T *)ctx->cd32;
-    T *wc = f64 ? (T *)ctx->wc64 : (T *)ctx->wc32;
-    T *zc = f64 ? (T *)ctx->zc64 : (T *)ctx->zc32;
-    const T *Ah = f64 ? (const T *)ctx->Ahat64 : (const T *)ctx->Ahat32;
+    const T *cd = sizeof(T) == 8 ? (const T *)ctx->cd64 : (const T *)ctx->cd32;
     const int n_agg = (int)ctx->n_agg, nc = (int)ctx->nc;
     hipStream_t s = ctx->stream;
-    k_coarse_restrict<T><<<n_agg, kBlock, 0, s>>>(n_agg, ctx->agg_ptr, ctx->agg_nodes, cd, r, wc, done);
-    k_coarse_gemv<T><<<nblk(nc, kBlock / 64), kBlock, 0, s>>>(nc, Ah, wc, zc, done);
+    k_coarse_restrict<T><<<n_agg, kBlock, 0, s>>>(n_agg, ctx->agg_ptr, ctx->agg_nodes, cd, r, ctx->wc32, done);
+    k_coarse_gemv<<<nblk(nc, kBlock / 64), kBlock, 0, s>>>(nc, ctx->Ahat32, ctx->wc32, ctx->zc32, done);
     const unsigned g = std::min<unsigned>(capped_grid(ctx->N, kBlock), 512);
     *npart = (int)g;
-    k_coarse_prolong<T><<<g, kBlock, 0, s>>>((int)ctx->N, ctx->node_agg, cd, ctx->fixed, zc, r, z, p, part, done);
+    k_coarse_prolong<T><<<g, kBlock, 0, s>>>((int)ctx->N, ctx->node_agg, cd, ctx->fixed, ctx->zc32, r, z, p, part, done);
     return hipGetLastError();
 }
 

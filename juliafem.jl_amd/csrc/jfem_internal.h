@@ -211,7 +211,8 @@ struct jfm_ctx {
     std::vector<double> h_X;       // host copy of the coordinates (aggregation)
     int coarse_built_nodes = 0;    // coarse_nodes the aggregation was built for
     int coarse_gen = 0;            // bumped per aggregation (keys captured graphs)
-    bool coarse_ready = false;     // Ahat matches the operator of the current solve
+    bool coarse_ready = false;     // Ahat holds a coarse level
+    int coarse_tag = -1;           // operator it was built from: 0 K, 1 the tangent
     int64_t n_agg = 0, nc = 0;
     int ncolor = 0;
     int32_t *node_agg = nullptr, *agg_ptr = nullptr, *agg_nodes = nullptr, *agg_color = nullptr;
@@ -221,9 +222,7 @@ struct jfm_ctx {
     float *cd32 = nullptr;
     double *Rinv = nullptr;        // n_agg x 36
     double *Ac = nullptr;          // nc x nc (factorised in place)
-    double *Ahat64 = nullptr;      // nc x nc: R^-1 A_c^-1 R^-T
-    float *Ahat32 = nullptr;
-    double *wc64 = nullptr, *zc64 = nullptr;
+    float *Ahat32 = nullptr;       // nc x nc: R^-1 A_c^-1 R^-T
     float *wc32 = nullptr, *zc32 = nullptr;
     void *blas = nullptr;          // rocblas_handle
     double *Muser64 = nullptr;     // N x 9 user blocks (JFM_PREC_USER)

@@ -623,12 +623,16 @@ extern "C" int jfm_learned_outputs(jfm_ctx *ctx, float *out, double *M) {
     return JFM_OK;
 }
 
-extern "C" int jfm_learned_grad(jfm_ctx *ctx, int32_t n_pairs, const double *r, const double *e, double *loss,
-                                float *grad) {
-    if (!ctx || n_pairs <= 0 || !r || !e || !grad) return JFM_EINVAL;
+extern "C" int jfm_learned_grad(jfm_ctx *ctx, int32_t n_pairs, const double *r, const double *e, int32_t coarse_nodes,
+                                double *loss, float *grad) {
+    if (!ctx || n_pairs <= 0 || !r || !e || !grad || coarse_nodes < 0) return JFM_EINVAL;
     JFM_HIP(ctx, hipSetDevice(ctx->device));
     ctx->learned_ready = false;
     if (int rc = jfem_learned_prepare(ctx, nullptr)) return rc;
+    // the blocks act inside the two-level preconditioner: z = M r + Q A_c^-1 Q' r
+    const bool coarse = coarse_nodes > 0;
+    if (coarse && !(ctx->coarse_ready && ctx->coarse_built_nodes == coarse_nodes && ctx->coarse_tag == 0))
+        if (int rc = jfem_coarse_setup(ctx, JFM_OP_MATFREE, coarse_nodes, nullptr)) return rc;
     const int64_t n = ctx->ndof, N = ctx->N;
     hipStream_t s = ctx->stream;
     double *dr = jfm_alloc<double>(ctx, n), *de = jfm_alloc<double>(ctx, n), *dz = jfm_alloc<double>(ctx, n),
@@ -648,6 +652,11 @@ extern "C" int jfm_learned_grad(jfm_ctx *ctx, int32_t n_pairs, const double *r, 
         JFM_HIP(ctx, hipMemcpyAsync(dr, r + (size_t)p * n, sizeof(double) * n, hipMemcpyHostToDevice, s));
         JFM_HIP(ctx, hipMemcpyAsync(de, e + (size_t)p * n, sizeof(double) * n, hipMemcpyHostToDevice, s));
         k_block_apply<<<nblk(N), kBlock, 0, s>>>((int)N, ctx->Mlearn64, dr, dz);
+        if (coarse) {
+            int np = 0;
+            hipError_t err = launch_coarse_correct<double>(ctx, dr, dz, nullptr, ctx->partials + kPartMisc, nullptr, &np);
+            if (err != hipSuccess) { rc = jfm_hip_fail(ctx, err, "coarse correction"); break; }
+        }
         if ((rc = jfem_apply(ctx, JFM_OP_MATFREE, dz, dw))) break;
         double zr = 0, zw = 0, er = 0;
         if ((rc = jfem_dot(ctx, dz, dr, n, &zr)) || (rc = jfem_dot(ctx, dz, dw, n, &zw)) || (rc = jfem_dot(ctx, de, dr, n, &er)))

@@ -56,7 +56,7 @@ class NewtonOpts(C.Structure):
     _fields_ = [("load_steps", C.c_int32), ("max_newton", C.c_int32), ("rtol", C.c_double), ("method", C.c_int32),
                 ("precond", C.c_int32), ("eta_max", C.c_double), ("maxit_linear", C.c_int32),
                 ("learned_probe", C.c_int32), ("inner_rtol", C.c_double), ("inner_maxit", C.c_int32),
-                ("check_every", C.c_int32)]
+                ("check_every", C.c_int32), ("coarse_nodes", C.c_int32)]
 
 
 class NewtonStats(C.Structure):
@@ -107,7 +107,7 @@ def load():
     lib.jfm_learned_n_params.restype = C.c_int32
     lib.jfm_learned_features.argtypes = [_P, _P]
     lib.jfm_learned_outputs.argtypes = [_P, _P, _P]
-    lib.jfm_learned_grad.argtypes = [_P, C.c_int32, _P, _P, C.POINTER(C.c_double), _P]
+    lib.jfm_learned_grad.argtypes = [_P, C.c_int32, _P, _P, C.c_int32, C.POINTER(C.c_double), _P]
     lib.jfm_learned_set_weights.argtypes = [_P, _P, C.c_size_t]
     lib.jfm_nl_residual.argtypes = [_P, _P, C.c_double, _P, C.POINTER(C.c_double)]
     lib.jfm_solve_nonlinear.argtypes = [_P, C.POINTER(NewtonOpts), _P, C.POINTER(NewtonStats)]
@@ -290,11 +290,13 @@ class Context:
         self._check(self._lib.jfm_learned_outputs(self._h, _p(o), _p(M)))
         return o, M
 
-    def learned_grad(self, R, E):
-        """(mean loss, gradient) over residual/error pairs (rows of R and E)."""
+    def learned_grad(self, R, E, coarse_nodes=0):
+        """(mean loss, gradient) over residual/error pairs (rows of R and E);
+        coarse_nodes > 0 trains the blocks as the fine level of the two-level
+        preconditioner."""
         R = np.ascontiguousarray(R, dtype=np.float64)
         E = np.ascontiguousarray(E, dtype=np.float64)
         g = np.empty(self._lib.jfm_learned_n_params(), np.float32)
         loss = C.c_double(0)
-        self._check(self._lib.jfm_learned_grad(self._h, R.shape[0], _p(R), _p(E), C.byref(loss), _p(g)))
+        self._check(self._lib.jfm_learned_grad(self._h, R.shape[0], _p(R), _p(E), coarse_nodes, C.byref(loss), _p(g)))
         return loss.value, g

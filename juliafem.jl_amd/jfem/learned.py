@@ -149,9 +149,10 @@ class LearnedPreconditioner:
         return lp
 
 
-def harvest_pairs(ctx, mesh, n_pairs: int = 8, rtol_star: float = 1e-10, seed: int = 0):
-    """(R, E): residual/error pairs from iterates of the plain FP64 PCG."""
-    xs, _, st = ctx.solve(method=0, op="matfree", rtol=rtol_star, raise_on_noconv=False)
+def harvest_pairs(ctx, mesh, n_pairs: int = 8, rtol_star: float = 1e-10, seed: int = 0, coarse_nodes: int = 0):
+    """(R, E): residual/error pairs from iterates of the FP64 block-Jacobi
+    PCG (two-level with coarse_nodes > 0)."""
+    xs, _, st = ctx.solve(method=0, op="matfree", rtol=rtol_star, coarse_nodes=coarse_nodes, raise_on_noconv=False)
     n_it = max(int(st["iterations"]), 4)
     rng = np.random.default_rng(seed)
     ks = np.unique(np.clip(np.geomspace(2, 0.98 * n_it, n_pairs).astype(int)
@@ -159,7 +160,8 @@ def harvest_pairs(ctx, mesh, n_pairs: int = 8, rtol_star: float = 1e-10, seed: i
     fixed = mesh.fixed_mask().astype(bool)
     R, E = [], []
     for k in ks:
-        xk, _, _ = ctx.solve(method=0, op="matfree", rtol=1e-30, maxit=int(k), raise_on_noconv=False)
+        xk, _, _ = ctx.solve(method=0, op="matfree", rtol=1e-30, maxit=int(k), coarse_nodes=coarse_nodes,
+                             raise_on_noconv=False)
         e = xs - xk
         e[fixed] = 0.0
         r = ctx.apply(e, op="matfree")     # = f - K x_k, consistent with e exactly
@@ -168,15 +170,19 @@ def harvest_pairs(ctx, mesh, n_pairs: int = 8, rtol_star: float = 1e-10, seed: i
     return np.array(R), np.array(E)
 
 
-def train(contexts, pair_sets, lp: LearnedPreconditioner, steps: int = 200, lr: float = 2e-3, log=None):
-    """Adam on the mean loss over every (context, pairs) set; returns the loss history."""
+def train(contexts, pair_sets, lp: LearnedPreconditioner, steps: int = 200, lr: float = 2e-3, log=None,
+          coarse_nodes=0):
+    """Adam on the mean loss over every (context, pairs) set; returns the loss
+    history.  coarse_nodes > 0 (one value, or one per context) trains the
+    blocks inside the two-level preconditioner."""
     for s in range(steps):
         blob = lp.blob()
         gsum = np.zeros(N_PARAMS)
         lsum = 0.0
-        for ctx, (R, E) in zip(contexts, pair_sets):
+        cns = coarse_nodes if isinstance(coarse_nodes, (list, tuple)) else [coarse_nodes] * len(contexts)
+        for ctx, (R, E), cn in zip(contexts, pair_sets, cns):
             ctx.set_learned_weights(blob)
-            loss, g = ctx.learned_grad(R, E)
+            loss, g = ctx.learned_grad(R, E, coarse_nodes=cn)
             gsum += g
             lsum += loss
         gsum /= len(contexts)

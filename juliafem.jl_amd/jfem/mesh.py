@@ -328,3 +328,156 @@ def read_abaqus_inp(path: str):
         conn = np.array([[remap[n] for n in r] for r in rows], dtype=np.int64)
         out[name] = (et, conn)
     return X, out
+
+
+# ------------------------------------------------------------------- gmsh ----
+# gmsh element type -> (JuliaFEM type, nodes, permutation gmsh -> reference
+# local order).  gmsh's 10-node tetrahedron puts the (3,2) edge node before
+# the (3,1) one; the reference order (src/basis/lagrange_generated.jl:261-266)
+# has them the other way round, so positions 8 and 9 swap.
+_GMSH_TYPES = {
+    2: ("Tri3", 3, None), 9: ("Tri6", 6, None), 3: ("Quad4", 4, None),
+    4: ("Tet4", 4, None), 11: ("Tet10", 10, (0, 1, 2, 3, 4, 5, 6, 7, 9, 8)), 5: ("Hex8", 8, None),
+}
+
+
+def read_gmsh(path: str):
+    """Read an ASCII gmsh 4.1 mesh (the format of src/gmsh_reader.jl).
+
+    Unlike the reference reader, which keeps Tet4 only and takes entity tags
+    for physical groups, this one maps entities to their physical groups
+    through $Entities and keeps Tri3/Tri6/Quad4/Tet4/Tet10/Hex8.  Returns
+    (X (N, 3), {physical name: (etype, conn (n, npe) 0-based)}); node tags are
+    renumbered densely in ascending tag order.
+    """
+    with open(path) as fh:
+        lines = [ln.strip() for ln in fh]
+    pos = 0
+
+    def section(name):
+        nonlocal pos
+        while pos < len(lines) and lines[pos] != name:
+            pos += 1
+        if pos == len(lines):
+            return False
+        pos += 1
+        return True
+
+    if not section("$MeshFormat"):
+        raise ValueError("not a gmsh file")
+    ver = lines[pos].split()
+    if not ver[0].startswith("4") or ver[1] != "0":
+        raise ValueError(f"only ASCII gmsh 4.x is supported (found {lines[pos]})")
+    names = {}
+    if section("$PhysicalNames"):
+        for k in range(int(lines[pos])):
+            parts = lines[pos + 1 + k].split(maxsplit=2)
+            names[(int(parts[0]), int(parts[1]))] = parts[2].strip('"')
+    phys = {}
+    pos = 0
+    if section("$Entities"):
+        counts = [int(v) for v in lines[pos].split()]
+        pos += 1
+        for dim in range(4):
+            for _ in range(counts[dim]):
+                parts = lines[pos].split()
+                pos += 1
+                tag = int(parts[0])
+                k = 4 if dim == 0 else 7
+                n_ph = int(parts[k])
+                phys[(dim, tag)] = [int(v) for v in parts[k + 1:k + 1 + n_ph]]
+    pos = 0
+    if not section("$Nodes"):
+        raise ValueError("no $Nodes section")
+    n_blocks, n_nodes = (int(v) for v in lines[pos].split()[:2])
+    pos += 1
+    tags, coords = [], []
+    for _ in range(n_blocks):
+        cnt = int(lines[pos].split()[3])
+        pos += 1
+        tags.extend(int(lines[pos + k]) for k in range(cnt))
+        coords.extend([float(v) for v in lines[pos + cnt + k].split()[:3]] for k in range(cnt))
+        pos += 2 * cnt
+    order = np.argsort(tags)
+    tag_sorted = np.asarray(tags, np.int64)[order]
+    X = np.asarray(coords, np.float64)[order]
+    if not section("$Elements"):
+        raise ValueError("no $Elements section")
+    n_blocks = int(lines[pos].split()[0])
+    pos += 1
+    groups = {}
+    for _ in range(n_blocks):
+        dim, etag, gtype, cnt = (int(v) for v in lines[pos].split())
+        pos += 1
+        rows = [lines[pos + k].split() for k in range(cnt)]
+        pos += cnt
+        if gtype not in _GMSH_TYPES:
+            continue
+        etype, npe, perm = _GMSH_TYPES[gtype]
+        conn = np.asarray([[int(v) for v in r[1:1 + npe]] for r in rows], np.int64)
+        conn = np.searchsorted(tag_sorted, conn)
+        if perm is not None:
+            conn = conn[:, list(perm)]
+        for ptag in phys.get((dim, etag), []) or [None]:
+            # a physical tag without a name falls back to the name filed under
+            # the entity tag, the reference reader's reading of such files
+            name = names.get((dim, ptag)) or names.get((dim, etag)) or (
+                f"entity{dim}_{etag}" if ptag is None else f"physical{dim}_{ptag}")
+            if name in groups:
+                prev_type, prev = groups[name]
+                if prev_type != etype:
+                    raise ValueError(f"group {name} mixes {prev_type} and {etype}")
+                groups[name] = (etype, np.vstack([prev, conn]))
+            else:
+                groups[name] = (etype, conn)
+    return X, groups
+
+
+def mesh_from_groups(X, groups, volume: str, clamp: str, load: str | None = None,
+                     traction=(0.0, 0.0, -1.0), E: float = 210e3, nu: float = 0.3) -> Mesh:
+    """A Mesh from reader output: `volume` group as elements, every node of
+    the `clamp` group clamped, the `load` group's faces carrying `traction`."""
+    etype, conn = groups[volume]
+    if etype not in ("Tet10", "Tet4", "Hex8"):
+        raise ValueError(f"{volume} holds {etype}, not volume elements")
+    used = np.unique(conn)
+    remap = -np.ones(X.shape[0], np.int64)
+    remap[used] = np.arange(used.size)
+    clamp_nodes = np.unique(groups[clamp][1])
+    if load is not None:
+        ftype, faces = groups[load]
+    else:
+        ftype, faces = {"Tet10": "Tri6", "Tet4": "Tri3", "Hex8": "Quad4"}[etype], np.zeros((0, 3), np.int64)
+    n = conn.shape[0]
+    return Mesh(X=X[used], conn=remap[conn], etype=etype, E=np.full(n, float(E)), nu=np.full(n, float(nu)),
+                faces=remap[faces] if faces.size else np.zeros((0, {"Tri6": 6, "Tri3": 3, "Quad4": 4}[ftype]), np.int64),
+                ftype=ftype, clamp_nodes=remap[clamp_nodes[np.isin(clamp_nodes, used)]], traction=tuple(traction))
+
+
+# ------------------------------------------------------------ renumbering ----
+def rcm_order(mesh: Mesh) -> np.ndarray:
+    """Reverse Cuthill-McKee node order of the mesh's node adjacency
+    (src/graph/graph_ordering.jl:38-90): perm[new] = old."""
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import reverse_cuthill_mckee
+    conn = np.asarray(mesh.conn, np.int64)
+    npe = conn.shape[1]
+    rows = np.repeat(conn, npe, axis=1).ravel()
+    cols = np.tile(conn, (1, npe)).ravel()
+    A = sp.csr_matrix((np.ones(rows.size, np.int8), (rows, cols)), shape=(mesh.n_nodes,) * 2)
+    return np.asarray(reverse_cuthill_mckee(A, symmetric_mode=True), np.int64)
+
+
+def renumber(mesh: Mesh, perm: np.ndarray) -> Mesh:
+    """The same mesh with node `perm[k]` renamed k."""
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(perm.size)
+    return Mesh(X=mesh.X[perm], conn=inv[mesh.conn], etype=mesh.etype, E=mesh.E, nu=mesh.nu,
+                faces=inv[mesh.faces] if mesh.faces.size else mesh.faces, ftype=mesh.ftype,
+                clamp_nodes=np.sort(inv[mesh.clamp_nodes]), traction=mesh.traction, seed=mesh.seed, grid=None)
+
+
+def node_bandwidth(mesh: Mesh) -> int:
+    """max |a - b| over node pairs sharing an element."""
+    c = np.asarray(mesh.conn, np.int64)
+    return int((c.max(axis=1) - c.min(axis=1)).max())

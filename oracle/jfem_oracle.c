@@ -1105,7 +1105,7 @@ double orc_learned_loss_grad(int64_t N, const int64_t *rowptr, const int64_t *co
                              const uint8_t *fixed, const double *Minv, const double *feat, const double *W1,
                              const double *b1, const double *W2, const double *b2, const double *W3, const double *b3,
                              const double *shift, const double *scale, double eps, int n_pairs, const double *R,
-                             const double *Er, double *grad, int emulate_bf16) {
+                             const double *Er, const double *Cc, double *grad, int emulate_bf16) {
     const int64_t n = 3 * N;
     double *out = (double *)malloc((size_t)N * LO * sizeof(double));
     double *M = (double *)malloc((size_t)N * 9 * sizeof(double));
@@ -1120,7 +1120,7 @@ double orc_learned_loss_grad(int64_t N, const int64_t *rowptr, const int64_t *co
             for (int i = 0; i < 3; ++i) {
                 double s = 0;
                 for (int j = 0; j < 3; ++j) s += M[9 * a + 3 * i + j] * r[3 * a + j];
-                z[3 * a + i] = s;
+                z[3 * a + i] = s + (Cc ? Cc[(size_t)p * n + 3 * a + i] : 0.0);
             }
         orc_spmv(n, rowptr, colind, vals, z, w, 1);
         double zr = 0, zw = 0, er = 0;

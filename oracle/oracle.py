@@ -296,7 +296,9 @@ def learned_blocks(feat, net, Minv, fixed, emulate_bf16=True):
     return out, M
 
 
-def learned_loss_grad(csr, fixed, Minv, feat, net, R, E, emulate_bf16=True):
+def learned_loss_grad(csr, fixed, Minv, feat, net, R, E, emulate_bf16=True, coarse=None):
+    """(mean loss, gradient) over the pairs; `coarse` (pairs x n) is added to
+    z = M r when the blocks act inside the two-level preconditioner."""
     keep, eps = _net_args(net)
     rowptr, colind, vals = csr
     N = feat.shape[0]
@@ -304,11 +306,13 @@ def learned_loss_grad(csr, fixed, Minv, feat, net, R, E, emulate_bf16=True):
     E = _f64(E)
     grad = np.zeros(6150)
     f = lib().orc_learned_loss_grad
-    f.argtypes = [_I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.c_double, C.c_int, _P, _P, _P,
+    f.argtypes = [_I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, C.c_double, C.c_int, _P, _P, _P, _P,
                   C.c_int]
     f.restype = C.c_double
+    Cc = None if coarse is None else _f64(coarse)
     loss = f(N, _p(rowptr), _p(colind), _p(vals), _p(np.ascontiguousarray(fixed, dtype=np.uint8)), _p(_f64(Minv)),
-             _p(_f64(feat)), *[_p(k) for k in keep], eps, R.shape[0], _p(R), _p(E), _p(grad), int(emulate_bf16))
+             _p(_f64(feat)), *[_p(k) for k in keep], eps, R.shape[0], _p(R), _p(E), _p(Cc) if Cc is not None else None,
+             _p(grad), int(emulate_bf16))
     return loss, grad
 
 

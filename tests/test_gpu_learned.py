@@ -118,3 +118,24 @@ def test_harmful_network_falls_back(setup):
     assert st["fallback"] == 1 and st["converged"] == 1
     u2, _, st2 = ctx.solve(method=1, op="matfree", op_outer="matfree", precond=1, learned_probe=20, rtol=1e-8)
     assert st2["fallback"] == 1 and st2["converged"] == 1
+
+
+def test_gradient_with_coarse_level_matches_oracle(setup, oracle):
+    """Blocks trained as the fine level of the two-level preconditioner:
+    z = M r + P A_c^-1 P' r, the correction from the oracle's restatement."""
+    m, ctx, csr, f, fx = setup
+    lp = trained_like_net(5)
+    feat = ctx.learned_features().astype(np.float64)
+    lp.fit_normalisation([feat])
+    ctx.set_learned_weights(lp.blob())
+    say("harvest two-level pairs")
+    R, E = harvest_pairs(ctx, m, n_pairs=4, coarse_nodes=32)
+    loss_g, grad_g = ctx.learned_grad(R, E, coarse_nodes=32)
+    P, Ainv, _ = oracle.twolevel_operator(m, csr, 32)
+    Cc = np.array([P @ (Ainv @ (P.T @ r)) for r in R])
+    Minv = oracle.block_jacobi(csr, fx)
+    loss_o, grad_o = oracle.learned_loss_grad(csr, fx, Minv, feat, lp.blob(), R, E, emulate_bf16=True, coarse=Cc)
+    assert loss_g == pytest.approx(loss_o, rel=1e-6)
+    assert np.linalg.norm(grad_g - grad_o) <= 1e-2 * np.linalg.norm(grad_o)
+    loss_1, _ = ctx.learned_grad(R, E)
+    assert loss_g < loss_1      # the coarse level removes part of the error
