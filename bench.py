@@ -28,9 +28,9 @@ FP32_PEAK_TFLOPS = 157.3
 
 # headline candidates: (jfm_solve method, preconditioner, extra options)
 METHODS = {
-    "mixed-learned-2l": (1, 1, dict(learned_probe=16, inner_rtol=1e-3, coarse_nodes=512)),
-    "mixed-bj-2l": (1, 0, dict(inner_rtol=1e-3, coarse_nodes=512)),
-    "fp64-bj-2l": (0, 0, dict(coarse_nodes=512)),
+    "mixed-learned-2l": (1, 1, dict(learned_probe=16, inner_rtol=1e-2, coarse_nodes=2048)),
+    "mixed-bj-2l": (1, 0, dict(inner_rtol=1e-2, coarse_nodes=2048)),
+    "fp64-bj-2l": (0, 0, dict(coarse_nodes=2048)),
     "mixed-learned": (1, 1, dict(learned_probe=16, inner_rtol=1e-2, first_rtol=1e-3)),
     "mixed-bj": (1, 0, dict(inner_rtol=1e-2, first_rtol=1e-3)),
     "mixed-bj-plain": (1, 0, dict(inner_rtol=1e-4)),
@@ -89,13 +89,17 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--kernels-only", action="store_true",
-                    help="time the K.u kernels and a 1 GiB device copy only (for counter runs)")
+                    help="time the K.u kernels and known-size copies only (for counter runs)")
+    ap.add_argument("--workload", default="linear", choices=("linear", "nonlinear"),
+                    help="linear: the 1.05M-DOF solve (default); nonlinear: the 8.1M-DOF neo-Hookean Newton")
     ap.add_argument("--method", default="mixed-learned-2l", choices=sorted(METHODS),
                     help="headline solve method")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "nonlinear":
+        return nonlinear(args)
 
     import numpy as np
     import jfem
@@ -164,6 +168,16 @@ def main():
     log(f"FP64 BJ-PCG matfree: {st64}")
     u64c, st64c = solve(0, "csr", "csr")
     log(f"FP64 BJ-PCG csr: {st64c}")
+    others = {}
+    for name in ("fp64-bj-2l", "mixed-bj-2l"):
+        if name == args.method:
+            continue
+        m_, p_, kw_ = METHODS[name]
+        solve(m_, "matfree", precond=p_, **kw_)          # warm
+        _, st_ = solve(m_, "matfree", precond=p_, **kw_)
+        others[name] = {"s": (st_["solve_ms"]) / 1e3, "setup_s": st_["setup_ms"] / 1e3,
+                        "iters": st_["iterations"], "relres": st_["relres"]}
+        log(f"{name}: {st_}")
 
     for _ in range(args.warmup):
         solve(method, "matfree", precond=precond, **extra)
@@ -171,7 +185,9 @@ def main():
     u_head = None
     for _ in range(args.steps):
         u_head, st = solve(method, "matfree", precond=precond, **extra)
-        times.append((st["solve_ms"] + st["setup_ms"]) / 1e3)
+        # solve_ms spans the whole call: preconditioner setup (coarse level,
+        # learned blocks and their probe) included
+        times.append(st["solve_ms"] / 1e3)
         walls.append(st["wall_s"])
         stats = st
         log(f"{args.method} step: {st}")
@@ -218,12 +234,90 @@ def main():
         "speedup_vs_fp64_matfree": st64["solve_ms"] / 1e3 / value,
         "speedup_vs_fp64_csr": st64c["solve_ms"] / 1e3 / value,
         "iters": stats["iterations"], "outer": stats["outer"], "relres": stats["relres"],
-        "learned_fallback": stats["fallback"], "setup_ms": stats["setup_ms"],
+        "learned_fallback": stats["fallback"], "setup_ms": stats["setup_ms"], "other_methods": others,
         "err_vs_fp64_l2": err_vs_fp64, "host_wall_s": statistics.median(walls),
         "dofs_per_s": inf["n_dofs"] / value, "upload_s": t_upload, "assembly_wall_s": t_asm,
         "kernels": kernels,
     }
     print(json.dumps(out), flush=True)
+
+
+NL_TRACTION = 52.5   # end traction giving a tip deflection of about 10% of the span
+
+
+def nonlinear(args):
+    """8.1M-DOF neo-Hookean cantilever: 5 load steps of Newton to FP64 1e-8,
+    tangent solves by FP32 matrix-free PCG (learned blocks with the runtime
+    probe, two-level) inside FP64 refinement.  One step = the whole
+    load-stepped Newton solve."""
+    import numpy as np
+    import jfem
+    from jfem.mesh import box_tet10
+    nx, ny, nz = (320, 32, 32) if args.grid == [160, 16, 16] else args.grid
+    t0 = time.perf_counter()
+    mesh = box_tet10(nx, ny, nz, traction=(0.0, 0.0, -NL_TRACTION))
+    log(f"mesh {nx}x{ny}x{nz}: {mesh.n_nodes} nodes, {mesh.n_el} Tet10, {mesh.n_dofs} DOFs "
+        f"({time.perf_counter() - t0:.2f}s)")
+    ctx = jfem.Context(0)
+    t0 = time.perf_counter()
+    ctx.upload(mesh)
+    inf = ctx.info()
+    log(f"upload {time.perf_counter() - t0:.2f}s, device {inf['device_bytes'] / 2**30:.2f} GiB")
+    from jfem.learned import DEFAULT_WEIGHTS
+    with open(DEFAULT_WEIGHTS, "rb") as fh:
+        ctx.set_learned_weights(fh.read())
+    opts = dict(load_steps=5, rtol=1e-8, method=1, precond=1, learned_probe=16, inner_rtol=1e-2,
+                coarse_nodes=2048, max_newton=20)
+    times, stats, u = [], None, None
+    for k in range(args.warmup + args.steps):
+        t = time.perf_counter()
+        u, st = ctx.solve_nonlinear(**opts)
+        st["wall_s"] = time.perf_counter() - t
+        log(f"{'warmup' if k < args.warmup else 'step'}: {st}")
+        if k >= args.warmup:
+            times.append(st["solve_ms"] / 1e3)
+            stats = st
+    value = statistics.median(times)
+    U = u.reshape(-1, 3)
+    tip = U[mesh.X[:, 0] > 10.0 - 1e-9].mean(axis=0)
+    t_apply = ctx.time_apply(op="tangent", precision=1, reps=20)
+    cpu = None
+    if not args.skip_cpu:
+        cpu = nl_cpu_baseline(mesh, u, stats["newton_iterations"])
+    out = {
+        "metric": "8.1M-DOF neo-Hookean Newton solve time @res 1e-8 (5 load steps)", "value": value, "unit": "s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": value * 1e3,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32-inner/f64-outer",
+        "data": "synthetic (seeded box mesh)",
+        "config": {"workload": f"neo-Hookean Tet10 cantilever {nx}x{ny}x{nz} hexes x6, {inf['n_dofs']} DOFs, "
+                               f"mu 80.77e3, lambda 121.15e3, end traction {NL_TRACTION}", "n_dofs": inf["n_dofs"],
+                   "options": opts, "parallelism": "single GPU"},
+        "newton_iterations": stats["newton_iterations"], "linear_iterations": stats["linear_iterations"],
+        "learned_fallbacks": stats["fallbacks"], "relres": stats["relres"], "converged": stats["converged"],
+        "tip_displacement": tip.tolist(), "tangent_apply_fp32_ms": t_apply[0],
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def nl_cpu_baseline(mesh, u, newton_iterations, sample_el=20000):
+    """Oracle TL residual + tangent for a bounded sample of elements, scaled to
+    the whole mesh and to the Newton iterations the GPU took.  A lower bound of
+    the CPU path: it leaves out the tangent solves themselves."""
+    import numpy as np
+    from oracle import oracle as O
+    from jfem.mesh import Mesh
+    k = min(sample_el, mesh.n_el)
+    sub = Mesh(X=mesh.X, conn=mesh.conn[:k], etype=mesh.etype, E=mesh.E[:k], nu=mesh.nu[:k],
+               faces=mesh.faces[:0], ftype=mesh.ftype, clamp_nodes=mesh.clamp_nodes)
+    t0 = time.perf_counter()
+    O.nh_assemble(sub, u, tangent=True)
+    dt = time.perf_counter() - t0
+    value = dt * mesh.n_el / k * newton_iterations
+    log(f"cpu oracle: TL residual+tangent of {k} elements {dt:.2f}s -> {value:.1f}s for {newton_iterations} Newton iterations")
+    return {"value": value, "unit": "s", "cores": 1, "kind": "port",
+            "sample": f"oracle TL residual and tangent (COO) of {k} of {mesh.n_el} elements ({dt:.2f} s, one core), "
+                      f"scaled to the mesh and to {newton_iterations} Newton iterations; tangent solves not included"}
 
 
 def copy_calibration(ctx, n_dofs):

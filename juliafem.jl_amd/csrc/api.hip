@@ -1164,6 +1164,10 @@ extern "C" int jfm_solve_nonlinear(jfm_ctx *ctx, const jfm_newton_opts *opts, do
     hipStream_t s = ctx->stream;
     const int check = std::max(2, opts->check_every > 0 ? opts->check_every : 32);
     const double eta_max = opts->eta_max > 0.0 && opts->eta_max < 1.0 ? opts->eta_max : 0.1;
+    static const bool verbose = [] {
+        const char *v = std::getenv("JFEM_VERBOSE");
+        return v && v[0] == '1';
+    }();
     jfm_newton_stats st{};
     ctx->hist_it.clear();
     ctx->hist_res.clear();
@@ -1217,7 +1221,10 @@ extern "C" int jfm_solve_nonlinear(jfm_ctx *ctx, const jfm_newton_opts *opts, do
             ctx->ws64.ok = ctx->ws32.ok = true;
             ctx->minv_tangent = true;
             const bool coarse = opts->coarse_nodes > 0;
-            if (coarse)  // coarse level of the tangent at this state
+            // coarse level of the tangent, rebuilt at the first iterate of each
+            // load step; later iterates keep it (Q A_c^-1 Q' stays SPD, only
+            // less exact as K_T moves)
+            if (coarse && it == 0)
                 if (int rc = jfem_coarse_setup(ctx, JFM_OP_TANGENT, opts->coarse_nodes, nullptr)) return rc;
             const double *M64 = (const double *)ctx->ws64.Minv;
             const float *M32 = (const float *)ctx->ws32.Minv;
@@ -1257,6 +1264,9 @@ extern "C" int jfm_solve_nonlinear(jfm_ctx *ctx, const jfm_newton_opts *opts, do
             if (int rc = tangent_solve(ctx, opts, M64, M32, eta, rn, check, coarse, &lits)) return rc;
             st.linear_iterations += lits;
             ++st.newton_iterations;
+            if (verbose)
+                std::fprintf(stderr, "[jfem] load step %d/%d, Newton %d: relres %.3e, eta %.2e, %d PCG iterations\n",
+                             step, opts->load_steps, it + 1, relres, eta, lits);
             // u += du; halve the step while some Gauss point leaves det F > 0
             JFM_HIP(ctx, hipMemcpyAsync(ctx->rr64, ctx->nl_u, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
             double rnew = 0.0;

@@ -109,7 +109,7 @@ __global__ __launch_bounds__(256) void k_mf_elem(int n_el, const int32_t *__rest
     using EL = Elem<NPE>;
     constexpr int NG = EL::NGP;
     if (done && *done) return;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int e = xcd_block() * blockDim.x + threadIdx.x;
     if (e >= n_el) return;
     T H[NG][9];
 #pragma unroll
@@ -197,7 +197,7 @@ __global__ __launch_bounds__(256) void k_ebe_elem(int n_el, const int32_t *__res
     using EL = Elem<NPE>;
     constexpr PairTable<NPE> PT = make_pairs<NPE>();
     if (done && *done) return;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int e = xcd_block() * blockDim.x + threadIdx.x;
     if (e >= n_el) return;
     // K_e annihilates translations: subtracting node 0's value keeps FP32 accurate
     T u[NPE][3], f[NPE][3], t[3];
@@ -614,11 +614,11 @@ template <int NPE, typename T>
 static void launch_elem_phase(jfm_ctx *ctx, int op, const T *x, T *ye, const int *done) {
     const int n = (int)ctx->n_el;
     if (op == JFM_OP_EBE) {
-        k_ebe_elem<NPE, T><<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->conn, ctx->inc_slot, ctx->Ke, x, ye, done);
+        k_ebe_elem<NPE, T><<<xcd_grid(nblk(n)), kBlock, 0, ctx->stream>>>(n, ctx->conn, ctx->inc_slot, ctx->Ke, x, ye, done);
     } else if (ctx->affine) {
-        k_mf_elem<NPE, T, true><<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->conn, ctx->inc_slot, gf_of<T>(ctx), x, ye, done);
+        k_mf_elem<NPE, T, true><<<xcd_grid(nblk(n)), kBlock, 0, ctx->stream>>>(n, ctx->conn, ctx->inc_slot, gf_of<T>(ctx), x, ye, done);
     } else {
-        k_mf_elem<NPE, T, false><<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->conn, ctx->inc_slot, gf_of<T>(ctx), x, ye, done);
+        k_mf_elem<NPE, T, false><<<xcd_grid(nblk(n)), kBlock, 0, ctx->stream>>>(n, ctx->conn, ctx->inc_slot, gf_of<T>(ctx), x, ye, done);
     }
 }
 

@@ -84,6 +84,18 @@ template <> __device__ __forceinline__ double dn_rt<8>(int q, int i, int c) { re
 
 // Grid size for a grid-stride producer kernel: enough blocks for `work`
 // items at `per_block` items per block, capped at kMaxPart.
+// Workgroups are dispatched to the 8 XCDs round-robin (block b runs on XCD
+// b % 8), and each XCD has its own L2.  xcd_block() renumbers the blocks of a
+// grid whose size is a multiple of 8 (xcd_grid) so that every XCD works on one
+// contiguous eighth of the logical blocks: neighbouring elements or rows, and
+// the parts of x they read, then share an L2.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block() {
+    const int q = (int)gridDim.x / kXcds;
+    return ((int)blockIdx.x % kXcds) * q + (int)blockIdx.x / kXcds;
+}
+inline unsigned xcd_grid(unsigned nb) { return (nb + kXcds - 1) / kXcds * kXcds; }
+
 inline unsigned capped_grid(int64_t work, int per_block) {
     int64_t g = (work + per_block - 1) / per_block;
     if (g < 1) g = 1;

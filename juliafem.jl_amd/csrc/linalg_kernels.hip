@@ -28,6 +28,11 @@ static inline unsigned nblk(int64_t n, int b = kBlock) { return (unsigned)((n + 
 #ifndef JFEM_SPMV_NT
 #define JFEM_SPMV_NT 0
 #endif
+// block-row order: 0 grid-stride, 1 one contiguous run per block (XCD-grouped),
+// 2 grid-stride sweeps with one contiguous window per XCD in each sweep
+#ifndef JFEM_SPMV_MAP
+#define JFEM_SPMV_MAP 0
+#endif
 template <typename T> __device__ __forceinline__ T ldnt(const T *p) {
 #if JFEM_SPMV_NT
     return __builtin_nontemporal_load(p);
@@ -47,10 +52,20 @@ __global__ __launch_bounds__(256) void k_bsr_spmv(int N, const int32_t *__restri
     constexpr int RPB = kBlock / G;  // rows per block sweep
     const int lane = threadIdx.x % G;
     double dv = 0.0;
-    for (int a0 = blockIdx.x * RPB; a0 < N; a0 += gridDim.x * RPB) {
+#if JFEM_SPMV_MAP == 1
+    // one contiguous run of block rows per block (XCD-grouped)
+    const int per = ((N + (int)gridDim.x - 1) / (int)gridDim.x + RPB - 1) / RPB * RPB;
+    const int begin = xcd_block() * per, end = min(N, begin + per), step = RPB;
+#elif JFEM_SPMV_MAP == 2
+    // grid-stride sweeps; within a sweep each XCD takes one contiguous window
+    const int begin = xcd_block() * RPB, end = N, step = (int)gridDim.x * RPB;
+#else
+    const int begin = (int)blockIdx.x * RPB, end = N, step = (int)gridDim.x * RPB;
+#endif
+    for (int a0 = begin; a0 < end; a0 += step) {
         const int a = a0 + threadIdx.x / G;
         T acc0 = T(0), acc1 = T(0), acc2 = T(0);
-        if (a < N) {
+        if (a < end) {
             const int r0 = brow[a], nb = brow[a + 1] - r0;
             const T *v = vals + 9 * (size_t)r0;
             const size_t rs = 3 * (size_t)nb;
@@ -71,7 +86,7 @@ __global__ __launch_bounds__(256) void k_bsr_spmv(int N, const int32_t *__restri
             acc1 += __shfl_xor(acc1, off, G);
             acc2 += __shfl_xor(acc2, off, G);
         }
-        if (a < N && lane < 3) {
+        if (a < end && lane < 3) {
             const size_t row = 3 * (size_t)a + lane;
             T val = lane == 0 ? acc0 : (lane == 1 ? acc1 : acc2);
             if (fixed && fixed[row]) val = T(0);
@@ -260,8 +275,11 @@ __global__ __launch_bounds__(256) void k_dot(const T *__restrict__ a, const T *_
 }
 
 // ------------------------------------------------------------- launchers ----
+// 16 lanes per block row measured fastest at 87M values (0.164 ms FP64
+// against 0.182 with 8 and 0.218 with 4); the grid-stride row order beat the
+// XCD-grouped orders (0.187, 0.194).
 #ifndef JFEM_SPMV_LANES
-#define JFEM_SPMV_LANES 8
+#define JFEM_SPMV_LANES 16
 #endif
 constexpr int kSpmvLanes = JFEM_SPMV_LANES;
 
@@ -269,7 +287,7 @@ template <typename T>
 hipError_t launch_bsr_spmv(jfm_ctx *ctx, const T *vals, const T *x, T *y, const T *dotv, double *part,
                            const int *done, int *npart) {
     const int N = (int)ctx->N;
-    const unsigned g = capped_grid((int64_t)N * kSpmvLanes, kBlock);
+    const unsigned g = xcd_grid(capped_grid((int64_t)N * kSpmvLanes, kBlock));
     if (npart) *npart = (int)g;
     k_bsr_spmv<T, kSpmvLanes><<<g, kBlock, 0, ctx->stream>>>(N, ctx->brow, ctx->bcol, vals, x, ctx->fixed, y,
                                                             dotv, part, done);

@@ -121,7 +121,7 @@ __global__ __launch_bounds__(256) void k_nh_elem(int n_el, const int32_t *__rest
     using EL = Elem<NPE>;
     constexpr int NG = EL::NGP;
     if (done && *done) return;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int e = xcd_block() * blockDim.x + threadIdx.x;
     if (e >= n_el) return;
     const int n0 = conn[e];
     const T u0[3] = {u[3 * (size_t)n0], u[3 * (size_t)n0 + 1], u[3 * (size_t)n0 + 2]};
@@ -294,9 +294,9 @@ template <int NPE, typename T, bool TANGENT>
 static void nh_elem(jfm_ctx *ctx, const T *gf, const T *u, const T *v, T *ye, const int *done) {
     const int n = (int)ctx->n_el;
     if (ctx->affine)
-        k_nh_elem<NPE, T, true, TANGENT><<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->conn, ctx->inc_slot, gf, u, v, ye, done);
+        k_nh_elem<NPE, T, true, TANGENT><<<xcd_grid(nblk(n)), kBlock, 0, ctx->stream>>>(n, ctx->conn, ctx->inc_slot, gf, u, v, ye, done);
     else
-        k_nh_elem<NPE, T, false, TANGENT><<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->conn, ctx->inc_slot, gf, u, v, ye, done);
+        k_nh_elem<NPE, T, false, TANGENT><<<xcd_grid(nblk(n)), kBlock, 0, ctx->stream>>>(n, ctx->conn, ctx->inc_slot, gf, u, v, ye, done);
 }
 
 template <typename T>

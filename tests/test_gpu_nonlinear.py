@@ -114,7 +114,10 @@ def test_small_load_newton_is_linear_solve(oracle):
     c = ctx_for(m)
     u, st = c.solve_nonlinear(load_steps=1, rtol=1e-9)
     ul, _, _ = c.solve(method=0, op="matfree", rtol=1e-10)
-    assert st["converged"] == 1 and st["newton_iterations"] <= 3
+    # inexact Newton: each tangent solve stops at the forcing tolerance
+    # (0.1 first, then Eisenstat-Walker), so a nearly linear step still takes
+    # a few iterates
+    assert st["converged"] == 1 and st["newton_iterations"] <= 6
     # geometric and material nonlinearity at strain ~1e-5 stay below 1e-4
     assert np.linalg.norm(u - ul) <= 1e-4 * np.linalg.norm(ul)
     c.close()
