@@ -158,7 +158,8 @@ int jfm_apply(jfm_ctx *ctx, int32_t op, int32_t precision, const double *x, doub
 /* Average device times (ms) over `reps` back-to-back applications on
  * device-resident vectors, timed with HIP events on the context's stream:
  * avg_ms[0] = whole K.u, avg_ms[1] = its first kernel (SpMV, or the element
- * phase of EBE/MATFREE), avg_ms[2] = the nodal gather (0 for CSR). */
+ * phase of EBE/MATFREE), avg_ms[2] = the nodal gather (0 for CSR); each as
+ * reps back-to-back launches between two events. */
 int jfm_time_apply(jfm_ctx *ctx, int32_t op, int32_t precision, int32_t reps, double *avg_ms);
 
 /* ---- solve (solve!(solver), src/solvers.jl:192-216, 273-344; and

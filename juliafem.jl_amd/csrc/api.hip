@@ -580,55 +580,41 @@ extern "C" int jfm_time_apply(jfm_ctx *ctx, int32_t op, int32_t precision, int32
     if (precision != JFM_FP64 && precision != JFM_FP32) return jfm_fail(ctx, JFM_EINVAL, "precision must be FP64 or FP32");
     JFM_HIP(ctx, hipSetDevice(ctx->device));
     if (int rc = prepare_op(ctx, op)) return rc;
-    std::vector<hipEvent_t> ev(3 * (size_t)reps);
-    for (auto &e : ev) JFM_HIP(ctx, hipEventCreate(&e));
     hipEvent_t t0, t1;
     JFM_HIP(ctx, hipEventCreate(&t0));
     JFM_HIP(ctx, hipEventCreate(&t1));
     const bool two = op != JFM_OP_CSR;
     const bool f64 = precision == JFM_FP64;
-    auto run = [&](int r, bool timed) -> hipError_t {
-        hipError_t e = hipSuccess;
-        if (timed) (void)hipEventRecord(ev[3 * r], ctx->stream);
+    // part 0: whole product, 1: SpMV or element phase, 2: gather
+    auto run = [&](int part) -> hipError_t {
         if (!two) {
-            e = f64 ? launch_bsr_spmv<double>(ctx, ctx->vals64, ctx->vx64, ctx->vy64, nullptr, nullptr, nullptr, nullptr)
-                    : launch_bsr_spmv<float>(ctx, ctx->vals32, ctx->vx32, ctx->vy32, nullptr, nullptr, nullptr, nullptr);
-            if (timed) (void)hipEventRecord(ev[3 * r + 1], ctx->stream);
-        } else {
-            e = f64 ? launch_elem_only<double>(ctx, op, ctx->vx64, nullptr) : launch_elem_only<float>(ctx, op, ctx->vx32, nullptr);
-            if (e != hipSuccess) return e;
-            if (timed) (void)hipEventRecord(ev[3 * r + 1], ctx->stream);
-            e = f64 ? launch_gather<double>(ctx, ctx->ye64, ctx->fixed, ctx->vy64, nullptr, nullptr, nullptr, nullptr)
-                    : launch_gather<float>(ctx, ctx->ye32, ctx->fixed, ctx->vy32, nullptr, nullptr, nullptr, nullptr);
+            return f64 ? launch_bsr_spmv<double>(ctx, ctx->vals64, ctx->vx64, ctx->vy64, nullptr, nullptr, nullptr, nullptr)
+                       : launch_bsr_spmv<float>(ctx, ctx->vals32, ctx->vx32, ctx->vy32, nullptr, nullptr, nullptr, nullptr);
         }
-        if (timed) (void)hipEventRecord(ev[3 * r + 2], ctx->stream);
-        return e;
+        hipError_t e = hipSuccess;
+        if (part != 2) e = f64 ? launch_elem_only<double>(ctx, op, ctx->vx64, nullptr) : launch_elem_only<float>(ctx, op, ctx->vx32, nullptr);
+        if (e != hipSuccess || part == 1) return e;
+        return f64 ? launch_gather<double>(ctx, ctx->ye64, ctx->fixed, ctx->vy64, nullptr, nullptr, nullptr, nullptr)
+                   : launch_gather<float>(ctx, ctx->ye32, ctx->fixed, ctx->vy32, nullptr, nullptr, nullptr, nullptr);
     };
-    CHECK_LAUNCH(ctx, run(0, false));  // warm
-    JFM_HIP(ctx, hipEventRecord(t0, ctx->stream));
-    for (int r = 0; r < reps; ++r) CHECK_LAUNCH(ctx, run(r, true));
-    JFM_HIP(ctx, hipEventRecord(t1, ctx->stream));
-    JFM_HIP(ctx, hipEventSynchronize(t1));
-    double p1 = 0, p2 = 0;
-    for (int r = 0; r < reps; ++r) {
-        float a = 0.f, b = 0.f;
-        JFM_HIP(ctx, hipEventElapsedTime(&a, ev[3 * r], ev[3 * r + 1]));
-        JFM_HIP(ctx, hipEventElapsedTime(&b, ev[3 * r + 1], ev[3 * r + 2]));
-        p1 += a;
-        p2 += b;
+    // each part as `reps` back-to-back launches between two events, so the
+    // averages are kernel durations plus launch gaps, without per-launch events
+    for (int part = 0; part < (two ? 3 : 2); ++part) {
+        CHECK_LAUNCH(ctx, run(part));  // warm
+        JFM_HIP(ctx, hipEventRecord(t0, ctx->stream));
+        for (int r = 0; r < reps; ++r) CHECK_LAUNCH(ctx, run(part));
+        JFM_HIP(ctx, hipEventRecord(t1, ctx->stream));
+        JFM_HIP(ctx, hipEventSynchronize(t1));
+        float ms = 0.f;
+        JFM_HIP(ctx, hipEventElapsedTime(&ms, t0, t1));
+        avg_ms[part] = ms / reps;
     }
-    float tot = 0.f;
-    JFM_HIP(ctx, hipEventElapsedTime(&tot, t0, t1));
-    avg_ms[0] = tot / reps;
-    avg_ms[1] = p1 / reps;
-    avg_ms[2] = two ? p2 / reps : 0.0;
-    for (auto &e : ev) (void)hipEventDestroy(e);
+    if (!two) avg_ms[2] = 0.0;
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
     return JFM_OK;
 }
 
-// =============================================================== assembly ==
 extern "C" int jfm_assemble(jfm_ctx *ctx, const double *u, double *vals, double *f) {
     if (!ctx) return JFM_EINVAL;
     if (!ctx->has_mesh) return jfm_fail(ctx, JFM_ESTATE, "no mesh");
