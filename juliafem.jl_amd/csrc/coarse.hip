@@ -114,6 +114,24 @@ __global__ __launch_bounds__(256) void k_coarse_probe_restrict(int n_agg, const 
     }
 }
 
+// zero the incidence slots of the listed elements and the listed rows of y
+__global__ __launch_bounds__(256) void k_coarse_clear(int ne, const int32_t *__restrict__ elist, int npe, int n_el,
+                                                     const int32_t *__restrict__ slot, double *__restrict__ ye, int nn,
+                                                     const int32_t *__restrict__ nlist, double *__restrict__ y) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < ne) {
+        const int e = elist[k];
+        for (int i = 0; i < npe; ++i) {
+            double *o = ye + 3 * (size_t)slot[(size_t)i * n_el + e];
+            o[0] = 0.0; o[1] = 0.0; o[2] = 0.0;
+        }
+    }
+    if (k < nn) {
+        const size_t o = 3 * (size_t)nlist[k];
+        y[o] = 0.0; y[o + 1] = 0.0; y[o + 2] = 0.0;
+    }
+}
+
 // symmetrise A_c; unit diagonal for dropped (all-zero) modes
 __global__ __launch_bounds__(256) void k_coarse_fix(int nc, double *__restrict__ Ac) {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -220,6 +238,8 @@ struct Aggregation {
     int ncolor = 0;
     std::vector<double> cd, Rinv;   // N x 3, n_agg x 36
     std::vector<int32_t> dropped;   // nc
+    // per colour: the elements touching its aggregates and those elements' nodes
+    std::vector<int32_t> ecol_ptr, ecol, ncol_ptr, ncol;
 };
 
 // grid binning of the coordinates, about `target` nodes per bin
@@ -409,6 +429,43 @@ static int coarse_host_setup(jfm_ctx *ctx, int coarse_nodes, const std::vector<u
             slot = h;
         }
     }
+    // probing touches, for colour c, only the elements with a node in a
+    // colour-c aggregate and those elements' nodes
+    const int64_t n_el = ctx->n_el;
+    const int npe = ctx->npe;
+    std::vector<std::vector<int32_t>> el(ncolor), nd(ncolor);
+    std::vector<int32_t> mark_e(ncolor, -1);
+    for (int64_t e = 0; e < n_el; ++e) {
+        for (int i = 0; i < npe; ++i) {
+            const int c = A.color[agg[ctx->h_conn[e * npe + i]]];
+            if (mark_e[c] != (int32_t)e) {
+                mark_e[c] = (int32_t)e;
+                el[c].push_back((int32_t)e);
+            }
+        }
+    }
+    std::vector<int32_t> seen(N, -1);
+    A.ecol_ptr.assign(ncolor + 1, 0);
+    A.ncol_ptr.assign(ncolor + 1, 0);
+    for (int c = 0; c < ncolor; ++c) {
+        for (int32_t e : el[c])
+            for (int i = 0; i < npe; ++i) {
+                const int32_t a = ctx->h_conn[(int64_t)e * npe + i];
+                if (seen[a] != c) {
+                    seen[a] = c;
+                    nd[c].push_back(a);
+                }
+            }
+        std::sort(nd[c].begin(), nd[c].end());
+        A.ecol_ptr[c + 1] = A.ecol_ptr[c] + (int32_t)el[c].size();
+        A.ncol_ptr[c + 1] = A.ncol_ptr[c] + (int32_t)nd[c].size();
+    }
+    A.ecol.reserve(A.ecol_ptr[ncolor]);
+    A.ncol.reserve(A.ncol_ptr[ncolor]);
+    for (int c = 0; c < ncolor; ++c) {
+        A.ecol.insert(A.ecol.end(), el[c].begin(), el[c].end());
+        A.ncol.insert(A.ncol.end(), nd[c].begin(), nd[c].end());
+    }
     return JFM_OK;
 }
 
@@ -422,7 +479,8 @@ void jfem_coarse_release(jfm_ctx *ctx) {
     free_dev(ctx, ctx->agg_color); free_dev(ctx, ctx->partner); free_dev(ctx, ctx->cd64); free_dev(ctx, ctx->cd32);
     free_dev(ctx, ctx->Rinv); free_dev(ctx, ctx->Ac); free_dev(ctx, ctx->Ahat32);
     free_dev(ctx, ctx->wc32); free_dev(ctx, ctx->zc32);
-    free_dev(ctx, ctx->dropped);
+    free_dev(ctx, ctx->dropped); free_dev(ctx, ctx->ecol); free_dev(ctx, ctx->ncol);
+    ctx->h_ecol_ptr.clear(); ctx->h_ncol_ptr.clear();
     ctx->coarse_ready = false;
     ctx->coarse_tag = -1;
     ctx->coarse_built_nodes = 0;
@@ -471,6 +529,8 @@ int jfem_coarse_setup(jfm_ctx *ctx, int op, int coarse_nodes, double *ms) {
         CALLOC(ctx->cd32, float, 3 * N);
         CALLOC(ctx->Rinv, double, 36 * (int64_t)A.n_agg);
         CALLOC(ctx->dropped, int32_t, nc);
+        CALLOC(ctx->ecol, int32_t, std::max<int64_t>(1, (int64_t)A.ecol.size()));
+        CALLOC(ctx->ncol, int32_t, std::max<int64_t>(1, (int64_t)A.ncol.size()));
         CALLOC(ctx->Ac, double, nc * nc);
         CALLOC(ctx->Ahat32, float, nc * nc);
         CALLOC(ctx->wc32, float, nc);
@@ -485,6 +545,10 @@ int jfem_coarse_setup(jfm_ctx *ctx, int op, int coarse_nodes, double *ms) {
         JFM_HIP(ctx, hipMemcpyAsync(ctx->cd32, cd32.data(), 4 * 3 * N, hipMemcpyHostToDevice, s));
         JFM_HIP(ctx, hipMemcpyAsync(ctx->Rinv, A.Rinv.data(), 8 * A.Rinv.size(), hipMemcpyHostToDevice, s));
         JFM_HIP(ctx, hipMemcpyAsync(ctx->dropped, A.dropped.data(), 4 * nc, hipMemcpyHostToDevice, s));
+        JFM_HIP(ctx, hipMemcpyAsync(ctx->ecol, A.ecol.data(), 4 * A.ecol.size(), hipMemcpyHostToDevice, s));
+        JFM_HIP(ctx, hipMemcpyAsync(ctx->ncol, A.ncol.data(), 4 * A.ncol.size(), hipMemcpyHostToDevice, s));
+        ctx->h_ecol_ptr = A.ecol_ptr;
+        ctx->h_ncol_ptr = A.ncol_ptr;
         JFM_HIP(ctx, hipStreamSynchronize(s));
         ctx->coarse_built_nodes = coarse_nodes;
         ++ctx->coarse_gen;
@@ -494,16 +558,31 @@ int jfem_coarse_setup(jfm_ctx *ctx, int op, int coarse_nodes, double *ms) {
     double *pv = (double *)ctx->ws64.p, *qv = (double *)ctx->ws64.q;
     if (!pv || !qv) return jfm_fail(ctx, JFM_ESTATE, "coarse setup needs the solver workspaces");
     JFM_HIP(ctx, hipMemsetAsync(ctx->Ac, 0, sizeof(double) * (size_t)nc * nc, s));
-    for (int c = 0; c < ctx->ncolor; ++c)
+    // The products touch only the elements next to colour c and their nodes:
+    // every other incidence slot and row of qv is kept at zero (cleared once
+    // here, and after each product for the rows and slots it wrote).
+    const bool tangent = op == JFM_OP_TANGENT;
+    if (tangent && !ctx->has_state) return jfm_fail(ctx, JFM_ESTATE, "tangent coarse level needs a state");
+    JFM_HIP(ctx, hipMemsetAsync(ctx->ye64, 0, sizeof(double) * 3 * (size_t)ctx->npe * ctx->n_el, s));
+    JFM_HIP(ctx, hipMemsetAsync(qv, 0, sizeof(double) * 3 * (size_t)N, s));
+    for (int c = 0; c < ctx->ncolor; ++c) {
+        const int e0 = ctx->h_ecol_ptr[c], ne = ctx->h_ecol_ptr[c + 1] - e0;
+        const int n0 = ctx->h_ncol_ptr[c], nn = ctx->h_ncol_ptr[c + 1] - n0;
         for (int j = 0; j < 6; ++j) {
             k_coarse_probe<<<nblk(N), kBlock, 0, s>>>((int)N, ctx->node_agg, ctx->agg_color, c, j, ctx->cd64, ctx->Rinv,
                                                       ctx->fixed, pv);
             JFM_HIP(ctx, hipGetLastError());
-            if (int rc = jfem_apply(ctx, op, pv, qv)) return rc;
+            JFM_HIP(ctx, launch_subset_apply(ctx, tangent, pv, qv, ctx->ecol + e0, ne, ctx->ncol + n0, nn));
             k_coarse_probe_restrict<<<n_agg, kBlock, 0, s>>>(n_agg, ctx->agg_ptr, ctx->agg_nodes, ctx->cd64, qv,
                                                              ctx->partner + (size_t)c * n_agg, ctx->Rinv, j, ctx->Ac, nc);
             JFM_HIP(ctx, hipGetLastError());
+            if (std::max(ne, nn) > 0) {
+                k_coarse_clear<<<nblk(std::max(ne, nn)), kBlock, 0, s>>>(ne, ctx->ecol + e0, ctx->npe, (int)ctx->n_el,
+                                                                          ctx->inc_slot, ctx->ye64, nn, ctx->ncol + n0, qv);
+                JFM_HIP(ctx, hipGetLastError());
+            }
         }
+    }
     const int64_t nn = (int64_t)nc * nc;
     k_coarse_fix<<<nblk(nn), kBlock, 0, s>>>(nc, ctx->Ac);
     JFM_HIP(ctx, hipGetLastError());

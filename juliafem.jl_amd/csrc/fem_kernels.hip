@@ -105,12 +105,14 @@ template <int NPE, typename T, bool AFF>
 __global__ __launch_bounds__(256) void k_mf_elem(int n_el, const int32_t *__restrict__ conn,
                                                 const int32_t *__restrict__ slot, const T *__restrict__ gf,
                                                 const T *__restrict__ x, T *__restrict__ ye,
-                                                const int *__restrict__ done) {
+                                                const int *__restrict__ done, const int32_t *__restrict__ elist,
+                                                int n_items) {
     using EL = Elem<NPE>;
     constexpr int NG = EL::NGP;
     if (done && *done) return;
-    const int e = xcd_block() * blockDim.x + threadIdx.x;
-    if (e >= n_el) return;
+    const int idx = xcd_block() * blockDim.x + threadIdx.x;
+    if (idx >= n_items) return;
+    const int e = elist ? elist[idx] : idx;   // optional element subset
     T H[NG][9];
 #pragma unroll
     for (int q = 0; q < NG; ++q)
@@ -241,10 +243,11 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_gather(int N, const int32_t *__restrict__ ptr, const T *__restrict__ yinc,
                                                const uint8_t *__restrict__ fixed, T *__restrict__ y,
                                                const T *__restrict__ dotv, double *__restrict__ part,
-                                               const int *__restrict__ done) {
+                                               const int *__restrict__ done, const int32_t *__restrict__ nlist) {
     if (done && *done) return;
     double dv = 0.0;
-    for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < N; a += gridDim.x * blockDim.x) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < N; k += gridDim.x * blockDim.x) {
+        const int a = nlist ? nlist[k] : k;   // optional node subset (N items)
         const int k0 = ptr[a], k1 = ptr[a + 1];
         T s0 = T(0), s1 = T(0), s2 = T(0);
         const T *v = yinc + 3 * (size_t)k0;
@@ -599,7 +602,42 @@ hipError_t launch_gather(jfm_ctx *ctx, const T *ye, const uint8_t *fixed, T *y, 
                          const int *done, int *npart) {
     const unsigned g = capped_grid(ctx->N, kBlock);
     if (npart) *npart = (int)g;
-    k_gather<T><<<g, kBlock, 0, ctx->stream>>>((int)ctx->N, ctx->inc_ptr, ye, fixed, y, dotv, part, done);
+    k_gather<T><<<g, kBlock, 0, ctx->stream>>>((int)ctx->N, ctx->inc_ptr, ye, fixed, y, dotv, part, done, nullptr);
+    return hipGetLastError();
+}
+
+// y = K x (masked, FP64) restricted to the elements `elist` and the nodes
+// `nlist`: only those element slots are written and only those rows of y.
+// Every other incidence slot must hold zero (the caller keeps it so).
+template <int NPE>
+static void subset_elem(jfm_ctx *ctx, const double *x, const int32_t *elist, int ne) {
+    const int n = (int)ctx->n_el;
+    if (ctx->affine)
+        k_mf_elem<NPE, double, true><<<xcd_grid(nblk(ne)), kBlock, 0, ctx->stream>>>(n, ctx->conn, ctx->inc_slot, ctx->gf64, x,
+                                                                                    ctx->ye64, nullptr, elist, ne);
+    else
+        k_mf_elem<NPE, double, false><<<xcd_grid(nblk(ne)), kBlock, 0, ctx->stream>>>(n, ctx->conn, ctx->inc_slot, ctx->gf64, x,
+                                                                                     ctx->ye64, nullptr, elist, ne);
+}
+
+hipError_t launch_subset_apply(jfm_ctx *ctx, bool tangent, const double *x, double *y, const int32_t *elist, int ne,
+                               const int32_t *nlist, int nn) {
+    if (ne > 0) {
+        if (tangent) {
+            hipError_t e = launch_nh_elem<double>(ctx, true, ctx->gf64, ctx->ustate64, x, ctx->ye64, nullptr, elist, ne);
+            if (e != hipSuccess) return e;
+        } else {
+            switch (ctx->npe) {
+            case 10: subset_elem<10>(ctx, x, elist, ne); break;
+            case 4: subset_elem<4>(ctx, x, elist, ne); break;
+            case 8: subset_elem<8>(ctx, x, elist, ne); break;
+            default: return hipErrorInvalidValue;
+            }
+        }
+    }
+    if (nn > 0)
+        k_gather<double><<<capped_grid(nn, kBlock), kBlock, 0, ctx->stream>>>(nn, ctx->inc_ptr, ctx->ye64, ctx->fixed, y,
+                                                                             nullptr, nullptr, nullptr, nlist);
     return hipGetLastError();
 }
 
@@ -616,9 +654,11 @@ static void launch_elem_phase(jfm_ctx *ctx, int op, const T *x, T *ye, const int
     if (op == JFM_OP_EBE) {
         k_ebe_elem<NPE, T><<<xcd_grid(nblk(n)), kBlock, 0, ctx->stream>>>(n, ctx->conn, ctx->inc_slot, ctx->Ke, x, ye, done);
     } else if (ctx->affine) {
-        k_mf_elem<NPE, T, true><<<xcd_grid(nblk(n)), kBlock, 0, ctx->stream>>>(n, ctx->conn, ctx->inc_slot, gf_of<T>(ctx), x, ye, done);
+        k_mf_elem<NPE, T, true><<<xcd_grid(nblk(n)), kBlock, 0, ctx->stream>>>(n, ctx->conn, ctx->inc_slot, gf_of<T>(ctx), x, ye, done,
+                                                                              nullptr, n);
     } else {
-        k_mf_elem<NPE, T, false><<<xcd_grid(nblk(n)), kBlock, 0, ctx->stream>>>(n, ctx->conn, ctx->inc_slot, gf_of<T>(ctx), x, ye, done);
+        k_mf_elem<NPE, T, false><<<xcd_grid(nblk(n)), kBlock, 0, ctx->stream>>>(n, ctx->conn, ctx->inc_slot, gf_of<T>(ctx), x, ye, done,
+                                                                               nullptr, n);
     }
 }
 

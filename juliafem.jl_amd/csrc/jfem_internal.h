@@ -218,6 +218,8 @@ struct jfm_ctx {
     int32_t *node_agg = nullptr, *agg_ptr = nullptr, *agg_nodes = nullptr, *agg_color = nullptr;
     int32_t *partner = nullptr;    // ncolor x n_agg: the coupled aggregate of each colour
     int32_t *dropped = nullptr;    // nc: mode dropped (dependent on its aggregate)
+    int32_t *ecol = nullptr, *ncol = nullptr;        // per colour: probed elements, nodes
+    std::vector<int32_t> h_ecol_ptr, h_ncol_ptr;
     double *cd64 = nullptr;        // N x 3 offsets from the aggregate centroid
     float *cd32 = nullptr;
     double *Rinv = nullptr;        // n_agg x 36
@@ -269,7 +271,11 @@ hipError_t launch_two_phase(jfm_ctx *ctx, int op, const T *x, T *y, const T *dot
 // nonlinear.hip (neo-Hookean): element phase of f_int(u) (tangent = false)
 // or K_T(u) v (tangent = true) into the incidence slots
 template <typename T>
-hipError_t launch_nh_elem(jfm_ctx *ctx, bool tangent, const T *gf, const T *u, const T *v, T *ye, const int *done);
+hipError_t launch_nh_elem(jfm_ctx *ctx, bool tangent, const T *gf, const T *u, const T *v, T *ye, const int *done,
+                          const int32_t *elist = nullptr, int n_items = 0);
+// FP64 masked K x (or K_T x) over an element subset and a node subset only
+hipError_t launch_subset_apply(jfm_ctx *ctx, bool tangent, const double *x, double *y, const int32_t *elist, int ne,
+                               const int32_t *nlist, int nn);
 hipError_t launch_nh_block_jacobi(jfm_ctx *ctx, const double *u, double *M64, float *M32, double *D);
 // coarse.hip
 int jfem_coarse_setup(jfm_ctx *ctx, int op, int coarse_nodes, double *ms);

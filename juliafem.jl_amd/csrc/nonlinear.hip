@@ -117,12 +117,14 @@ template <int NPE, typename T, bool AFF, bool TANGENT>
 __global__ __launch_bounds__(256) void k_nh_elem(int n_el, const int32_t *__restrict__ conn,
                                                 const int32_t *__restrict__ slot, const T *__restrict__ gf,
                                                 const T *__restrict__ u, const T *__restrict__ v, T *__restrict__ ye,
-                                                const int *__restrict__ done) {
+                                                const int *__restrict__ done, const int32_t *__restrict__ elist,
+                                                int n_items) {
     using EL = Elem<NPE>;
     constexpr int NG = EL::NGP;
     if (done && *done) return;
-    const int e = xcd_block() * blockDim.x + threadIdx.x;
-    if (e >= n_el) return;
+    const int idx = xcd_block() * blockDim.x + threadIdx.x;
+    if (idx >= n_items) return;
+    const int e = elist ? elist[idx] : idx;   // optional element subset
     const int n0 = conn[e];
     const T u0[3] = {u[3 * (size_t)n0], u[3 * (size_t)n0 + 1], u[3 * (size_t)n0 + 2]};
     T v0[3] = {T(0), T(0), T(0)};
@@ -291,21 +293,27 @@ __global__ __launch_bounds__(256) void k_nh_block_jacobi(int N, const int32_t *_
 
 // --------------------------------------------------------------- launchers ---
 template <int NPE, typename T, bool TANGENT>
-static void nh_elem(jfm_ctx *ctx, const T *gf, const T *u, const T *v, T *ye, const int *done) {
+static void nh_elem(jfm_ctx *ctx, const T *gf, const T *u, const T *v, T *ye, const int *done, const int32_t *elist,
+                    int m) {
     const int n = (int)ctx->n_el;
     if (ctx->affine)
-        k_nh_elem<NPE, T, true, TANGENT><<<xcd_grid(nblk(n)), kBlock, 0, ctx->stream>>>(n, ctx->conn, ctx->inc_slot, gf, u, v, ye, done);
+        k_nh_elem<NPE, T, true, TANGENT><<<xcd_grid(nblk(m)), kBlock, 0, ctx->stream>>>(n, ctx->conn, ctx->inc_slot, gf, u, v, ye,
+                                                                                        done, elist, m);
     else
-        k_nh_elem<NPE, T, false, TANGENT><<<xcd_grid(nblk(n)), kBlock, 0, ctx->stream>>>(n, ctx->conn, ctx->inc_slot, gf, u, v, ye, done);
+        k_nh_elem<NPE, T, false, TANGENT><<<xcd_grid(nblk(m)), kBlock, 0, ctx->stream>>>(n, ctx->conn, ctx->inc_slot, gf, u, v, ye,
+                                                                                         done, elist, m);
 }
 
 template <typename T>
-hipError_t launch_nh_elem(jfm_ctx *ctx, bool tangent, const T *gf, const T *u, const T *v, T *ye, const int *done) {
+hipError_t launch_nh_elem(jfm_ctx *ctx, bool tangent, const T *gf, const T *u, const T *v, T *ye, const int *done,
+                          const int32_t *elist, int n_items) {
     if (ctx->n_el == 0) return hipSuccess;
+    const int m = elist ? n_items : (int)ctx->n_el;
+    if (m == 0) return hipSuccess;
     switch (ctx->npe) {
-    case 10: tangent ? nh_elem<10, T, true>(ctx, gf, u, v, ye, done) : nh_elem<10, T, false>(ctx, gf, u, v, ye, done); break;
-    case 4: tangent ? nh_elem<4, T, true>(ctx, gf, u, v, ye, done) : nh_elem<4, T, false>(ctx, gf, u, v, ye, done); break;
-    case 8: tangent ? nh_elem<8, T, true>(ctx, gf, u, v, ye, done) : nh_elem<8, T, false>(ctx, gf, u, v, ye, done); break;
+    case 10: tangent ? nh_elem<10, T, true>(ctx, gf, u, v, ye, done, elist, m) : nh_elem<10, T, false>(ctx, gf, u, v, ye, done, elist, m); break;
+    case 4: tangent ? nh_elem<4, T, true>(ctx, gf, u, v, ye, done, elist, m) : nh_elem<4, T, false>(ctx, gf, u, v, ye, done, elist, m); break;
+    case 8: tangent ? nh_elem<8, T, true>(ctx, gf, u, v, ye, done, elist, m) : nh_elem<8, T, false>(ctx, gf, u, v, ye, done, elist, m); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -322,7 +330,9 @@ hipError_t launch_nh_block_jacobi(jfm_ctx *ctx, const double *u, double *M64, fl
     return hipGetLastError();
 }
 
-template hipError_t launch_nh_elem<double>(jfm_ctx *, bool, const double *, const double *, const double *, double *, const int *);
-template hipError_t launch_nh_elem<float>(jfm_ctx *, bool, const float *, const float *, const float *, float *, const int *);
+template hipError_t launch_nh_elem<double>(jfm_ctx *, bool, const double *, const double *, const double *, double *, const int *,
+                                           const int32_t *, int);
+template hipError_t launch_nh_elem<float>(jfm_ctx *, bool, const float *, const float *, const float *, float *, const int *,
+                                          const int32_t *, int);
 
 }  // namespace jfem
