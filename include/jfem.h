@@ -222,6 +222,17 @@ int jfm_nl_residual(jfm_ctx *ctx, const double *u, double load_factor, double *R
  * displacement. */
 int jfm_solve_nonlinear(jfm_ctx *ctx, const jfm_newton_opts *opts, double *u, jfm_newton_stats *stats);
 
+/* ---- postprocessing (postprocess!(problem::Problem{Elasticity}, time, Val{:strain/:stress}),
+ * src/problems_elasticity.jl:520-594) ------------------------------------------
+ * Small-strain strain and linear-elastic stress of displacement u (n_dofs) at
+ * every Gauss point, ABAQUS order 11, 22, 33, 12, 23, 13 with tensor shear:
+ * strain_gp / stress_gp hold n_el x n_gauss x 6 doubles (n_gauss = 4 Tet10,
+ * 1 Tet4, 8 Hex8).  strain_nodal / stress_nodal (n_nodes x 6) receive the
+ * least-squares nodal fits (consistent mass solve, as lsq_fit).  Any output
+ * may be NULL. */
+int jfm_postprocess(jfm_ctx *ctx, const double *u, double *strain_gp, double *stress_gp, double *strain_nodal,
+                    double *stress_nodal);
+
 /* ---- learned preconditioner (no reference counterpart) ---------------
  * Per-node MLP 24 -> 64 -> 64 -> 6 (GELU), BF16 weights, FP32 accumulation.
  * The weight blob is the packed layout described in jfem_learned.h of the

@@ -529,7 +529,7 @@ static int ensure_assembled(jfm_ctx *ctx) {
 static int prepare_op(jfm_ctx *ctx, int op) {
     if (op == JFM_OP_CSR) return ensure_assembled(ctx);
     if (op == JFM_OP_EBE) return ensure_element_matrices(ctx);
-    if (op == JFM_OP_MATFREE) return JFM_OK;
+    if (op == JFM_OP_MATFREE || op == kOpMass) return JFM_OK;
     if (op == JFM_OP_TANGENT)
         return ctx->has_state ? JFM_OK
                               : jfm_fail(ctx, JFM_ESTATE, "tangent operator needs a state: call jfm_nl_residual first");
@@ -1286,4 +1286,65 @@ extern "C" int jfm_solve_nonlinear(jfm_ctx *ctx, const jfm_newton_opts *opts, do
         return jfm_fail(ctx, JFM_ENOCONV, msg);
     }
     return JFM_OK;
+}
+
+// ========================================================= postprocessing ==
+extern "C" int jfm_postprocess(jfm_ctx *ctx, const double *u, double *strain_gp, double *stress_gp,
+                               double *strain_nodal, double *stress_nodal) {
+    if (!ctx || !u) return JFM_EINVAL;
+    if (!ctx->has_mesh) return jfm_fail(ctx, JFM_ESTATE, "no mesh");
+    JFM_HIP(ctx, hipSetDevice(ctx->device));
+    if (int rc = ensure_workspaces(ctx)) return rc;
+    const int64_t n = ctx->ndof, N = ctx->N, nq = ctx->n_el * ctx->ngp * 6;
+    hipStream_t s = ctx->stream;
+    double *du = jfm_alloc<double>(ctx, n), *eps = jfm_alloc<double>(ctx, nq), *sig = jfm_alloc<double>(ctx, nq),
+           *Mm = jfm_alloc<double>(ctx, 9 * N);
+    auto cleanup = [&] {
+        for (void *p : {(void *)du, (void *)eps, (void *)sig, (void *)Mm}) jfm_free(ctx, p);
+    };
+    if (!du || !eps || !sig || !Mm) {
+        cleanup();
+        return jfm_fail(ctx, JFM_ENOMEM, "device allocation failed in jfm_postprocess");
+    }
+    int rc = JFM_OK;
+    auto fail_hip = [&](hipError_t e, const char *what) {
+        rc = jfm_hip_fail(ctx, e, what);
+        return rc;
+    };
+    hipError_t e = hipMemcpyAsync(du, u, sizeof(double) * n, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = launch_post_gp(ctx, du, eps, sig);
+    if (e != hipSuccess) fail_hip(e, "launch_post_gp");
+    if (rc == JFM_OK && strain_gp) {
+        e = hipMemcpyAsync(strain_gp, eps, sizeof(double) * nq, hipMemcpyDeviceToHost, s);
+        if (e != hipSuccess) fail_hip(e, "strain copy");
+    }
+    if (rc == JFM_OK && stress_gp) {
+        e = hipMemcpyAsync(stress_gp, sig, sizeof(double) * nq, hipMemcpyDeviceToHost, s);
+        if (e != hipSuccess) fail_hip(e, "stress copy");
+    }
+    // nodal least-squares fits (src/problems_elasticity.jl:550-594): M x = b
+    // per three components, PCG with the mass operator and its diagonal
+    if (rc == JFM_OK && (strain_nodal || stress_nodal)) {
+        e = launch_mass_diag(ctx, Mm);
+        if (e != hipSuccess) fail_hip(e, "launch_mass_diag");
+        std::vector<double> part(n);
+        double *outs[2] = {strain_nodal, stress_nodal};
+        for (int f = 0; f < 2 && rc == JFM_OK; ++f) {
+            if (!outs[f]) continue;
+            for (int comp0 = 0; comp0 < 6 && rc == JFM_OK; comp0 += 3) {
+                e = launch_post_rhs(ctx, du, f, comp0, (double *)ctx->ws64.b);
+                if (e != hipSuccess) { fail_hip(e, "launch_post_rhs"); break; }
+                int it = 0, status = 0;
+                if ((rc = pcg_run<double>(ctx, kOpMass, Mm, 1e-13, 2000, 32, &it, &status))) break;
+                e = hipMemcpyAsync(part.data(), ctx->ws64.x, sizeof(double) * n, hipMemcpyDeviceToHost, s);
+                if (e == hipSuccess) e = hipStreamSynchronize(s);
+                if (e != hipSuccess) { fail_hip(e, "nodal copy"); break; }
+                for (int64_t a = 0; a < N; ++a)
+                    for (int c = 0; c < 3; ++c) outs[f][6 * a + comp0 + c] = part[3 * a + c];
+            }
+        }
+    }
+    (void)hipStreamSynchronize(s);
+    cleanup();
+    return rc;
 }

@@ -665,6 +665,10 @@ static void launch_elem_phase(jfm_ctx *ctx, int op, const T *x, T *ye, const int
 template <typename T>
 hipError_t launch_elem_only(jfm_ctx *ctx, int op, const T *x, const int *done) {
     if (ctx->n_el == 0) return hipSuccess;
+    if (op == kOpMass) {  // consistent mass (least-squares recovery), FP64 only
+        if (sizeof(T) != 8) return hipErrorInvalidValue;
+        return launch_mass_elem(ctx, (const double *)x, done);
+    }
     if (op == JFM_OP_TANGENT) {  // neo-Hookean K_T(u) x at the stored state
         const T *u = sizeof(T) == 8 ? (const T *)ctx->ustate64 : (const T *)ctx->ustate32;
         return launch_nh_elem<T>(ctx, true, gf_of<T>(ctx), u, x, ye_of<T>(ctx), done);
@@ -683,7 +687,7 @@ hipError_t launch_two_phase(jfm_ctx *ctx, int op, const T *x, T *y, const T *dot
                             int *npart) {
     hipError_t e = launch_elem_only<T>(ctx, op, x, done);
     if (e != hipSuccess) return e;
-    return launch_gather<T>(ctx, ye_of<T>(ctx), ctx->fixed, y, dotv, part, done, npart);
+    return launch_gather<T>(ctx, ye_of<T>(ctx), op == kOpMass ? nullptr : ctx->fixed, y, dotv, part, done, npart);
 }
 
 template hipError_t launch_elem_only<double>(jfm_ctx *, int, const double *, const int *);

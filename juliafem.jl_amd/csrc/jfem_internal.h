@@ -57,6 +57,21 @@ template <> struct Elem<10> {
         default: return c == 1 ? 4.0 * s : (c == 2 ? 4.0 * v : 0.0);
         }
     }
+    __host__ __device__ static constexpr double N(int q, int i) {
+        const double u = xi(q, 0), v = xi(q, 1), s = xi(q, 2), l = 1.0 - u - v - s;
+        switch (i) {
+        case 0: return l * (2.0 * l - 1.0);
+        case 1: return u * (2.0 * u - 1.0);
+        case 2: return v * (2.0 * v - 1.0);
+        case 3: return s * (2.0 * s - 1.0);
+        case 4: return 4.0 * u * l;
+        case 5: return 4.0 * u * v;
+        case 6: return 4.0 * v * l;
+        case 7: return 4.0 * s * l;
+        case 8: return 4.0 * u * s;
+        default: return 4.0 * v * s;
+        }
+    }
 };
 
 template <> struct Elem<4> {
@@ -65,6 +80,7 @@ template <> struct Elem<4> {
     __host__ __device__ static constexpr double dN(int, int i, int c) {
         return i == 0 ? -1.0 : ((i - 1) == c ? 1.0 : 0.0);
     }
+    __host__ __device__ static constexpr double N(int, int) { return 0.25; }  // at the centroid
 };
 
 template <> struct Elem<8> {
@@ -86,6 +102,9 @@ template <> struct Elem<8> {
         const double a = 1.0 + ref(i, 0) * xi(q, 0), b = 1.0 + ref(i, 1) * xi(q, 1),
                      d = 1.0 + ref(i, 2) * xi(q, 2);
         return c == 0 ? 0.125 * ref(i, 0) * b * d : (c == 1 ? 0.125 * a * ref(i, 1) * d : 0.125 * a * b * ref(i, 2));
+    }
+    __host__ __device__ static constexpr double N(int q, int i) {
+        return 0.125 * (1.0 + ref(i, 0) * xi(q, 0)) * (1.0 + ref(i, 1) * xi(q, 1)) * (1.0 + ref(i, 2) * xi(q, 2));
     }
 };
 
@@ -283,6 +302,12 @@ void jfem_coarse_release(jfm_ctx *ctx);
 void jfem_coarse_destroy_handle(jfm_ctx *ctx);
 template <typename T>
 hipError_t launch_coarse_correct(jfm_ctx *ctx, const T *r, T *z, T *p, double *part, const int *done, int *npart);
+// postprocess.hip (kOpMass: consistent scalar mass, three components, no row mask)
+constexpr int kOpMass = 16;
+hipError_t launch_mass_elem(jfm_ctx *ctx, const double *x, const int *done);
+hipError_t launch_post_gp(jfm_ctx *ctx, const double *u, double *strain, double *stress);
+hipError_t launch_post_rhs(jfm_ctx *ctx, const double *u, int field, int comp0, double *b);
+hipError_t launch_mass_diag(jfm_ctx *ctx, double *Minv);
 // linalg_kernels.hip
 template <typename T>
 hipError_t launch_bsr_spmv(jfm_ctx *ctx, const T *vals, const T *x, T *y, const T *dotv, double *part,

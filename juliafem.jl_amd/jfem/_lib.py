@@ -27,7 +27,7 @@ SYMBOLS = ("jfm_create", "jfm_destroy", "jfm_last_error", "jfm_get_info", "jfm_u
            "jfm_upload_bcs", "jfm_csr_pattern", "jfm_assemble", "jfm_apply", "jfm_time_apply",
            "jfm_solve", "jfm_solve_history", "jfm_precond_set_blocks", "jfm_learned_blob_bytes", "jfm_learned_n_params",
            "jfm_learned_set_weights", "jfm_learned_get_weights", "jfm_learned_features", "jfm_learned_outputs",
-           "jfm_learned_grad", "jfm_nl_residual", "jfm_solve_nonlinear")
+           "jfm_learned_grad", "jfm_nl_residual", "jfm_solve_nonlinear", "jfm_postprocess")
 
 
 class JfemError(RuntimeError):
@@ -110,6 +110,7 @@ def load():
     lib.jfm_learned_grad.argtypes = [_P, C.c_int32, _P, _P, C.c_int32, C.POINTER(C.c_double), _P]
     lib.jfm_learned_set_weights.argtypes = [_P, _P, C.c_size_t]
     lib.jfm_nl_residual.argtypes = [_P, _P, C.c_double, _P, C.POINTER(C.c_double)]
+    lib.jfm_postprocess.argtypes = [_P, _P, _P, _P, _P, _P]
     lib.jfm_solve_nonlinear.argtypes = [_P, C.POINTER(NewtonOpts), _P, C.POINTER(NewtonStats)]
     lib.jfm_learned_get_weights.argtypes = [_P, _P, C.c_size_t]
     for name in SYMBOLS:
@@ -250,6 +251,23 @@ class Context:
         if rc != 0 and not (rc == -4 and not raise_on_noconv):
             self._check(rc)
         return u, st.as_dict()
+
+    def postprocess(self, u, gauss=True, nodal=True):
+        """Gauss-point strain/stress (n_el, n_gauss, 6) and their least-squares
+        nodal fits (n_nodes, 6), ABAQUS order 11, 22, 33, 12, 23, 13."""
+        inf = self.info()
+        ngp = {10: 4, 4: 1, 8: 8}[inf["npe"]]
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        out = {}
+        if gauss:
+            out["strain"] = np.empty((inf["n_el"], ngp, 6))
+            out["stress"] = np.empty((inf["n_el"], ngp, 6))
+        if nodal:
+            out["strain_nodal"] = np.empty((inf["n_nodes"], 6))
+            out["stress_nodal"] = np.empty((inf["n_nodes"], 6))
+        self._check(self._lib.jfm_postprocess(self._h, _p(u), _p(out.get("strain")), _p(out.get("stress")),
+                                              _p(out.get("strain_nodal")), _p(out.get("stress_nodal"))))
+        return out
 
     def history(self, max_entries: int = 64):
         """[(cumulative iterations, FP64 relative residual)] of the last solve

@@ -518,3 +518,98 @@ def twolevel_pcg(mesh, csr, b, coarse_nodes: int, rtol: float = 1e-8, maxit: int
         p = z + (rz2 / rz) * p
         rz = rz2
     return x, maxit
+
+
+# ------------------------------------------------------------ postprocessing --
+def lsq_rule(etype: str):
+    """Quadrature of the nodal least-squares fit.  The reference integrates it
+    with the element's own rule (src/problems_elasticity.jl:555-560), which
+    for Tet10 (4 points) and Tet4 (1 point) makes the fit matrix singular: a
+    constant field is not recovered (see tests/test_postprocess_oracle.py).
+    Tetrahedra use a collapsed (Duffy) Gauss product rule instead, 3 x 3 x 4
+    points, exact to degree 4 (the mass entries of Tet10 are degree 4); Hex8
+    keeps its 2 x 2 x 2 rule, which is exact for its mass entries."""
+    if etype == "Hex8":
+        return volume_rule(etype)
+    ga, wa = np.polynomial.legendre.leggauss(3)
+    gc, wc = np.polynomial.legendre.leggauss(4)
+    ga, wa, gc, wc = 0.5 * (ga + 1), 0.5 * wa, 0.5 * (gc + 1), 0.5 * wc
+    pts, wts = [], []
+    for a, w1 in zip(ga, wa):
+        for b, w2 in zip(ga, wa):
+            for c, w3 in zip(gc, wc):
+                pts.append([a * (1 - b) * (1 - c), b * (1 - c), c])
+                wts.append(w1 * w2 * w3 * (1 - b) * (1 - c) ** 2)
+    return np.array(wts), np.array(pts)
+
+
+def _strain_stress_at(mesh, u, xi):
+    conn = np.asarray(mesh.conn, np.int64)
+    Xe = np.asarray(mesh.X)[conn]
+    ue = np.asarray(u, np.float64).reshape(-1, 3)[conn]
+    E, nu = np.asarray(mesh.E), np.asarray(mesh.nu)
+    la = (E * nu / ((1 + nu) * (1 - 2 * nu)))[:, None]
+    mu = (E / (2 * (1 + nu)))[:, None]
+    strain = np.empty((conn.shape[0], len(xi), 6))
+    stress = np.empty_like(strain)
+    detJ = np.empty((conn.shape[0], len(xi)))
+    for q in range(len(xi)):
+        _, dN = basis(mesh.etype, xi[q])
+        J = np.einsum("ia,eib->eab", dN, Xe)          # J[a][b] = sum_i dN_i/dxi_a X_i,b
+        detJ[:, q] = np.linalg.det(J)
+        g = np.einsum("eab,ib->eia", np.linalg.inv(J), dN)
+        G = np.einsum("eic,eia->eca", ue, g)            # G[c][a] = du_c / dX_a
+        ev = np.stack([G[:, 0, 0], G[:, 1, 1], G[:, 2, 2], 0.5 * (G[:, 0, 1] + G[:, 1, 0]),
+                       0.5 * (G[:, 1, 2] + G[:, 2, 1]), 0.5 * (G[:, 0, 2] + G[:, 2, 0])], axis=1)
+        tr = ev[:, :3].sum(axis=1, keepdims=True)
+        strain[:, q] = ev
+        stress[:, q] = 2 * mu * ev + np.hstack([la * tr] * 3 + [np.zeros_like(tr)] * 3)
+    return strain, stress, detJ
+
+
+def post_gauss(mesh, u):
+    """Gauss-point strain and stress (n_el, n_gauss, 6), ABAQUS order 11, 22,
+    33, 12, 23, 13 with tensor shear: eps = (grad u + grad u')/2, linear
+    isotropic stress (src/problems_elasticity.jl:520-548)."""
+    _, xi = volume_rule(mesh.etype)
+    strain, stress, _ = _strain_stress_at(mesh, u, xi)
+    return strain, stress
+
+
+def lsq_fit(mesh, u, field: str = "strain", rule=None):
+    """Nodal least-squares fit of the strain or stress of u: A x = b with
+    A = sum w N N', b = sum w f N, w = weight * detJ, solved by a sparse direct
+    method (src/problems_elasticity.jl:550-583), on the points of `rule`
+    (default lsq_rule)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+    w, xi = lsq_rule(mesh.etype) if rule is None else rule
+    strain, stress, detJ = _strain_stress_at(mesh, u, xi)
+    fgp = strain if field == "strain" else stress
+    conn = np.asarray(mesh.conn, np.int64)
+    n_el, npe = conn.shape
+    N_nodes = mesh.X.shape[0]
+    rows, cols, vals = [], [], []
+    b = np.zeros((N_nodes, 6))
+    for q in range(len(w)):
+        Nq, _ = basis(mesh.etype, xi[q])
+        wq = w[q] * detJ[:, q]
+        rows.append(np.repeat(conn, npe, axis=1).ravel())
+        cols.append(np.tile(conn, (1, npe)).ravel())
+        vals.append((wq[:, None, None] * np.outer(Nq, Nq)[None]).ravel())
+        np.add.at(b, conn.ravel(), (wq[:, None, None] * Nq[None, :, None] * fgp[:, q, None, :]).reshape(-1, 6))
+    A = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(N_nodes, N_nodes))
+    A = 0.5 * (A + A.T)
+    return np.column_stack([sla.spsolve(A.tocsc(), b[:, k]) for k in range(6)])
+
+
+def mass_volume(mesh):
+    """sum of the consistent mass entries = the body's volume (a check)."""
+    w, xi = lsq_rule(mesh.etype)
+    conn = np.asarray(mesh.conn, np.int64)
+    Xe = np.asarray(mesh.X)[conn]
+    tot = 0.0
+    for q in range(w.size):
+        Nq, dN = basis(mesh.etype, xi[q])
+        tot += (w[q] * np.linalg.det(np.einsum("ia,eib->eab", dN, Xe))).sum() * Nq.sum() ** 2
+    return tot
