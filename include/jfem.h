@@ -233,6 +233,20 @@ int jfm_solve_nonlinear(jfm_ctx *ctx, const jfm_newton_opts *opts, double *u, jf
 int jfm_postprocess(jfm_ctx *ctx, const double *u, double *strain_gp, double *stress_gp, double *strain_nodal,
                     double *stress_nodal);
 
+/* ---- steady heat conduction (Problem{Heat}, src/problems_heat.jl:36-113) --
+ * On the uploaded mesh (element type and geometry; E and nu unused):
+ *     K T = f,  K = sum w detJ k dN'dN  +  sum_faces w detJ h N N',
+ *               f = sum w detJ q N      +  sum_faces w detJ (g + h T_ext) N,
+ * with T = T_fixed on nodes where fixed_node is set.  k, q (may be NULL):
+ * per element; flux, h, T_ext (each may be NULL; h and T_ext together):
+ * per boundary face (n_faces x nodes_per_face connectivity, index_base).
+ * Block-Jacobi PCG in FP64 to rtol; T receives n_nodes temperatures.
+ * Does not change the elasticity boundary conditions. */
+int jfm_heat_solve(jfm_ctx *ctx, const double *k, const double *q, int64_t n_faces, int32_t nodes_per_face,
+                   const int64_t *face_conn, const double *flux, const double *h, const double *T_ext,
+                   const uint8_t *fixed_node, const double *T_fixed, int32_t index_base, double rtol, int32_t maxit,
+                   double *T, jfm_solve_stats *stats);
+
 /* ---- learned preconditioner (no reference counterpart) ---------------
  * Per-node MLP 24 -> 64 -> 64 -> 6 (GELU), BF16 weights, FP32 accumulation.
  * The weight blob is the packed layout described in jfem_learned.h of the

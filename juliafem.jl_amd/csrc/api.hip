@@ -147,6 +147,14 @@ static void drop_graphs(jfm_ctx *ctx) {
     ctx->graphs.clear();
 }
 
+static void release_heat(jfm_ctx *ctx) {
+    free_ptr(ctx, ctx->heat_k); free_ptr(ctx, ctx->heat_fixed); free_ptr(ctx, ctx->heat_g); free_ptr(ctx, ctx->heat_Minv);
+    free_ptr(ctx, ctx->hf_conn); free_ptr(ctx, ctx->hf_ptr); free_ptr(ctx, ctx->hf_inc); free_ptr(ctx, ctx->hf_h);
+    free_ptr(ctx, ctx->hf_g); free_ptr(ctx, ctx->hf_text); free_ptr(ctx, ctx->hf_fe);
+    ctx->hf_n = 0;
+    ctx->hf_nfn = 0;
+}
+
 static void release_mesh(jfm_ctx *ctx) {
     free_ptr(ctx, ctx->X); free_ptr(ctx, ctx->conn); free_ptr(ctx, ctx->E); free_ptr(ctx, ctx->nu);
     free_ptr(ctx, ctx->gf64); free_ptr(ctx, ctx->gf32); free_ptr(ctx, ctx->inc_ptr); free_ptr(ctx, ctx->inc);
@@ -170,6 +178,7 @@ static void release_mesh(jfm_ctx *ctx) {
     free_ptr(ctx, ctx->nl_u); free_ptr(ctx, ctx->nl_r); free_ptr(ctx, ctx->nl_du); free_ptr(ctx, ctx->nl_zero);
     ctx->has_state = ctx->minv_tangent = false;
     jfem_coarse_release(ctx);
+    release_heat(ctx);
     ctx->h_X.clear();
     ctx->has_user = false;
     ctx->has_mesh = ctx->has_bcs = ctx->assembled = ctx->has_load = false;
@@ -530,6 +539,7 @@ static int prepare_op(jfm_ctx *ctx, int op) {
     if (op == JFM_OP_CSR) return ensure_assembled(ctx);
     if (op == JFM_OP_EBE) return ensure_element_matrices(ctx);
     if (op == JFM_OP_MATFREE || op == kOpMass) return JFM_OK;
+    if (op == kOpHeat) return ctx->heat_k ? JFM_OK : jfm_fail(ctx, JFM_ESTATE, "no heat problem uploaded");
     if (op == JFM_OP_TANGENT)
         return ctx->has_state ? JFM_OK
                               : jfm_fail(ctx, JFM_ESTATE, "tangent operator needs a state: call jfm_nl_residual first");
@@ -542,6 +552,10 @@ template <typename T>
 static hipError_t op_apply(jfm_ctx *ctx, int op, const T *x, T *y, const T *dotv, double *part, const int *done,
                            int *npart) {
     if (op == JFM_OP_CSR) return launch_bsr_spmv<T>(ctx, vals_of<T>(ctx), x, y, dotv, part, done, npart);
+    if (op == kOpHeat) {
+        if (sizeof(T) != 8) return hipErrorInvalidValue;
+        return launch_heat_apply(ctx, (const double *)x, (double *)y, (const double *)dotv, part, done, npart);
+    }
     return launch_two_phase<T>(ctx, op, x, y, dotv, part, done, npart);
 }
 
@@ -1347,4 +1361,133 @@ extern "C" int jfm_postprocess(jfm_ctx *ctx, const double *u, double *strain_gp,
     (void)hipStreamSynchronize(s);
     cleanup();
     return rc;
+}
+
+// ================================================================== heat ==
+extern "C" int jfm_heat_solve(jfm_ctx *ctx, const double *k, const double *q, int64_t n_faces, int32_t nodes_per_face,
+                              const int64_t *face_conn, const double *flux, const double *h, const double *T_ext,
+                              const uint8_t *fixed_node, const double *T_fixed, int32_t index_base, double rtol,
+                              int32_t maxit, double *T, jfm_solve_stats *stats) {
+    if (!ctx || !k || !T || !(rtol > 0.0) || maxit <= 0) return JFM_EINVAL;
+    if (!ctx->has_mesh) return jfm_fail(ctx, JFM_ESTATE, "no mesh");
+    if (n_faces < 0 || (n_faces > 0 && !face_conn)) return jfm_fail(ctx, JFM_EINVAL, "invalid face arguments");
+    if (n_faces > 0 && nodes_per_face != 6 && nodes_per_face != 3 && nodes_per_face != 4)
+        return jfm_fail(ctx, JFM_EUNSUPPORTED, "nodes_per_face must be 6 (Tri6), 3 (Tri3) or 4 (Quad4)");
+    if ((h == nullptr) != (T_ext == nullptr)) return jfm_fail(ctx, JFM_EINVAL, "convection needs both h and T_ext");
+    JFM_HIP(ctx, hipSetDevice(ctx->device));
+    if (int rc = ensure_workspaces(ctx)) return rc;
+    const int64_t N = ctx->N, n = ctx->ndof, n_el = ctx->n_el;
+    hipStream_t s = ctx->stream;
+    for (int64_t e = 0; e < n_el; ++e)
+        if (!(k[e] > 0.0)) return jfm_fail(ctx, JFM_EINVAL, "thermal conductivity must be positive");
+    // the captured PCG graphs of a previous heat problem point at its buffers
+    drop_graphs(ctx);
+    release_heat(ctx);
+    ALLOC(ctx, ctx->heat_k, double, n_el);
+    ALLOC(ctx, ctx->heat_fixed, uint8_t, n);
+    ALLOC(ctx, ctx->heat_g, double, n);
+    ALLOC(ctx, ctx->heat_Minv, double, 9 * N);
+    JFM_HIP(ctx, hipMemcpyAsync(ctx->heat_k, k, sizeof(double) * n_el, hipMemcpyHostToDevice, s));
+    std::vector<uint8_t> fx(n, 1);
+    std::vector<double> g(n, 0.0);
+    for (int64_t a = 0; a < N; ++a) {
+        fx[3 * a] = fixed_node ? fixed_node[a] : 0;
+        if (fx[3 * a] && T_fixed) g[3 * a] = T_fixed[a];
+    }
+    JFM_HIP(ctx, hipMemcpyAsync(ctx->heat_fixed, fx.data(), n, hipMemcpyHostToDevice, s));
+    JFM_HIP(ctx, hipMemcpyAsync(ctx->heat_g, g.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+    double *src = nullptr;
+    if (q) {
+        src = jfm_alloc<double>(ctx, n_el);
+        if (!src) return jfm_fail(ctx, JFM_ENOMEM, "device allocation failed in jfm_heat_solve");
+        JFM_HIP(ctx, hipMemcpyAsync(src, q, sizeof(double) * n_el, hipMemcpyHostToDevice, s));
+    }
+    if (n_faces > 0) {
+        const int nfn = nodes_per_face;
+        std::vector<int32_t> soa((size_t)n_faces * nfn), fptr(N + 1, 0);
+        for (int64_t f = 0; f < n_faces; ++f)
+            for (int i = 0; i < nfn; ++i) {
+                const int64_t v = face_conn[f * nfn + i] - index_base;
+                if (v < 0 || v >= N) {
+                    if (src) jfm_free(ctx, src);
+                    return jfm_fail(ctx, JFM_EINVAL, "face connectivity out of range");
+                }
+                soa[(size_t)i * n_faces + f] = (int32_t)v;
+                fptr[v + 1]++;
+            }
+        for (int64_t a = 0; a < N; ++a) fptr[a + 1] += fptr[a];
+        std::vector<int32_t> inc((size_t)n_faces * nfn), pos(fptr.begin(), fptr.end() - 1);
+        for (int64_t f = 0; f < n_faces; ++f)  // ascending face order per node
+            for (int i = 0; i < nfn; ++i) inc[pos[soa[(size_t)i * n_faces + f]]++] = (int32_t)(f * nfn + i);
+        ctx->hf_n = n_faces;
+        ctx->hf_nfn = nfn;
+        ALLOC(ctx, ctx->hf_conn, int32_t, n_faces * nfn);
+        ALLOC(ctx, ctx->hf_ptr, int32_t, N + 1);
+        ALLOC(ctx, ctx->hf_inc, int32_t, n_faces * nfn);
+        ALLOC(ctx, ctx->hf_fe, double, n_faces * nfn * 3);
+        JFM_HIP(ctx, hipMemcpyAsync(ctx->hf_conn, soa.data(), 4 * soa.size(), hipMemcpyHostToDevice, s));
+        JFM_HIP(ctx, hipMemcpyAsync(ctx->hf_ptr, fptr.data(), 4 * (N + 1), hipMemcpyHostToDevice, s));
+        JFM_HIP(ctx, hipMemcpyAsync(ctx->hf_inc, inc.data(), 4 * inc.size(), hipMemcpyHostToDevice, s));
+        for (auto [dst, v] : {std::pair<double **, const double *>{&ctx->hf_g, flux}, {&ctx->hf_h, h}, {&ctx->hf_text, T_ext}})
+            if (v) {
+                ALLOC(ctx, *dst, double, n_faces);
+                JFM_HIP(ctx, hipMemcpyAsync(*dst, v, sizeof(double) * n_faces, hipMemcpyHostToDevice, s));
+            }
+    }
+    // the heat row mask stands in for the elasticity one for this solve
+    struct MaskSwap {
+        jfm_ctx *c;
+        uint8_t *saved;
+        ~MaskSwap() { c->fixed = saved; }
+    } swap{ctx, ctx->fixed};
+    ctx->fixed = ctx->heat_fixed;
+    hipEvent_t e0, e1;
+    JFM_HIP(ctx, hipEventCreate(&e0));
+    JFM_HIP(ctx, hipEventCreate(&e1));
+    JFM_HIP(ctx, hipEventRecord(e0, s));
+    jfm_solve_stats st{};
+    // b = f - K g on the free temperatures
+    CHECK_LAUNCH(ctx, launch_heat_load(ctx, src, ctx->vx64));
+    double *b = (double *)ctx->ws64.b;
+    double bnorm = 0.0;
+    if (int rc = true_residual(ctx, kOpHeat, ctx->vx64, ctx->heat_g, b, &bnorm)) return rc;
+    CHECK_LAUNCH(ctx, launch_heat_jacobi(ctx, ctx->heat_Minv));
+    const int check = 32;
+    if (bnorm > 0.0) {
+        int it = 0, status = 0, total = 0;
+        if (int rc = pcg_run<double>(ctx, kOpHeat, ctx->heat_Minv, rtol, maxit, check, &it, &status)) return rc;
+        total = it;
+        for (int restart = 0; restart < 4 && total < maxit; ++restart) {  // residual replacement, as jfm_solve
+            double rn = 0.0;
+            if (int rc = true_residual(ctx, kOpHeat, b, (const double *)ctx->ws64.x, ctx->rr64, &rn)) return rc;
+            if (rn <= rtol * bnorm) break;
+            if (int rc = pcg_run<double>(ctx, kOpHeat, ctx->heat_Minv, rtol, maxit - total, check, &it, &status, true))
+                return rc;
+            total += it;
+            st.outer++;
+        }
+        st.iterations = total;
+        CHECK_LAUNCH(ctx, launch_add(ctx, ctx->heat_g, (const double *)ctx->ws64.x, ctx->u64, n));
+    } else {
+        JFM_HIP(ctx, hipMemcpyAsync(ctx->u64, ctx->heat_g, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    }
+    double rn = 0.0;
+    if (int rc = true_residual(ctx, kOpHeat, ctx->vx64, ctx->u64, ctx->rr64, &rn)) return rc;
+    JFM_HIP(ctx, hipEventRecord(e1, s));
+    JFM_HIP(ctx, hipEventSynchronize(e1));
+    float ms = 0.f;
+    JFM_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    std::vector<double> u(n);
+    JFM_HIP(ctx, hipMemcpyAsync(u.data(), ctx->u64, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    if (int rc = sync(ctx)) return rc;
+    if (src) jfm_free(ctx, src);
+    for (int64_t a = 0; a < N; ++a) T[a] = u[3 * a];
+    st.relres = bnorm > 0.0 ? rn / bnorm : 0.0;
+    st.converged = st.relres <= rtol ? 1 : 0;
+    st.solve_ms = ms;
+    if (stats) *stats = st;
+    if (!st.converged) return jfm_fail(ctx, JFM_ENOCONV, "heat solve did not reach rtol");
+    return JFM_OK;
 }

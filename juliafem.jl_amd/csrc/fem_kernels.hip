@@ -392,34 +392,6 @@ __global__ __launch_bounds__(256) void k_asm_vals(int nnzb, const int32_t *__res
 // Consistent traction load per face (src/problems_elasticity.jl:454-502):
 // f_i += w |dX/du x dX/dv| t N_i.  Tri6: 3-pt rule (gauss_points.jl:100-106),
 // Tri3: 1-pt (:93-97), Quad4: 2x2 (:133-142).
-__device__ void surf_basis(int nfn, double u, double v, double *N, double (*dN)[2]) {
-    if (nfn == 6) {
-        N[0] = 1 - 3.0 * u - 3.0 * v + 2.0 * u * u + 4.0 * u * v + 2.0 * v * v;
-        N[1] = -u + 2.0 * u * u;
-        N[2] = -v + 2.0 * v * v;
-        N[3] = 4.0 * u - 4.0 * u * u - 4.0 * u * v;
-        N[4] = 4.0 * u * v;
-        N[5] = 4.0 * v - 4.0 * u * v - 4.0 * v * v;
-        dN[0][0] = -3.0 + 4.0 * u + 4.0 * v; dN[0][1] = -3.0 + 4.0 * u + 4.0 * v;
-        dN[1][0] = -1.0 + 4.0 * u; dN[1][1] = 0.0;
-        dN[2][0] = 0.0; dN[2][1] = -1.0 + 4.0 * v;
-        dN[3][0] = 4.0 - 8.0 * u - 4.0 * v; dN[3][1] = -4.0 * u;
-        dN[4][0] = 4.0 * v; dN[4][1] = 4.0 * u;
-        dN[5][0] = -4.0 * v; dN[5][1] = 4.0 - 4.0 * u - 8.0 * v;
-    } else if (nfn == 3) {
-        N[0] = 1 - u - v; N[1] = u; N[2] = v;
-        dN[0][0] = -1; dN[0][1] = -1; dN[1][0] = 1; dN[1][1] = 0; dN[2][0] = 0; dN[2][1] = 1;
-    } else {
-        const double r[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
-        for (int i = 0; i < 4; ++i) {
-            const double a = 1 + r[i][0] * u, b = 1 + r[i][1] * v;
-            N[i] = 0.25 * a * b;
-            dN[i][0] = 0.25 * r[i][0] * b;
-            dN[i][1] = 0.25 * a * r[i][1];
-        }
-    }
-}
-
 __global__ __launch_bounds__(256) void k_face(int nf, int nfn, const int32_t *__restrict__ fconn,
                                              const double *__restrict__ X, const double *__restrict__ ft,
                                              double *__restrict__ fe) {
@@ -431,22 +403,7 @@ __global__ __launch_bounds__(256) void k_face(int nf, int nfn, const int32_t *__
         for (int c = 0; c < 3; ++c) Xf[i][c] = X[3 * (size_t)nd + c];
     }
     double w[4], pu[4], pv[4];
-    int ng;
-    if (nfn == 6) {
-        ng = 3;
-        w[0] = w[1] = w[2] = 1.0 / 6;
-        pu[0] = 1.0 / 6; pv[0] = 1.0 / 6;
-        pu[1] = 2.0 / 3; pv[1] = 1.0 / 6;
-        pu[2] = 1.0 / 6; pv[2] = 2.0 / 3;
-    } else if (nfn == 3) {
-        ng = 1;
-        w[0] = 0.5; pu[0] = pv[0] = 1.0 / 3;
-    } else {
-        ng = 4;
-        const double a = 1.0 / sqrt(3.0);
-        w[0] = w[1] = w[2] = w[3] = 1.0;
-        pu[0] = -a; pv[0] = -a; pu[1] = a; pv[1] = -a; pu[2] = -a; pv[2] = a; pu[3] = a; pv[3] = a;
-    }
+    const int ng = surf_rule(nfn, w, pu, pv);
     double out[6][3];
     for (int i = 0; i < nfn; ++i) out[i][0] = out[i][1] = out[i][2] = 0.0;
     const double t0 = ft[3 * (size_t)f], t1 = ft[3 * (size_t)f + 1], t2 = ft[3 * (size_t)f + 2];

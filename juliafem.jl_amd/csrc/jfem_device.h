@@ -96,6 +96,54 @@ __device__ __forceinline__ int xcd_block() {
 }
 inline unsigned xcd_grid(unsigned nb) { return (nb + kXcds - 1) / kXcds * kXcds; }
 
+// Surface elements: Tri6 / Tri3 / Quad4 basis (src/basis/lagrange_generated.jl:
+// 89-171) and their rules (Tri 3-point and 1-point, Quad 2x2 Gauss).
+__device__ inline void surf_basis(int nfn, double u, double v, double *N, double (*dN)[2]) {
+    if (nfn == 6) {
+        N[0] = 1 - 3.0 * u - 3.0 * v + 2.0 * u * u + 4.0 * u * v + 2.0 * v * v;
+        N[1] = -u + 2.0 * u * u;
+        N[2] = -v + 2.0 * v * v;
+        N[3] = 4.0 * u - 4.0 * u * u - 4.0 * u * v;
+        N[4] = 4.0 * u * v;
+        N[5] = 4.0 * v - 4.0 * u * v - 4.0 * v * v;
+        dN[0][0] = -3.0 + 4.0 * u + 4.0 * v; dN[0][1] = -3.0 + 4.0 * u + 4.0 * v;
+        dN[1][0] = -1.0 + 4.0 * u; dN[1][1] = 0.0;
+        dN[2][0] = 0.0; dN[2][1] = -1.0 + 4.0 * v;
+        dN[3][0] = 4.0 - 8.0 * u - 4.0 * v; dN[3][1] = -4.0 * u;
+        dN[4][0] = 4.0 * v; dN[4][1] = 4.0 * u;
+        dN[5][0] = -4.0 * v; dN[5][1] = 4.0 - 4.0 * u - 8.0 * v;
+    } else if (nfn == 3) {
+        N[0] = 1 - u - v; N[1] = u; N[2] = v;
+        dN[0][0] = -1; dN[0][1] = -1; dN[1][0] = 1; dN[1][1] = 0; dN[2][0] = 0; dN[2][1] = 1;
+    } else {
+        const double r[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
+        for (int i = 0; i < 4; ++i) {
+            const double a = 1 + r[i][0] * u, b = 1 + r[i][1] * v;
+            N[i] = 0.25 * a * b;
+            dN[i][0] = 0.25 * r[i][0] * b;
+            dN[i][1] = 0.25 * a * r[i][1];
+        }
+    }
+}
+
+__device__ inline int surf_rule(int nfn, double (&w)[4], double (&pu)[4], double (&pv)[4]) {
+    if (nfn == 6) {
+        w[0] = w[1] = w[2] = 1.0 / 6;
+        pu[0] = 1.0 / 6; pv[0] = 1.0 / 6;
+        pu[1] = 2.0 / 3; pv[1] = 1.0 / 6;
+        pu[2] = 1.0 / 6; pv[2] = 2.0 / 3;
+        return 3;
+    }
+    if (nfn == 3) {
+        w[0] = 0.5; pu[0] = pv[0] = 1.0 / 3;
+        return 1;
+    }
+    const double a = 0.57735026918962576451;  // 1/sqrt(3)
+    w[0] = w[1] = w[2] = w[3] = 1.0;
+    pu[0] = -a; pv[0] = -a; pu[1] = a; pv[1] = -a; pu[2] = -a; pv[2] = a; pu[3] = a; pv[3] = a;
+    return 4;
+}
+
 inline unsigned capped_grid(int64_t work, int per_block) {
     int64_t g = (work + per_block - 1) / per_block;
     if (g < 1) g = 1;

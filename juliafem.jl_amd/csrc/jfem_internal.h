@@ -246,6 +246,15 @@ struct jfm_ctx {
     float *Ahat32 = nullptr;       // nc x nc: R^-1 A_c^-1 R^-T
     float *wc32 = nullptr, *zc32 = nullptr;
     void *blas = nullptr;          // rocblas_handle
+    // heat conduction (heat.hip): temperature in component 0
+    double *heat_k = nullptr;      // n_el conductivities
+    uint8_t *heat_fixed = nullptr; // 3N row mask of the heat system
+    double *heat_g = nullptr;      // 3N prescribed temperatures (component 0)
+    double *heat_Minv = nullptr;   // N x 9 Jacobi blocks
+    int64_t hf_n = 0;              // boundary faces with flux / convection
+    int hf_nfn = 0;
+    int32_t *hf_conn = nullptr, *hf_ptr = nullptr, *hf_inc = nullptr;
+    double *hf_h = nullptr, *hf_g = nullptr, *hf_text = nullptr, *hf_fe = nullptr;
     double *Muser64 = nullptr;     // N x 9 user blocks (JFM_PREC_USER)
     float *Muser32 = nullptr;
     bool has_user = false;
@@ -302,6 +311,12 @@ void jfem_coarse_release(jfm_ctx *ctx);
 void jfem_coarse_destroy_handle(jfm_ctx *ctx);
 template <typename T>
 hipError_t launch_coarse_correct(jfm_ctx *ctx, const T *r, T *z, T *p, double *part, const int *done, int *npart);
+// heat.hip (kOpHeat: conduction + convection, FP64, row mask ctx->fixed)
+constexpr int kOpHeat = 17;
+hipError_t launch_heat_apply(jfm_ctx *ctx, const double *x, double *y, const double *dotv, double *part, const int *done,
+                             int *npart);
+hipError_t launch_heat_load(jfm_ctx *ctx, const double *src, double *f);
+hipError_t launch_heat_jacobi(jfm_ctx *ctx, double *Minv);
 // postprocess.hip (kOpMass: consistent scalar mass, three components, no row mask)
 constexpr int kOpMass = 16;
 hipError_t launch_mass_elem(jfm_ctx *ctx, const double *x, const int *done);

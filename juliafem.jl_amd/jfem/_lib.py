@@ -27,7 +27,7 @@ SYMBOLS = ("jfm_create", "jfm_destroy", "jfm_last_error", "jfm_get_info", "jfm_u
            "jfm_upload_bcs", "jfm_csr_pattern", "jfm_assemble", "jfm_apply", "jfm_time_apply",
            "jfm_solve", "jfm_solve_history", "jfm_precond_set_blocks", "jfm_learned_blob_bytes", "jfm_learned_n_params",
            "jfm_learned_set_weights", "jfm_learned_get_weights", "jfm_learned_features", "jfm_learned_outputs",
-           "jfm_learned_grad", "jfm_nl_residual", "jfm_solve_nonlinear", "jfm_postprocess")
+           "jfm_learned_grad", "jfm_nl_residual", "jfm_solve_nonlinear", "jfm_postprocess", "jfm_heat_solve")
 
 
 class JfemError(RuntimeError):
@@ -111,6 +111,8 @@ def load():
     lib.jfm_learned_set_weights.argtypes = [_P, _P, C.c_size_t]
     lib.jfm_nl_residual.argtypes = [_P, _P, C.c_double, _P, C.POINTER(C.c_double)]
     lib.jfm_postprocess.argtypes = [_P, _P, _P, _P, _P, _P]
+    lib.jfm_heat_solve.argtypes = [_P, _P, _P, C.c_int64, C.c_int32, _P, _P, _P, _P, _P, _P, C.c_int32, C.c_double,
+                                   C.c_int32, _P, C.POINTER(SolveStats)]
     lib.jfm_solve_nonlinear.argtypes = [_P, C.POINTER(NewtonOpts), _P, C.POINTER(NewtonStats)]
     lib.jfm_learned_get_weights.argtypes = [_P, _P, C.c_size_t]
     for name in SYMBOLS:
@@ -268,6 +270,30 @@ class Context:
         self._check(self._lib.jfm_postprocess(self._h, _p(u), _p(out.get("strain")), _p(out.get("stress")),
                                               _p(out.get("strain_nodal")), _p(out.get("stress_nodal"))))
         return out
+
+    def heat_solve(self, k, q=None, faces=None, flux=None, h=None, T_ext=None, fixed=None, T_fixed=None,
+                   rtol=1e-10, maxit=20000, raise_on_noconv=True):
+        """Steady conduction on the uploaded mesh; returns (T per node, stats).
+        k, q per element; faces (n_faces, nfn) 0-based with flux / h / T_ext per
+        face; fixed (per node, bool) with T_fixed (per node)."""
+        inf = self.info()
+        N, n_el = inf["n_nodes"], inf["n_el"]
+        f64 = lambda a, m: None if a is None else np.ascontiguousarray(np.broadcast_to(a, (m,)), dtype=np.float64)
+        kk, qq = f64(k, n_el), f64(q, n_el)
+        if faces is None:
+            faces = np.zeros((0, 3), np.int64)
+        faces = np.ascontiguousarray(faces, dtype=np.int64)
+        nf = faces.shape[0]
+        fl, hh, te = f64(flux, nf), f64(h, nf), f64(T_ext, nf)
+        fx = None if fixed is None else np.ascontiguousarray(fixed, dtype=np.uint8)
+        tf = f64(T_fixed, N)
+        T = np.empty(N)
+        st = SolveStats()
+        rc = self._lib.jfm_heat_solve(self._h, _p(kk), _p(qq), nf, faces.shape[1], _p(faces) if nf else None,
+                                      _p(fl), _p(hh), _p(te), _p(fx), _p(tf), 0, rtol, maxit, _p(T), C.byref(st))
+        if rc != 0 and not (rc == -4 and not raise_on_noconv):
+            self._check(rc)
+        return T, st.as_dict()
 
     def history(self, max_entries: int = 64):
         """[(cumulative iterations, FP64 relative residual)] of the last solve

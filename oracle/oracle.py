@@ -613,3 +613,62 @@ def mass_volume(mesh):
         Nq, dN = basis(mesh.etype, xi[q])
         tot += (w[q] * np.linalg.det(np.einsum("ia,eib->eab", dN, Xe))).sum() * Nq.sum() ** 2
     return tot
+
+
+# ---------------------------------------------------------------------- heat --
+def heat_system(mesh, k, q=None, faces=None, ftype=None, flux=None, h=None, T_ext=None):
+    """(K, f) of steady conduction (src/problems_heat.jl:36-113): volume
+    Ke += w detJ k dN'dN, fe += w detJ q N; faces fe += w detJ (g + h T_ext) N,
+    Ke += w detJ h N N'.  K as scipy CSR, one DOF per node."""
+    import scipy.sparse as sp
+    w, xi = volume_rule(mesh.etype)
+    conn = np.asarray(mesh.conn, np.int64)
+    Xe = np.asarray(mesh.X)[conn]
+    n_el, npe = conn.shape
+    N = mesh.X.shape[0]
+    k = np.broadcast_to(np.asarray(k, np.float64), (n_el,))
+    rows, cols, vals = [], [], []
+    f = np.zeros(N)
+    for qp in range(w.size):
+        Nq, dN = basis(mesh.etype, xi[qp])
+        J = np.einsum("ia,eib->eab", dN, Xe)
+        wd = w[qp] * np.linalg.det(J)
+        g = np.einsum("eab,ib->eia", np.linalg.inv(J), dN)
+        Ke = np.einsum("e,eia,eja->eij", wd * k, g, g)
+        rows.append(np.repeat(conn, npe, axis=1).ravel())
+        cols.append(np.tile(conn, (1, npe)).ravel())
+        vals.append(Ke.ravel())
+        if q is not None:
+            qq = np.broadcast_to(np.asarray(q, np.float64), (n_el,))
+            np.add.at(f, conn.ravel(), (wd * qq)[:, None].repeat(npe, 1).ravel() * np.tile(Nq, n_el))
+    if faces is not None and len(faces):
+        faces = np.asarray(faces, np.int64)
+        nf, nfn = faces.shape
+        Xf = np.asarray(mesh.X)[faces]
+        ws, us = surface_rule(ftype)
+        gg = np.zeros(nf) if flux is None else np.broadcast_to(np.asarray(flux, np.float64), (nf,))
+        hh = np.zeros(nf) if h is None else np.broadcast_to(np.asarray(h, np.float64), (nf,))
+        tt = np.zeros(nf) if T_ext is None else np.broadcast_to(np.asarray(T_ext, np.float64), (nf,))
+        for qp in range(ws.size):
+            Nq, dN = basis(ftype, us[qp], surface=True)
+            Jf = np.einsum("ia,eib->eab", dN, Xf)
+            area = np.linalg.norm(np.cross(Jf[:, 0], Jf[:, 1]), axis=1)
+            wq = ws[qp] * area
+            np.add.at(f, faces.ravel(), ((wq * (gg + hh * tt))[:, None] * Nq[None, :]).ravel())
+            rows.append(np.repeat(faces, nfn, axis=1).ravel())
+            cols.append(np.tile(faces, (1, nfn)).ravel())
+            vals.append(((wq * hh)[:, None, None] * np.outer(Nq, Nq)[None]).ravel())
+    K = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(N, N))
+    return K, f
+
+
+def heat_solve_direct(K, f, fixed, T_fixed):
+    """Dirichlet elimination and a sparse direct solve (as src/solvers.jl:192-216)."""
+    import scipy.sparse.linalg as sla
+    fixed = np.asarray(fixed, bool)
+    T = np.where(fixed, T_fixed, 0.0)
+    free = ~fixed
+    Kc = K.tocsc()
+    b = f[free] - Kc[free][:, fixed] @ T[fixed]
+    T[free] = sla.spsolve(Kc[free][:, free], b)
+    return T
