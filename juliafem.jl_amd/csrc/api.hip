@@ -116,7 +116,6 @@ extern "C" void jfm_destroy(jfm_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     drop_graphs(ctx);
-    jfem_coarse_destroy_handle(ctx);
     for (void *p : ctx->allocs) (void)hipFree(p);
     ctx->allocs.clear();
     if (ctx->S_host) (void)hipHostFree(ctx->S_host);
@@ -728,11 +727,21 @@ static hipError_t pcg_iteration(jfm_ctx *ctx, int op, int slot, const T *Minv, b
     if (e != hipSuccess) return e;
     e = launch_pcg_update<T>(ctx, slot, np_pq, x, r, z, p, q, Minv, &np_rz);
     if (e != hipSuccess) return e;
-    if (coarse) {  // z += Q A_c^-1 Q' r; the (r.z, r.r) partials are rewritten
-        e = launch_coarse_correct<T>(ctx, r, z, (T *)nullptr, ctx->partials + part_rz(slot), &ctx->S->done[slot], &np_rz);
+    if (coarse) {  // zc = A_c^-1 Q' r, applied inside the direction update
+        int np_wz = 0;
+        e = launch_coarse_product<T>(ctx, r, &ctx->S->done[slot], &np_wz);
         if (e != hipSuccess) return e;
+        return launch_pcg_direction_coarse<T>(ctx, slot, np_rz, np_wz, p, z);
     }
     return launch_pcg_direction<T>(ctx, slot, np_rz, p, z);
+}
+
+bool jfem::jfem_use_graphs() {
+    static const bool use = [] {
+        const char *v = std::getenv("JFEM_GRAPHS");
+        return !(v && v[0] == '0');
+    }();
+    return use;
 }
 
 // A batch of `batch` (even) iterations captured once as a HIP graph and
@@ -746,11 +755,7 @@ static int pcg_batch(jfm_ctx *ctx, int op, const T *Minv, int batch, bool coarse
             JFM_HIP(ctx, hipGraphLaunch(g.exec, ctx->stream));
             return JFM_OK;
         }
-    static const bool use_graphs = [] {
-        const char *v = std::getenv("JFEM_GRAPHS");
-        return !(v && v[0] == '0');
-    }();
-    if (!use_graphs) {
+    if (!jfem_use_graphs()) {
         for (int it = 0; it < batch; ++it) CHECK_LAUNCH(ctx, pcg_iteration<T>(ctx, op, it & 1, Minv, coarse));
         return JFM_OK;
     }
@@ -975,7 +980,13 @@ extern "C" int jfm_solve(jfm_ctx *ctx, const jfm_solve_opts *opts, const double 
             int it = 0, status = 0;
             int cap = opts->maxit - total;
             if (opts->inner_maxit > 0) cap = std::min(cap, opts->inner_maxit);
-            if (int rc = pcg_run<float>(ctx, op, Minv32, inner, cap, check, &it, &status, false, coarse)) return rc;
+            // when one inner solve tightened to at most inner^1.5 reaches the
+            // target, tighten it (with a factor 2 margin) rather than leave a
+            // residual just above the target for one more outer step
+            double rt = inner;
+            const double need = 0.5 * opts->rtol * bnorm / rn;
+            if (need < inner && need >= std::pow(inner, 1.5)) rt = need;
+            if (int rc = pcg_run<float>(ctx, op, Minv32, rt, cap, check, &it, &status, false, coarse)) return rc;
             total += it;
             ++outer;
             CHECK_LAUNCH(ctx, launch_axpy_scaled(ctx, ctx->u64, (const float *)ctx->ws32.x, rn, n));

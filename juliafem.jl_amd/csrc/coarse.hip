@@ -22,23 +22,27 @@
 //    one colour are at distance >= 3, so the restriction of that product to
 //    aggregate g is Q_g' K Q_{p,j} for the single colour-c aggregate p coupled
 //    to g.  No atomics; the result is deterministic.
-//  * rocSOLVER potrf + potri invert A_c (dense, a few thousand rows); the
-//    inverse is folded with the R_g^-1 into Ahat = R^-1 A_c^-1 R^-T (stored
-//    FP32), so the iteration needs only the raw modes: Q A_c^-1 Q' r =
-//    Bm Ahat Bm' r.
+//  * a block Gauss-Jordan sweep inverts A_c in place (dense, a few thousand
+//    rows; two launches per 32-row pivot block); the inverse is folded with
+//    the R_g^-1 into Ahat = R^-1 A_c^-1 R^-T (stored FP32), so the iteration
+//    needs only the raw modes: Q A_c^-1 Q' r = Bm Ahat Bm' r.
+//  * the whole device part is one launch sequence per aggregation and
+//    operator, captured once as a HIP graph and replayed.
 //
-// Per PCG iteration (after the update kernel has written z = B r):
-//  k_coarse_restrict  w_g = sum_{a in g} [r_a ; d_a x r_a]   (one block per g)
-//  k_coarse_gemv      zc = Ahat w                           (one wave per row)
-//  k_coarse_prolong   z_a += m_a o (t_g + omega_g x d_a), partials of r.z, r.r
+// Per PCG iteration (after the update kernel has written z = B r and the
+// partials of r.Br and r.r):
+//  k_coarse_restrict  w_g = sum_{a in g} [m_a r_a ; d_a x m_a r_a]  (block per g)
+//  k_coarse_gemv      zc = Ahat w, partials of w.zc             (wave per row)
+//  k_pcg_dir_coarse   p_a = z_a + m_a o (t_g + omega_g x d_a) + beta p_a
+// The prolongation is folded into the direction update, and r.z of the full
+// preconditioned residual is r.Br + w.zc (r.(m o Q zc) = (Q' m r).zc), so the
+// corrected z is never stored.  At the start of PCG, k_coarse_prolong adds
+// the correction to z explicitly.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <numeric>
 #include <vector>
-
-#include <rocblas/rocblas.h>
-#include <rocsolver/rocsolver.h>
 
 #include "jfem_device.h"
 
@@ -48,13 +52,20 @@ namespace jfem {
 
 static inline unsigned nblk(int64_t n, int b = kBlock) { return (unsigned)((n + b - 1) / b); }
 
-// v = sum over aggregates of colour c of Q_g e_j (masked), 0 elsewhere
-__global__ __launch_bounds__(256) void k_coarse_probe(int N, const int32_t *__restrict__ node_agg,
-                                                     const int32_t *__restrict__ agg_color, int c, int j,
+// v_j = sum over aggregates of colour c of Q_g e_j (masked) on the listed
+// nodes (a superset of the colour's nodes; v_j is zero elsewhere and stays
+// so); mode j = blockIdx.y, v_j = v + j vs
+__global__ __launch_bounds__(256) void k_coarse_probe(int nn, const int32_t *__restrict__ nlist,
+                                                     const int32_t *__restrict__ node_agg,
+                                                     const int32_t *__restrict__ agg_color, int c,
                                                      const double *__restrict__ cd, const double *__restrict__ Rinv,
-                                                     const uint8_t *__restrict__ fixed, double *__restrict__ v) {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= N) return;
+                                                     const uint8_t *__restrict__ fixed, double *__restrict__ v,
+                                                     int64_t vs) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nn) return;
+    const int j = blockIdx.y;
+    v += j * vs;
+    const int a = nlist[k];
     const int g = node_agg[a];
     double out[3] = {0.0, 0.0, 0.0};
     if (agg_color[g] == c) {
@@ -72,15 +83,17 @@ __global__ __launch_bounds__(256) void k_coarse_probe(int N, const int32_t *__re
     for (int k = 0; k < 3; ++k) v[3 * (size_t)a + k] = fixed[3 * (size_t)a + k] ? 0.0 : out[k];
 }
 
-// w~_g = sum_{a in g} [y_a ; d_a x y_a] in a fixed order (block per aggregate)
-template <typename T>
+// w~_g = sum_{a in g} [y_a ; d_a x y_a] in a fixed order (block per aggregate);
+// with CLEAR the rows read are set to zero (every node is in one aggregate)
+template <typename T, bool CLEAR = false>
 __device__ __forceinline__ void agg_sums(int g, const int32_t *__restrict__ agg_ptr, const int32_t *__restrict__ agg_nodes,
-                                         const T *__restrict__ cd, const T *__restrict__ y, double (&w)[6]) {
+                                         const T *__restrict__ cd, T *__restrict__ y, double (&w)[6]) {
 #pragma unroll
     for (int k = 0; k < 6; ++k) w[k] = 0.0;
     for (int k = agg_ptr[g] + threadIdx.x; k < agg_ptr[g + 1]; k += blockDim.x) {
         const size_t o = 3 * (size_t)agg_nodes[k];
         const double y0 = y[o], y1 = y[o + 1], y2 = y[o + 2];
+        if (CLEAR) { y[o] = T(0); y[o + 1] = T(0); y[o + 2] = T(0); }
         const double d0 = cd[o], d1 = cd[o + 1], d2 = cd[o + 2];
         w[0] += y0;
         w[1] += y1;
@@ -92,18 +105,22 @@ __device__ __forceinline__ void agg_sums(int g, const int32_t *__restrict__ agg_
     block_sum_all<6>(w);
 }
 
-// A_c rows of aggregate g for probe (c, j): Rinv_g' w~_g into column 6p + j
+// A_c rows of aggregate g for probe (c, j): Rinv_g' w~_g into column 6p + j;
+// mode j = blockIdx.y, its product at y + j ys
 __global__ __launch_bounds__(256) void k_coarse_probe_restrict(int n_agg, const int32_t *__restrict__ agg_ptr,
                                                               const int32_t *__restrict__ agg_nodes,
-                                                              const double *__restrict__ cd, const double *__restrict__ y,
-                                                              const int32_t *__restrict__ partner_c,
-                                                              const double *__restrict__ Rinv, int j, double *__restrict__ Ac,
+                                                              const double *__restrict__ cd, double *__restrict__ y,
+                                                              int64_t ys, const int32_t *__restrict__ partner_c,
+                                                              const double *__restrict__ Rinv, double *__restrict__ Ac,
                                                               int nc) {
-    const int g = blockIdx.x;
+    const int g = blockIdx.x, j = blockIdx.y;
+    y += j * ys;
     const int p = partner_c[g];
-    if (p < 0) return;  // uniform per block
+    // uniform per block; an aggregate with no colour-c partner has no node
+    // next to colour c, so its rows of y are zero and need no clearing
+    if (p < 0) return;
     double w[6];
-    agg_sums<double>(g, agg_ptr, agg_nodes, cd, y, w);
+    agg_sums<double, true>(g, agg_ptr, agg_nodes, cd, y, w);
     if (threadIdx.x < 6) {
         const int i = threadIdx.x;
         const double *R = Rinv + 36 * (size_t)g;
@@ -114,11 +131,19 @@ __global__ __launch_bounds__(256) void k_coarse_probe_restrict(int n_agg, const 
     }
 }
 
-// zero the incidence slots of the listed elements and the listed rows of y
+__global__ __launch_bounds__(256) void k_fill_zero(double *__restrict__ p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0.0;
+}
+
+// zero the incidence slots of the listed elements and the listed rows of y,
+// for vector blockIdx.y (slots at ye + j ys, rows at y + j vs)
 __global__ __launch_bounds__(256) void k_coarse_clear(int ne, const int32_t *__restrict__ elist, int npe, int n_el,
-                                                     const int32_t *__restrict__ slot, double *__restrict__ ye, int nn,
-                                                     const int32_t *__restrict__ nlist, double *__restrict__ y) {
+                                                     const int32_t *__restrict__ slot, double *__restrict__ ye, int64_t ys,
+                                                     int nn, const int32_t *__restrict__ nlist, double *__restrict__ y,
+                                                     int64_t vs) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    ye += blockIdx.y * ys;
+    y += blockIdx.y * vs;
     if (k < ne) {
         const int e = elist[k];
         for (int i = 0; i < npe; ++i) {
@@ -147,13 +172,168 @@ __global__ __launch_bounds__(256) void k_coarse_fix(int nc, double *__restrict__
     Ac[(size_t)j * nc + i] = s;
 }
 
-// Ahat = R^-1 A^-1 R^-T from the lower triangle (column-major) that potri
-// leaves: entry (r, s) of A^-1 with r >= s is at L[s * nc + r].
-__global__ __launch_bounds__(256) void k_coarse_fold(int nc, const double *__restrict__ L, const double *__restrict__ Rinv,
-                                                    const int32_t *__restrict__ dropped, float *__restrict__ A32) {
+// ---- inverse of A_c by block Gauss-Jordan (in place, no pivoting) ----
+// A_c is SPD, so every pivot block -- a Schur complement of A_c -- is SPD and
+// needs no pivoting.  Step K (rows/columns k0..k0+bk-1), with P = A_KK:
+//   A_KK <- P^-1,  A_Kj <- P^-1 A_Kj,  A_iK <- -A_iK P^-1,
+//   A_ij <- A_ij - (A_iK P^-1) A_Kj          (i, j outside K)
+// After the last step A holds A_c^-1.  A pivot of the small in-LDS sweep that
+// is not positive and finite flags the matrix as not positive definite.
+constexpr int kGJ = 32;  // pivot block
+
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+
+// P^-1 of the bk x bk block at (k0, k0) (padded to the identity) by scalar
+// Gauss-Jordan in the registers of wave 0: lane j holds column j & 31, and
+// column k -- the multipliers of step k -- is read from lane k.  No LDS or
+// barrier per step.  The result goes to P; every block of the panel kernel
+// computes it for itself.  Returns false (in wave 0) for a pivot that is not
+// positive and finite.
+__device__ __forceinline__ bool gj_pivot_inverse(int n, int k0, int bk, const double *__restrict__ A,
+                                                 double (*P)[kGJ + 1]) {
+    bool ok = true;
+    if (threadIdx.x < 64) {
+        const int j = threadIdx.x & (kGJ - 1);
+        double a[kGJ];
+#pragma unroll
+        for (int i = 0; i < kGJ; ++i)
+            a[i] = (i < bk && j < bk) ? A[(size_t)(k0 + i) * n + k0 + j] : (i == j ? 1.0 : 0.0);
+#pragma unroll
+        for (int k = 0; k < kGJ; ++k) {
+            if (k < bk) {
+                const double p = readlane_f64(a[k], k);
+                ok = ok && p > 0.0 && p < INFINITY;
+                const double ip = 1.0 / p;
+                const double rk = a[k] * ip;  // a_kj / p
+                const bool piv = j == k;
+#pragma unroll
+                for (int i = 0; i < kGJ; ++i) {
+                    if (i == k) continue;
+                    const double cik = readlane_f64(a[i], k);  // a_ik, read before lane k updates it
+                    a[i] = piv ? -cik * ip : a[i] - cik * rk;
+                }
+                a[k] = piv ? ip : rk;
+            }
+        }
+        if (threadIdx.x < kGJ)
+#pragma unroll
+            for (int i = 0; i < kGJ; ++i) P[i][j] = a[i];
+    }
+    __syncthreads();
+    return ok;
+}
+
+// Cbuf (n x kGJ) = A_:K P^-1, Rbuf (kGJ x n) = A_K:, Pbuf = P^-1; 8 rows of
+// Cbuf per block of 256 threads.
+__global__ __launch_bounds__(256) void k_gj_panel(int n, int k0, int bk, const double *__restrict__ A,
+                                                 double *__restrict__ Cbuf, double *__restrict__ Rbuf,
+                                                 double *__restrict__ Pbuf, int *__restrict__ flag) {
+    __shared__ double P[kGJ][kGJ + 1];
+    const bool ok = gj_pivot_inverse(n, k0, bk, A, P);  // valid in wave 0
+    if (blockIdx.x == 0) {
+        for (int t = threadIdx.x; t < kGJ * kGJ; t += blockDim.x) Pbuf[t] = P[t / kGJ][t - kGJ * (t / kGJ)];
+        if (threadIdx.x == 0 && !ok) flag[0] = 1;
+    }
+    const int row = blockIdx.x * 8 + threadIdx.x / kGJ, m = threadIdx.x & (kGJ - 1);
+    if (row < n) {
+        const double *Ar = A + (size_t)row * n + k0;
+        double c = 0.0;
+        for (int l = 0; l < bk; ++l) c += Ar[l] * P[l][m];
+        Cbuf[(size_t)row * kGJ + m] = c;
+    }
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < (int64_t)bk * n; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t mm = t / n, j = t - mm * n;
+        Rbuf[mm * n + j] = A[(size_t)(k0 + mm) * n + j];
+    }
+}
+
+// the update of step K over 64 x 64 tiles, 4 x 4 entries per thread
+__global__ __launch_bounds__(256) void k_gj_update(int n, int k0, int bk, double *__restrict__ A,
+                                                  const double *__restrict__ Cbuf, const double *__restrict__ Rbuf,
+                                                  const double *__restrict__ Pbuf) {
+    __shared__ double Cs[64][kGJ + 1], Rs[kGJ][65], Ps[kGJ][kGJ + 1];
+    const int tiles = (n + 63) / 64;
+    const int ti = blockIdx.x / tiles, tj = blockIdx.x - tiles * (blockIdx.x / tiles);
+    const int i0 = 64 * ti, j0 = 64 * tj;
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    // this thread's entries of A, in flight while the panels are staged
+    double av[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int i = i0 + ty + 16 * a, j = j0 + tx + 16 * b;
+            av[a][b] = (i < n && j < n) ? A[(size_t)i * n + j] : 0.0;
+        }
+    for (int t = threadIdx.x; t < 64 * kGJ; t += blockDim.x) {
+        const int a = t / kGJ, m = t - kGJ * (t / kGJ);
+        Cs[a][m] = (i0 + a < n && m < bk) ? Cbuf[(size_t)(i0 + a) * kGJ + m] : 0.0;
+        const int mm = t / 64, b = t - 64 * (t / 64);
+        Rs[mm][b] = (mm < bk && j0 + b < n) ? Rbuf[(size_t)mm * n + j0 + b] : 0.0;
+    }
+    for (int t = threadIdx.x; t < kGJ * kGJ; t += blockDim.x) Ps[t / kGJ][t - kGJ * (t / kGJ)] = Pbuf[t];
+    __syncthreads();
+    double acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+    for (int m = 0; m < bk; ++m) {
+        double cv[4], rv[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) cv[a] = Cs[ty + 16 * a][m];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) rv[b] = Rs[m][tx + 16 * b];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] += cv[a] * rv[b];
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const int i = i0 + ty + 16 * a;
+        if (i >= n) continue;
+        const bool inI = i >= k0 && i < k0 + bk;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int j = j0 + tx + 16 * b;
+            if (j >= n) continue;
+            const bool inJ = j >= k0 && j < k0 + bk;
+            double *o = A + (size_t)i * n + j;
+            if (!inI && !inJ) {
+                *o = av[a][b] - acc[a][b];
+            } else if (!inI) {
+                *o = -Cs[ty + 16 * a][j - k0];
+            } else if (!inJ) {
+                double v = 0.0;
+                for (int l = 0; l < bk; ++l) v += Ps[i - k0][l] * Rs[l][tx + 16 * b];
+                *o = v;
+            } else {
+                *o = Ps[i - k0][j - k0];
+            }
+        }
+    }
+}
+
+__global__ void k_flag_clear(int *flag) { flag[0] = 0; }
+
+// Ahat = R^-1 A^-1 R^-T from the upper triangle (row-major) of the inverse:
+// entry (r, s) of A^-1 with r >= s is read at L[s * nc + r].
+// Rows have stride ld; columns nc..ld-1 are set to zero.
+__global__ __launch_bounds__(256) void k_coarse_fold(int nc, int ld, const double *__restrict__ L,
+                                                    const double *__restrict__ Rinv, const int32_t *__restrict__ dropped,
+                                                    float *__restrict__ A32) {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (int64_t)nc * nc) return;
-    const int r = (int)(idx / nc), s = (int)(idx - (int64_t)r * nc);
+    if (idx >= (int64_t)nc * ld) return;
+    const int r = (int)(idx / ld), s = (int)(idx - (int64_t)r * ld);
+    if (s >= nc) {
+        A32[idx] = 0.f;
+        return;
+    }
     const int g = r / 6, i = r - 6 * g, h = s / 6, j = s - 6 * h;
     const double *Rg = Rinv + 36 * (size_t)g, *Rh = Rinv + 36 * (size_t)h;
     double acc = 0.0;
@@ -173,33 +353,70 @@ __global__ __launch_bounds__(256) void k_coarse_fold(int nc, const double *__res
     A32[idx] = (float)acc;
 }
 
+// w_g = sum_{a in g} [m_a r_a ; d_a x (m_a r_a)], one 1024-thread block per
+// aggregate (about two nodes per thread); cdA is cd in agg_nodes order so its
+// reads are contiguous.  Written in FP32 (padded row of the gemv) and FP64
+// (for w.zc).
 template <typename T>
-__global__ __launch_bounds__(256) void k_coarse_restrict(int n_agg, const int32_t *__restrict__ agg_ptr,
-                                                        const int32_t *__restrict__ agg_nodes, const T *__restrict__ cd,
-                                                        const T *__restrict__ r, float *__restrict__ wc,
-                                                        const int *__restrict__ done) {
-    if (done && *done) return;
+__global__ __launch_bounds__(1024) void k_coarse_restrict(int n_agg, const int32_t *__restrict__ agg_ptr,
+                                                         const int32_t *__restrict__ agg_nodes, const T *__restrict__ cdA,
+                                                         const uint8_t *__restrict__ fixed, const T *__restrict__ r,
+                                                         float *__restrict__ wc, double *__restrict__ wc64,
+                                                         const int *__restrict__ done) {
+    // the reads below are harmless past convergence, so the flag is tested
+    // after them and its latency overlaps theirs
+    const int dn = done ? *done : 0;
     const int g = blockIdx.x;
-    double w[6];
-    agg_sums<T>(g, agg_ptr, agg_nodes, cd, r, w);
-    if (threadIdx.x < 6) wc[6 * g + threadIdx.x] = (float)w[threadIdx.x];
+    double w[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int k = agg_ptr[g] + threadIdx.x; k < agg_ptr[g + 1]; k += blockDim.x) {
+        const size_t o = 3 * (size_t)agg_nodes[k];
+        const double y0 = fixed[o] ? 0.0 : (double)r[o];
+        const double y1 = fixed[o + 1] ? 0.0 : (double)r[o + 1];
+        const double y2 = fixed[o + 2] ? 0.0 : (double)r[o + 2];
+        const double d0 = cdA[3 * (size_t)k], d1 = cdA[3 * (size_t)k + 1], d2 = cdA[3 * (size_t)k + 2];
+        w[0] += y0; w[1] += y1; w[2] += y2;
+        w[3] += d1 * y2 - d2 * y1;
+        w[4] += d2 * y0 - d0 * y2;
+        w[5] += d0 * y1 - d1 * y0;
+    }
+    if (dn) return;  // uniform
+    block_sum_all<6>(w);
+    if (threadIdx.x < 6) {
+        wc[6 * g + threadIdx.x] = (float)w[threadIdx.x];
+        wc64[6 * g + threadIdx.x] = w[threadIdx.x];
+    }
 }
 
-// zc = Ahat w, one 64-lane wave per row, fixed-order lane sums.  Ahat is
+// zc = Ahat w, one 64-lane wave per row (grid-stride over rows), float4 loads
+// of the zero-padded rows, FP64 lane sums added in a fixed order.  Ahat is
 // kept in FP32 for both precisions: it only scales the coarse correction of a
 // fixed SPD preconditioner, so its rounding changes the iteration count at
 // most, never the answer, and it halves the dominant read of the level.
-__global__ __launch_bounds__(256) void k_coarse_gemv(int nc, const float *__restrict__ A, const float *__restrict__ w,
-                                                    float *__restrict__ zc, const int *__restrict__ done) {
-    if (done && *done) return;
-    const int row = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+// Block partials of w.zc (w in FP64) go to `part`.
+__global__ __launch_bounds__(256) void k_coarse_gemv(int nc, int ld4, const float4 *__restrict__ A,
+                                                    const float4 *__restrict__ w, const double *__restrict__ w64,
+                                                    float *__restrict__ zc, double *__restrict__ part,
+                                                    const int *__restrict__ done) {
+    const int dn = done ? *done : 0;  // tested once the first row is in flight
     const int lane = threadIdx.x & 63;
-    if (row >= nc) return;  // whole wave leaves together
-    const float *Ar = A + (size_t)row * nc;
-    double s = 0.0;
-    for (int k = lane; k < nc; k += 64) s += (double)Ar[k] * (double)w[k];
-    s = wave_sum(s);
-    if (lane == 0) zc[row] = (float)s;
+    double v[1] = {0.0};
+    for (int row = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; row < nc; row += gridDim.x * (blockDim.x / 64)) {
+        const float4 *Ar = A + (size_t)row * ld4;
+        double s = 0.0;
+        for (int k = lane; k < ld4; k += 64) {
+            const float4 a = Ar[k], b = w[k];
+            s += (double)a.x * b.x + (double)a.y * b.y + (double)a.z * b.z + (double)a.w * b.w;
+        }
+        if (dn) return;  // uniform
+        const double sd = wave_sum(s);
+        const float z = (float)sd;
+        if (lane == 0) {
+            zc[row] = z;
+            v[0] += w64[row] * (double)z;
+        }
+    }
+    if (dn) return;  // every thread leaves before the block reduction
+    write_partials<1>(v, part);
 }
 
 // z_a += m_a o (t_g + omega_g x d_a); optionally p = z; partials (r.z, r.r)
@@ -478,18 +695,75 @@ void jfem_coarse_release(jfm_ctx *ctx) {
     free_dev(ctx, ctx->node_agg); free_dev(ctx, ctx->agg_ptr); free_dev(ctx, ctx->agg_nodes);
     free_dev(ctx, ctx->agg_color); free_dev(ctx, ctx->partner); free_dev(ctx, ctx->cd64); free_dev(ctx, ctx->cd32);
     free_dev(ctx, ctx->Rinv); free_dev(ctx, ctx->Ac); free_dev(ctx, ctx->Ahat32);
-    free_dev(ctx, ctx->wc32); free_dev(ctx, ctx->zc32);
+    free_dev(ctx, ctx->wc32); free_dev(ctx, ctx->zc32); free_dev(ctx, ctx->wc64);
+    free_dev(ctx, ctx->cdA64); free_dev(ctx, ctx->cdA32);
+    free_dev(ctx, ctx->gjC); free_dev(ctx, ctx->gjR); free_dev(ctx, ctx->gjP);
+    free_dev(ctx, ctx->pv6); free_dev(ctx, ctx->qv6); free_dev(ctx, ctx->ye6);
     free_dev(ctx, ctx->dropped); free_dev(ctx, ctx->ecol); free_dev(ctx, ctx->ncol);
     ctx->h_ecol_ptr.clear(); ctx->h_ncol_ptr.clear();
+    for (int k = 0; k < 2; ++k)
+        if (ctx->coarse_graph[k]) {
+            (void)hipGraphExecDestroy(ctx->coarse_graph[k]);
+            ctx->coarse_graph[k] = nullptr;
+        }
     ctx->coarse_ready = false;
     ctx->coarse_tag = -1;
     ctx->coarse_built_nodes = 0;
     ctx->n_agg = ctx->nc = 0;
 }
 
-void jfem_coarse_destroy_handle(jfm_ctx *ctx) {
-    if (ctx->blas) (void)rocblas_destroy_handle((rocblas_handle)ctx->blas);
-    ctx->blas = nullptr;
+// Zero A_c, then one batched product of the six modes per colour over the
+// colour's elements and nodes only; the slots and rows a colour wrote are
+// cleared before the next colour, so pv6, qv6 and ye6 are zero between
+// colours (as after their allocation).  No memset nodes, so the sequence can
+// be captured.
+static hipError_t probe_sequence(jfm_ctx *ctx, bool tangent) {
+    hipStream_t s = ctx->stream;
+    const int64_t N3 = 3 * ctx->N;
+    const int nc = (int)ctx->nc, n_agg = (int)ctx->n_agg;
+    const int64_t nslots = 3 * (int64_t)ctx->npe * ctx->n_el;
+    k_fill_zero<<<1024, kBlock, 0, s>>>(ctx->Ac, (int64_t)nc * nc);
+    hipError_t e = hipGetLastError();
+    for (int c = 0; c < ctx->ncolor && e == hipSuccess; ++c) {
+        const int e0 = ctx->h_ecol_ptr[c], ne = ctx->h_ecol_ptr[c + 1] - e0;
+        const int n0 = ctx->h_ncol_ptr[c], nn = ctx->h_ncol_ptr[c + 1] - n0;
+        if (nn == 0) continue;
+        k_coarse_probe<<<dim3(nblk(nn), 6), kBlock, 0, s>>>(nn, ctx->ncol + n0, ctx->node_agg, ctx->agg_color, c,
+                                                            ctx->cd64, ctx->Rinv, ctx->fixed, ctx->pv6, N3);
+        e = launch_subset_apply(ctx, tangent, ctx->pv6, ctx->qv6, ctx->ye6, 6, N3, nslots, ctx->ecol + e0, ne,
+                                ctx->ncol + n0, nn);
+        if (e != hipSuccess) break;
+        // reads qv6 and sets the rows it read back to zero
+        k_coarse_probe_restrict<<<dim3(n_agg, 6), kBlock, 0, s>>>(n_agg, ctx->agg_ptr, ctx->agg_nodes, ctx->cd64,
+                                                                  ctx->qv6, N3, ctx->partner + (size_t)c * n_agg,
+                                                                  ctx->Rinv, ctx->Ac, nc);
+        k_coarse_clear<<<dim3(nblk(std::max(ne, nn)), 6), kBlock, 0, s>>>(ne, ctx->ecol + e0, ctx->npe, (int)ctx->n_el,
+                                                                          ctx->inc_slot, ctx->ye6, nslots, nn,
+                                                                          ctx->ncol + n0, ctx->pv6, N3);
+        e = hipGetLastError();
+    }
+    return e;
+}
+
+// A_c by probing, its inverse and the fold into Ahat32; a non-SPD pivot sets
+// the flag at ctx->counter + 48.
+static hipError_t setup_sequence(jfm_ctx *ctx, bool tangent) {
+    hipStream_t s = ctx->stream;
+    const int nc = (int)ctx->nc;
+    int *flag = (int *)(ctx->counter + 48);
+    k_flag_clear<<<1, 1, 0, s>>>(flag);
+    hipError_t e = probe_sequence(ctx, tangent);
+    if (e != hipSuccess) return e;
+    const int64_t nn = (int64_t)nc * nc;
+    k_coarse_fix<<<nblk(nn), kBlock, 0, s>>>(nc, ctx->Ac);
+    const int tiles = (nc + 63) / 64;
+    for (int k0 = 0; k0 < nc; k0 += kGJ) {
+        const int bk = std::min(kGJ, nc - k0);
+        k_gj_panel<<<(nc + 7) / 8, 256, 0, s>>>(nc, k0, bk, ctx->Ac, ctx->gjC, ctx->gjR, ctx->gjP, flag);
+        k_gj_update<<<tiles * tiles, 256, 0, s>>>(nc, k0, bk, ctx->Ac, ctx->gjC, ctx->gjR, ctx->gjP);
+    }
+    k_coarse_fold<<<nblk(nc * ctx->ldc), kBlock, 0, s>>>(nc, (int)ctx->ldc, ctx->Ac, ctx->Rinv, ctx->dropped, ctx->Ahat32);
+    return hipGetLastError();
 }
 
 // Builds the coarse level for operator `op` (FP64, masked).  The aggregation
@@ -532,9 +806,24 @@ int jfem_coarse_setup(jfm_ctx *ctx, int op, int coarse_nodes, double *ms) {
         CALLOC(ctx->ecol, int32_t, std::max<int64_t>(1, (int64_t)A.ecol.size()));
         CALLOC(ctx->ncol, int32_t, std::max<int64_t>(1, (int64_t)A.ncol.size()));
         CALLOC(ctx->Ac, double, nc * nc);
-        CALLOC(ctx->Ahat32, float, nc * nc);
-        CALLOC(ctx->wc32, float, nc);
+        CALLOC(ctx->gjC, double, nc * kGJ);
+        CALLOC(ctx->gjR, double, nc * kGJ);
+        CALLOC(ctx->gjP, double, kGJ * kGJ);
+        const int64_t nslots = 3 * (int64_t)ctx->npe * ctx->n_el;
+        CALLOC(ctx->pv6, double, 6 * 3 * N);
+        CALLOC(ctx->qv6, double, 6 * 3 * N);
+        CALLOC(ctx->ye6, double, 6 * nslots);
+        JFM_HIP(ctx, hipMemsetAsync(ctx->pv6, 0, sizeof(double) * 18 * N, s));
+        JFM_HIP(ctx, hipMemsetAsync(ctx->qv6, 0, sizeof(double) * 18 * N, s));
+        JFM_HIP(ctx, hipMemsetAsync(ctx->ye6, 0, sizeof(double) * 6 * nslots, s));
+        const int64_t ldc = (nc + 255) / 256 * 256;
+        ctx->ldc = ldc;
+        CALLOC(ctx->Ahat32, float, nc * ldc);
+        CALLOC(ctx->wc32, float, ldc);
+        CALLOC(ctx->wc64, double, nc);
         CALLOC(ctx->zc32, float, nc);
+        CALLOC(ctx->cdA64, double, 3 * N);
+        CALLOC(ctx->cdA32, float, 3 * N);
 #undef CALLOC
         JFM_HIP(ctx, hipMemcpyAsync(ctx->node_agg, A.node_agg.data(), 4 * N, hipMemcpyHostToDevice, s));
         JFM_HIP(ctx, hipMemcpyAsync(ctx->agg_ptr, A.agg_ptr.data(), 4 * (A.n_agg + 1), hipMemcpyHostToDevice, s));
@@ -543,6 +832,13 @@ int jfem_coarse_setup(jfm_ctx *ctx, int op, int coarse_nodes, double *ms) {
         JFM_HIP(ctx, hipMemcpyAsync(ctx->partner, A.partner.data(), 4 * A.partner.size(), hipMemcpyHostToDevice, s));
         JFM_HIP(ctx, hipMemcpyAsync(ctx->cd64, A.cd.data(), 8 * 3 * N, hipMemcpyHostToDevice, s));
         JFM_HIP(ctx, hipMemcpyAsync(ctx->cd32, cd32.data(), 4 * 3 * N, hipMemcpyHostToDevice, s));
+        std::vector<double> cdA(3 * N);
+        for (int64_t k = 0; k < N; ++k)
+            for (int c = 0; c < 3; ++c) cdA[3 * k + c] = A.cd[3 * (size_t)A.agg_nodes[k] + c];
+        std::vector<float> cdA32(cdA.begin(), cdA.end());
+        JFM_HIP(ctx, hipMemcpyAsync(ctx->cdA64, cdA.data(), 8 * 3 * N, hipMemcpyHostToDevice, s));
+        JFM_HIP(ctx, hipMemcpyAsync(ctx->cdA32, cdA32.data(), 4 * 3 * N, hipMemcpyHostToDevice, s));
+        JFM_HIP(ctx, hipMemsetAsync(ctx->wc32, 0, 4 * ldc, s));  // padding stays zero
         JFM_HIP(ctx, hipMemcpyAsync(ctx->Rinv, A.Rinv.data(), 8 * A.Rinv.size(), hipMemcpyHostToDevice, s));
         JFM_HIP(ctx, hipMemcpyAsync(ctx->dropped, A.dropped.data(), 4 * nc, hipMemcpyHostToDevice, s));
         JFM_HIP(ctx, hipMemcpyAsync(ctx->ecol, A.ecol.data(), 4 * A.ecol.size(), hipMemcpyHostToDevice, s));
@@ -553,58 +849,48 @@ int jfem_coarse_setup(jfm_ctx *ctx, int op, int coarse_nodes, double *ms) {
         ctx->coarse_built_nodes = coarse_nodes;
         ++ctx->coarse_gen;
     }
-    const int nc = (int)ctx->nc, n_agg = (int)ctx->n_agg;
-    // A_c by probing; the FP64 PCG direction vectors are free before a solve
-    double *pv = (double *)ctx->ws64.p, *qv = (double *)ctx->ws64.q;
-    if (!pv || !qv) return jfm_fail(ctx, JFM_ESTATE, "coarse setup needs the solver workspaces");
-    JFM_HIP(ctx, hipMemsetAsync(ctx->Ac, 0, sizeof(double) * (size_t)nc * nc, s));
-    // The products touch only the elements next to colour c and their nodes:
-    // every other incidence slot and row of qv is kept at zero (cleared once
-    // here, and after each product for the rows and slots it wrote).
+    // the same launch sequence every time for a given aggregation and
+    // operator, so it is captured once as a HIP graph
     const bool tangent = op == JFM_OP_TANGENT;
     if (tangent && !ctx->has_state) return jfm_fail(ctx, JFM_ESTATE, "tangent coarse level needs a state");
-    JFM_HIP(ctx, hipMemsetAsync(ctx->ye64, 0, sizeof(double) * 3 * (size_t)ctx->npe * ctx->n_el, s));
-    JFM_HIP(ctx, hipMemsetAsync(qv, 0, sizeof(double) * 3 * (size_t)N, s));
-    for (int c = 0; c < ctx->ncolor; ++c) {
-        const int e0 = ctx->h_ecol_ptr[c], ne = ctx->h_ecol_ptr[c + 1] - e0;
-        const int n0 = ctx->h_ncol_ptr[c], nn = ctx->h_ncol_ptr[c + 1] - n0;
-        for (int j = 0; j < 6; ++j) {
-            k_coarse_probe<<<nblk(N), kBlock, 0, s>>>((int)N, ctx->node_agg, ctx->agg_color, c, j, ctx->cd64, ctx->Rinv,
-                                                      ctx->fixed, pv);
-            JFM_HIP(ctx, hipGetLastError());
-            JFM_HIP(ctx, launch_subset_apply(ctx, tangent, pv, qv, ctx->ecol + e0, ne, ctx->ncol + n0, nn));
-            k_coarse_probe_restrict<<<n_agg, kBlock, 0, s>>>(n_agg, ctx->agg_ptr, ctx->agg_nodes, ctx->cd64, qv,
-                                                             ctx->partner + (size_t)c * n_agg, ctx->Rinv, j, ctx->Ac, nc);
-            JFM_HIP(ctx, hipGetLastError());
-            if (std::max(ne, nn) > 0) {
-                k_coarse_clear<<<nblk(std::max(ne, nn)), kBlock, 0, s>>>(ne, ctx->ecol + e0, ctx->npe, (int)ctx->n_el,
-                                                                          ctx->inc_slot, ctx->ye64, nn, ctx->ncol + n0, qv);
-                JFM_HIP(ctx, hipGetLastError());
+    hipError_t seq_err = hipSuccess;
+    if (jfem_use_graphs()) {
+        hipGraphExec_t &ex = ctx->coarse_graph[tangent ? 1 : 0];
+        int &gen = ctx->coarse_graph_gen[tangent ? 1 : 0];
+        if (ex && gen != ctx->coarse_gen) {
+            (void)hipGraphExecDestroy(ex);
+            ex = nullptr;
+        }
+        if (!ex) {
+            hipGraph_t graph = nullptr;
+            hipError_t le = hipSuccess;
+            bool ok = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess;
+            if (ok) {
+                le = setup_sequence(ctx, tangent);
+                ok = hipStreamEndCapture(s, &graph) == hipSuccess && le == hipSuccess;
+            }
+            if (ok) ok = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0) == hipSuccess;
+            if (graph) (void)hipGraphDestroy(graph);
+            (void)hipGetLastError();
+            if (!ok) {
+                if (ex) (void)hipGraphExecDestroy(ex);
+                ex = nullptr;
+            } else {
+                gen = ctx->coarse_gen;
             }
         }
+        if (ex) JFM_HIP(ctx, hipGraphLaunch(ex, s));
+        else seq_err = setup_sequence(ctx, tangent);
+    } else {
+        seq_err = setup_sequence(ctx, tangent);
     }
-    const int64_t nn = (int64_t)nc * nc;
-    k_coarse_fix<<<nblk(nn), kBlock, 0, s>>>(nc, ctx->Ac);
-    JFM_HIP(ctx, hipGetLastError());
-    // Cholesky inverse (rocSOLVER, column-major lower = our upper; A_c is symmetric)
-    if (!ctx->blas) {
-        rocblas_handle h = nullptr;
-        if (rocblas_create_handle(&h) != rocblas_status_success) return jfm_fail(ctx, JFM_EHIP, "rocblas_create_handle failed");
-        ctx->blas = h;
+    if (seq_err != hipSuccess) {
+        // the probe buffers may no longer be zero: drop the level
+        jfem_coarse_release(ctx);
+        JFM_HIP(ctx, seq_err);
     }
-    rocblas_handle h = (rocblas_handle)ctx->blas;
-    if (rocblas_set_stream(h, s) != rocblas_status_success) return jfm_fail(ctx, JFM_EHIP, "rocblas_set_stream failed");
-    int *info = (int *)(ctx->counter + 48);  // small integer scratch
-    if (rocsolver_dpotrf(h, rocblas_fill_lower, nc, ctx->Ac, nc, info) != rocblas_status_success)
-        return jfm_fail(ctx, JFM_EHIP, "rocsolver_dpotrf failed");
-    int hinfo = 0;
-    JFM_HIP(ctx, hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, s));
-    JFM_HIP(ctx, hipStreamSynchronize(s));
-    if (hinfo != 0) return jfm_fail(ctx, JFM_ENOTSPD, "coarse matrix is not positive definite");
-    if (rocsolver_dpotri(h, rocblas_fill_lower, nc, ctx->Ac, nc, info) != rocblas_status_success)
-        return jfm_fail(ctx, JFM_EHIP, "rocsolver_dpotri failed");
-    k_coarse_fold<<<nblk(nn), kBlock, 0, s>>>(nc, ctx->Ac, ctx->Rinv, ctx->dropped, ctx->Ahat32);
-    JFM_HIP(ctx, hipGetLastError());
+    int hflag = 0;
+    JFM_HIP(ctx, hipMemcpyAsync(&hflag, ctx->counter + 48, sizeof(int), hipMemcpyDeviceToHost, s));
     JFM_HIP(ctx, hipEventRecord(e1, s));
     JFM_HIP(ctx, hipEventSynchronize(e1));
     float t = 0.f;
@@ -612,6 +898,7 @@ int jfem_coarse_setup(jfm_ctx *ctx, int op, int coarse_nodes, double *ms) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (ms) *ms += t;
+    if (hflag != 0) return jfm_fail(ctx, JFM_ENOTSPD, "coarse matrix is not positive definite");
     ctx->coarse_ready = true;
     ctx->coarse_tag = op == JFM_OP_TANGENT ? 1 : 0;
     return JFM_OK;
@@ -619,19 +906,41 @@ int jfem_coarse_setup(jfm_ctx *ctx, int op, int coarse_nodes, double *ms) {
 
 // coarse correction after the update kernel of a PCG iteration (or after the
 // init kernel with set_p): z += Q A_c^-1 Q' r, partials (r.z, r.r) to `part`
+static unsigned gemv_grid(int64_t nc) { return (unsigned)std::min<int64_t>((nc + 3) / 4, kMaxPart); }
+
+template <typename T>
+static void coarse_product(jfm_ctx *ctx, const T *r, const int *done, double *part_wz) {
+    const T *cdA = sizeof(T) == 8 ? (const T *)ctx->cdA64 : (const T *)ctx->cdA32;
+    const int n_agg = (int)ctx->n_agg, nc = (int)ctx->nc;
+    hipStream_t s = ctx->stream;
+    k_coarse_restrict<T><<<n_agg, 1024, 0, s>>>(n_agg, ctx->agg_ptr, ctx->agg_nodes, cdA, ctx->fixed, r, ctx->wc32,
+                                               ctx->wc64, done);
+    k_coarse_gemv<<<gemv_grid(nc), kBlock, 0, s>>>(nc, (int)(ctx->ldc / 4), (const float4 *)ctx->Ahat32,
+                                                  (const float4 *)ctx->wc32, ctx->wc64, ctx->zc32, part_wz, done);
+}
+
+template <typename T>
+hipError_t launch_coarse_product(jfm_ctx *ctx, const T *r, const int *done, int *npart_wz) {
+    coarse_product<T>(ctx, r, done, ctx->partials + kPartCoarse);
+    *npart_wz = (int)gemv_grid(ctx->nc);
+    return hipGetLastError();
+}
+
+// coarse correction outside the iteration (PCG start, training):
+// z += Q A_c^-1 Q' r, optionally p = z; partials (r.z, r.r) to `part`
 template <typename T>
 hipError_t launch_coarse_correct(jfm_ctx *ctx, const T *r, T *z, T *p, double *part, const int *done, int *npart) {
     const T *cd = sizeof(T) == 8 ? (const T *)ctx->cd64 : (const T *)ctx->cd32;
-    const int n_agg = (int)ctx->n_agg, nc = (int)ctx->nc;
     hipStream_t s = ctx->stream;
-    k_coarse_restrict<T><<<n_agg, kBlock, 0, s>>>(n_agg, ctx->agg_ptr, ctx->agg_nodes, cd, r, ctx->wc32, done);
-    k_coarse_gemv<<<nblk(nc, kBlock / 64), kBlock, 0, s>>>(nc, ctx->Ahat32, ctx->wc32, ctx->zc32, done);
+    coarse_product<T>(ctx, r, done, ctx->partials + kPartCoarse);
     const unsigned g = std::min<unsigned>(capped_grid(ctx->N, kBlock), 512);
     *npart = (int)g;
     k_coarse_prolong<T><<<g, kBlock, 0, s>>>((int)ctx->N, ctx->node_agg, cd, ctx->fixed, ctx->zc32, r, z, p, part, done);
     return hipGetLastError();
 }
 
+template hipError_t launch_coarse_product<double>(jfm_ctx *, const double *, const int *, int *);
+template hipError_t launch_coarse_product<float>(jfm_ctx *, const float *, const int *, int *);
 template hipError_t launch_coarse_correct<double>(jfm_ctx *, const double *, double *, double *, double *, const int *, int *);
 template hipError_t launch_coarse_correct<float>(jfm_ctx *, const float *, float *, float *, double *, const int *, int *);
 

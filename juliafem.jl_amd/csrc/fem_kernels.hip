@@ -106,13 +106,15 @@ __global__ __launch_bounds__(256) void k_mf_elem(int n_el, const int32_t *__rest
                                                 const int32_t *__restrict__ slot, const T *__restrict__ gf,
                                                 const T *__restrict__ x, T *__restrict__ ye,
                                                 const int *__restrict__ done, const int32_t *__restrict__ elist,
-                                                int n_items) {
+                                                int n_items, int64_t xs = 0, int64_t ys = 0) {
     using EL = Elem<NPE>;
     constexpr int NG = EL::NGP;
     if (done && *done) return;
     const int idx = xcd_block() * blockDim.x + threadIdx.x;
     if (idx >= n_items) return;
     const int e = elist ? elist[idx] : idx;   // optional element subset
+    x += blockIdx.y * xs;                     // vector blockIdx.y of a batch
+    ye += blockIdx.y * ys;
     T H[NG][9];
 #pragma unroll
     for (int q = 0; q < NG; ++q)
@@ -243,8 +245,11 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_gather(int N, const int32_t *__restrict__ ptr, const T *__restrict__ yinc,
                                                const uint8_t *__restrict__ fixed, T *__restrict__ y,
                                                const T *__restrict__ dotv, double *__restrict__ part,
-                                               const int *__restrict__ done, const int32_t *__restrict__ nlist) {
+                                               const int *__restrict__ done, const int32_t *__restrict__ nlist,
+                                               int64_t ys = 0, int64_t ns = 0) {
     if (done && *done) return;
+    yinc += blockIdx.y * ys;   // vector blockIdx.y of a batch
+    y += blockIdx.y * ns;
     double dv = 0.0;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < N; k += gridDim.x * blockDim.x) {
         const int a = nlist ? nlist[k] : k;   // optional node subset (N items)
@@ -563,38 +568,42 @@ hipError_t launch_gather(jfm_ctx *ctx, const T *ye, const uint8_t *fixed, T *y, 
     return hipGetLastError();
 }
 
-// y = K x (masked, FP64) restricted to the elements `elist` and the nodes
-// `nlist`: only those element slots are written and only those rows of y.
-// Every other incidence slot must hold zero (the caller keeps it so).
+// y_j = K x_j (masked, FP64) for the `nv` vectors x_j = x + j xs (y_j, and
+// the incidence slots ye_j = ye + j ys, likewise), restricted to the elements
+// `elist` and the nodes `nlist`: only those element slots are written and only
+// those rows of y_j.  Every other incidence slot must hold zero (the caller
+// keeps it so).
 template <int NPE>
-static void subset_elem(jfm_ctx *ctx, const double *x, const int32_t *elist, int ne) {
+static void subset_elem(jfm_ctx *ctx, const double *x, double *ye, int nv, int64_t xs, int64_t ys,
+                        const int32_t *elist, int ne) {
     const int n = (int)ctx->n_el;
+    const dim3 grid(xcd_grid(nblk(ne)), nv);
     if (ctx->affine)
-        k_mf_elem<NPE, double, true><<<xcd_grid(nblk(ne)), kBlock, 0, ctx->stream>>>(n, ctx->conn, ctx->inc_slot, ctx->gf64, x,
-                                                                                    ctx->ye64, nullptr, elist, ne);
+        k_mf_elem<NPE, double, true><<<grid, kBlock, 0, ctx->stream>>>(n, ctx->conn, ctx->inc_slot, ctx->gf64, x, ye,
+                                                                      nullptr, elist, ne, xs, ys);
     else
-        k_mf_elem<NPE, double, false><<<xcd_grid(nblk(ne)), kBlock, 0, ctx->stream>>>(n, ctx->conn, ctx->inc_slot, ctx->gf64, x,
-                                                                                     ctx->ye64, nullptr, elist, ne);
+        k_mf_elem<NPE, double, false><<<grid, kBlock, 0, ctx->stream>>>(n, ctx->conn, ctx->inc_slot, ctx->gf64, x, ye,
+                                                                       nullptr, elist, ne, xs, ys);
 }
 
-hipError_t launch_subset_apply(jfm_ctx *ctx, bool tangent, const double *x, double *y, const int32_t *elist, int ne,
-                               const int32_t *nlist, int nn) {
+hipError_t launch_subset_apply(jfm_ctx *ctx, bool tangent, const double *x, double *y, double *ye, int nv, int64_t xs,
+                               int64_t ys, const int32_t *elist, int ne, const int32_t *nlist, int nn) {
     if (ne > 0) {
         if (tangent) {
-            hipError_t e = launch_nh_elem<double>(ctx, true, ctx->gf64, ctx->ustate64, x, ctx->ye64, nullptr, elist, ne);
+            hipError_t e = launch_nh_elem<double>(ctx, true, ctx->gf64, ctx->ustate64, x, ye, nullptr, elist, ne, nv, xs, ys);
             if (e != hipSuccess) return e;
         } else {
             switch (ctx->npe) {
-            case 10: subset_elem<10>(ctx, x, elist, ne); break;
-            case 4: subset_elem<4>(ctx, x, elist, ne); break;
-            case 8: subset_elem<8>(ctx, x, elist, ne); break;
+            case 10: subset_elem<10>(ctx, x, ye, nv, xs, ys, elist, ne); break;
+            case 4: subset_elem<4>(ctx, x, ye, nv, xs, ys, elist, ne); break;
+            case 8: subset_elem<8>(ctx, x, ye, nv, xs, ys, elist, ne); break;
             default: return hipErrorInvalidValue;
             }
         }
     }
     if (nn > 0)
-        k_gather<double><<<capped_grid(nn, kBlock), kBlock, 0, ctx->stream>>>(nn, ctx->inc_ptr, ctx->ye64, ctx->fixed, y,
-                                                                             nullptr, nullptr, nullptr, nlist);
+        k_gather<double><<<dim3(capped_grid(nn, kBlock), nv), kBlock, 0, ctx->stream>>>(
+            nn, ctx->inc_ptr, ye, ctx->fixed, y, nullptr, nullptr, nullptr, nlist, ys, xs);
     return hipGetLastError();
 }
 

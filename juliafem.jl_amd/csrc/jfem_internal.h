@@ -133,7 +133,8 @@ constexpr int kMaxPart = 1024;      // producer grids are capped at this many bl
 inline constexpr int part_pq(int slot) { return slot * kMaxPart; }
 inline constexpr int part_rz(int slot) { return 2 * kMaxPart + slot * 2 * kMaxPart; }
 constexpr int kPartMisc = 6 * kMaxPart;
-constexpr int kPartTotal = 8 * kMaxPart;
+constexpr int kPartCoarse = 8 * kMaxPart;  // w.zc partials of the coarse product
+constexpr int kPartTotal = 9 * kMaxPart;
 
 // --------------------------------------------------------------- context --
 struct Workspace {        // PCG vectors of one precision
@@ -238,14 +239,22 @@ struct jfm_ctx {
     int32_t *partner = nullptr;    // ncolor x n_agg: the coupled aggregate of each colour
     int32_t *dropped = nullptr;    // nc: mode dropped (dependent on its aggregate)
     int32_t *ecol = nullptr, *ncol = nullptr;        // per colour: probed elements, nodes
+    hipGraphExec_t coarse_graph[2] = {nullptr, nullptr};  // probe sequence for K, K_T
+    int coarse_graph_gen[2] = {0, 0};
     std::vector<int32_t> h_ecol_ptr, h_ncol_ptr;
     double *cd64 = nullptr;        // N x 3 offsets from the aggregate centroid
     float *cd32 = nullptr;
     double *Rinv = nullptr;        // n_agg x 36
-    double *Ac = nullptr;          // nc x nc (factorised in place)
-    float *Ahat32 = nullptr;       // nc x nc: R^-1 A_c^-1 R^-T
-    float *wc32 = nullptr, *zc32 = nullptr;
-    void *blas = nullptr;          // rocblas_handle
+    double *Ac = nullptr;          // nc x nc (inverted in place)
+    double *cdA64 = nullptr;       // cd in aggregate order (agg_nodes), for the restriction
+    float *cdA32 = nullptr;
+    int64_t ldc = 0;               // row stride of Ahat32: nc rounded up to 256, zero padded
+    float *Ahat32 = nullptr;       // nc x ldc: R^-1 A_c^-1 R^-T
+    float *wc32 = nullptr, *zc32 = nullptr;  // wc32 has ldc entries, zero past nc
+    double *wc64 = nullptr;        // Q' m r of the last restriction
+    double *gjC = nullptr, *gjR = nullptr, *gjP = nullptr;  // Gauss-Jordan panels
+    double *pv6 = nullptr, *qv6 = nullptr;  // probe vectors, 6 x 3N (zero outside the probed rows)
+    double *ye6 = nullptr;                  // probe incidence slots, 6 x 3 npe n_el (zero outside the probed elements)
     // heat conduction (heat.hip): temperature in component 0
     double *heat_k = nullptr;      // n_el conductivities
     uint8_t *heat_fixed = nullptr; // 3N row mask of the heat system
@@ -300,17 +309,25 @@ hipError_t launch_two_phase(jfm_ctx *ctx, int op, const T *x, T *y, const T *dot
 // or K_T(u) v (tangent = true) into the incidence slots
 template <typename T>
 hipError_t launch_nh_elem(jfm_ctx *ctx, bool tangent, const T *gf, const T *u, const T *v, T *ye, const int *done,
-                          const int32_t *elist = nullptr, int n_items = 0);
-// FP64 masked K x (or K_T x) over an element subset and a node subset only
-hipError_t launch_subset_apply(jfm_ctx *ctx, bool tangent, const double *x, double *y, const int32_t *elist, int ne,
-                               const int32_t *nlist, int nn);
+                          const int32_t *elist = nullptr, int n_items = 0, int nv = 1, int64_t vs = 0, int64_t ys = 0);
+// FP64 masked K x (or K_T x) over an element subset and a node subset only,
+// for nv vectors at once (x_j = x + j xs, y_j = y + j xs, slots ye + j ys)
+hipError_t launch_subset_apply(jfm_ctx *ctx, bool tangent, const double *x, double *y, double *ye, int nv, int64_t xs,
+                               int64_t ys, const int32_t *elist, int ne, const int32_t *nlist, int nn);
 hipError_t launch_nh_block_jacobi(jfm_ctx *ctx, const double *u, double *M64, float *M32, double *D);
 // coarse.hip
 int jfem_coarse_setup(jfm_ctx *ctx, int op, int coarse_nodes, double *ms);
+bool jfem_use_graphs();  // false when JFEM_GRAPHS=0 (profilers)
 void jfem_coarse_release(jfm_ctx *ctx);
-void jfem_coarse_destroy_handle(jfm_ctx *ctx);
 template <typename T>
 hipError_t launch_coarse_correct(jfm_ctx *ctx, const T *r, T *z, T *p, double *part, const int *done, int *npart);
+// the coarse product of a PCG iteration: zc = Ahat Q' m r, with the w.zc
+// partials (npart_wz of them) at partials + kPartCoarse
+template <typename T>
+hipError_t launch_coarse_product(jfm_ctx *ctx, const T *r, const int *done, int *npart_wz);
+// p = z + m o Q zc + beta p, beta from the (r.z, r.r) and w.zc partials
+template <typename T>
+hipError_t launch_pcg_direction_coarse(jfm_ctx *ctx, int slot, int npart_rz, int npart_wz, T *p, const T *z);
 // heat.hip (kOpHeat: conduction + convection, FP64, row mask ctx->fixed)
 constexpr int kOpHeat = 17;
 hipError_t launch_heat_apply(jfm_ctx *ctx, const double *x, double *y, const double *dotv, double *part, const int *done,

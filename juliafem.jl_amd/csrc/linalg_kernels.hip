@@ -170,6 +170,21 @@ __global__ __launch_bounds__(256) void k_pcg_update(int N, int slot, int npart_p
     }
 }
 
+// the next iteration's scalars and status, written by one thread at the end
+// of the direction kernel
+__device__ __forceinline__ void pcg_record(PcgScalars *S, int slot, double rz, double rr, double beta) {
+    const int it = S->iter + 1;
+    S->iter = it;
+    S->rz[slot ^ 1] = rz;
+    S->rr = rr;
+    S->beta = beta;
+    int nd = DONE_RUNNING;
+    if (rr <= S->tol2) nd = DONE_CONVERGED;
+    else if (!(S->pq > 0.0) || !(rz > 0.0) || !finite(rz)) nd = DONE_BREAKDOWN;
+    else if (it >= S->maxit) nd = DONE_MAXIT;
+    S->done[slot ^ 1] = nd;
+}
+
 // p = z + beta p; block 0 records the next iteration's scalars and status
 template <typename T>
 __global__ __launch_bounds__(256) void k_pcg_dir(int n3, int slot, int npart_rz, T *__restrict__ p,
@@ -190,18 +205,39 @@ __global__ __launch_bounds__(256) void k_pcg_dir(int n3, int slot, int npart_rz,
     const T beta = (T)beta_d;
     if (i0 < n3) p[i0] = z0 + beta * p0;
     for (int i = i0 + stride; i < n3; i += stride) p[i] = z[i] + beta * p[i];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        const int it = S->iter + 1;
-        S->iter = it;
-        S->rz[slot ^ 1] = t[0];
-        S->rr = t[1];
-        S->beta = beta_d;
-        int nd = DONE_RUNNING;
-        if (t[1] <= S->tol2) nd = DONE_CONVERGED;
-        else if (!(S->pq > 0.0) || !(t[0] > 0.0) || !finite(t[0])) nd = DONE_BREAKDOWN;
-        else if (it >= S->maxit) nd = DONE_MAXIT;
-        S->done[slot ^ 1] = nd;
+    if (blockIdx.x == 0 && threadIdx.x == 0) pcg_record(S, slot, t[0], t[1], beta_d);
+}
+
+// Two-level form: p_a = z_a + m_a o (t_g + omega_g x d_a) + beta p_a with
+// (t_g, omega_g) = zc of the node's aggregate, one thread per node; r.z is
+// r.Br (update partials) + w.zc (coarse product partials).
+template <typename T>
+__global__ __launch_bounds__(256) void k_pcg_dir_coarse(int N, int slot, int npart_rz, int npart_wz, T *__restrict__ p,
+                                                       const T *__restrict__ z, const int32_t *__restrict__ node_agg,
+                                                       const T *__restrict__ cd, const uint8_t *__restrict__ fixed,
+                                                       const float *__restrict__ zc, const double *__restrict__ partials,
+                                                       PcgScalars *S) {
+    const int d = S->done[slot];
+    if (d) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) S->done[slot ^ 1] = d;
+        return;
     }
+    double t[2], u[1];
+    sum_partials<2>(partials + part_rz(slot), npart_rz, t);
+    sum_partials<1>(partials + kPartCoarse, npart_wz, u);
+    const double rz = t[0] + u[0];
+    const double beta_d = rz / S->rz[slot];
+    const T beta = (T)beta_d;
+    for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < N; a += gridDim.x * blockDim.x) {
+        const size_t o = 3 * (size_t)a;
+        const float *yf = zc + 6 * (size_t)node_agg[a];
+        const T y[6] = {(T)yf[0], (T)yf[1], (T)yf[2], (T)yf[3], (T)yf[4], (T)yf[5]};
+        const T d0 = cd[o], d1 = cd[o + 1], d2 = cd[o + 2];
+        const T c[3] = {y[0] + y[4] * d2 - y[5] * d1, y[1] + y[5] * d0 - y[3] * d2, y[2] + y[3] * d1 - y[4] * d0};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) p[o + k] = (z[o + k] + (fixed[o + k] ? T(0) : c[k])) + beta * p[o + k];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) pcg_record(S, slot, rz, t[1], beta_d);
 }
 
 // one block: sum partials and apply `fin`
@@ -302,7 +338,14 @@ hipError_t launch_pcg_init(jfm_ctx *ctx, T *r, T *z, T *p, const T *Minv, int *n
     return hipGetLastError();
 }
 
-constexpr int kVecBlocks = 512;  // grid of the PCG vector kernels
+#ifndef JFEM_VEC_BLOCKS
+#define JFEM_VEC_BLOCKS 512
+#endif
+#ifndef JFEM_DIR_BLOCKS
+#define JFEM_DIR_BLOCKS 1024
+#endif
+constexpr int kVecBlocks = JFEM_VEC_BLOCKS;  // grid of the PCG update kernel
+constexpr int kDirBlocks = JFEM_DIR_BLOCKS;  // grid of the PCG direction kernels
 
 template <typename T>
 hipError_t launch_pcg_update(jfm_ctx *ctx, int slot, int npart_pq, T *x, T *r, T *z, const T *p, const T *q,
@@ -317,10 +360,20 @@ hipError_t launch_pcg_update(jfm_ctx *ctx, int slot, int npart_pq, T *x, T *r, T
 template <typename T>
 hipError_t launch_pcg_direction(jfm_ctx *ctx, int slot, int npart_rz, T *p, const T *z) {
     const int64_t n3 = 3 * ctx->N;
-    k_pcg_dir<T><<<std::min<unsigned>(capped_grid(n3, kBlock), 2 * kVecBlocks), kBlock, 0, ctx->stream>>>((int)n3, slot, npart_rz, p, z, ctx->partials,
+    k_pcg_dir<T><<<std::min<unsigned>(capped_grid(n3, kBlock), kDirBlocks), kBlock, 0, ctx->stream>>>((int)n3, slot, npart_rz, p, z, ctx->partials,
                                                                       ctx->S);
     return hipGetLastError();
 }
+
+template <typename T>
+hipError_t launch_pcg_direction_coarse(jfm_ctx *ctx, int slot, int npart_rz, int npart_wz, T *p, const T *z) {
+    const T *cd = sizeof(T) == 8 ? (const T *)ctx->cd64 : (const T *)ctx->cd32;
+    k_pcg_dir_coarse<T><<<std::min<unsigned>(capped_grid(ctx->N, kBlock), kDirBlocks), kBlock, 0, ctx->stream>>>(
+        (int)ctx->N, slot, npart_rz, npart_wz, p, z, ctx->node_agg, cd, ctx->fixed, ctx->zc32, ctx->partials, ctx->S);
+    return hipGetLastError();
+}
+template hipError_t launch_pcg_direction_coarse<double>(jfm_ctx *, int, int, int, double *, const double *);
+template hipError_t launch_pcg_direction_coarse<float>(jfm_ctx *, int, int, int, float *, const float *);
 
 hipError_t launch_finalize(jfm_ctx *ctx, const double *part, int npart, int nv, int fin) {
     k_finalize<<<1, kBlock, 0, ctx->stream>>>(part, npart, nv, fin, ctx->S);

@@ -118,13 +118,15 @@ __global__ __launch_bounds__(256) void k_nh_elem(int n_el, const int32_t *__rest
                                                 const int32_t *__restrict__ slot, const T *__restrict__ gf,
                                                 const T *__restrict__ u, const T *__restrict__ v, T *__restrict__ ye,
                                                 const int *__restrict__ done, const int32_t *__restrict__ elist,
-                                                int n_items) {
+                                                int n_items, int64_t vs = 0, int64_t ys = 0) {
     using EL = Elem<NPE>;
     constexpr int NG = EL::NGP;
     if (done && *done) return;
     const int idx = xcd_block() * blockDim.x + threadIdx.x;
     if (idx >= n_items) return;
     const int e = elist ? elist[idx] : idx;   // optional element subset
+    if (TANGENT) v += blockIdx.y * vs;        // vector blockIdx.y of a batch
+    ye += blockIdx.y * ys;
     const int n0 = conn[e];
     const T u0[3] = {u[3 * (size_t)n0], u[3 * (size_t)n0 + 1], u[3 * (size_t)n0 + 2]};
     T v0[3] = {T(0), T(0), T(0)};
@@ -294,26 +296,27 @@ __global__ __launch_bounds__(256) void k_nh_block_jacobi(int N, const int32_t *_
 // --------------------------------------------------------------- launchers ---
 template <int NPE, typename T, bool TANGENT>
 static void nh_elem(jfm_ctx *ctx, const T *gf, const T *u, const T *v, T *ye, const int *done, const int32_t *elist,
-                    int m) {
+                    int m, int nv, int64_t vs, int64_t ys) {
     const int n = (int)ctx->n_el;
+    const dim3 grid(xcd_grid(nblk(m)), nv);
     if (ctx->affine)
-        k_nh_elem<NPE, T, true, TANGENT><<<xcd_grid(nblk(m)), kBlock, 0, ctx->stream>>>(n, ctx->conn, ctx->inc_slot, gf, u, v, ye,
-                                                                                        done, elist, m);
+        k_nh_elem<NPE, T, true, TANGENT><<<grid, kBlock, 0, ctx->stream>>>(n, ctx->conn, ctx->inc_slot, gf, u, v, ye, done,
+                                                                           elist, m, vs, ys);
     else
-        k_nh_elem<NPE, T, false, TANGENT><<<xcd_grid(nblk(m)), kBlock, 0, ctx->stream>>>(n, ctx->conn, ctx->inc_slot, gf, u, v, ye,
-                                                                                         done, elist, m);
+        k_nh_elem<NPE, T, false, TANGENT><<<grid, kBlock, 0, ctx->stream>>>(n, ctx->conn, ctx->inc_slot, gf, u, v, ye, done,
+                                                                            elist, m, vs, ys);
 }
 
 template <typename T>
 hipError_t launch_nh_elem(jfm_ctx *ctx, bool tangent, const T *gf, const T *u, const T *v, T *ye, const int *done,
-                          const int32_t *elist, int n_items) {
+                          const int32_t *elist, int n_items, int nv, int64_t vs, int64_t ys) {
     if (ctx->n_el == 0) return hipSuccess;
     const int m = elist ? n_items : (int)ctx->n_el;
     if (m == 0) return hipSuccess;
     switch (ctx->npe) {
-    case 10: tangent ? nh_elem<10, T, true>(ctx, gf, u, v, ye, done, elist, m) : nh_elem<10, T, false>(ctx, gf, u, v, ye, done, elist, m); break;
-    case 4: tangent ? nh_elem<4, T, true>(ctx, gf, u, v, ye, done, elist, m) : nh_elem<4, T, false>(ctx, gf, u, v, ye, done, elist, m); break;
-    case 8: tangent ? nh_elem<8, T, true>(ctx, gf, u, v, ye, done, elist, m) : nh_elem<8, T, false>(ctx, gf, u, v, ye, done, elist, m); break;
+    case 10: tangent ? nh_elem<10, T, true>(ctx, gf, u, v, ye, done, elist, m, nv, vs, ys) : nh_elem<10, T, false>(ctx, gf, u, v, ye, done, elist, m, nv, vs, ys); break;
+    case 4: tangent ? nh_elem<4, T, true>(ctx, gf, u, v, ye, done, elist, m, nv, vs, ys) : nh_elem<4, T, false>(ctx, gf, u, v, ye, done, elist, m, nv, vs, ys); break;
+    case 8: tangent ? nh_elem<8, T, true>(ctx, gf, u, v, ye, done, elist, m, nv, vs, ys) : nh_elem<8, T, false>(ctx, gf, u, v, ye, done, elist, m, nv, vs, ys); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -331,8 +334,8 @@ hipError_t launch_nh_block_jacobi(jfm_ctx *ctx, const double *u, double *M64, fl
 }
 
 template hipError_t launch_nh_elem<double>(jfm_ctx *, bool, const double *, const double *, const double *, double *, const int *,
-                                           const int32_t *, int);
+                                           const int32_t *, int, int, int64_t, int64_t);
 template hipError_t launch_nh_elem<float>(jfm_ctx *, bool, const float *, const float *, const float *, float *, const int *,
-                                          const int32_t *, int);
+                                          const int32_t *, int, int, int64_t, int64_t);
 
 }  // namespace jfem
