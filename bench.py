@@ -51,6 +51,18 @@ def kernel_bytes(inf, op, prec):
     if op == "csr":
         spmv = 9 * nnzb * T + 4 * nnzb + 4 * (N + 1) + 2 * 3 * N * T + 3 * N
         return {"spmv": spmv}
+    if op == "matfree" and inf.get("n_patches", 0) > 0:
+        # patch kernel: connectivity (int32, npe per element), the geometric
+        # factors, x once, the patch-local incidences (uint16, npe per element),
+        # per patch node its count (uint16) and target (int32); the fixed mask
+        # and y of the interior nodes, the partial sums of the shared entries
+        npn, nb, ns = inf["patch_nodes"], inf["shared_nodes"], inf["shared_slots"]
+        interior = npn - ns
+        patch = (4 * npe * n_el + 11 * ngf * n_el * T + 3 * N * T + 2 * npe * n_el + 6 * npn
+                 + 3 * interior + 3 * interior * T + 3 * ns * T)
+        # shared-node kernel: slot ranges and node ids, the partials, mask, y
+        combine = 8 * nb + 4 + 3 * ns * T + 3 * nb + 3 * nb * T
+        return {"patch": patch, "combine": combine}
     # element phase: connectivity + incidence slots (int32, npe each), the
     # geometric factors (11 per factor set) or the stored K_e blocks, x once,
     # the element results (3 npe per element) written once
@@ -128,12 +140,19 @@ def main():
             if op == "ebe" and prec == 1:
                 continue
             tot, k1, k2 = ctx.time_apply(op=op, precision=prec, reps=50)
+            inf = ctx.info()  # the operator's structure exists once it has run
             b = kernel_bytes(inf, op, prec)
             name = f"{op}_{'fp64' if prec == 0 else 'fp32'}"
             ent = {"ms": tot}
             if op == "csr":
                 ent["spmv_ms"] = k1
                 ent["spmv_GBps"] = b["spmv"] / (k1 * 1e-3) / 1e9
+            elif "patch" in b:
+                ent["patch_ms"], ent["combine_ms"] = k1, k2
+                ent["patch_GBps"] = b["patch"] / (k1 * 1e-3) / 1e9
+                ent["combine_GBps"] = b["combine"] / (k2 * 1e-3) / 1e9
+                ent["Ku_GBps"] = (b["patch"] + b["combine"]) / (tot * 1e-3) / 1e9
+                ent["patch_TFLOPs"] = flops_matfree_elem(inf) / (k1 * 1e-3) / 1e12
             else:
                 ent["element_ms"], ent["gather_ms"] = k1, k2
                 ent["element_GBps"] = b["element"] / (k1 * 1e-3) / 1e9
@@ -197,7 +216,9 @@ def main():
     # ---- roofline of the dominant kernel of the headline solve ---------------
     hk = kernels["matfree_fp32" if method == 1 else "matfree_fp64"]
     tname = "float" if method == 1 else "double"
-    if hk["element_ms"] >= hk["gather_ms"]:
+    if "patch_ms" in hk:
+        dom, dom_ms, dom_bytes = f"k_ku_patch<10,{tname},AFF>", hk["patch_ms"], hk["bytes"]["patch"]
+    elif hk["element_ms"] >= hk["gather_ms"]:
         dom, dom_ms, dom_bytes = f"k_mf_elem<10,{tname},AFF>", hk["element_ms"], hk["bytes"]["element"]
     else:
         dom, dom_ms, dom_bytes = f"k_gather<{tname}>", hk["gather_ms"], hk["bytes"]["gather"]

@@ -108,6 +108,10 @@ typedef struct {
     int64_t n_nodes, n_el, n_dofs, nnz, nnzb, n_faces;
     int32_t npe, affine;
     int64_t device_bytes;  /* bytes currently allocated by the context */
+    /* patch form of the matrix-free operator (0 until it is first used):
+     * patches of 256 elements, their (patch, node) entries, the nodes shared
+     * between patches and their partial sums */
+    int64_t n_patches, patch_nodes, shared_nodes, shared_slots;
 } jfm_info;
 
 /* ---- context --------------------------------------------------------- */
@@ -157,8 +161,9 @@ int jfm_apply(jfm_ctx *ctx, int32_t op, int32_t precision, const double *x, doub
 
 /* Average device times (ms) over `reps` back-to-back applications on
  * device-resident vectors, timed with HIP events on the context's stream:
- * avg_ms[0] = whole K.u, avg_ms[1] = its first kernel (SpMV, or the element
- * phase of EBE/MATFREE), avg_ms[2] = the nodal gather (0 for CSR); each as
+ * avg_ms[0] = whole K.u, avg_ms[1] = its first kernel (SpMV; the element
+ * phase of EBE; the patch kernel of MATFREE), avg_ms[2] = the second (the
+ * nodal gather of EBE; the shared-node kernel of MATFREE; 0 for CSR); each as
  * reps back-to-back launches between two events. */
 int jfm_time_apply(jfm_ctx *ctx, int32_t op, int32_t precision, int32_t reps, double *avg_ms);
 

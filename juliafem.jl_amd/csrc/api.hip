@@ -36,6 +36,7 @@ template <typename T> T *jfm_alloc(jfm_ctx *ctx, int64_t count) {
     ctx->bytes += (int64_t)bytes;
     return (T *)p;
 }
+template uint16_t *jfm_alloc<uint16_t>(jfm_ctx *, int64_t);
 
 void jfm_free(jfm_ctx *ctx, void *p) {
     if (!p) return;
@@ -136,6 +137,10 @@ extern "C" int jfm_get_info(jfm_ctx *ctx, jfm_info *info) {
     info->npe = ctx->npe;
     info->affine = ctx->affine ? 1 : 0;
     info->device_bytes = ctx->bytes;
+    info->n_patches = ctx->patch_ready ? ctx->npatch : 0;
+    info->patch_nodes = ctx->patch_ready ? ctx->npn : 0;
+    info->shared_nodes = ctx->patch_ready ? ctx->nbnode : 0;
+    info->shared_slots = ctx->patch_ready ? ctx->nslot : 0;
     return JFM_OK;
 }
 
@@ -177,6 +182,7 @@ static void release_mesh(jfm_ctx *ctx) {
     free_ptr(ctx, ctx->nl_u); free_ptr(ctx, ctx->nl_r); free_ptr(ctx, ctx->nl_du); free_ptr(ctx, ctx->nl_zero);
     ctx->has_state = ctx->minv_tangent = false;
     jfem_coarse_release(ctx);
+    jfem_patch_release(ctx);
     release_heat(ctx);
     ctx->h_X.clear();
     ctx->has_user = false;
@@ -537,7 +543,8 @@ static int ensure_assembled(jfm_ctx *ctx) {
 static int prepare_op(jfm_ctx *ctx, int op) {
     if (op == JFM_OP_CSR) return ensure_assembled(ctx);
     if (op == JFM_OP_EBE) return ensure_element_matrices(ctx);
-    if (op == JFM_OP_MATFREE || op == kOpMass) return JFM_OK;
+    if (op == JFM_OP_MATFREE) return jfem_patch_enabled() ? jfem_patch_setup(ctx) : JFM_OK;
+    if (op == kOpMass) return JFM_OK;
     if (op == kOpHeat) return ctx->heat_k ? JFM_OK : jfm_fail(ctx, JFM_ESTATE, "no heat problem uploaded");
     if (op == JFM_OP_TANGENT)
         return ctx->has_state ? JFM_OK
@@ -605,6 +612,14 @@ extern "C" int jfm_time_apply(jfm_ctx *ctx, int32_t op, int32_t precision, int32
                        : launch_bsr_spmv<float>(ctx, ctx->vals32, ctx->vx32, ctx->vy32, nullptr, nullptr, nullptr, nullptr);
         }
         hipError_t e = hipSuccess;
+        if (op == JFM_OP_MATFREE && ctx->patch_ready) {  // patch kernel, then the shared-node kernel
+            if (part != 2)
+                e = f64 ? launch_patch_phase<double>(ctx, ctx->vx64, ctx->vy64, nullptr, nullptr)
+                        : launch_patch_phase<float>(ctx, ctx->vx32, ctx->vy32, nullptr, nullptr);
+            if (e != hipSuccess || part == 1) return e;
+            return f64 ? launch_patch_combine<double>(ctx, ctx->vy64, nullptr, nullptr, nullptr, nullptr)
+                       : launch_patch_combine<float>(ctx, ctx->vy32, nullptr, nullptr, nullptr, nullptr);
+        }
         if (part != 2) e = f64 ? launch_elem_only<double>(ctx, op, ctx->vx64, nullptr) : launch_elem_only<float>(ctx, op, ctx->vx32, nullptr);
         if (e != hipSuccess || part == 1) return e;
         return f64 ? launch_gather<double>(ctx, ctx->ye64, ctx->fixed, ctx->vy64, nullptr, nullptr, nullptr, nullptr)

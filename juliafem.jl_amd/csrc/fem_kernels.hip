@@ -18,6 +18,7 @@
 #include <algorithm>
 
 #include "jfem_device.h"
+#include "mf_element.h"
 
 namespace jfem {
 
@@ -95,100 +96,26 @@ __global__ __launch_bounds__(256) void k_geom(int n_el, int ngf, const int32_t *
 }
 
 // ---------------------------------------------------- matrix-free element --
-// Phase 1 of the two-phase operator: y_e = K_e x_e from the geometric factors.
-// H_q = sum_i (x_i - x_0) (x) dN_i(xi_q) (reference gradient; sum_i dN_i = 0,
-// so subtracting node 0's value removes the element's translation, which in
-// FP32 would otherwise cancel away the strain), grad x = H_q invJ',
-// sigma_w = w detJ (lambda tr(eps) I + 2 mu eps), P_q = invJ' sigma_w,
-// y_i = sum_q P_q' dN_i(xi_q).
+// Phase 1 of the two-phase operator: y_e = K_e x_e from the geometric factors
+// (mf_element.h), written to the element's incidence slots.
 template <int NPE, typename T, bool AFF>
 __global__ __launch_bounds__(256) void k_mf_elem(int n_el, const int32_t *__restrict__ conn,
                                                 const int32_t *__restrict__ slot, const T *__restrict__ gf,
                                                 const T *__restrict__ x, T *__restrict__ ye,
                                                 const int *__restrict__ done, const int32_t *__restrict__ elist,
                                                 int n_items, int64_t xs = 0, int64_t ys = 0) {
-    using EL = Elem<NPE>;
-    constexpr int NG = EL::NGP;
     if (done && *done) return;
     const int idx = xcd_block() * blockDim.x + threadIdx.x;
     if (idx >= n_items) return;
     const int e = elist ? elist[idx] : idx;   // optional element subset
     x += blockIdx.y * xs;                     // vector blockIdx.y of a batch
     ye += blockIdx.y * ys;
-    T H[NG][9];
-#pragma unroll
-    for (int q = 0; q < NG; ++q)
-#pragma unroll
-        for (int k = 0; k < 9; ++k) H[q][k] = T(0);
-    const int n0 = conn[e];
-    const T t0 = x[3 * (size_t)n0], t1 = x[3 * (size_t)n0 + 1], t2 = x[3 * (size_t)n0 + 2];
-#pragma unroll
-    for (int i = 1; i < NPE; ++i) {
-        const int nd = conn[(size_t)i * n_el + e];
-        const T u0 = x[3 * (size_t)nd] - t0, u1 = x[3 * (size_t)nd + 1] - t1, u2 = x[3 * (size_t)nd + 2] - t2;
-#pragma unroll
-        for (int q = 0; q < NG; ++q)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const double d = EL::dN(q, i, c);
-                if (d != 0.0) {
-                    H[q][c] += u0 * T(d);
-                    H[q][3 + c] += u1 * T(d);
-                    H[q][6 + c] += u2 * T(d);
-                }
-            }
-    }
-    T Ji[9], cl, cm;
-    auto load = [&](int q) {
-        const T *g = gf + (size_t)q * GF * n_el + e;
-#pragma unroll
-        for (int k = 0; k < 9; ++k) Ji[k] = g[(size_t)k * n_el];
-        cl = g[(size_t)9 * n_el];
-        cm = g[(size_t)10 * n_el];
-    };
-    if (AFF) load(0);
-#pragma unroll
-    for (int q = 0; q < NG; ++q) {
-        if (!AFF) load(q);
-        T G[9];
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-#pragma unroll
-            for (int b = 0; b < 3; ++b)
-                G[a * 3 + b] = H[q][a * 3] * Ji[b * 3] + H[q][a * 3 + 1] * Ji[b * 3 + 1] + H[q][a * 3 + 2] * Ji[b * 3 + 2];
-        const T tr = cl * (G[0] + G[4] + G[8]);
-        T s[9];
-        s[0] = tr + T(2) * cm * G[0];
-        s[4] = tr + T(2) * cm * G[4];
-        s[8] = tr + T(2) * cm * G[8];
-        s[1] = s[3] = cm * (G[1] + G[3]);
-        s[2] = s[6] = cm * (G[2] + G[6]);
-        s[5] = s[7] = cm * (G[5] + G[7]);
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-                H[q][d * 3 + c] = Ji[d] * s[c] + Ji[3 + d] * s[3 + c] + Ji[6 + d] * s[6 + c];
-    }
-#pragma unroll
-    for (int i = 0; i < NPE; ++i) {
-        T *out = ye + 3 * (size_t)slot[(size_t)i * n_el + e];
-        T f0 = T(0), f1 = T(0), f2 = T(0);
-#pragma unroll
-        for (int q = 0; q < NG; ++q)
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                const double dd = EL::dN(q, i, d);
-                if (dd != 0.0) {
-                    f0 += T(dd) * H[q][d * 3];
-                    f1 += T(dd) * H[q][d * 3 + 1];
-                    f2 += T(dd) * H[q][d * 3 + 2];
-                }
-            }
-        out[0] = f0;
-        out[1] = f1;
-        out[2] = f2;
-    }
+    mf_element<NPE, T, AFF>(n_el, e, conn, gf, x, [&](int i, T f0, T f1, T f2) {
+        T *o = ye + 3 * (size_t)slot[(size_t)i * n_el + e];
+        o[0] = f0;
+        o[1] = f1;
+        o[2] = f2;
+    });
 }
 
 // ------------------------------------------------------------- EBE element --
@@ -651,6 +578,7 @@ hipError_t launch_elem_only(jfm_ctx *ctx, int op, const T *x, const int *done) {
 template <typename T>
 hipError_t launch_two_phase(jfm_ctx *ctx, int op, const T *x, T *y, const T *dotv, double *part, const int *done,
                             int *npart) {
+    if (op == JFM_OP_MATFREE && ctx->patch_ready) return launch_patch_apply<T>(ctx, x, y, dotv, part, done, npart);
     hipError_t e = launch_elem_only<T>(ctx, op, x, done);
     if (e != hipSuccess) return e;
     return launch_gather<T>(ctx, ye_of<T>(ctx), op == kOpMass ? nullptr : ctx->fixed, y, dotv, part, done, npart);

@@ -253,6 +253,20 @@ struct jfm_ctx {
     float *wc32 = nullptr, *zc32 = nullptr;  // wc32 has ldc entries, zero past nc
     double *wc64 = nullptr;        // Q' m r of the last restriction
     double *gjC = nullptr, *gjR = nullptr, *gjP = nullptr;  // Gauss-Jordan panels
+    // patch operator (patch.hip): elements in Morton order, 256 per patch
+    bool patch_ready = false;
+    int npatch = 0, npn = 0, nbnode = 0, nslot = 0;
+    int32_t *eperm = nullptr;                 // patch order -> element id
+    int32_t *connP = nullptr;                 // npe x n_el, patch order
+    double *gfP64 = nullptr;                  // geometric factors, patch order
+    float *gfP32 = nullptr;
+    int32_t *pn_ptr = nullptr, *pn_out = nullptr;  // patch nodes: range per patch, target
+    uint16_t *pn_cnt = nullptr;               // patch node: incidences in the patch
+    int32_t *rstart = nullptr, *rbase = nullptr;  // rounds of 256 patch nodes
+    uint16_t *linT = nullptr;                 // patch-local incidences, per round transposed
+    int32_t *bptr = nullptr, *bnode = nullptr;  // shared nodes, their partial slots
+    double *bpart = nullptr;                  // 3 x slots (FP32 or FP64 view)
+    double *ppart = nullptr;                  // per-patch partials of y.dotv
     double *pv6 = nullptr, *qv6 = nullptr;  // probe vectors, 6 x 3N (zero outside the probed rows)
     double *ye6 = nullptr;                  // probe incidence slots, 6 x 3 npe n_el (zero outside the probed elements)
     // heat conduction (heat.hip): temperature in component 0
@@ -315,6 +329,16 @@ hipError_t launch_nh_elem(jfm_ctx *ctx, bool tangent, const T *gf, const T *u, c
 hipError_t launch_subset_apply(jfm_ctx *ctx, bool tangent, const double *x, double *y, double *ye, int nv, int64_t xs,
                                int64_t ys, const int32_t *elist, int ne, const int32_t *nlist, int nn);
 hipError_t launch_nh_block_jacobi(jfm_ctx *ctx, const double *u, double *M64, float *M32, double *D);
+// patch.hip: fused matrix-free K.u (JFM_OP_MATFREE)
+bool jfem_patch_enabled();  // false when JFEM_PATCH=0 (two-phase operator)
+int jfem_patch_setup(jfm_ctx *ctx);
+void jfem_patch_release(jfm_ctx *ctx);
+template <typename T>
+hipError_t launch_patch_phase(jfm_ctx *ctx, const T *x, T *y, const T *dotv, const int *done);
+template <typename T>
+hipError_t launch_patch_combine(jfm_ctx *ctx, T *y, const T *dotv, double *part, const int *done, int *npart);
+template <typename T>
+hipError_t launch_patch_apply(jfm_ctx *ctx, const T *x, T *y, const T *dotv, double *part, const int *done, int *npart);
 // coarse.hip
 int jfem_coarse_setup(jfm_ctx *ctx, int op, int coarse_nodes, double *ms);
 bool jfem_use_graphs();  // false when JFEM_GRAPHS=0 (profilers)

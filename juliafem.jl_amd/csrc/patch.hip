@@ -1,0 +1,465 @@
+// patch.hip -- fused matrix-free K.u over element patches (gfx950).
+//
+// The two-phase operator (fem_kernels.hip) writes every element's 3 x npe
+// terms to HBM and reads them back in the nodal gather.  Here the elements are
+// ordered along a Morton curve of their centroids and cut into patches of 256
+// (one workgroup each, one element per thread).  A workgroup computes its
+// elements' terms with the same element product (mf_element.h) into LDS and
+// then sums them per patch node, in ascending local element order, from a
+// patch-local incidence list:
+//  * a node whose incidences all lie in the patch (interior) is finished and
+//    written to y directly;
+//  * any other node writes the patch's partial sum to a slot; a second kernel
+//    adds each such node's partials in ascending patch order.
+// Every node is written exactly once (nodes of no element through the second
+// kernel, as zero), there are no floating-point atomics and the result does
+// not depend on the launch: the order of every sum is fixed by the patch
+// structure.  The result equals the two-phase operator's up to the order of
+// the nodal sums.
+//
+// Reference: the operator of ext/JuliaFEMCUDAExt.jl:179-193 / 293-361 (element
+// loop and nodal assembly), elements per src/problems_elasticity.jl:203-451.
+#include <algorithm>
+#include <cstdlib>
+#include <numeric>
+#include <vector>
+
+#include "jfem_device.h"
+#include "mf_element.h"
+
+namespace jfem {
+
+constexpr int kPatch = 256;  // elements per patch = threads per workgroup
+
+// Fused element product and patch-local nodal sums.  LDS holds the patch's
+// element terms as ys[(3 i + c) * kPatch + local element].  The patch's nodes
+// are taken in rounds of kPatch, one per thread; round q lists the j-th
+// incidence of its thread t's node at linT[rbase[q] + j kPatch + t] (j below
+// the node's count), so the lanes of a wave read consecutive entries.  Nodes
+// are sorted by count within a patch, so a round's counts are close.  A
+// shared node's partial sum goes to its slot s (pn_out = -(s + 1)):
+// bpart[3 s + c].
+// Block partials of y.dotv over the interior nodes go to ppart.
+template <int NPE, typename T, bool AFF>
+__global__ __launch_bounds__(kPatch) void k_ku_patch(int n_el, int npatch, int npn, const int32_t *__restrict__ connP,
+                                                    const T *__restrict__ gfP, const T *__restrict__ x,
+                                                    const int32_t *__restrict__ pn_ptr, const uint16_t *__restrict__ pn_cnt,
+                                                    const int32_t *__restrict__ pn_out, const int32_t *__restrict__ rstart,
+                                                    const int32_t *__restrict__ rbase, const uint16_t *__restrict__ linT,
+                                                    const uint8_t *__restrict__ fixed, T *__restrict__ y,
+                                                    T *__restrict__ bpart, const T *__restrict__ dotv,
+                                                    double *__restrict__ ppart, const int *__restrict__ done) {
+    __shared__ T ys[3 * NPE * kPatch];
+    const int p = xcd_block();
+    if (p >= npatch) return;  // whole workgroup (grid padded to a multiple of 8)
+    if (done && *done) return;
+    const int t = threadIdx.x;
+    const int k = p * kPatch + t;
+    // round 0's node data and first incidences, loaded before the element
+    // product so that their latency overlaps it
+    const int lb = pn_ptr[p], le = pn_ptr[p + 1], q0 = rstart[p];
+    constexpr int kPre = 4;
+    int cnt = 0, o = 0, mpre[kPre] = {0, 0, 0, 0};
+    if (lb + t < le) {
+        cnt = pn_cnt[lb + t];
+        o = pn_out[lb + t];
+        const int base = rbase[q0];
+#pragma unroll
+        for (int j = 0; j < kPre; ++j)
+            if (j < cnt) mpre[j] = linT[base + j * kPatch + t];
+    }
+    if (k < n_el)
+        mf_element<NPE, T, AFF>(n_el, k, connP, gfP, x, [&](int i, T f0, T f1, T f2) {
+            ys[(3 * i) * kPatch + t] = f0;
+            ys[(3 * i + 1) * kPatch + t] = f1;
+            ys[(3 * i + 2) * kPatch + t] = f2;
+        });
+    __syncthreads();
+    double dv = 0.0;
+    for (int r = 0, l = lb + t; lb + r * kPatch < le; ++r, l += kPatch) {
+        const int base = rbase[q0 + r];
+        if (r > 0 && l < le) {
+            cnt = pn_cnt[l];
+            o = pn_out[l];
+#pragma unroll
+            for (int j = 0; j < kPre; ++j)
+                if (j < cnt) mpre[j] = linT[base + j * kPatch + t];
+        }
+        if (l >= le) continue;
+        T s0 = T(0), s1 = T(0), s2 = T(0);
+#pragma unroll
+        for (int j = 0; j < kPre; ++j)
+            if (j < cnt) {
+                const int m = mpre[j];  // (3 i) * kPatch + local element
+                s0 += ys[m];
+                s1 += ys[m + kPatch];
+                s2 += ys[m + 2 * kPatch];
+            }
+        for (int j = kPre; j < cnt; ++j) {
+            const int m = linT[base + j * kPatch + t];
+            s0 += ys[m];
+            s1 += ys[m + kPatch];
+            s2 += ys[m + 2 * kPatch];
+        }
+        if (o >= 0) {
+            const size_t b = 3 * (size_t)o;
+            if (fixed) {
+                if (fixed[b]) s0 = T(0);
+                if (fixed[b + 1]) s1 = T(0);
+                if (fixed[b + 2]) s2 = T(0);
+            }
+            y[b] = s0;
+            y[b + 1] = s1;
+            y[b + 2] = s2;
+            if (dotv) dv += (double)s0 * (double)dotv[b] + (double)s1 * (double)dotv[b + 1] + (double)s2 * (double)dotv[b + 2];
+        } else {
+            const size_t b = 3 * (size_t)(-o - 1);
+            bpart[b] = s0;
+            bpart[b + 1] = s1;
+            bpart[b + 2] = s2;
+        }
+    }
+    if (ppart) {
+        double v[1] = {dv};
+        block_sum_all<1>(v);
+        if (t == 0) ppart[p] = v[0];
+    }
+}
+
+// Nodes shared between patches (and nodes of no element): y = sum of the
+// patch partials in slots bptr[b]..bptr[b+1] (ascending patch order).  The
+// shared nodes are listed in the order of their first patch and each block
+// takes a contiguous chunk, with the XCD mapping of the patch kernel, so that
+// most partials are read from the L2 that received them.  Block partials of
+// y.dotv also take in a fixed slice of the patch kernel's partials.
+template <typename T>
+__global__ __launch_bounds__(256) void k_ku_combine(int nb, int chunk, const int32_t *__restrict__ bptr,
+                                                   const int32_t *__restrict__ bnode, const T *__restrict__ bpart,
+                                                   const uint8_t *__restrict__ fixed, T *__restrict__ y,
+                                                   const T *__restrict__ dotv, const double *__restrict__ ppart,
+                                                   int npatch, double *__restrict__ part, const int *__restrict__ done) {
+    if (done && *done) return;
+    double dv = 0.0;
+    if (dotv)
+        for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < npatch; q += gridDim.x * blockDim.x) dv += ppart[q];
+    const int b0 = xcd_block() * chunk, b1 = min(b0 + chunk, nb);
+    for (int b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
+        T s0 = T(0), s1 = T(0), s2 = T(0);
+        const int r1 = bptr[b + 1];
+        for (int r = bptr[b]; r < r1; ++r) {
+            s0 += bpart[3 * (size_t)r];
+            s1 += bpart[3 * (size_t)r + 1];
+            s2 += bpart[3 * (size_t)r + 2];
+        }
+        const size_t o = 3 * (size_t)bnode[b];
+        if (fixed) {
+            if (fixed[o]) s0 = T(0);
+            if (fixed[o + 1]) s1 = T(0);
+            if (fixed[o + 2]) s2 = T(0);
+        }
+        y[o] = s0;
+        y[o + 1] = s1;
+        y[o + 2] = s2;
+        if (dotv) dv += (double)s0 * (double)dotv[o] + (double)s1 * (double)dotv[o + 1] + (double)s2 * (double)dotv[o + 2];
+    }
+    if (part) {
+        double v[1] = {dv};
+        write_partials<1>(v, part);
+    }
+}
+
+// geometric factors in patch order: dst[(r * n_el) + k] = src[r * n_el + perm[k]]
+template <typename T>
+__global__ __launch_bounds__(256) void k_permute_gf(int64_t rows, int n_el, const int32_t *__restrict__ perm,
+                                                   const T *__restrict__ src, T *__restrict__ dst) {
+    const int64_t total = rows * n_el;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = idx / n_el, k = idx - r * n_el;
+        dst[idx] = src[r * n_el + perm[k]];
+    }
+}
+
+// ------------------------------------------------------------ host setup ---
+namespace {
+
+uint64_t spread3(uint64_t v) {  // 21 bits -> every third bit
+    v &= 0x1fffff;
+    v = (v | v << 32) & 0x1f00000000ffffULL;
+    v = (v | v << 16) & 0x1f0000ff0000ffULL;
+    v = (v | v << 8) & 0x100f00f00f00f00fULL;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+    v = (v | v << 2) & 0x1249249249249249ULL;
+    return v;
+}
+
+}  // namespace
+
+bool jfem_patch_enabled() {
+    static const bool use = [] {
+        const char *v = std::getenv("JFEM_PATCH");
+        return !(v && v[0] == '0');
+    }();
+    return use;
+}
+
+void jfem_patch_release(jfm_ctx *ctx) {
+    for (void **p : {(void **)&ctx->connP, (void **)&ctx->gfP64, (void **)&ctx->gfP32, (void **)&ctx->pn_ptr,
+                     (void **)&ctx->pn_cnt, (void **)&ctx->pn_out, (void **)&ctx->rstart, (void **)&ctx->rbase,
+                     (void **)&ctx->linT, (void **)&ctx->bptr, (void **)&ctx->bnode,
+                     (void **)&ctx->bpart, (void **)&ctx->ppart, (void **)&ctx->eperm}) {
+        jfm_free(ctx, *p);
+        *p = nullptr;
+    }
+    ctx->patch_ready = false;
+    ctx->npatch = ctx->npn = ctx->nbnode = ctx->nslot = 0;
+}
+
+// The patch structure on the host: element order, patch-local incidences in
+// rounds and the shared-node partial lists.  Returns false if an index does
+// not fit its type.
+struct PatchHost {
+    int npatch = 0;
+    std::vector<int32_t> perm, connP, pn_ptr, pn_out, rstart, rbase, bnode, bptr;
+    std::vector<uint16_t> pn_cnt, linT;
+};
+
+static bool build_patches(const std::vector<int32_t> &conn, const std::vector<double> &X, int64_t n_el, int64_t N,
+                          int npe, PatchHost &P) {
+    // Morton order of the element centroids (stable in element id)
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int64_t a = 0; a < N; ++a)
+        for (int c = 0; c < 3; ++c) {
+            lo[c] = std::min(lo[c], X[3 * a + c]);
+            hi[c] = std::max(hi[c], X[3 * a + c]);
+        }
+    std::vector<uint64_t> code(n_el);
+    for (int64_t e = 0; e < n_el; ++e) {
+        double cen[3] = {0.0, 0.0, 0.0};
+        for (int i = 0; i < npe; ++i)
+            for (int c = 0; c < 3; ++c) cen[c] += X[3 * (size_t)conn[e * npe + i] + c];
+        uint64_t q[3];
+        for (int c = 0; c < 3; ++c) {
+            const double ext = hi[c] - lo[c];
+            const double f = ext > 0.0 ? (cen[c] / npe - lo[c]) / ext : 0.0;
+            q[c] = (uint64_t)std::min(std::max(f, 0.0) * 2097151.0, 2097151.0);
+        }
+        code[e] = spread3(q[0]) << 2 | spread3(q[1]) << 1 | spread3(q[2]);
+    }
+    std::vector<int32_t> &perm = P.perm;
+    perm.resize(n_el);
+    std::iota(perm.begin(), perm.end(), 0);
+    std::stable_sort(perm.begin(), perm.end(), [&](int32_t a, int32_t b) { return code[a] < code[b]; });
+    code.clear();
+    code.shrink_to_fit();
+    std::vector<int32_t> val(N, 0);  // node valences
+    for (int32_t v : conn) ++val[v];
+    const int npatch = (int)((n_el + kPatch - 1) / kPatch);
+    P.npatch = npatch;
+    P.connP.assign((size_t)npe * n_el, 0);
+    P.pn_ptr.assign(npatch + 1, 0);
+    P.rstart.assign(npatch + 1, 0);
+    P.pn_out.clear();
+    P.pn_cnt.clear();
+    P.rbase.clear();
+    P.linT.clear();
+    std::vector<std::vector<int32_t>> shared_l(N);  // per node: patch-node indices, ascending patch
+    std::vector<std::pair<int64_t, int32_t>> pairs;  // (node * kPatch + local element, i)
+    struct PNode { int32_t a, first, cnt; };
+    std::vector<PNode> nodes;
+    for (int p = 0; p < npatch; ++p) {
+        const int64_t k0 = (int64_t)p * kPatch, k1 = std::min<int64_t>(k0 + kPatch, n_el);
+        pairs.clear();
+        for (int64_t k = k0; k < k1; ++k) {
+            const int32_t e = perm[k];
+            for (int i = 0; i < npe; ++i) {
+                const int32_t a = conn[(size_t)e * npe + i];
+                P.connP[(size_t)i * n_el + k] = a;
+                pairs.emplace_back((int64_t)a * kPatch + (k - k0), i);
+            }
+        }
+        std::sort(pairs.begin(), pairs.end());  // by node, then local element
+        nodes.clear();
+        for (size_t r = 0; r < pairs.size();) {
+            const int32_t a = (int32_t)(pairs[r].first / kPatch);
+            size_t r2 = r;
+            while (r2 < pairs.size() && pairs[r2].first / kPatch == a) ++r2;
+            nodes.push_back({a, (int32_t)r, (int32_t)(r2 - r)});
+            r = r2;
+        }
+        // most incidences first, so that a round's counts are close
+        std::stable_sort(nodes.begin(), nodes.end(), [](const PNode &u, const PNode &v) { return u.cnt > v.cnt; });
+        const int64_t l0 = (int64_t)P.pn_out.size();
+        for (size_t u = 0; u < nodes.size(); u += kPatch) {
+            const int cntmax = nodes[u].cnt;
+            const int64_t base = (int64_t)P.linT.size();
+            if (base > INT32_MAX) return false;
+            P.rbase.push_back((int32_t)base);
+            P.linT.resize(base + (size_t)cntmax * kPatch, 0);
+            for (size_t t = 0; t < (size_t)kPatch && u + t < nodes.size(); ++t) {
+                const PNode &nd = nodes[u + t];
+                for (int j = 0; j < nd.cnt; ++j) {
+                    const auto &pr = pairs[nd.first + j];
+                    const int kl = (int)(pr.first % kPatch), i = pr.second;
+                    P.linT[base + (size_t)j * kPatch + t] = (uint16_t)(3 * i * kPatch + kl);
+                }
+            }
+        }
+        for (size_t u = 0; u < nodes.size(); ++u) {
+            const PNode &nd = nodes[u];
+            if (nd.cnt > 65535) return false;
+            P.pn_cnt.push_back((uint16_t)nd.cnt);
+            if (nd.cnt == val[nd.a]) {
+                P.pn_out.push_back(nd.a);  // interior
+            } else {
+                P.pn_out.push_back(-1);  // shared
+                shared_l[nd.a].push_back((int32_t)(l0 + (int64_t)u));
+            }
+        }
+        P.pn_ptr[p + 1] = (int32_t)P.pn_out.size();
+        P.rstart[p + 1] = (int32_t)P.rbase.size();
+    }
+    // shared nodes in the order of their first patch entry, then the nodes
+    // of no element
+    std::vector<int32_t> order;
+    for (int64_t a = 0; a < N; ++a)
+        if (!shared_l[a].empty()) order.push_back((int32_t)a);
+    std::sort(order.begin(), order.end(), [&](int32_t u, int32_t v) { return shared_l[u][0] < shared_l[v][0]; });
+    for (int64_t a = 0; a < N; ++a)
+        if (val[a] == 0) order.push_back((int32_t)a);
+    P.bnode.clear();
+    P.bptr.assign(1, 0);
+    for (int32_t a : order) {
+        P.bnode.push_back(a);
+        // slots of node a: consecutive, in ascending patch order
+        for (size_t j = 0; j < shared_l[a].size(); ++j) P.pn_out[shared_l[a][j]] = -(P.bptr.back() + (int32_t)j + 1);
+        P.bptr.push_back(P.bptr.back() + (int32_t)shared_l[a].size());
+    }
+    return 3 * (npe - 1) * kPatch + kPatch <= 65536 && (int64_t)P.pn_out.size() < INT32_MAX / 3;
+}
+
+// Builds the patch structure for the uploaded mesh (once per mesh).
+int jfem_patch_setup(jfm_ctx *ctx) {
+    if (ctx->patch_ready) return JFM_OK;
+    const int64_t n_el = ctx->n_el, N = ctx->N;
+    const int npe = ctx->npe;
+    if (n_el == 0) return JFM_OK;
+    if ((int64_t)ctx->h_conn.size() != n_el * npe || (int64_t)ctx->h_X.size() != 3 * N)
+        return jfm_fail(ctx, JFM_ESTATE, "patch setup needs the host mesh");
+    PatchHost P;
+    if (!build_patches(ctx->h_conn, ctx->h_X, n_el, N, npe, P))
+        return jfm_fail(ctx, JFM_EINVAL, "patch structure out of range");
+    const int npatch = P.npatch;
+    const int64_t npn = (int64_t)P.pn_out.size(), nb = (int64_t)P.bnode.size();
+    hipStream_t s = ctx->stream;
+    const int64_t ngfr = (int64_t)ctx->ngf * GF;  // rows of the geometric factors
+#define PALLOC(ptr, T, cnt)                                                                 \
+    do {                                                                                    \
+        ptr = jfm_alloc<T>(ctx, std::max<int64_t>(1, (cnt)));                               \
+        if (!ptr) {                                                                         \
+            jfem_patch_release(ctx);                                                        \
+            return jfm_fail(ctx, JFM_ENOMEM, "patch operator allocation failed");            \
+        }                                                                                   \
+    } while (0)
+#define PUP(ptr, vec)                                                                       \
+    if (!(vec).empty())                                                                     \
+        JFM_HIP(ctx, hipMemcpyAsync(ptr, (vec).data(), sizeof((vec)[0]) * (vec).size(), hipMemcpyHostToDevice, s))
+    PALLOC(ctx->connP, int32_t, npe * n_el);
+    PALLOC(ctx->eperm, int32_t, n_el);
+    PALLOC(ctx->gfP64, double, ngfr * n_el);
+    PALLOC(ctx->gfP32, float, ngfr * n_el);
+    PALLOC(ctx->pn_ptr, int32_t, npatch + 1);
+    PALLOC(ctx->pn_cnt, uint16_t, npn);
+    PALLOC(ctx->pn_out, int32_t, npn);
+    PALLOC(ctx->rstart, int32_t, npatch + 1);
+    PALLOC(ctx->rbase, int32_t, (int64_t)P.rbase.size());
+    PALLOC(ctx->linT, uint16_t, (int64_t)P.linT.size());
+    PALLOC(ctx->bptr, int32_t, nb + 1);
+    PALLOC(ctx->bnode, int32_t, nb);
+    PALLOC(ctx->bpart, double, 3 * (int64_t)P.bptr.back());
+    PALLOC(ctx->ppart, double, npatch);
+#undef PALLOC
+    PUP(ctx->connP, P.connP);
+    PUP(ctx->eperm, P.perm);
+    PUP(ctx->pn_ptr, P.pn_ptr);
+    PUP(ctx->pn_cnt, P.pn_cnt);
+    PUP(ctx->pn_out, P.pn_out);
+    PUP(ctx->rstart, P.rstart);
+    PUP(ctx->rbase, P.rbase);
+    PUP(ctx->linT, P.linT);
+    PUP(ctx->bptr, P.bptr);
+    PUP(ctx->bnode, P.bnode);
+#undef PUP
+    k_permute_gf<double><<<1024, 256, 0, s>>>(ngfr, (int)n_el, ctx->eperm, ctx->gf64, ctx->gfP64);
+    k_permute_gf<float><<<1024, 256, 0, s>>>(ngfr, (int)n_el, ctx->eperm, ctx->gf32, ctx->gfP32);
+    JFM_HIP(ctx, hipGetLastError());
+    JFM_HIP(ctx, hipStreamSynchronize(s));  // host vectors go out of scope
+    ctx->npatch = npatch;
+    ctx->npn = (int)npn;
+    ctx->nbnode = (int)nb;
+    ctx->nslot = P.bptr.back();
+    ctx->patch_ready = true;
+    return JFM_OK;
+}
+
+// ------------------------------------------------------------- launchers ---
+template <typename T> static const T *gfP_of(jfm_ctx *ctx);
+template <> const double *gfP_of<double>(jfm_ctx *ctx) { return ctx->gfP64; }
+template <> const float *gfP_of<float>(jfm_ctx *ctx) { return ctx->gfP32; }
+
+template <int NPE, typename T>
+static void patch_kernel(jfm_ctx *ctx, const T *x, T *y, const T *dotv, const int *done) {
+    const dim3 grid(xcd_grid(ctx->npatch));
+    const int n = (int)ctx->n_el;
+    double *pp = dotv ? ctx->ppart : nullptr;
+    if (ctx->affine)
+        k_ku_patch<NPE, T, true><<<grid, kPatch, 0, ctx->stream>>>(n, ctx->npatch, ctx->npn, ctx->connP, gfP_of<T>(ctx), x,
+                                                                   ctx->pn_ptr, ctx->pn_cnt, ctx->pn_out, ctx->rstart,
+                                                                   ctx->rbase, ctx->linT, ctx->fixed, y, (T *)ctx->bpart,
+                                                                   dotv, pp, done);
+    else
+        k_ku_patch<NPE, T, false><<<grid, kPatch, 0, ctx->stream>>>(n, ctx->npatch, ctx->npn, ctx->connP, gfP_of<T>(ctx), x,
+                                                                    ctx->pn_ptr, ctx->pn_cnt, ctx->pn_out, ctx->rstart,
+                                                                    ctx->rbase, ctx->linT, ctx->fixed, y, (T *)ctx->bpart,
+                                                                    dotv, pp, done);
+}
+
+template <typename T>
+hipError_t launch_patch_phase(jfm_ctx *ctx, const T *x, T *y, const T *dotv, const int *done) {
+    if (ctx->n_el == 0) return hipSuccess;
+    switch (ctx->npe) {
+    case 10: patch_kernel<10, T>(ctx, x, y, dotv, done); break;
+    case 4: patch_kernel<4, T>(ctx, x, y, dotv, done); break;
+    case 8: patch_kernel<8, T>(ctx, x, y, dotv, done); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_patch_combine(jfm_ctx *ctx, T *y, const T *dotv, double *part, const int *done, int *npart) {
+    // a multiple of 8 blocks (XCD mapping), at most 512 partials
+    const int nb = ctx->nbnode;
+    const unsigned g = std::min<unsigned>(xcd_grid((unsigned)((std::max(nb, 1) + kBlock - 1) / kBlock)), 512);
+    const int chunk = (int)((nb + g - 1) / g);
+    if (npart) *npart = (int)g;
+    k_ku_combine<T><<<g, kBlock, 0, ctx->stream>>>(nb, chunk, ctx->bptr, ctx->bnode, (const T *)ctx->bpart,
+                                                   ctx->fixed, y, dotv, ctx->n_el ? ctx->ppart : nullptr,
+                                                   ctx->n_el ? ctx->npatch : 0, part, done);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_patch_apply(jfm_ctx *ctx, const T *x, T *y, const T *dotv, double *part, const int *done, int *npart) {
+    hipError_t e = launch_patch_phase<T>(ctx, x, y, dotv, done);
+    if (e != hipSuccess) return e;
+    return launch_patch_combine<T>(ctx, y, dotv, part, done, npart);
+}
+
+template hipError_t launch_patch_phase<double>(jfm_ctx *, const double *, double *, const double *, const int *);
+template hipError_t launch_patch_phase<float>(jfm_ctx *, const float *, float *, const float *, const int *);
+template hipError_t launch_patch_combine<double>(jfm_ctx *, double *, const double *, double *, const int *, int *);
+template hipError_t launch_patch_combine<float>(jfm_ctx *, float *, const float *, double *, const int *, int *);
+template hipError_t launch_patch_apply<double>(jfm_ctx *, const double *, double *, const double *, double *, const int *, int *);
+template hipError_t launch_patch_apply<float>(jfm_ctx *, const float *, float *, const float *, double *, const int *, int *);
+
+}  // namespace jfem
