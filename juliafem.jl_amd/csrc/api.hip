@@ -177,7 +177,7 @@ static void release_mesh(jfm_ctx *ctx) {
         w->ok = false;
     }
     free_ptr(ctx, ctx->Mlearn64); free_ptr(ctx, ctx->Mlearn32); free_ptr(ctx, ctx->feat); free_ptr(ctx, ctx->lout);
-    free_ptr(ctx, ctx->Muser64); free_ptr(ctx, ctx->Muser32);
+    free_ptr(ctx, ctx->Muser64); free_ptr(ctx, ctx->Muser32); free_ptr(ctx, ctx->xprobe);
     free_ptr(ctx, ctx->ustate64); free_ptr(ctx, ctx->ustate32); free_ptr(ctx, ctx->Dtan);
     free_ptr(ctx, ctx->nl_u); free_ptr(ctx, ctx->nl_r); free_ptr(ctx, ctx->nl_du); free_ptr(ctx, ctx->nl_zero);
     ctx->has_state = ctx->minv_tangent = false;
@@ -873,14 +873,19 @@ static int learned_select(jfm_ctx *ctx, int op, int probe, int check, bool *use,
     if (probe <= 0) return JFM_OK;
     double phi[2] = {0.0, 0.0};
     const double *cand[2] = {(const double *)ctx->ws64.Minv, ctx->Mlearn64};
+    if (!ctx->xprobe) ALLOC(ctx, ctx->xprobe, double, ctx->ndof);
+    const size_t bytes = sizeof(double) * ctx->ndof;
     for (int c = 0; c < 2; ++c) {
         int it = 0, status = 0;
         if (int rc = pcg_run<double>(ctx, op, cand[c], 1e-30, probe, check, &it, &status, false, coarse)) return rc;
         if (int rc = jfem_dot(ctx, (const double *)ctx->ws64.b, (const double *)ctx->ws64.x, ctx->ndof, &phi[c]))
             return rc;
         *iters += it;
+        if (c == 0) JFM_HIP(ctx, hipMemcpyAsync(ctx->xprobe, ctx->ws64.x, bytes, hipMemcpyDeviceToDevice, ctx->stream));
     }
     *use = phi[1] > phi[0];
+    // the winner's probe iterate starts the solve
+    if (*use) JFM_HIP(ctx, hipMemcpyAsync(ctx->xprobe, ctx->ws64.x, bytes, hipMemcpyDeviceToDevice, ctx->stream));
     return JFM_OK;
 }
 
@@ -929,12 +934,14 @@ extern "C" int jfm_solve(jfm_ctx *ctx, const jfm_solve_opts *opts, const double 
         Minv64 = ctx->Muser64;
         Minv32 = ctx->Muser32;
     }
+    const double *x0 = nullptr;  // initial iterate (b = f - K g system), when a probe made one
     if (opts->precond == JFM_PREC_LEARNED && bnorm > 0.0) {
         bool use = true;
         int its = 0;
         if (int rc = learned_select(ctx, op_outer, opts->learned_probe, check, &use, &its, &st.setup_ms, coarse))
             return rc;
         st.iterations += its;
+        x0 = opts->learned_probe > 0 ? ctx->xprobe : nullptr;
         if (use) {
             Minv64 = ctx->Mlearn64;
             Minv32 = ctx->Mlearn32;
@@ -947,7 +954,9 @@ extern "C" int jfm_solve(jfm_ctx *ctx, const jfm_solve_opts *opts, const double 
         JFM_HIP(ctx, hipMemcpyAsync(ctx->u64, ctx->g, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     } else if (opts->method == JFM_SOLVE_PCG) {
         int it = 0, status = 0, total = 0, restarts = 0;
-        if (int rc = pcg_run<double>(ctx, op, Minv64, opts->rtol, opts->maxit, check, &it, &status, false, coarse))
+        if (x0) JFM_HIP(ctx, hipMemcpyAsync(ctx->ws64.x, x0, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        if (int rc = pcg_run<double>(ctx, op, Minv64, opts->rtol, opts->maxit, check, &it, &status, x0 != nullptr,
+                                     coarse))
             return rc;
         total = it;
         // The recursive residual drifts from the true one near the FP64 floor:
@@ -972,15 +981,17 @@ extern "C" int jfm_solve(jfm_ctx *ctx, const jfm_solve_opts *opts, const double 
     } else {
         // FP64 iterative refinement around FP32 PCG: r = f - K u in FP64,
         // K d = r / |r| in FP32 to inner_rtol, u += |r| d.
-        JFM_HIP(ctx, hipMemcpyAsync(ctx->u64, ctx->g, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        if (x0) CHECK_LAUNCH(ctx, launch_add(ctx, ctx->g, x0, ctx->u64, n));
+        else JFM_HIP(ctx, hipMemcpyAsync(ctx->u64, ctx->g, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
         int total = 0, outer = 0;
         const double inner = opts->inner_rtol > 0 ? opts->inner_rtol : 1e-4;
         const int max_outer = opts->max_outer > 0 ? opts->max_outer : 20;
         if (opts->first_rtol > 0.0) {
             // FP64 phase for the smooth low modes that FP32 cannot resolve
             int it = 0, status = 0;
-            if (int rc = pcg_run<double>(ctx, op_outer, Minv64, opts->first_rtol, opts->maxit, check, &it, &status, false,
-                                         coarse))
+            if (x0) JFM_HIP(ctx, hipMemcpyAsync(ctx->ws64.x, x0, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+            if (int rc = pcg_run<double>(ctx, op_outer, Minv64, opts->first_rtol, opts->maxit, check, &it, &status,
+                                         x0 != nullptr, coarse))
                 return rc;
             total += it;
             CHECK_LAUNCH(ctx, launch_add(ctx, ctx->g, (const double *)ctx->ws64.x, ctx->u64, n));

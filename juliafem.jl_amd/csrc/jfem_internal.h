@@ -217,6 +217,7 @@ struct jfm_ctx {
     bool has_mlp = false;
     bool learned_ready = false;    // Mlearn matches the current mesh/BCs/weights
     double *Mlearn64 = nullptr;    // N x 9 learned SPD blocks
+    double *xprobe = nullptr;      // the learned-selection probe's winning iterate
     float *Mlearn32 = nullptr;
     float *feat = nullptr;         // N x 24 node features
     float *lout = nullptr;         // N x 6 network outputs
