@@ -28,7 +28,7 @@ FP32_PEAK_TFLOPS = 157.3
 
 # headline candidates: (jfm_solve method, preconditioner, extra options)
 METHODS = {
-    "mixed-learned-2l": (1, 1, dict(learned_probe=16, inner_rtol=1e-2, coarse_nodes=2048)),
+    "mixed-learned-2l": (1, 1, dict(learned_probe=16, inner_rtol=2e-3, coarse_nodes=2048)),
     "mixed-bj-2l": (1, 0, dict(inner_rtol=1e-2, coarse_nodes=2048)),
     "fp64-bj-2l": (0, 0, dict(coarse_nodes=2048)),
     "mixed-learned": (1, 1, dict(learned_probe=16, inner_rtol=1e-2, first_rtol=1e-3)),

@@ -1006,12 +1006,16 @@ extern "C" int jfm_solve(jfm_ctx *ctx, const jfm_solve_opts *opts, const double 
             int it = 0, status = 0;
             int cap = opts->maxit - total;
             if (opts->inner_maxit > 0) cap = std::min(cap, opts->inner_maxit);
-            // when one inner solve tightened to at most inner^1.5 reaches the
-            // target, tighten it (with a factor 2 margin) rather than leave a
-            // residual just above the target for one more outer step
+            // spread the remaining reduction (to 0.3 x the target) evenly over
+            // the fewest inner solves that each reduce by at most inner_rtol:
+            // the last solve then ends well below the target instead of just
+            // above it, and no solve asks FP32 for more than inner_rtol
             double rt = inner;
-            const double need = 0.5 * opts->rtol * bnorm / rn;
-            if (need < inner && need >= std::pow(inner, 1.5)) rt = need;
+            const double R = 0.3 * opts->rtol * bnorm / rn;
+            if (R < 1.0 && inner < 1.0) {
+                const int k = std::max(1, (int)std::ceil(std::log(R) / std::log(inner) - 1e-9));
+                rt = std::pow(R, 1.0 / k);
+            }
             if (int rc = pcg_run<float>(ctx, op, Minv32, rt, cap, check, &it, &status, false, coarse)) return rc;
             total += it;
             ++outer;

@@ -105,23 +105,35 @@ __device__ __forceinline__ void agg_sums(int g, const int32_t *__restrict__ agg_
     block_sum_all<6>(w);
 }
 
-// A_c rows of aggregate g for probe (c, j): Rinv_g' w~_g into column 6p + j;
-// mode j = blockIdx.y, its product at y + j ys
-__global__ __launch_bounds__(256) void k_coarse_probe_restrict(int n_agg, const int32_t *__restrict__ agg_ptr,
-                                                              const int32_t *__restrict__ agg_nodes,
+// A_c rows of aggregate g for probe (c, j): Rinv_g' w~_g into column 6p + j,
+// with w~_g summed over g's nodes next to colour c only (segment blockIdx.x of
+// the colour's node list; the product is zero on g's other nodes).  Mode j =
+// blockIdx.y, its product at y + j ys; the rows read are set back to zero.
+__global__ __launch_bounds__(256) void k_coarse_probe_restrict(const int32_t *__restrict__ segb,
+                                                              const int32_t *__restrict__ sega,
+                                                              const int32_t *__restrict__ ncol,
                                                               const double *__restrict__ cd, double *__restrict__ y,
                                                               int64_t ys, const int32_t *__restrict__ partner_c,
                                                               const double *__restrict__ Rinv, double *__restrict__ Ac,
                                                               int nc) {
-    const int g = blockIdx.x, j = blockIdx.y;
+    const int sg = blockIdx.x, j = blockIdx.y;
     y += j * ys;
+    const int g = sega[sg];
+    double w[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int k = segb[sg] + threadIdx.x; k < segb[sg + 1]; k += blockDim.x) {
+        const size_t o = 3 * (size_t)ncol[k];
+        const double y0 = y[o], y1 = y[o + 1], y2 = y[o + 2];
+        y[o] = 0.0; y[o + 1] = 0.0; y[o + 2] = 0.0;
+        const double d0 = cd[o], d1 = cd[o + 1], d2 = cd[o + 2];
+        w[0] += y0; w[1] += y1; w[2] += y2;
+        w[3] += d1 * y2 - d2 * y1;
+        w[4] += d2 * y0 - d0 * y2;
+        w[5] += d0 * y1 - d1 * y0;
+    }
+    block_sum_all<6>(w);
+    // an aggregate next to colour c always has its colour-c partner
     const int p = partner_c[g];
-    // uniform per block; an aggregate with no colour-c partner has no node
-    // next to colour c, so its rows of y are zero and need no clearing
-    if (p < 0) return;
-    double w[6];
-    agg_sums<double, true>(g, agg_ptr, agg_nodes, cd, y, w);
-    if (threadIdx.x < 6) {
+    if (p >= 0 && threadIdx.x < 6) {
         const int i = threadIdx.x;
         const double *R = Rinv + 36 * (size_t)g;
         double s = 0.0;
@@ -457,6 +469,9 @@ struct Aggregation {
     std::vector<int32_t> dropped;   // nc
     // per colour: the elements touching its aggregates and those elements' nodes
     std::vector<int32_t> ecol_ptr, ecol, ncol_ptr, ncol;
+    // a colour's nodes are listed by aggregate: segment s (colours' segments
+    // at cseg[c]..cseg[c+1]) is ncol[segb[s]..segb[s+1]) of aggregate sega[s]
+    std::vector<int32_t> cseg, segb, sega;
 };
 
 // grid binning of the coordinates, about `target` nodes per bin
@@ -673,16 +688,27 @@ static int coarse_host_setup(jfm_ctx *ctx, int coarse_nodes, const std::vector<u
                     nd[c].push_back(a);
                 }
             }
-        std::sort(nd[c].begin(), nd[c].end());
+        std::sort(nd[c].begin(), nd[c].end(), [&](int32_t u, int32_t v) {
+            return agg[u] != agg[v] ? agg[u] < agg[v] : u < v;
+        });
         A.ecol_ptr[c + 1] = A.ecol_ptr[c] + (int32_t)el[c].size();
         A.ncol_ptr[c + 1] = A.ncol_ptr[c] + (int32_t)nd[c].size();
     }
     A.ecol.reserve(A.ecol_ptr[ncolor]);
     A.ncol.reserve(A.ncol_ptr[ncolor]);
+    A.cseg.assign(1, 0);
     for (int c = 0; c < ncolor; ++c) {
         A.ecol.insert(A.ecol.end(), el[c].begin(), el[c].end());
-        A.ncol.insert(A.ncol.end(), nd[c].begin(), nd[c].end());
+        for (size_t k = 0; k < nd[c].size(); ++k) {
+            if (k == 0 || agg[nd[c][k]] != agg[nd[c][k - 1]]) {
+                A.segb.push_back((int32_t)A.ncol.size());
+                A.sega.push_back(agg[nd[c][k]]);
+            }
+            A.ncol.push_back(nd[c][k]);
+        }
+        A.cseg.push_back((int32_t)A.sega.size());
     }
+    A.segb.push_back((int32_t)A.ncol.size());
     return JFM_OK;
 }
 
@@ -700,7 +726,8 @@ void jfem_coarse_release(jfm_ctx *ctx) {
     free_dev(ctx, ctx->gjC); free_dev(ctx, ctx->gjR); free_dev(ctx, ctx->gjP);
     free_dev(ctx, ctx->pv6); free_dev(ctx, ctx->qv6); free_dev(ctx, ctx->ye6);
     free_dev(ctx, ctx->dropped); free_dev(ctx, ctx->ecol); free_dev(ctx, ctx->ncol);
-    ctx->h_ecol_ptr.clear(); ctx->h_ncol_ptr.clear();
+    ctx->h_ecol_ptr.clear(); ctx->h_ncol_ptr.clear(); ctx->h_cseg.clear();
+    free_dev(ctx, ctx->segb); free_dev(ctx, ctx->sega);
     for (int k = 0; k < 2; ++k)
         if (ctx->coarse_graph[k]) {
             (void)hipGraphExecDestroy(ctx->coarse_graph[k]);
@@ -734,9 +761,10 @@ static hipError_t probe_sequence(jfm_ctx *ctx, bool tangent) {
                                 ctx->ncol + n0, nn);
         if (e != hipSuccess) break;
         // reads qv6 and sets the rows it read back to zero
-        k_coarse_probe_restrict<<<dim3(n_agg, 6), kBlock, 0, s>>>(n_agg, ctx->agg_ptr, ctx->agg_nodes, ctx->cd64,
-                                                                  ctx->qv6, N3, ctx->partner + (size_t)c * n_agg,
-                                                                  ctx->Rinv, ctx->Ac, nc);
+        const int s0 = ctx->h_cseg[c], ns = ctx->h_cseg[c + 1] - s0;
+        k_coarse_probe_restrict<<<dim3(ns, 6), kBlock, 0, s>>>(ctx->segb + s0, ctx->sega + s0, ctx->ncol, ctx->cd64,
+                                                               ctx->qv6, N3, ctx->partner + (size_t)c * n_agg,
+                                                               ctx->Rinv, ctx->Ac, nc);
         k_coarse_clear<<<dim3(nblk(std::max(ne, nn)), 6), kBlock, 0, s>>>(ne, ctx->ecol + e0, ctx->npe, (int)ctx->n_el,
                                                                           ctx->inc_slot, ctx->ye6, nslots, nn,
                                                                           ctx->ncol + n0, ctx->pv6, N3);
@@ -805,6 +833,8 @@ int jfem_coarse_setup(jfm_ctx *ctx, int op, int coarse_nodes, double *ms) {
         CALLOC(ctx->dropped, int32_t, nc);
         CALLOC(ctx->ecol, int32_t, std::max<int64_t>(1, (int64_t)A.ecol.size()));
         CALLOC(ctx->ncol, int32_t, std::max<int64_t>(1, (int64_t)A.ncol.size()));
+        CALLOC(ctx->segb, int32_t, (int64_t)A.segb.size());
+        CALLOC(ctx->sega, int32_t, std::max<int64_t>(1, (int64_t)A.sega.size()));
         CALLOC(ctx->Ac, double, nc * nc);
         CALLOC(ctx->gjC, double, nc * kGJ);
         CALLOC(ctx->gjR, double, nc * kGJ);
@@ -843,8 +873,12 @@ int jfem_coarse_setup(jfm_ctx *ctx, int op, int coarse_nodes, double *ms) {
         JFM_HIP(ctx, hipMemcpyAsync(ctx->dropped, A.dropped.data(), 4 * nc, hipMemcpyHostToDevice, s));
         JFM_HIP(ctx, hipMemcpyAsync(ctx->ecol, A.ecol.data(), 4 * A.ecol.size(), hipMemcpyHostToDevice, s));
         JFM_HIP(ctx, hipMemcpyAsync(ctx->ncol, A.ncol.data(), 4 * A.ncol.size(), hipMemcpyHostToDevice, s));
+        JFM_HIP(ctx, hipMemcpyAsync(ctx->segb, A.segb.data(), 4 * A.segb.size(), hipMemcpyHostToDevice, s));
+        if (!A.sega.empty())
+            JFM_HIP(ctx, hipMemcpyAsync(ctx->sega, A.sega.data(), 4 * A.sega.size(), hipMemcpyHostToDevice, s));
         ctx->h_ecol_ptr = A.ecol_ptr;
         ctx->h_ncol_ptr = A.ncol_ptr;
+        ctx->h_cseg = A.cseg;
         JFM_HIP(ctx, hipStreamSynchronize(s));
         ctx->coarse_built_nodes = coarse_nodes;
         ++ctx->coarse_gen;

@@ -242,7 +242,8 @@ struct jfm_ctx {
     int32_t *ecol = nullptr, *ncol = nullptr;        // per colour: probed elements, nodes
     hipGraphExec_t coarse_graph[2] = {nullptr, nullptr};  // probe sequence for K, K_T
     int coarse_graph_gen[2] = {0, 0};
-    std::vector<int32_t> h_ecol_ptr, h_ncol_ptr;
+    std::vector<int32_t> h_ecol_ptr, h_ncol_ptr, h_cseg;
+    int32_t *segb = nullptr, *sega = nullptr;        // colour node lists by aggregate
     double *cd64 = nullptr;        // N x 3 offsets from the aggregate centroid
     float *cd32 = nullptr;
     double *Rinv = nullptr;        // n_agg x 36
