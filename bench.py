@@ -302,6 +302,20 @@ def nonlinear(args):
     U = u.reshape(-1, 3)
     tip = U[mesh.X[:, 0] > 10.0 - 1e-9].mean(axis=0)
     t_apply = ctx.time_apply(op="tangent", precision=1, reps=20)
+    # the linear matrix-free K.u at this size, where the launch latency that
+    # bounds it at 1M DOFs is spread over 8x the work
+    ku_large = {}
+    for prec in (0, 1):
+        tot, k1, k2 = ctx.time_apply(op="matfree", precision=prec, reps=20)
+        inf2 = ctx.info()
+        b = kernel_bytes(inf2, "matfree", prec)
+        sv = survey_ku(inf2, prec)
+        ku_large["fp64" if prec == 0 else "fp32"] = {
+            "ms": tot, "patch_ms": k1, "combine_ms": k2,
+            "patch_GBps": b["patch"] / (k1 * 1e-3) / 1e9, "patch_frac": b["patch"] / (k1 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "Ku_GBps": (b["patch"] + b["combine"]) / (tot * 1e-3) / 1e9,
+            "ku_ideal_GBps": sv["bytes"] / (tot * 1e-3) / 1e9, "bytes": b}
+        log(f"K.u {'fp64' if prec == 0 else 'fp32'} at {inf2['n_dofs']} DOFs: {ku_large['fp64' if prec == 0 else 'fp32']}")
     cpu = None
     if not args.skip_cpu:
         cpu = nl_cpu_baseline(mesh, u, stats["newton_iterations"])
@@ -315,7 +329,7 @@ def nonlinear(args):
                    "options": opts, "parallelism": "single GPU"},
         "newton_iterations": stats["newton_iterations"], "linear_iterations": stats["linear_iterations"],
         "learned_fallbacks": stats["fallbacks"], "relres": stats["relres"], "converged": stats["converged"],
-        "tip_displacement": tip.tolist(), "tangent_apply_fp32_ms": t_apply[0],
+        "tip_displacement": tip.tolist(), "tangent_apply_fp32_ms": t_apply[0], "linear_ku": ku_large,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
