@@ -309,11 +309,12 @@ def nonlinear(args):
         tot, k1, k2 = ctx.time_apply(op="matfree", precision=prec, reps=20)
         inf2 = ctx.info()
         b = kernel_bytes(inf2, "matfree", prec)
+        (n1, b1), (n2, b2) = list(b.items())   # patch/combine, or element/gather
         sv = survey_ku(inf2, prec)
         ku_large["fp64" if prec == 0 else "fp32"] = {
-            "ms": tot, "patch_ms": k1, "combine_ms": k2,
-            "patch_GBps": b["patch"] / (k1 * 1e-3) / 1e9, "patch_frac": b["patch"] / (k1 * 1e-3) / 1e9 / HBM_PEAK_GBS,
-            "Ku_GBps": (b["patch"] + b["combine"]) / (tot * 1e-3) / 1e9,
+            "ms": tot, f"{n1}_ms": k1, f"{n2}_ms": k2,
+            f"{n1}_GBps": b1 / (k1 * 1e-3) / 1e9, f"{n1}_frac": b1 / (k1 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "Ku_GBps": (b1 + b2) / (tot * 1e-3) / 1e9,
             "ku_ideal_GBps": sv["bytes"] / (tot * 1e-3) / 1e9, "bytes": b}
         log(f"K.u {'fp64' if prec == 0 else 'fp32'} at {inf2['n_dofs']} DOFs: {ku_large['fp64' if prec == 0 else 'fp32']}")
     cpu = None
