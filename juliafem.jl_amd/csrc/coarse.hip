@@ -940,7 +940,13 @@ int jfem_coarse_setup(jfm_ctx *ctx, int op, int coarse_nodes, double *ms) {
 
 // coarse correction after the update kernel of a PCG iteration (or after the
 // init kernel with set_p): z += Q A_c^-1 Q' r, partials (r.z, r.r) to `part`
-static unsigned gemv_grid(int64_t nc) { return (unsigned)std::min<int64_t>((nc + 3) / 4, kMaxPart); }
+#ifndef JFEM_GEMV_ROWS
+#define JFEM_GEMV_ROWS 4
+#endif
+constexpr int kGemvRows = JFEM_GEMV_ROWS;  // rows (waves) per block of the coarse GEMV
+static unsigned gemv_grid(int64_t nc) {
+    return (unsigned)std::min<int64_t>((nc + kGemvRows - 1) / kGemvRows, kMaxPart);
+}
 
 template <typename T>
 static void coarse_product(jfm_ctx *ctx, const T *r, const int *done, double *part_wz) {
@@ -949,7 +955,7 @@ static void coarse_product(jfm_ctx *ctx, const T *r, const int *done, double *pa
     hipStream_t s = ctx->stream;
     k_coarse_restrict<T><<<n_agg, 1024, 0, s>>>(n_agg, ctx->agg_ptr, ctx->agg_nodes, cdA, ctx->fixed, r, ctx->wc32,
                                                ctx->wc64, done);
-    k_coarse_gemv<<<gemv_grid(nc), kBlock, 0, s>>>(nc, (int)(ctx->ldc / 4), (const float4 *)ctx->Ahat32,
+    k_coarse_gemv<<<gemv_grid(nc), 64 * kGemvRows, 0, s>>>(nc, (int)(ctx->ldc / 4), (const float4 *)ctx->Ahat32,
                                                   (const float4 *)ctx->wc32, ctx->wc64, ctx->zc32, part_wz, done);
 }
 
