@@ -36,7 +36,8 @@ constexpr int kPatch = 256;  // elements per patch = threads per workgroup
 // are taken in rounds of kPatch, one per thread; round q lists the j-th
 // incidence of its thread t's node at linT[rbase[q] + j kPatch + t] (j below
 // the node's count), so the lanes of a wave read consecutive entries.  Nodes
-// are sorted by count within a patch, so a round's counts are close.  A
+// are sorted by count within a patch, so a round's counts are close; the
+// first rounds' node data is loaded before the element product.  A
 // shared node's partial sum goes to its slot s (pn_out = -(s + 1)):
 // bpart[3 s + c].
 // Block partials of y.dotv over the interior nodes go to ppart.
@@ -55,18 +56,26 @@ __global__ __launch_bounds__(kPatch) void k_ku_patch(int n_el, int npatch, int n
     if (done && *done) return;
     const int t = threadIdx.x;
     const int k = p * kPatch + t;
-    // round 0's node data and first incidences, loaded before the element
-    // product so that their latency overlaps it
+    // the first kPreR rounds' node data and first incidences, loaded before
+    // the element product so that their latency overlaps it
     const int lb = pn_ptr[p], le = pn_ptr[p + 1], q0 = rstart[p];
-    constexpr int kPre = 4;
-    int cnt = 0, o = 0, mpre[kPre] = {0, 0, 0, 0};
-    if (lb + t < le) {
-        cnt = pn_cnt[lb + t];
-        o = pn_out[lb + t];
-        const int base = rbase[q0];
+    constexpr int kPre = 4, kPreR = sizeof(T) == 4 ? 3 : 1;  // FP64: registers are short
+    int cnt[kPreR], oo[kPreR], mpre[kPreR][kPre];
+    auto fetch = [&](int r, int l, int &c, int &o, int (&mp)[kPre]) {
+        c = pn_cnt[l];
+        o = pn_out[l];
+        const int base = rbase[q0 + r];
 #pragma unroll
-        for (int j = 0; j < kPre; ++j)
-            if (j < cnt) mpre[j] = linT[base + j * kPatch + t];
+        for (int j = 0; j < kPre; ++j) mp[j] = j < c ? (int)linT[base + j * kPatch + t] : 0;
+    };
+#pragma unroll
+    for (int r = 0; r < kPreR; ++r) {
+        cnt[r] = 0;
+        oo[r] = 0;
+#pragma unroll
+        for (int j = 0; j < kPre; ++j) mpre[r][j] = 0;
+        const int l = lb + r * kPatch + t;
+        if (l < le) fetch(r, l, cnt[r], oo[r], mpre[r]);
     }
     if (k < n_el)
         mf_element<NPE, T, AFF>(n_el, k, connP, gfP, x, [&](int i, T f0, T f1, T f2) {
@@ -76,30 +85,25 @@ __global__ __launch_bounds__(kPatch) void k_ku_patch(int n_el, int npatch, int n
         });
     __syncthreads();
     double dv = 0.0;
-    for (int r = 0, l = lb + t; lb + r * kPatch < le; ++r, l += kPatch) {
-        const int base = rbase[q0 + r];
-        if (r > 0 && l < le) {
-            cnt = pn_cnt[l];
-            o = pn_out[l];
-#pragma unroll
-            for (int j = 0; j < kPre; ++j)
-                if (j < cnt) mpre[j] = linT[base + j * kPatch + t];
-        }
-        if (l >= le) continue;
+    // node l of round r: the sum of its incidences, to y or to its slot
+    auto node = [&](int r, int l, int c, int o, const int (&mp)[kPre]) {
         T s0 = T(0), s1 = T(0), s2 = T(0);
 #pragma unroll
         for (int j = 0; j < kPre; ++j)
-            if (j < cnt) {
-                const int m = mpre[j];  // (3 i) * kPatch + local element
+            if (j < c) {
+                const int m = mp[j];  // (3 i) * kPatch + local element
                 s0 += ys[m];
                 s1 += ys[m + kPatch];
                 s2 += ys[m + 2 * kPatch];
             }
-        for (int j = kPre; j < cnt; ++j) {
-            const int m = linT[base + j * kPatch + t];
-            s0 += ys[m];
-            s1 += ys[m + kPatch];
-            s2 += ys[m + 2 * kPatch];
+        if (c > kPre) {
+            const int base = rbase[q0 + r];
+            for (int j = kPre; j < c; ++j) {
+                const int m = linT[base + j * kPatch + t];
+                s0 += ys[m];
+                s1 += ys[m + kPatch];
+                s2 += ys[m + 2 * kPatch];
+            }
         }
         if (o >= 0) {
             const size_t b = 3 * (size_t)o;
@@ -117,6 +121,19 @@ __global__ __launch_bounds__(kPatch) void k_ku_patch(int n_el, int npatch, int n
             bpart[b] = s0;
             bpart[b + 1] = s1;
             bpart[b + 2] = s2;
+        }
+    };
+#pragma unroll
+    for (int r = 0; r < kPreR; ++r) {
+        const int l = lb + r * kPatch + t;
+        if (l < le) node(r, l, cnt[r], oo[r], mpre[r]);
+    }
+    for (int r = kPreR; lb + r * kPatch < le; ++r) {  // patches of more than kPreR rounds
+        const int l = lb + r * kPatch + t;
+        if (l < le) {
+            int c, o, mp[kPre];
+            fetch(r, l, c, o, mp);
+            node(r, l, c, o, mp);
         }
     }
     if (ppart) {
