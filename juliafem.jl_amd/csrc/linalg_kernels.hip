@@ -129,7 +129,6 @@ __global__ __launch_bounds__(256) void k_pcg_update(int N, int slot, int npart_p
                                                    T *__restrict__ r, T *__restrict__ z, const T *__restrict__ p,
                                                    const T *__restrict__ q, const T *__restrict__ Minv,
                                                    double *__restrict__ partials, PcgScalars *S) {
-    if (S->done[slot]) return;
     const int stride = gridDim.x * blockDim.x;
     int a = blockIdx.x * blockDim.x + threadIdx.x;
     T xv[3], pv[3], rv[3], qv[3], M[9];
@@ -140,7 +139,8 @@ __global__ __launch_bounds__(256) void k_pcg_update(int N, int slot, int npart_p
 #pragma unroll
         for (int m = 0; m < 9; ++m) M[m] = Minv[9 * (size_t)aa + m];
     };
-    if (a < N) load(a);
+    if (a < N) load(a);  // in flight while the flag is read
+    if (S->done[slot]) return;
     double pq[1];
     sum_partials<1>(partials + part_pq(slot), npart_pq, pq);
     const double alpha_d = (pq[0] > 0.0 && finite(pq[0])) ? S->rz[slot] / pq[0] : 0.0;
@@ -190,15 +190,15 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_pcg_dir(int n3, int slot, int npart_rz, T *__restrict__ p,
                                                 const T *__restrict__ z, const double *__restrict__ partials,
                                                 PcgScalars *S) {
+    const int stride = gridDim.x * blockDim.x;
+    const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
+    // first operands in flight before the flag test and the beta reduction
+    T z0 = i0 < n3 ? z[i0] : T(0), p0 = i0 < n3 ? p[i0] : T(0);
     const int d = S->done[slot];
     if (d) {
         if (blockIdx.x == 0 && threadIdx.x == 0) S->done[slot ^ 1] = d;
         return;
     }
-    const int stride = gridDim.x * blockDim.x;
-    const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
-    // first operands in flight before the beta reduction
-    T z0 = i0 < n3 ? z[i0] : T(0), p0 = i0 < n3 ? p[i0] : T(0);
     double t[2];
     sum_partials<2>(partials + part_rz(slot), npart_rz, t);
     const double beta_d = t[0] / S->rz[slot];
@@ -217,6 +217,16 @@ __global__ __launch_bounds__(256) void k_pcg_dir_coarse(int N, int slot, int npa
                                                        const T *__restrict__ cd, const uint8_t *__restrict__ fixed,
                                                        const float *__restrict__ zc, const double *__restrict__ partials,
                                                        PcgScalars *S) {
+    // the first node's operands in flight before the flag test and the reductions
+    const int a0 = blockIdx.x * blockDim.x + threadIdx.x;
+    T z0[3] = {T(0), T(0), T(0)}, p0[3] = {T(0), T(0), T(0)}, d0[3] = {T(0), T(0), T(0)};
+    int g0 = 0;
+    if (a0 < N) {
+        const size_t o = 3 * (size_t)a0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { z0[k] = z[o + k]; p0[k] = p[o + k]; d0[k] = cd[o + k]; }
+        g0 = node_agg[a0];
+    }
     const int d = S->done[slot];
     if (d) {
         if (blockIdx.x == 0 && threadIdx.x == 0) S->done[slot ^ 1] = d;
@@ -228,14 +238,19 @@ __global__ __launch_bounds__(256) void k_pcg_dir_coarse(int N, int slot, int npa
     const double rz = t[0] + u[0];
     const double beta_d = rz / S->rz[slot];
     const T beta = (T)beta_d;
-    for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < N; a += gridDim.x * blockDim.x) {
+    for (int a = a0; a < N; a += gridDim.x * blockDim.x) {
         const size_t o = 3 * (size_t)a;
-        const float *yf = zc + 6 * (size_t)node_agg[a];
-        const T y[6] = {(T)yf[0], (T)yf[1], (T)yf[2], (T)yf[3], (T)yf[4], (T)yf[5]};
-        const T d0 = cd[o], d1 = cd[o + 1], d2 = cd[o + 2];
-        const T c[3] = {y[0] + y[4] * d2 - y[5] * d1, y[1] + y[5] * d0 - y[3] * d2, y[2] + y[3] * d1 - y[4] * d0};
+        if (a != a0) {
 #pragma unroll
-        for (int k = 0; k < 3; ++k) p[o + k] = (z[o + k] + (fixed[o + k] ? T(0) : c[k])) + beta * p[o + k];
+            for (int k = 0; k < 3; ++k) { z0[k] = z[o + k]; p0[k] = p[o + k]; d0[k] = cd[o + k]; }
+            g0 = node_agg[a];
+        }
+        const float *yf = zc + 6 * (size_t)g0;
+        const T y[6] = {(T)yf[0], (T)yf[1], (T)yf[2], (T)yf[3], (T)yf[4], (T)yf[5]};
+        const T c[3] = {y[0] + y[4] * d0[2] - y[5] * d0[1], y[1] + y[5] * d0[0] - y[3] * d0[2],
+                        y[2] + y[3] * d0[1] - y[4] * d0[0]};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) p[o + k] = (z0[k] + (fixed[o + k] ? T(0) : c[k])) + beta * p0[k];
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) pcg_record(S, slot, rz, t[1], beta_d);
 }

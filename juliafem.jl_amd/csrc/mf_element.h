@@ -13,11 +13,11 @@
 
 namespace jfem {
 
-// Element e (SoA connectivity conn[i * n_el + e], geometric factors
-// gf[(q * GF + k) * n_el + e]); out(i, f0, f1, f2) receives node i's terms.
-template <int NPE, typename T, bool AFF, typename Out>
-__device__ __forceinline__ void mf_element(int n_el, int e, const int32_t *__restrict__ conn, const T *__restrict__ gf,
-                                           const T *__restrict__ x, Out &&out) {
+// Element e with node values read through xl(i, v): v = x at local node i;
+// geometric factors gf[(q * GF + k) * n_el + e]; out(i, f0, f1, f2) receives
+// node i's terms.
+template <int NPE, typename T, bool AFF, typename XL, typename Out>
+__device__ __forceinline__ void mf_element_x(int n_el, int e, const T *__restrict__ gf, XL &&xl, Out &&out) {
     using EL = Elem<NPE>;
     constexpr int NG = EL::NGP;
     T H[NG][9];
@@ -25,12 +25,13 @@ __device__ __forceinline__ void mf_element(int n_el, int e, const int32_t *__res
     for (int q = 0; q < NG; ++q)
 #pragma unroll
         for (int k = 0; k < 9; ++k) H[q][k] = T(0);
-    const int n0 = conn[e];
-    const T t0 = x[3 * (size_t)n0], t1 = x[3 * (size_t)n0 + 1], t2 = x[3 * (size_t)n0 + 2];
+    T v0[3];
+    xl(0, v0);
 #pragma unroll
     for (int i = 1; i < NPE; ++i) {
-        const int nd = conn[(size_t)i * n_el + e];
-        const T u0 = x[3 * (size_t)nd] - t0, u1 = x[3 * (size_t)nd + 1] - t1, u2 = x[3 * (size_t)nd + 2] - t2;
+        T v[3];
+        xl(i, v);
+        const T u0 = v[0] - v0[0], u1 = v[1] - v0[1], u2 = v[2] - v0[2];
 #pragma unroll
         for (int q = 0; q < NG; ++q)
 #pragma unroll
@@ -91,6 +92,18 @@ __device__ __forceinline__ void mf_element(int n_el, int e, const int32_t *__res
             }
         out(i, f0, f1, f2);
     }
+}
+
+// Element e with SoA connectivity conn[i * n_el + e] and x gathered from HBM.
+template <int NPE, typename T, bool AFF, typename Out>
+__device__ __forceinline__ void mf_element(int n_el, int e, const int32_t *__restrict__ conn, const T *__restrict__ gf,
+                                           const T *__restrict__ x, Out &&out) {
+    mf_element_x<NPE, T, AFF>(n_el, e, gf, [&](int i, T (&v)[3]) {
+        const size_t o = 3 * (size_t)conn[(size_t)i * n_el + e];
+        v[0] = x[o];
+        v[1] = x[o + 1];
+        v[2] = x[o + 2];
+    }, out);
 }
 
 }  // namespace jfem
