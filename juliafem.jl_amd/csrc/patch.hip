@@ -53,7 +53,7 @@ __global__ __launch_bounds__(kPatch) void k_ku_patch(int n_el, int npatch, int n
     __shared__ T ys[3 * NPE * kPatch];
     const int p = xcd_block();
     if (p >= npatch) return;  // whole workgroup (grid padded to a multiple of 8)
-    if (done && *done) return;
+    const int dn = done ? *done : 0;  // tested once the prefetches below are in flight
     const int t = threadIdx.x;
     const int k = p * kPatch + t;
     // the first kPreR rounds' node data and first incidences, loaded before
@@ -77,6 +77,7 @@ __global__ __launch_bounds__(kPatch) void k_ku_patch(int n_el, int npatch, int n
         const int l = lb + r * kPatch + t;
         if (l < le) fetch(r, l, cnt[r], oo[r], mpre[r]);
     }
+    if (dn) return;  // uniform
     if (k < n_el)
         mf_element<NPE, T, AFF>(n_el, k, connP, gfP, x, [&](int i, T f0, T f1, T f2) {
             ys[(3 * i) * kPatch + t] = f0;
@@ -155,15 +156,24 @@ __global__ __launch_bounds__(256) void k_ku_combine(int nb, int chunk, const int
                                                    const uint8_t *__restrict__ fixed, T *__restrict__ y,
                                                    const T *__restrict__ dotv, const double *__restrict__ ppart,
                                                    int npatch, double *__restrict__ part, const int *__restrict__ done) {
+    // the first node's slot range is read with the convergence flag
+    const int b0 = xcd_block() * chunk, b1 = min(b0 + chunk, nb);
+    int ra = 0, rb = 0;
+    if (b0 + (int)threadIdx.x < b1) {
+        ra = bptr[b0 + threadIdx.x];
+        rb = bptr[b0 + threadIdx.x + 1];
+    }
     if (done && *done) return;
     double dv = 0.0;
     if (dotv)
         for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < npatch; q += gridDim.x * blockDim.x) dv += ppart[q];
-    const int b0 = xcd_block() * chunk, b1 = min(b0 + chunk, nb);
     for (int b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
         T s0 = T(0), s1 = T(0), s2 = T(0);
-        const int r1 = bptr[b + 1];
-        for (int r = bptr[b]; r < r1; ++r) {
+        if (b != b0 + (int)threadIdx.x) {
+            ra = bptr[b];
+            rb = bptr[b + 1];
+        }
+        for (int r = ra; r < rb; ++r) {
             s0 += bpart[3 * (size_t)r];
             s1 += bpart[3 * (size_t)r + 1];
             s2 += bpart[3 * (size_t)r + 2];
