@@ -837,10 +837,27 @@ static int pcg_run(jfm_ctx *ctx, int op, const T *Minv, double rtol, int maxit, 
     if (coarse) CHECK_LAUNCH(ctx, launch_coarse_correct<T>(ctx, r, z, p, ctx->partials + kPartMisc, nullptr, &np));
     if (int rc = finalize_misc(ctx, np, 2, FIN_INIT)) return rc;
     if (int rc = read_scalars(ctx)) return rc;
-    const int batch = std::max(2, (check_every + 1) & ~1);
+    // Batches of up to check_every iterations; once the residual falls, the
+    // next batch is the power of two that covers the iterations its rate
+    // predicts are left, so that few launches run past convergence.
+    const int bmax = std::max(2, (check_every + 1) & ~1);
+    int batch = bmax;
+    double rr_prev = ctx->S_host->rr;
+    int it_prev = ctx->S_host->iter;
     while (ctx->S_host->done[0] == DONE_RUNNING) {
         if (int rc = pcg_batch<T>(ctx, op, Minv, batch, coarse)) return rc;
         if (int rc = read_scalars(ctx)) return rc;
+        const double rr = ctx->S_host->rr, tol2 = ctx->S_host->tol2;
+        const int it = ctx->S_host->iter;
+        batch = bmax;
+        if (rr > tol2 && tol2 > 0.0 && rr < rr_prev && it > it_prev) {
+            const double left = std::log(tol2 / rr) / (std::log(rr / rr_prev) / (it - it_prev));
+            int nb = 2;
+            while (nb < left + 1.0 && nb < bmax) nb *= 2;
+            batch = std::min(nb, bmax);
+        }
+        rr_prev = rr;
+        it_prev = it;
     }
     *iters = ctx->S_host->iter;
     *status = ctx->S_host->done[0];
