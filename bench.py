@@ -288,7 +288,7 @@ def nonlinear(args):
     with open(DEFAULT_WEIGHTS, "rb") as fh:
         ctx.set_learned_weights(fh.read())
     opts = dict(load_steps=5, rtol=1e-8, method=1, precond=1, learned_probe=16, inner_rtol=1e-2,
-                coarse_nodes=2048, max_newton=20)
+                coarse_nodes=4096, max_newton=20)
     times, stats, u = [], None, None
     for k in range(args.warmup + args.steps):
         t = time.perf_counter()

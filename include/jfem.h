@@ -203,11 +203,14 @@ typedef struct {
     double eta_max;       /* first and largest linear tolerance of Eisenstat-Walker
                              choice 2 (0: 0.1) */
     int32_t maxit_linear; /* PCG iteration cap per tangent solve */
-    int32_t learned_probe;/* as in jfm_solve_opts */
+    int32_t learned_probe;/* as in jfm_solve_opts, at the first Newton iterate of each load
+                             step; the step's later iterates keep the outcome (accepted
+                             blocks are recomputed from each new tangent diagonal) */
     double inner_rtol;    /* JFM_SOLVE_MIXED_IR: FP32 inner tolerance (0: 1e-2) */
     int32_t inner_maxit;  /* JFM_SOLVE_MIXED_IR: FP32 iterations per inner solve (0: no cap) */
     int32_t check_every;  /* PCG iterations between host checks (0: 32) */
-    int32_t coarse_nodes; /* as in jfm_solve_opts; A_c = Q' K_T Q per Newton iteration */
+    int32_t coarse_nodes; /* as in jfm_solve_opts; A_c = Q' K_T Q at the first Newton
+                             iterate of each load step */
 } jfm_newton_opts;
 
 typedef struct {

@@ -1267,6 +1267,7 @@ extern "C" int jfm_solve_nonlinear(jfm_ctx *ctx, const jfm_newton_opts *opts, do
         const double ref = lf * fnorm > 0.0 ? lf * fnorm : (rn > 0.0 ? rn : 1.0);
         bool ok = false;
         double rn_prev = rn, eta_prev = eta_max;
+        bool step_learned = true;  // this load step's probe outcome
         for (int it = 0;; ++it) {
             relres = rn / ref;
             ctx->hist_it.push_back(st.linear_iterations);
@@ -1287,12 +1288,20 @@ extern "C" int jfm_solve_nonlinear(jfm_ctx *ctx, const jfm_newton_opts *opts, do
             const double *M64 = (const double *)ctx->ws64.Minv;
             const float *M32 = (const float *)ctx->ws32.Minv;
             if (opts->precond == JFM_PREC_LEARNED) {
+                // The probe runs at the first iterate of each load step, like the
+                // coarse rebuild; later iterates of the step keep its outcome.
+                // Accepted blocks are recomputed from the new tangent diagonal at
+                // every Newton step.
                 ctx->learned_ready = false;
-                bool use = true;
+                bool use = false;
                 int its = 0;
-                JFM_HIP(ctx, hipMemcpyAsync(ctx->ws64.b, ctx->nl_r, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-                if (int rc = learned_select(ctx, JFM_OP_TANGENT, opts->learned_probe, check, &use, &its, nullptr, coarse))
-                    return rc;
+                if (it == 0 || step_learned) {
+                    JFM_HIP(ctx, hipMemcpyAsync(ctx->ws64.b, ctx->nl_r, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+                    const int probe = it == 0 ? opts->learned_probe : 0;
+                    if (int rc = learned_select(ctx, JFM_OP_TANGENT, probe, check, &use, &its, nullptr, coarse))
+                        return rc;
+                    if (it == 0) step_learned = use;
+                }
                 st.linear_iterations += its;
                 if (use) {
                     M64 = ctx->Mlearn64;
