@@ -888,21 +888,24 @@ static int learned_select(jfm_ctx *ctx, int op, int probe, int check, bool *use,
     if (setup_ms) *setup_ms += lms;
     *use = true;
     if (probe <= 0) return JFM_OK;
+    // The probe runs in FP32 (FP32 operator and blocks) on b rounded to FP32:
+    // it only ranks the two preconditioners by the energy b'x_k.
     double phi[2] = {0.0, 0.0};
-    const double *cand[2] = {(const double *)ctx->ws64.Minv, ctx->Mlearn64};
-    if (!ctx->xprobe) ALLOC(ctx, ctx->xprobe, double, ctx->ndof);
-    const size_t bytes = sizeof(double) * ctx->ndof;
+    const float *cand[2] = {(const float *)ctx->ws32.Minv, ctx->Mlearn32};
+    const int64_t n = ctx->ndof;
+    if (!ctx->xprobe) ALLOC(ctx, ctx->xprobe, double, n);
+    double *xd[2] = {ctx->xprobe, ctx->vy64};  // the iterates in FP64 (vy64 is free here)
+    CHECK_LAUNCH(ctx, (launch_convert<double, float>(ctx, (const double *)ctx->ws64.b, (float *)ctx->ws32.b, n, 1.0)));
     for (int c = 0; c < 2; ++c) {
         int it = 0, status = 0;
-        if (int rc = pcg_run<double>(ctx, op, cand[c], 1e-30, probe, check, &it, &status, false, coarse)) return rc;
-        if (int rc = jfem_dot(ctx, (const double *)ctx->ws64.b, (const double *)ctx->ws64.x, ctx->ndof, &phi[c]))
-            return rc;
+        if (int rc = pcg_run<float>(ctx, op, cand[c], 1e-30, probe, check, &it, &status, false, coarse)) return rc;
+        CHECK_LAUNCH(ctx, (launch_convert<float, double>(ctx, (const float *)ctx->ws32.x, xd[c], n, 1.0)));
+        if (int rc = jfem_dot(ctx, (const double *)ctx->ws64.b, xd[c], n, &phi[c])) return rc;
         *iters += it;
-        if (c == 0) JFM_HIP(ctx, hipMemcpyAsync(ctx->xprobe, ctx->ws64.x, bytes, hipMemcpyDeviceToDevice, ctx->stream));
     }
     *use = phi[1] > phi[0];
     // the winner's probe iterate starts the solve
-    if (*use) JFM_HIP(ctx, hipMemcpyAsync(ctx->xprobe, ctx->ws64.x, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    if (*use) JFM_HIP(ctx, hipMemcpyAsync(ctx->xprobe, xd[1], sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
     return JFM_OK;
 }
 
