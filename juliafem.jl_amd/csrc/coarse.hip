@@ -108,11 +108,11 @@ __device__ __forceinline__ void agg_sums(int g, const int32_t *__restrict__ agg_
 // A_c rows of aggregate g for probe (c, j): Rinv_g' w~_g into column 6p + j,
 // with w~_g summed over g's nodes next to colour c only (segment blockIdx.x of
 // the colour's node list; the product is zero on g's other nodes).  Mode j =
-// blockIdx.y, its product at y + j ys; the rows read are set back to zero.
+// blockIdx.y, its product at y + j ys.
 __global__ __launch_bounds__(256) void k_coarse_probe_restrict(const int32_t *__restrict__ segb,
                                                               const int32_t *__restrict__ sega,
                                                               const int32_t *__restrict__ ncol,
-                                                              const double *__restrict__ cd, double *__restrict__ y,
+                                                              const double *__restrict__ cd, const double *__restrict__ y,
                                                               int64_t ys, const int32_t *__restrict__ partner_c,
                                                               const double *__restrict__ Rinv, double *__restrict__ Ac,
                                                               int nc) {
@@ -123,7 +123,6 @@ __global__ __launch_bounds__(256) void k_coarse_probe_restrict(const int32_t *__
     for (int k = segb[sg] + threadIdx.x; k < segb[sg + 1]; k += blockDim.x) {
         const size_t o = 3 * (size_t)ncol[k];
         const double y0 = y[o], y1 = y[o + 1], y2 = y[o + 2];
-        y[o] = 0.0; y[o + 1] = 0.0; y[o + 2] = 0.0;
         const double d0 = cd[o], d1 = cd[o + 1], d2 = cd[o + 2];
         w[0] += y0; w[1] += y1; w[2] += y2;
         w[3] += d1 * y2 - d2 * y1;
@@ -147,26 +146,30 @@ __global__ __launch_bounds__(256) void k_fill_zero(double *__restrict__ p, int64
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0.0;
 }
 
-// zero the incidence slots of the listed elements and the listed rows of y,
-// for vector blockIdx.y (slots at ye + j ys, rows at y + j vs)
-__global__ __launch_bounds__(256) void k_coarse_clear(int ne, const int32_t *__restrict__ elist, int npe, int n_el,
-                                                     const int32_t *__restrict__ slot, double *__restrict__ ye, int64_t ys,
-                                                     int nn, const int32_t *__restrict__ nlist, double *__restrict__ y,
-                                                     int64_t vs) {
+// Rows of the colour's nodes of y_j = K x_j: the sums of their incidence
+// slots in the colour's elements only (ascending element id), so the slots of
+// other elements, stale from earlier colours, are never read; mode j =
+// blockIdx.y (slots at ye + j ys, rows at y + j vs).  Fixed rows are zero.
+__global__ __launch_bounds__(256) void k_coarse_gather(int nn, const int32_t *__restrict__ nlist,
+                                                      const int32_t *__restrict__ sptr, const int32_t *__restrict__ sidx,
+                                                      const double *__restrict__ ye, int64_t ys,
+                                                      const uint8_t *__restrict__ fixed, double *__restrict__ y,
+                                                      int64_t vs) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nn) return;
     ye += blockIdx.y * ys;
     y += blockIdx.y * vs;
-    if (k < ne) {
-        const int e = elist[k];
-        for (int i = 0; i < npe; ++i) {
-            double *o = ye + 3 * (size_t)slot[(size_t)i * n_el + e];
-            o[0] = 0.0; o[1] = 0.0; o[2] = 0.0;
-        }
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int r = sptr[k]; r < sptr[k + 1]; ++r) {
+        const size_t o = 3 * (size_t)sidx[r];
+        s0 += ye[o];
+        s1 += ye[o + 1];
+        s2 += ye[o + 2];
     }
-    if (k < nn) {
-        const size_t o = 3 * (size_t)nlist[k];
-        y[o] = 0.0; y[o + 1] = 0.0; y[o + 2] = 0.0;
-    }
+    const size_t o = 3 * (size_t)nlist[k];
+    y[o] = fixed[o] ? 0.0 : s0;
+    y[o + 1] = fixed[o + 1] ? 0.0 : s1;
+    y[o + 2] = fixed[o + 2] ? 0.0 : s2;
 }
 
 // symmetrise A_c; unit diagonal for dropped (all-zero) modes
@@ -472,6 +475,9 @@ struct Aggregation {
     // a colour's nodes are listed by aggregate: segment s (colours' segments
     // at cseg[c]..cseg[c+1]) is ncol[segb[s]..segb[s+1]) of aggregate sega[s]
     std::vector<int32_t> cseg, segb, sega;
+    // per listed node (ncol order): its incidence slots in the colour's
+    // elements, ascending element id: csub[csub_ptr[k]..csub_ptr[k+1])
+    std::vector<int32_t> csub_ptr, csub;
 };
 
 // grid binning of the coordinates, about `target` nodes per bin
@@ -535,7 +541,8 @@ std::vector<std::vector<int32_t>> agg_adjacency(const std::vector<int32_t> &conn
 }  // namespace
 
 // Host-side aggregation, colouring and per-aggregate mode bases.
-static int coarse_host_setup(jfm_ctx *ctx, int coarse_nodes, const std::vector<uint8_t> &fixed, Aggregation &A) {
+static int coarse_host_setup(jfm_ctx *ctx, int coarse_nodes, const std::vector<uint8_t> &fixed,
+                             const std::vector<int32_t> &slot, Aggregation &A) {
     const int64_t N = ctx->N;
     const std::vector<double> &X = ctx->h_X;
     int n_agg = 0;
@@ -709,6 +716,27 @@ static int coarse_host_setup(jfm_ctx *ctx, int coarse_nodes, const std::vector<u
         A.cseg.push_back((int32_t)A.sega.size());
     }
     A.segb.push_back((int32_t)A.ncol.size());
+    // per colour and listed node: its incidence slots in the colour's
+    // elements, ascending element id (slot[i * n_el + e] is incidence (e, i))
+    A.csub_ptr.assign(1, 0);
+    std::vector<int32_t> posn(N, -1), cnt, fill;
+    for (int c = 0; c < ncolor; ++c) {
+        const int32_t k0 = A.ncol_ptr[c], k1 = A.ncol_ptr[c + 1];
+        for (int32_t k = k0; k < k1; ++k) posn[A.ncol[k]] = k - k0;
+        cnt.assign(k1 - k0 + 1, 0);
+        for (int32_t e : el[c])
+            for (int i = 0; i < npe; ++i) ++cnt[posn[ctx->h_conn[(int64_t)e * npe + i]] + 1];
+        for (int32_t k = 0; k < k1 - k0; ++k) cnt[k + 1] += cnt[k];
+        const size_t base = A.csub.size();
+        A.csub.resize(base + cnt[k1 - k0]);
+        fill.assign(cnt.begin(), cnt.end() - 1);
+        for (int32_t e : el[c])  // ascending element id
+            for (int i = 0; i < npe; ++i) {
+                const int32_t a = ctx->h_conn[(int64_t)e * npe + i];
+                A.csub[base + fill[posn[a]]++] = slot[(size_t)i * n_el + e];
+            }
+        for (int32_t k = 0; k < k1 - k0; ++k) A.csub_ptr.push_back((int32_t)(base + cnt[k + 1]));
+    }
     return JFM_OK;
 }
 
@@ -727,7 +755,7 @@ void jfem_coarse_release(jfm_ctx *ctx) {
     free_dev(ctx, ctx->pv6); free_dev(ctx, ctx->qv6); free_dev(ctx, ctx->ye6);
     free_dev(ctx, ctx->dropped); free_dev(ctx, ctx->ecol); free_dev(ctx, ctx->ncol);
     ctx->h_ecol_ptr.clear(); ctx->h_ncol_ptr.clear(); ctx->h_cseg.clear();
-    free_dev(ctx, ctx->segb); free_dev(ctx, ctx->sega);
+    free_dev(ctx, ctx->segb); free_dev(ctx, ctx->sega); free_dev(ctx, ctx->csub_ptr); free_dev(ctx, ctx->csub);
     for (int k = 0; k < 2; ++k)
         if (ctx->coarse_graph[k]) {
             (void)hipGraphExecDestroy(ctx->coarse_graph[k]);
@@ -740,10 +768,11 @@ void jfem_coarse_release(jfm_ctx *ctx) {
 }
 
 // Zero A_c, then one batched product of the six modes per colour over the
-// colour's elements and nodes only; the slots and rows a colour wrote are
-// cleared before the next colour, so pv6, qv6 and ye6 are zero between
-// colours (as after their allocation).  No memset nodes, so the sequence can
-// be captured.
+// colour's elements and nodes only.  Each colour writes every row of pv6 and
+// qv6 and every slot of ye6 that it reads (the probe covers the colour's
+// nodes, which hold all nodes of its elements; the gather reads only the
+// colour's element slots), so nothing needs clearing between colours.  No
+// memset nodes, so the sequence can be captured.
 static hipError_t probe_sequence(jfm_ctx *ctx, bool tangent) {
     hipStream_t s = ctx->stream;
     const int64_t N3 = 3 * ctx->N;
@@ -757,17 +786,16 @@ static hipError_t probe_sequence(jfm_ctx *ctx, bool tangent) {
         if (nn == 0) continue;
         k_coarse_probe<<<dim3(nblk(nn), 6), kBlock, 0, s>>>(nn, ctx->ncol + n0, ctx->node_agg, ctx->agg_color, c,
                                                             ctx->cd64, ctx->Rinv, ctx->fixed, ctx->pv6, N3);
+        // element products only: the gather below reads just this colour's slots
         e = launch_subset_apply(ctx, tangent, ctx->pv6, ctx->qv6, ctx->ye6, 6, N3, nslots, ctx->ecol + e0, ne,
-                                ctx->ncol + n0, nn);
+                                nullptr, 0);
         if (e != hipSuccess) break;
-        // reads qv6 and sets the rows it read back to zero
+        k_coarse_gather<<<dim3(nblk(nn), 6), kBlock, 0, s>>>(nn, ctx->ncol + n0, ctx->csub_ptr + n0, ctx->csub,
+                                                             ctx->ye6, nslots, ctx->fixed, ctx->qv6, N3);
         const int s0 = ctx->h_cseg[c], ns = ctx->h_cseg[c + 1] - s0;
         k_coarse_probe_restrict<<<dim3(ns, 6), kBlock, 0, s>>>(ctx->segb + s0, ctx->sega + s0, ctx->ncol, ctx->cd64,
                                                                ctx->qv6, N3, ctx->partner + (size_t)c * n_agg,
                                                                ctx->Rinv, ctx->Ac, nc);
-        k_coarse_clear<<<dim3(nblk(std::max(ne, nn)), 6), kBlock, 0, s>>>(ne, ctx->ecol + e0, ctx->npe, (int)ctx->n_el,
-                                                                          ctx->inc_slot, ctx->ye6, nslots, nn,
-                                                                          ctx->ncol + n0, ctx->pv6, N3);
         e = hipGetLastError();
     }
     return e;
@@ -809,8 +837,11 @@ int jfem_coarse_setup(jfm_ctx *ctx, int op, int coarse_nodes, double *ms) {
         std::vector<uint8_t> fx(n);
         JFM_HIP(ctx, hipMemcpyAsync(fx.data(), ctx->fixed, n, hipMemcpyDeviceToHost, s));
         JFM_HIP(ctx, hipStreamSynchronize(s));
+        std::vector<int32_t> slot((size_t)ctx->npe * ctx->n_el);
+        JFM_HIP(ctx, hipMemcpyAsync(slot.data(), ctx->inc_slot, 4 * slot.size(), hipMemcpyDeviceToHost, s));
+        JFM_HIP(ctx, hipStreamSynchronize(s));
         Aggregation A;
-        if (int rc = coarse_host_setup(ctx, coarse_nodes, fx, A)) return rc;
+        if (int rc = coarse_host_setup(ctx, coarse_nodes, fx, slot, A)) return rc;
         const int64_t nc = 6 * (int64_t)A.n_agg;
         if (nc > 20000) return jfm_fail(ctx, JFM_EINVAL, "coarse level too large: raise coarse_nodes");
         ctx->n_agg = A.n_agg;
@@ -834,6 +865,8 @@ int jfem_coarse_setup(jfm_ctx *ctx, int op, int coarse_nodes, double *ms) {
         CALLOC(ctx->ecol, int32_t, std::max<int64_t>(1, (int64_t)A.ecol.size()));
         CALLOC(ctx->ncol, int32_t, std::max<int64_t>(1, (int64_t)A.ncol.size()));
         CALLOC(ctx->segb, int32_t, (int64_t)A.segb.size());
+        CALLOC(ctx->csub_ptr, int32_t, (int64_t)A.csub_ptr.size());
+        CALLOC(ctx->csub, int32_t, std::max<int64_t>(1, (int64_t)A.csub.size()));
         CALLOC(ctx->sega, int32_t, std::max<int64_t>(1, (int64_t)A.sega.size()));
         CALLOC(ctx->Ac, double, nc * nc);
         CALLOC(ctx->gjC, double, nc * kGJ);
@@ -874,6 +907,9 @@ int jfem_coarse_setup(jfm_ctx *ctx, int op, int coarse_nodes, double *ms) {
         JFM_HIP(ctx, hipMemcpyAsync(ctx->ecol, A.ecol.data(), 4 * A.ecol.size(), hipMemcpyHostToDevice, s));
         JFM_HIP(ctx, hipMemcpyAsync(ctx->ncol, A.ncol.data(), 4 * A.ncol.size(), hipMemcpyHostToDevice, s));
         JFM_HIP(ctx, hipMemcpyAsync(ctx->segb, A.segb.data(), 4 * A.segb.size(), hipMemcpyHostToDevice, s));
+        JFM_HIP(ctx, hipMemcpyAsync(ctx->csub_ptr, A.csub_ptr.data(), 4 * A.csub_ptr.size(), hipMemcpyHostToDevice, s));
+        if (!A.csub.empty())
+            JFM_HIP(ctx, hipMemcpyAsync(ctx->csub, A.csub.data(), 4 * A.csub.size(), hipMemcpyHostToDevice, s));
         if (!A.sega.empty())
             JFM_HIP(ctx, hipMemcpyAsync(ctx->sega, A.sega.data(), 4 * A.sega.size(), hipMemcpyHostToDevice, s));
         ctx->h_ecol_ptr = A.ecol_ptr;

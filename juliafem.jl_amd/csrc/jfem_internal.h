@@ -244,6 +244,7 @@ struct jfm_ctx {
     int coarse_graph_gen[2] = {0, 0};
     std::vector<int32_t> h_ecol_ptr, h_ncol_ptr, h_cseg;
     int32_t *segb = nullptr, *sega = nullptr;        // colour node lists by aggregate
+    int32_t *csub_ptr = nullptr, *csub = nullptr;    // colour node lists: their slots in the colour's elements
     double *cd64 = nullptr;        // N x 3 offsets from the aggregate centroid
     float *cd32 = nullptr;
     double *Rinv = nullptr;        // n_agg x 36
