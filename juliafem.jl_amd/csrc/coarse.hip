@@ -83,28 +83,6 @@ __global__ __launch_bounds__(256) void k_coarse_probe(int nn, const int32_t *__r
     for (int k = 0; k < 3; ++k) v[3 * (size_t)a + k] = fixed[3 * (size_t)a + k] ? 0.0 : out[k];
 }
 
-// w~_g = sum_{a in g} [y_a ; d_a x y_a] in a fixed order (block per aggregate);
-// with CLEAR the rows read are set to zero (every node is in one aggregate)
-template <typename T, bool CLEAR = false>
-__device__ __forceinline__ void agg_sums(int g, const int32_t *__restrict__ agg_ptr, const int32_t *__restrict__ agg_nodes,
-                                         const T *__restrict__ cd, T *__restrict__ y, double (&w)[6]) {
-#pragma unroll
-    for (int k = 0; k < 6; ++k) w[k] = 0.0;
-    for (int k = agg_ptr[g] + threadIdx.x; k < agg_ptr[g + 1]; k += blockDim.x) {
-        const size_t o = 3 * (size_t)agg_nodes[k];
-        const double y0 = y[o], y1 = y[o + 1], y2 = y[o + 2];
-        if (CLEAR) { y[o] = T(0); y[o + 1] = T(0); y[o + 2] = T(0); }
-        const double d0 = cd[o], d1 = cd[o + 1], d2 = cd[o + 2];
-        w[0] += y0;
-        w[1] += y1;
-        w[2] += y2;
-        w[3] += d1 * y2 - d2 * y1;
-        w[4] += d2 * y0 - d0 * y2;
-        w[5] += d0 * y1 - d1 * y0;
-    }
-    block_sum_all<6>(w);
-}
-
 // A_c rows of aggregate g for probe (c, j): Rinv_g' w~_g into column 6p + j,
 // with w~_g summed over g's nodes next to colour c only (segment blockIdx.x of
 // the colour's node list; the product is zero on g's other nodes).  Mode j =

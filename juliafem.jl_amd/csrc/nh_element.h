@@ -16,8 +16,6 @@
 
 namespace jfem {
 
-template <typename T> struct Mat3 { T m[9]; };
-
 // det(I + M) - 1 = I1(M) + I2(M) + I3(M), exact at small M
 template <typename T>
 __device__ __forceinline__ T det_ipm_minus1(const T *M) {
