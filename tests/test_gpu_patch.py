@@ -117,5 +117,8 @@ def test_same_as_two_phase_operator(jf, tmp_path):
     r = np.load(out)
     assert int(r["patches"]) == 0
     assert np.abs(y - r["y"]).max() <= 1e-13 * np.abs(y).max()
-    assert abs(st["iterations"] - int(r["it"])) <= 2
+    # block-Jacobi PCG runs ~950 iterations here, past the loss of
+    # orthogonality: rounding-level differences (the order of the nodal sums
+    # and of p.q) move the count by a few percent, not the answer
+    assert abs(st["iterations"] - int(r["it"])) <= 0.03 * int(r["it"]) + 2
     assert np.linalg.norm(u - r["u"]) <= 1e-9 * np.linalg.norm(u)
