@@ -109,7 +109,7 @@ typedef struct {
     int32_t npe, affine;
     int64_t device_bytes;  /* bytes currently allocated by the context */
     /* patch form of the matrix-free operator (0 until it is first used):
-     * patches of 256 elements, their (patch, node) entries, the nodes shared
+     * patches of 512 elements, their (patch, node) entries, the nodes shared
      * between patches and their partial sums */
     int64_t n_patches, patch_nodes, shared_nodes, shared_slots;
 } jfm_info;

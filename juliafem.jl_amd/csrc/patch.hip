@@ -2,7 +2,7 @@
 //
 // The two-phase operator (fem_kernels.hip) writes every element's 3 x npe
 // terms to HBM and reads them back in the nodal gather.  Here the elements are
-// ordered along a Morton curve of their centroids and cut into patches of 256
+// ordered along a Morton curve of their centroids and cut into patches of 512
 // (one workgroup each, one element per thread).  A workgroup computes its
 // elements' terms with the same element product (mf_element.h) into LDS and
 // then sums them per patch node, in ascending local element order, from a
@@ -29,7 +29,7 @@
 
 namespace jfem {
 
-constexpr int kPatch = 256;  // elements per patch = threads per workgroup
+constexpr int kPatch = 512;  // elements per patch = threads per workgroup
 
 // Fused element product and patch-local nodal sums.  LDS holds the patch's
 // element terms as ys[(3 i + c) * kPatch + local element].  The patch's nodes

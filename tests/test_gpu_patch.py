@@ -1,12 +1,12 @@
 """The patch form of the matrix-free operator (patch.hip), through the C ABI.
 
-The elements are cut into Morton-ordered patches of 256; patch-interior
+The elements are cut into Morton-ordered patches of 512; patch-interior
 nodes are summed inside a workgroup and shared nodes by a second kernel.
 Checked here beyond the parity suite (which runs every matrix-free product
 through it):
   * the structure's invariant: every node is either interior to exactly one
     patch or listed once among the shared nodes (nodes of no element
-    included), and there is one patch per 256 elements;
+    included), and there is one patch per 512 elements;
   * K.u against the oracle's restatement (1e-12 FP64, 2e-6 FP32) on meshes
     whose element count is not a multiple of 256 and that hold nodes of no
     element, with and without the row mask (reactions);
@@ -24,9 +24,11 @@ from jfem import mesh as M
 
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PATCH = 512  # elements per patch (kPatch in patch.hip)
 
 MESHES = {
     "tet10_13x3x2": lambda: M.box_tet10(13, 3, 2, seed=7, jitter=0.1, E_sigma=0.4),   # 468 elements
+    "tet10_17x4x3": lambda: M.box_tet10(17, 4, 3, seed=9, jitter=0.1, E_sigma=0.4),   # 1,224 elements
     "tet10_40x4x4": lambda: M.box_tet10(40, 4, 4),                                   # 3,840, unused lattice nodes
     "hex8_9x5x4": lambda: M.box_hex8(9, 5, 4, seed=3, jitter=0.1, E_sigma=0.3),
     "tet4_11x4x3": lambda: M.box_tet4(11, 4, 3, seed=1, jitter=0.1, E_sigma=0.3),
@@ -53,7 +55,7 @@ def test_structure_and_apply(jf, oracle, name):
     x = np.random.default_rng(11).standard_normal(m.n_dofs)
     y64 = ctx.apply(x, op="matfree")
     inf = ctx.info()
-    assert inf["n_patches"] == (m.n_el + 255) // 256
+    assert inf["n_patches"] == (m.n_el + PATCH - 1) // PATCH
     interior = inf["patch_nodes"] - inf["shared_slots"]
     assert interior + inf["shared_nodes"] == m.n_nodes
     assert inf["shared_slots"] >= 2 * (inf["shared_nodes"] - (m.n_nodes - len(np.unique(m.conn))))
