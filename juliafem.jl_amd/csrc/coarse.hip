@@ -350,7 +350,9 @@ __global__ __launch_bounds__(256) void k_coarse_fold(int nc, int ld, const doubl
 // aggregate (about two nodes per thread); cdA is cd in agg_nodes order so its
 // reads are contiguous.  Written in FP32 (padded row of the gemv) and FP64
 // (for w.zc).
-template <typename T>
+// With AGG, r is already m o r in aggregate order (rA, position k), so the
+// reads are contiguous and need no node list or mask.
+template <typename T, bool AGG>
 __global__ __launch_bounds__(1024) void k_coarse_restrict(int n_agg, const int32_t *__restrict__ agg_ptr,
                                                          const int32_t *__restrict__ agg_nodes, const T *__restrict__ cdA,
                                                          const uint8_t *__restrict__ fixed, const T *__restrict__ r,
@@ -362,10 +364,17 @@ __global__ __launch_bounds__(1024) void k_coarse_restrict(int n_agg, const int32
     const int g = blockIdx.x;
     double w[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     for (int k = agg_ptr[g] + threadIdx.x; k < agg_ptr[g + 1]; k += blockDim.x) {
-        const size_t o = 3 * (size_t)agg_nodes[k];
-        const double y0 = fixed[o] ? 0.0 : (double)r[o];
-        const double y1 = fixed[o + 1] ? 0.0 : (double)r[o + 1];
-        const double y2 = fixed[o + 2] ? 0.0 : (double)r[o + 2];
+        double y0, y1, y2;
+        if (AGG) {
+            y0 = (double)r[3 * (size_t)k];
+            y1 = (double)r[3 * (size_t)k + 1];
+            y2 = (double)r[3 * (size_t)k + 2];
+        } else {
+            const size_t o = 3 * (size_t)agg_nodes[k];
+            y0 = fixed[o] ? 0.0 : (double)r[o];
+            y1 = fixed[o + 1] ? 0.0 : (double)r[o + 1];
+            y2 = fixed[o + 2] ? 0.0 : (double)r[o + 2];
+        }
         const double d0 = cdA[3 * (size_t)k], d1 = cdA[3 * (size_t)k + 1], d2 = cdA[3 * (size_t)k + 2];
         w[0] += y0; w[1] += y1; w[2] += y2;
         w[3] += d1 * y2 - d2 * y1;
@@ -728,7 +737,7 @@ void jfem_coarse_release(jfm_ctx *ctx) {
     free_dev(ctx, ctx->agg_color); free_dev(ctx, ctx->partner); free_dev(ctx, ctx->cd64); free_dev(ctx, ctx->cd32);
     free_dev(ctx, ctx->Rinv); free_dev(ctx, ctx->Ac); free_dev(ctx, ctx->Ahat32);
     free_dev(ctx, ctx->wc32); free_dev(ctx, ctx->zc32); free_dev(ctx, ctx->wc64);
-    free_dev(ctx, ctx->cdA64); free_dev(ctx, ctx->cdA32);
+    free_dev(ctx, ctx->cdA64); free_dev(ctx, ctx->cdA32); free_dev(ctx, ctx->apos); free_dev(ctx, ctx->rA);
     free_dev(ctx, ctx->gjC); free_dev(ctx, ctx->gjR); free_dev(ctx, ctx->gjP);
     free_dev(ctx, ctx->pv6); free_dev(ctx, ctx->qv6); free_dev(ctx, ctx->ye6);
     free_dev(ctx, ctx->dropped); free_dev(ctx, ctx->ecol); free_dev(ctx, ctx->ncol);
@@ -864,6 +873,8 @@ int jfem_coarse_setup(jfm_ctx *ctx, int op, int coarse_nodes, double *ms) {
         CALLOC(ctx->wc64, double, nc);
         CALLOC(ctx->zc32, float, nc);
         CALLOC(ctx->cdA64, double, 3 * N);
+        CALLOC(ctx->apos, int32_t, N);
+        CALLOC(ctx->rA, double, 3 * N);
         CALLOC(ctx->cdA32, float, 3 * N);
 #undef CALLOC
         JFM_HIP(ctx, hipMemcpyAsync(ctx->node_agg, A.node_agg.data(), 4 * N, hipMemcpyHostToDevice, s));
@@ -878,6 +889,12 @@ int jfem_coarse_setup(jfm_ctx *ctx, int op, int coarse_nodes, double *ms) {
             for (int c = 0; c < 3; ++c) cdA[3 * k + c] = A.cd[3 * (size_t)A.agg_nodes[k] + c];
         std::vector<float> cdA32(cdA.begin(), cdA.end());
         JFM_HIP(ctx, hipMemcpyAsync(ctx->cdA64, cdA.data(), 8 * 3 * N, hipMemcpyHostToDevice, s));
+        {
+            std::vector<int32_t> apos(N);
+            for (int64_t k = 0; k < N; ++k) apos[A.agg_nodes[k]] = (int32_t)k;
+            JFM_HIP(ctx, hipMemcpyAsync(ctx->apos, apos.data(), 4 * N, hipMemcpyHostToDevice, s));
+            JFM_HIP(ctx, hipStreamSynchronize(s));  // apos goes out of scope
+        }
         JFM_HIP(ctx, hipMemcpyAsync(ctx->cdA32, cdA32.data(), 4 * 3 * N, hipMemcpyHostToDevice, s));
         JFM_HIP(ctx, hipMemsetAsync(ctx->wc32, 0, 4 * ldc, s));  // padding stays zero
         JFM_HIP(ctx, hipMemcpyAsync(ctx->Rinv, A.Rinv.data(), 8 * A.Rinv.size(), hipMemcpyHostToDevice, s));
@@ -962,23 +979,33 @@ static unsigned gemv_grid(int64_t nc) {
     return (unsigned)std::min<int64_t>((nc + kGemvRows - 1) / kGemvRows, kMaxPart);
 }
 
-template <typename T>
+template <typename T, bool AGG>
 static void coarse_product(jfm_ctx *ctx, const T *r, const int *done, double *part_wz) {
     const T *cdA = sizeof(T) == 8 ? (const T *)ctx->cdA64 : (const T *)ctx->cdA32;
     const int n_agg = (int)ctx->n_agg, nc = (int)ctx->nc;
     hipStream_t s = ctx->stream;
-    k_coarse_restrict<T><<<n_agg, 1024, 0, s>>>(n_agg, ctx->agg_ptr, ctx->agg_nodes, cdA, ctx->fixed, r, ctx->wc32,
-                                               ctx->wc64, done);
+    k_coarse_restrict<T, AGG><<<n_agg, 1024, 0, s>>>(n_agg, ctx->agg_ptr, ctx->agg_nodes, cdA, ctx->fixed, r, ctx->wc32,
+                                                    ctx->wc64, done);
     k_coarse_gemv<<<gemv_grid(nc), 64 * kGemvRows, 0, s>>>(nc, (int)(ctx->ldc / 4), (const float4 *)ctx->Ahat32,
                                                   (const float4 *)ctx->wc32, ctx->wc64, ctx->zc32, part_wz, done);
 }
 
 template <typename T>
 hipError_t launch_coarse_product(jfm_ctx *ctx, const T *r, const int *done, int *npart_wz) {
-    coarse_product<T>(ctx, r, done, ctx->partials + kPartCoarse);
+    coarse_product<T, false>(ctx, r, done, ctx->partials + kPartCoarse);
     *npart_wz = (int)gemv_grid(ctx->nc);
     return hipGetLastError();
 }
+
+template <typename T>
+hipError_t launch_coarse_product_agg(jfm_ctx *ctx, const T *rA, const int *done, int *npart_wz) {
+    coarse_product<T, true>(ctx, rA, done, ctx->partials + kPartCoarse);
+    *npart_wz = (int)gemv_grid(ctx->nc);
+    return hipGetLastError();
+}
+
+template <> double *coarse_rA<double>(jfm_ctx *ctx) { return ctx->rA; }
+template <> float *coarse_rA<float>(jfm_ctx *ctx) { return (float *)ctx->rA; }
 
 // coarse correction outside the iteration (PCG start, training):
 // z += Q A_c^-1 Q' r, optionally p = z; partials (r.z, r.r) to `part`
@@ -986,7 +1013,7 @@ template <typename T>
 hipError_t launch_coarse_correct(jfm_ctx *ctx, const T *r, T *z, T *p, double *part, const int *done, int *npart) {
     const T *cd = sizeof(T) == 8 ? (const T *)ctx->cd64 : (const T *)ctx->cd32;
     hipStream_t s = ctx->stream;
-    coarse_product<T>(ctx, r, done, ctx->partials + kPartCoarse);
+    coarse_product<T, false>(ctx, r, done, ctx->partials + kPartCoarse);
     const unsigned g = std::min<unsigned>(capped_grid(ctx->N, kBlock), 512);
     *npart = (int)g;
     k_coarse_prolong<T><<<g, kBlock, 0, s>>>((int)ctx->N, ctx->node_agg, cd, ctx->fixed, ctx->zc32, r, z, p, part, done);
@@ -995,6 +1022,8 @@ hipError_t launch_coarse_correct(jfm_ctx *ctx, const T *r, T *z, T *p, double *p
 
 template hipError_t launch_coarse_product<double>(jfm_ctx *, const double *, const int *, int *);
 template hipError_t launch_coarse_product<float>(jfm_ctx *, const float *, const int *, int *);
+template hipError_t launch_coarse_product_agg<double>(jfm_ctx *, const double *, const int *, int *);
+template hipError_t launch_coarse_product_agg<float>(jfm_ctx *, const float *, const int *, int *);
 template hipError_t launch_coarse_correct<double>(jfm_ctx *, const double *, double *, double *, double *, const int *, int *);
 template hipError_t launch_coarse_correct<float>(jfm_ctx *, const float *, float *, float *, double *, const int *, int *);
 

@@ -249,6 +249,8 @@ struct jfm_ctx {
     float *cd32 = nullptr;
     double *Rinv = nullptr;        // n_agg x 36
     double *Ac = nullptr;          // nc x nc (inverted in place)
+    int32_t *apos = nullptr;       // node -> its position in agg_nodes
+    double *rA = nullptr;          // 3 x N: the masked residual in aggregate order (FP64 or FP32 view)
     double *cdA64 = nullptr;       // cd in aggregate order (agg_nodes), for the restriction
     float *cdA32 = nullptr;
     int64_t ldc = 0;               // row stride of Ahat32: nc rounded up to 256, zero padded
@@ -352,6 +354,11 @@ hipError_t launch_coarse_correct(jfm_ctx *ctx, const T *r, T *z, T *p, double *p
 // partials (npart_wz of them) at partials + kPartCoarse
 template <typename T>
 hipError_t launch_coarse_product(jfm_ctx *ctx, const T *r, const int *done, int *npart_wz);
+// the same from m o r in aggregate order, as the update kernel writes it to
+// coarse_rA<T>(ctx)
+template <typename T>
+hipError_t launch_coarse_product_agg(jfm_ctx *ctx, const T *rA, const int *done, int *npart_wz);
+template <typename T> T *coarse_rA(jfm_ctx *ctx);
 // p = z + m o Q zc + beta p, beta from the (r.z, r.r) and w.zc partials
 template <typename T>
 hipError_t launch_pcg_direction_coarse(jfm_ctx *ctx, int slot, int npart_rz, int npart_wz, T *p, const T *z);
@@ -375,7 +382,7 @@ template <typename T>
 hipError_t launch_pcg_init(jfm_ctx *ctx, T *r, T *z, T *p, const T *Minv, int *npart);
 template <typename T>
 hipError_t launch_pcg_update(jfm_ctx *ctx, int slot, int npart_pq, T *x, T *r, T *z, const T *p, const T *q,
-                             const T *Minv, int *npart);
+                             const T *Minv, int *npart, T *rA = nullptr);
 template <typename T>
 hipError_t launch_pcg_direction(jfm_ctx *ctx, int slot, int npart_rz, T *p, const T *z);
 hipError_t launch_finalize(jfm_ctx *ctx, const double *part, int npart, int nv, int fin);

@@ -124,11 +124,16 @@ __global__ __launch_bounds__(256) void k_pcg_init(int N, const T *__restrict__ r
 // x += alpha p, r -= alpha q, z = Minv r; partials (r.z, r.r).  The first
 // node's operands are loaded before the alpha reduction so that their
 // latency overlaps it.
+// With rA, the masked residual also goes to rA in aggregate order (node a at
+// position apos[a]) for the coarse restriction, which then reads it
+// contiguously.
 template <typename T>
 __global__ __launch_bounds__(256) void k_pcg_update(int N, int slot, int npart_pq, T *__restrict__ x,
                                                    T *__restrict__ r, T *__restrict__ z, const T *__restrict__ p,
                                                    const T *__restrict__ q, const T *__restrict__ Minv,
-                                                   double *__restrict__ partials, PcgScalars *S) {
+                                                   double *__restrict__ partials, PcgScalars *S,
+                                                   T *__restrict__ rA = nullptr, const int32_t *__restrict__ apos = nullptr,
+                                                   const uint8_t *__restrict__ fixed = nullptr) {
     const int stride = gridDim.x * blockDim.x;
     int a = blockIdx.x * blockDim.x + threadIdx.x;
     T xv[3], pv[3], rv[3], qv[3], M[9];
@@ -158,6 +163,11 @@ __global__ __launch_bounds__(256) void k_pcg_update(int N, int slot, int npart_p
         const T z1 = M[3] * rv[0] + M[4] * rv[1] + M[5] * rv[2];
         const T z2 = M[6] * rv[0] + M[7] * rv[1] + M[8] * rv[2];
         z[o] = z0; z[o + 1] = z1; z[o + 2] = z2;
+        if (rA) {
+            const size_t oa = 3 * (size_t)apos[a];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) rA[oa + c] = fixed[o + c] ? T(0) : rv[c];
+        }
         v[0] += (double)rv[0] * z0 + (double)rv[1] * z1 + (double)rv[2] * z2;
         v[1] += (double)rv[0] * rv[0] + (double)rv[1] * rv[1] + (double)rv[2] * rv[2];
         a += stride;
@@ -364,11 +374,11 @@ constexpr int kDirBlocks = JFEM_DIR_BLOCKS;  // grid of the PCG direction kernel
 
 template <typename T>
 hipError_t launch_pcg_update(jfm_ctx *ctx, int slot, int npart_pq, T *x, T *r, T *z, const T *p, const T *q,
-                             const T *Minv, int *npart) {
+                             const T *Minv, int *npart, T *rA) {
     const unsigned g = std::min<unsigned>(capped_grid(ctx->N, kBlock), kVecBlocks);
     *npart = (int)g;
     k_pcg_update<T><<<g, kBlock, 0, ctx->stream>>>((int)ctx->N, slot, npart_pq, x, r, z, p, q, Minv,
-                                                  ctx->partials, ctx->S);
+                                                  ctx->partials, ctx->S, rA, rA ? ctx->apos : nullptr, ctx->fixed);
     return hipGetLastError();
 }
 
@@ -437,8 +447,8 @@ template hipError_t launch_bsr_spmv<double>(jfm_ctx *, const double *, const dou
 template hipError_t launch_bsr_spmv<float>(jfm_ctx *, const float *, const float *, float *, const float *, double *, const int *, int *);
 template hipError_t launch_pcg_init<double>(jfm_ctx *, double *, double *, double *, const double *, int *);
 template hipError_t launch_pcg_init<float>(jfm_ctx *, float *, float *, float *, const float *, int *);
-template hipError_t launch_pcg_update<double>(jfm_ctx *, int, int, double *, double *, double *, const double *, const double *, const double *, int *);
-template hipError_t launch_pcg_update<float>(jfm_ctx *, int, int, float *, float *, float *, const float *, const float *, const float *, int *);
+template hipError_t launch_pcg_update<double>(jfm_ctx *, int, int, double *, double *, double *, const double *, const double *, const double *, int *, double *);
+template hipError_t launch_pcg_update<float>(jfm_ctx *, int, int, float *, float *, float *, const float *, const float *, const float *, int *, float *);
 template hipError_t launch_pcg_direction<double>(jfm_ctx *, int, int, double *, const double *);
 template hipError_t launch_pcg_direction<float>(jfm_ctx *, int, int, float *, const float *);
 template hipError_t launch_convert<double, float>(jfm_ctx *, const double *, float *, int64_t, double);
