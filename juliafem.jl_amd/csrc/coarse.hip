@@ -29,11 +29,16 @@
 //  * the whole device part is one launch sequence per aggregation and
 //    operator, captured once as a HIP graph and replayed.
 //
-// Per PCG iteration (after the update kernel has written z = B r and the
-// partials of r.Br and r.r):
-//  k_coarse_restrict  w_g = sum_{a in g} [m_a r_a ; d_a x m_a r_a]  (block per g)
+// Per PCG iteration:
+//  k_pcg_update       x, r, z = B r and the partials of r.Br and r.r; extra
+//                     blocks (one per aggregate) form the restriction by
+//                     recurrence, w_g <- w_g - alpha sum_{a in g} [m_a q_a ;
+//                     d_a x m_a q_a] (q = K p), as r <- r - alpha q; w is
+//                     formed directly from r at the PCG start
 //  k_coarse_gemv      zc = Ahat w, partials of w.zc             (wave per row)
 //  k_pcg_dir_coarse   p_a = z_a + m_a o (t_g + omega_g x d_a) + beta p_a
+// (JFEM_W_RECURRENCE=0: the update writes m o r in aggregate order and
+// k_coarse_restrict sums it after the update instead.)
 // The prolongation is folded into the direction update, and r.z of the full
 // preconditioned residual is r.Br + w.zc (r.(m o Q zc) = (Q' m r).zc), so the
 // corrected z is never stored.  At the start of PCG, k_coarse_prolong adds
@@ -420,6 +425,7 @@ __global__ __launch_bounds__(256) void k_coarse_gemv(int nc, int ld4, const floa
     if (dn) return;  // every thread leaves before the block reduction
     write_partials<1>(v, part);
 }
+
 
 // z_a += m_a o (t_g + omega_g x d_a); optionally p = z; partials (r.z, r.r)
 template <typename T>
@@ -870,7 +876,7 @@ int jfem_coarse_setup(jfm_ctx *ctx, int op, int coarse_nodes, double *ms) {
         ctx->ldc = ldc;
         CALLOC(ctx->Ahat32, float, nc * ldc);
         CALLOC(ctx->wc32, float, ldc);
-        CALLOC(ctx->wc64, double, nc);
+        CALLOC(ctx->wc64, double, 2 * nc);
         CALLOC(ctx->zc32, float, nc);
         CALLOC(ctx->cdA64, double, 3 * N);
         CALLOC(ctx->apos, int32_t, N);
@@ -1001,6 +1007,25 @@ template <typename T>
 hipError_t launch_coarse_product_agg(jfm_ctx *ctx, const T *rA, const int *done, int *npart_wz) {
     coarse_product<T, true>(ctx, rA, done, ctx->partials + kPartCoarse);
     *npart_wz = (int)gemv_grid(ctx->nc);
+    return hipGetLastError();
+}
+
+bool jfem_coarse_w_recurrence(const jfm_ctx *ctx) {
+    static const bool on = [] {
+        const char *v = std::getenv("JFEM_W_RECURRENCE");
+        return !(v && v[0] == '0');
+    }();
+    return on && ctx->n_agg > 0;
+}
+
+// zc = Ahat w_new for the iteration of parity `slot`, w_new as the update
+// kernel's restriction blocks wrote it (wc32, and wc64 + (slot ^ 1) nc)
+hipError_t launch_coarse_gemv_rec(jfm_ctx *ctx, int slot, int *npart_wz) {
+    const int nc = (int)ctx->nc;
+    k_coarse_gemv<<<gemv_grid(nc), 64 * kGemvRows, 0, ctx->stream>>>(
+        nc, (int)(ctx->ldc / 4), (const float4 *)ctx->Ahat32, (const float4 *)ctx->wc32,
+        ctx->wc64 + (size_t)(slot ^ 1) * nc, ctx->zc32, ctx->partials + kPartCoarse, &ctx->S->done[slot]);
+    *npart_wz = (int)gemv_grid(nc);
     return hipGetLastError();
 }
 

@@ -41,12 +41,15 @@ template <int NV> __device__ __forceinline__ void block_sum_all(double (&v)[NV])
 }
 
 // Block sums of v[] -> part[k*kMaxPart + blockIdx.x].  Called by all threads.
-template <int NV> __device__ __forceinline__ void write_partials(double (&v)[NV], double *part) {
+template <int NV> __device__ __forceinline__ void write_partials(double (&v)[NV], double *part, int slot) {
     block_sum_all<NV>(v);
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int k = 0; k < NV; ++k) part[k * kMaxPart + blockIdx.x] = v[k];
+        for (int k = 0; k < NV; ++k) part[k * kMaxPart + slot] = v[k];
     }
+}
+template <int NV> __device__ __forceinline__ void write_partials(double (&v)[NV], double *part) {
+    write_partials<NV>(v, part, (int)blockIdx.x);
 }
 
 // Sum of npart partials per value, fixed order, returned to every thread.
@@ -94,7 +97,7 @@ __device__ __forceinline__ int xcd_block() {
     const int q = (int)gridDim.x / kXcds;
     return ((int)blockIdx.x % kXcds) * q + (int)blockIdx.x / kXcds;
 }
-inline unsigned xcd_grid(unsigned nb) { return (nb + kXcds - 1) / kXcds * kXcds; }
+__host__ __device__ inline unsigned xcd_grid(unsigned nb) { return (nb + kXcds - 1) / kXcds * kXcds; }
 
 // Surface elements: Tri6 / Tri3 / Quad4 basis (src/basis/lagrange_generated.jl:
 // 89-171) and their rules (Tri 3-point and 1-point, Quad 2x2 Gauss).

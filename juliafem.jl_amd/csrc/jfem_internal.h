@@ -256,7 +256,7 @@ struct jfm_ctx {
     int64_t ldc = 0;               // row stride of Ahat32: nc rounded up to 256, zero padded
     float *Ahat32 = nullptr;       // nc x ldc: R^-1 A_c^-1 R^-T
     float *wc32 = nullptr, *zc32 = nullptr;  // wc32 has ldc entries, zero past nc
-    double *wc64 = nullptr;        // Q' m r of the last restriction
+    double *wc64 = nullptr;        // 2 x nc: Q' m r by iteration parity (slot 0 at the PCG start)
     double *gjC = nullptr, *gjR = nullptr, *gjP = nullptr;  // Gauss-Jordan panels
     // patch operator (patch.hip): elements in Morton order, 512 per patch
     bool patch_ready = false;
@@ -359,6 +359,14 @@ hipError_t launch_coarse_product(jfm_ctx *ctx, const T *r, const int *done, int 
 template <typename T>
 hipError_t launch_coarse_product_agg(jfm_ctx *ctx, const T *rA, const int *done, int *npart_wz);
 template <typename T> T *coarse_rA(jfm_ctx *ctx);
+// The iteration's coarse right-hand side by recurrence: w_{k+1} = w_k -
+// alpha Q' m q_k, formed by extra blocks of the update kernel
+// (launch_pcg_update(..., restrict_q = true)) from the previous parity's w
+// in FP64 (wc64 + slot nc; slot 0 is the exact restriction of the PCG
+// start); the GEMV then reads it as the direct restriction leaves it.  Used
+// when jfem_coarse_w_recurrence(ctx) (JFEM_W_RECURRENCE=0 turns it off).
+bool jfem_coarse_w_recurrence(const jfm_ctx *ctx);
+hipError_t launch_coarse_gemv_rec(jfm_ctx *ctx, int slot, int *npart_wz);
 // p = z + m o Q zc + beta p, beta from the (r.z, r.r) and w.zc partials
 template <typename T>
 hipError_t launch_pcg_direction_coarse(jfm_ctx *ctx, int slot, int npart_rz, int npart_wz, T *p, const T *z);
@@ -382,7 +390,7 @@ template <typename T>
 hipError_t launch_pcg_init(jfm_ctx *ctx, T *r, T *z, T *p, const T *Minv, int *npart);
 template <typename T>
 hipError_t launch_pcg_update(jfm_ctx *ctx, int slot, int npart_pq, T *x, T *r, T *z, const T *p, const T *q,
-                             const T *Minv, int *npart, T *rA = nullptr);
+                             const T *Minv, int *npart, T *rA = nullptr, bool restrict_q = false);
 template <typename T>
 hipError_t launch_pcg_direction(jfm_ctx *ctx, int slot, int npart_rz, T *p, const T *z);
 hipError_t launch_finalize(jfm_ctx *ctx, const double *part, int npart, int nv, int fin);

@@ -121,21 +121,117 @@ __global__ __launch_bounds__(256) void k_pcg_init(int N, const T *__restrict__ r
     write_partials<2>(v, part);
 }
 
+// The coarse right-hand side by recurrence, one block per aggregate g:
+// w_new,g = w_old,g - alpha u_g with u_g = sum_{a in g} [m_a q_a ; d_a x
+// (m_a q_a)] over agg_nodes[agg_ptr[g] .. agg_ptr[g + 1]) (cdA: the offsets
+// d in that order), alpha as the update blocks form it (rounded to T as they
+// apply it), written in FP32 (wc, the GEMV's operand) and FP64 (w_new).  A
+// thread's nodes are loaded four at a time.
+template <typename T>
+__device__ __forceinline__ void restrict_q(int g, int slot, int npart_pq, const int32_t *__restrict__ agg_ptr,
+                                           const int32_t *__restrict__ agg_nodes, const T *__restrict__ cdA,
+                                           const uint8_t *__restrict__ fixed, const T *__restrict__ q,
+                                           const double *__restrict__ partials, const PcgScalars *S,
+                                           const double *__restrict__ w_old, double *__restrict__ w_new,
+                                           float *__restrict__ wc) {
+    constexpr int kR = 4;  // nodes per thread whose loads are in flight together
+    const int k1 = agg_ptr[g + 1];
+    T y[kR][3], d[kR][3];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < kR; ++u) {
+            const int kk = k0 + u * (int)blockDim.x;
+            if (kk < k1) {
+                const size_t o = 3 * (size_t)agg_nodes[kk];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    y[u][c] = fixed[o + c] ? T(0) : q[o + c];
+                    d[u][c] = cdA[3 * (size_t)kk + c];
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) y[u][c] = d[u][c] = T(0);
+            }
+        }
+    };
+    int k = agg_ptr[g] + threadIdx.x;
+    load(k);
+    if (S->done[slot]) return;  // uniform
+    double w[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (;;) {
+#pragma unroll
+        for (int u = 0; u < kR; ++u) {
+            const double y0 = y[u][0], y1 = y[u][1], y2 = y[u][2];
+            const double d0 = d[u][0], d1 = d[u][1], d2 = d[u][2];
+            w[0] += y0;
+            w[1] += y1;
+            w[2] += y2;
+            w[3] += d1 * y2 - d2 * y1;
+            w[4] += d2 * y0 - d0 * y2;
+            w[5] += d0 * y1 - d1 * y0;
+        }
+        k += kR * (int)blockDim.x;
+        if (k >= k1) break;
+        load(k);
+    }
+    double pq[1];
+    sum_partials<1>(partials + part_pq(slot), npart_pq, pq);
+    const double alpha_d = (pq[0] > 0.0 && finite(pq[0])) ? S->rz[slot] / pq[0] : 0.0;
+    const double alpha = (double)(T)alpha_d;
+    block_sum_all<6>(w);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            const size_t o = 6 * (size_t)g + j;
+            const double wn = w_old[o] - alpha * w[j];
+            w_new[o] = wn;
+            wc[o] = (float)wn;
+        }
+    }
+}
+
 // x += alpha p, r -= alpha q, z = Minv r; partials (r.z, r.r).  The first
 // node's operands are loaded before the alpha reduction so that their
 // latency overlaps it.
 // With rA, the masked residual also goes to rA in aggregate order (node a at
 // position apos[a]) for the coarse restriction, which then reads it
 // contiguously.
+// With w_new, extra blocks form the coarse right-hand side by recurrence
+// (restrict_q) instead.
 template <typename T>
-__global__ __launch_bounds__(256) void k_pcg_update(int N, int slot, int npart_pq, T *__restrict__ x,
+__global__ __launch_bounds__(512) void k_pcg_update(int N, int slot, int npart_pq, T *__restrict__ x,
                                                    T *__restrict__ r, T *__restrict__ z, const T *__restrict__ p,
                                                    const T *__restrict__ q, const T *__restrict__ Minv,
                                                    double *__restrict__ partials, PcgScalars *S,
                                                    T *__restrict__ rA = nullptr, const int32_t *__restrict__ apos = nullptr,
-                                                   const uint8_t *__restrict__ fixed = nullptr) {
-    const int stride = gridDim.x * blockDim.x;
-    int a = blockIdx.x * blockDim.x + threadIdx.x;
+                                                   const uint8_t *__restrict__ fixed = nullptr, int n_agg = 0,
+                                                   const int32_t *__restrict__ agg_ptr = nullptr,
+                                                   const int32_t *__restrict__ agg_nodes = nullptr,
+                                                   const T *__restrict__ cdA = nullptr,
+                                                   const double *__restrict__ w_old = nullptr,
+                                                   double *__restrict__ w_new = nullptr, float *__restrict__ wc = nullptr) {
+    // With w_new, the first xcd_grid(n_agg) blocks (dispatched first, so that
+    // they run beside the update) form the coarse right-hand side by
+    // recurrence, and both parts give each XCD a contiguous eighth of their
+    // aggregates and of the nodes: aggregates are numbered along x as the
+    // nodes usually are, so an aggregate, its neighbours that share cache
+    // lines of q with it and the update of the same nodes meet in one L2.
+    int b = (int)blockIdx.x;
+    int nb = (int)gridDim.x;
+    if (w_new) {
+        const int na = (int)xcd_grid((unsigned)n_agg);
+        if (b < na) {
+            const int g = (b % kXcds) * (na / kXcds) + b / kXcds;
+            if (g < n_agg)
+                restrict_q<T>(g, slot, npart_pq, agg_ptr, agg_nodes, cdA, fixed, q, partials, S, w_old, w_new, wc);
+            return;
+        }
+        b -= na;
+        nb -= na;  // a multiple of kXcds
+        b = (b % kXcds) * (nb / kXcds) + b / kXcds;
+    }
+    const int stride = nb * blockDim.x;
+    int a = b * blockDim.x + threadIdx.x;
     T xv[3], pv[3], rv[3], qv[3], M[9];
     auto load = [&](int aa) {
         const size_t o = 3 * (size_t)aa;
@@ -173,8 +269,8 @@ __global__ __launch_bounds__(256) void k_pcg_update(int N, int slot, int npart_p
         a += stride;
         if (a < N) load(a);
     }
-    write_partials<2>(v, partials + part_rz(slot));
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    write_partials<2>(v, partials + part_rz(slot), b);
+    if (b == 0 && threadIdx.x == 0) {
         S->alpha = alpha_d;
         S->pq = pq[0];
     }
@@ -374,9 +470,22 @@ constexpr int kDirBlocks = JFEM_DIR_BLOCKS;  // grid of the PCG direction kernel
 
 template <typename T>
 hipError_t launch_pcg_update(jfm_ctx *ctx, int slot, int npart_pq, T *x, T *r, T *z, const T *p, const T *q,
-                             const T *Minv, int *npart, T *rA) {
+                             const T *Minv, int *npart, T *rA, bool restrict_q) {
     const unsigned g = std::min<unsigned>(capped_grid(ctx->N, kBlock), kVecBlocks);
     *npart = (int)g;
+    if (restrict_q) {  // xcd_grid(n_agg) more blocks form the coarse right-hand side (wc64 by parity)
+        const T *cdA = sizeof(T) == 8 ? (const T *)ctx->cdA64 : (const T *)ctx->cdA32;
+        const int n_agg = (int)ctx->n_agg;
+        // 512-thread blocks: an aggregate's nodes in one pass of four per
+        // thread; the update part keeps the thread count of the plain update
+        const unsigned gv = xcd_grid((g + 1) / 2);
+        *npart = (int)gv;
+        k_pcg_update<T><<<gv + xcd_grid((unsigned)n_agg), 2 * kBlock, 0, ctx->stream>>>(
+            (int)ctx->N, slot, npart_pq, x, r, z, p, q, Minv, ctx->partials, ctx->S, nullptr, nullptr, ctx->fixed,
+            n_agg, ctx->agg_ptr, ctx->agg_nodes, cdA, ctx->wc64 + (size_t)slot * ctx->nc,
+            ctx->wc64 + (size_t)(slot ^ 1) * ctx->nc, ctx->wc32);
+        return hipGetLastError();
+    }
     k_pcg_update<T><<<g, kBlock, 0, ctx->stream>>>((int)ctx->N, slot, npart_pq, x, r, z, p, q, Minv,
                                                   ctx->partials, ctx->S, rA, rA ? ctx->apos : nullptr, ctx->fixed);
     return hipGetLastError();
@@ -447,8 +556,8 @@ template hipError_t launch_bsr_spmv<double>(jfm_ctx *, const double *, const dou
 template hipError_t launch_bsr_spmv<float>(jfm_ctx *, const float *, const float *, float *, const float *, double *, const int *, int *);
 template hipError_t launch_pcg_init<double>(jfm_ctx *, double *, double *, double *, const double *, int *);
 template hipError_t launch_pcg_init<float>(jfm_ctx *, float *, float *, float *, const float *, int *);
-template hipError_t launch_pcg_update<double>(jfm_ctx *, int, int, double *, double *, double *, const double *, const double *, const double *, int *, double *);
-template hipError_t launch_pcg_update<float>(jfm_ctx *, int, int, float *, float *, float *, const float *, const float *, const float *, int *, float *);
+template hipError_t launch_pcg_update<double>(jfm_ctx *, int, int, double *, double *, double *, const double *, const double *, const double *, int *, double *, bool);
+template hipError_t launch_pcg_update<float>(jfm_ctx *, int, int, float *, float *, float *, const float *, const float *, const float *, int *, float *, bool);
 template hipError_t launch_pcg_direction<double>(jfm_ctx *, int, int, double *, const double *);
 template hipError_t launch_pcg_direction<float>(jfm_ctx *, int, int, float *, const float *);
 template hipError_t launch_convert<double, float>(jfm_ctx *, const double *, float *, int64_t, double);

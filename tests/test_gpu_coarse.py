@@ -7,6 +7,8 @@ and the coarse matrix (a wrong A_c still converges, but not in the same
 number of iterations); bitwise determinism; mixed precision and learned
 blocks with the coarse level; the Newton path with it.
 """
+import os
+import subprocess
 import sys
 
 import numpy as np
@@ -15,6 +17,7 @@ import pytest
 from jfem import mesh as M
 
 pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def say(msg):
@@ -120,3 +123,50 @@ def test_two_level_config1():
         assert np.linalg.norm(u - u0) / np.linalg.norm(u0) <= 1e-6
         assert st["iterations"] * 5 < s0["iterations"]
 
+
+_CHILD = r"""
+import sys
+import numpy as np
+sys.path[:0] = [{root!r}, {pkg!r}]
+import jfem
+from jfem import mesh as M
+m = M.box_tet10(40, 4, 4, seed=1, jitter=0.1, E_sigma=0.5)
+c = jfem.Context(0)
+c.upload(m)
+out = {{}}
+for meth in (0, 1):
+    u, _, st = c.solve(method=meth, op="matfree", op_outer="matfree", rtol=1e-10, inner_rtol=1e-3, coarse_nodes=32)
+    out["u%d" % meth] = u
+    out["it%d" % meth] = st["iterations"]
+np.savez({out!r}, **out)
+"""
+
+
+def test_restriction_recurrence_matches_direct_restriction(cfg0, tmp_path):
+    """Per iteration the coarse right-hand side follows w <- w - alpha Q' m q
+    (q = K p, restricted by extra blocks of the update kernel), and is formed
+    exactly from r at every PCG start.  Against the direct restriction of r
+    at every iteration (JFEM_W_RECURRENCE=0, a child process): the same
+    answer, iteration counts within rounding (3% + 2), in FP64 PCG and in the
+    FP32 inner solves."""
+    m, ctx, csr, f, ud = cfg0
+    res = {}
+    for meth in (0, 1):
+        say(f"restriction by recurrence, method={meth}")
+        u, _, st = ctx.solve(method=meth, op="matfree", op_outer="matfree", rtol=1e-10, inner_rtol=1e-3,
+                             coarse_nodes=32)
+        assert st["converged"] == 1
+        res[meth] = (u, st["iterations"])
+    out = str(tmp_path / "direct.npz")
+    code = _CHILD.format(root=ROOT, pkg=os.path.join(ROOT, "juliafem.jl_amd"), out=out)
+    say("direct restriction in a child process (JFEM_W_RECURRENCE=0)")
+    env = dict(os.environ, JFEM_W_RECURRENCE="0")
+    subprocess.run([sys.executable, "-c", code], env=env, check=True, timeout=120)
+    r = np.load(out)
+    for meth in (0, 1):
+        u, it = res[meth]
+        it_d = int(r["it%d" % meth])
+        say(f"  method={meth}: iterations recurrence={it} direct={it_d}")
+        assert abs(it - it_d) <= 2 + 0.03 * it_d
+        assert np.linalg.norm(u - r["u%d" % meth]) <= 1e-8 * np.linalg.norm(u)
+        assert np.linalg.norm(u - ud) <= 1e-8 * np.linalg.norm(ud)
