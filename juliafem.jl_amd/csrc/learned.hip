@@ -265,13 +265,29 @@ __global__ __launch_bounds__(256) void k_mlp(int N, const float *__restrict__ fe
             gW3[0][i] = f32x4{0.f, 0.f, 0.f, 0.f};
         }
     }
-    for (int base = blockIdx.x * kWaves * kTile; base < N; base += gridDim.x * kWaves * kTile) {
-        const int n0 = base + wv * kTile;
-        for (int i = l; i < kTile * kXPad; i += 64) {
-            const int m = i / kXPad, c = i % kXPad, node = n0 + m;
-            const float v = (c < kFeat && node < N) ? (feat[(size_t)node * kFeat + c] - S.shift[c]) * S.scale[c] : 0.f;
-            W.X[m][c] = f2bf(v);
+    // A lane reads feature column fc of nodes (l + 64 u) / kXPad of the tile,
+    // all kFL loads at once, and the next tile's while this one is computed.
+    constexpr int kFL = kTile * kXPad / 64;
+    const int fc = l % kXPad;
+    float fv[kFL];
+    auto fetch = [&](int tile_base) {
+        const int t0 = tile_base + wv * kTile;
+#pragma unroll
+        for (int u = 0; u < kFL; ++u) {
+            const int node = t0 + (l + 64 * u) / kXPad;
+            fv[u] = (fc < kFeat && node < N) ? feat[(size_t)node * kFeat + fc] : 0.f;
         }
+    };
+    const int stride = gridDim.x * kWaves * kTile;
+    if (blockIdx.x * kWaves * kTile < N) fetch(blockIdx.x * kWaves * kTile);
+    for (int base = blockIdx.x * kWaves * kTile; base < N; base += stride) {
+        const int n0 = base + wv * kTile;
+#pragma unroll
+        for (int u = 0; u < kFL; ++u) {
+            const int m = (l + 64 * u) / kXPad, node = n0 + m;
+            W.X[m][fc] = f2bf((fc < kFeat && node < N) ? (fv[u] - S.shift[fc]) * S.scale[fc] : 0.f);
+        }
+        if (base + stride < N) fetch(base + stride);
         __syncthreads();
         {
             f32x4 acc[4];
