@@ -258,7 +258,7 @@ struct jfm_ctx {
     float *wc32 = nullptr, *zc32 = nullptr;  // wc32 has ldc entries, zero past nc
     double *wc64 = nullptr;        // 2 x nc: Q' m r by iteration parity (slot 0 at the PCG start)
     double *gjC = nullptr, *gjR = nullptr, *gjP = nullptr;  // Gauss-Jordan panels
-    // patch operator (patch.hip): elements in Morton order, 512 per patch
+    // patch operator (patch.hip): elements in Morton order, 480 per patch
     bool patch_ready = false;
     int npatch = 0, npn = 0, nbnode = 0, nslot = 0;
     int32_t *eperm = nullptr;                 // patch order -> element id

@@ -2,8 +2,8 @@
 //
 // The two-phase operator (fem_kernels.hip) writes every element's 3 x npe
 // terms to HBM and reads them back in the nodal gather.  Here the elements are
-// ordered along a Morton curve of their centroids and cut into patches of 512
-// (one workgroup each, one element per thread).  A workgroup computes its
+// ordered along a Morton curve of their centroids and cut into patches of
+// kPatch = 480 (one workgroup each, one element per thread).  A workgroup computes its
 // elements' terms with the same element product (mf_element.h) into LDS and
 // then sums them per patch node, in ascending local element order, from a
 // patch-local incidence list:
@@ -29,7 +29,15 @@
 
 namespace jfem {
 
-constexpr int kPatch = 512;  // elements per patch = threads per workgroup
+// Elements per patch = threads per workgroup.  The LDS image (57.6 KB FP32,
+// 115 KB FP64 for Tet10) allows two workgroups per CU, so the patch count is
+// best a multiple of 2 x 256: 480 gives exactly 512 patches for 245,760
+// elements and 4,096 for 1,966,080 (512 elements: 480 patches, 32 CUs with
+// one, and 4% slower).
+#ifndef JFEM_PATCH_ELEMS
+#define JFEM_PATCH_ELEMS 480
+#endif
+constexpr int kPatch = JFEM_PATCH_ELEMS;
 
 // Fused element product and patch-local nodal sums.  LDS holds the patch's
 // element terms as ys[(3 i + c) * kPatch + local element].  The patch's nodes

@@ -1,14 +1,14 @@
 """The patch form of the matrix-free operator (patch.hip), through the C ABI.
 
-The elements are cut into Morton-ordered patches of 512; patch-interior
+The elements are cut into Morton-ordered patches of 480; patch-interior
 nodes are summed inside a workgroup and shared nodes by a second kernel.
 Checked here beyond the parity suite (which runs every matrix-free product
 through it):
   * the structure's invariant: every node is either interior to exactly one
     patch or listed once among the shared nodes (nodes of no element
-    included), and there is one patch per 512 elements;
+    included), and there is one patch per 480 elements;
   * K.u against the oracle's restatement (1e-12 FP64, 2e-6 FP32) on meshes
-    whose element count is not a multiple of 256 and that hold nodes of no
+    whose element count is not a multiple of 480 and that hold nodes of no
     element, with and without the row mask (reactions);
   * the two-phase operator (JFEM_PATCH=0, a separate process): the same
     product up to the order of the nodal sums, the same PCG answer.
@@ -24,10 +24,10 @@ from jfem import mesh as M
 
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-PATCH = 512  # elements per patch (kPatch in patch.hip)
+PATCH = 480  # elements per patch (kPatch in patch.hip)
 
 MESHES = {
-    "tet10_13x3x2": lambda: M.box_tet10(13, 3, 2, seed=7, jitter=0.1, E_sigma=0.4),   # 468 elements
+    "tet10_13x3x2": lambda: M.box_tet10(13, 3, 2, seed=7, jitter=0.1, E_sigma=0.4),   # 468 elements, one patch
     "tet10_17x4x3": lambda: M.box_tet10(17, 4, 3, seed=9, jitter=0.1, E_sigma=0.4),   # 1,224 elements
     "tet10_40x4x4": lambda: M.box_tet10(40, 4, 4),                                   # 3,840, unused lattice nodes
     "hex8_9x5x4": lambda: M.box_hex8(9, 5, 4, seed=3, jitter=0.1, E_sigma=0.3),
