@@ -185,21 +185,20 @@ __device__ __forceinline__ double readlane_f64(double v, int lane) {
     return __hiloint2double(hi, lo);
 }
 
-// P^-1 of the bk x bk block at (k0, k0) (padded to the identity) by scalar
-// Gauss-Jordan in the registers of wave 0: lane j holds column j & 31, and
-// column k -- the multipliers of step k -- is read from lane k.  No LDS or
-// barrier per step.  The result goes to P; every block of the panel kernel
-// computes it for itself.  Returns false (in wave 0) for a pivot that is not
-// positive and finite.
-__device__ __forceinline__ bool gj_pivot_inverse(int n, int k0, int bk, const double *__restrict__ A,
-                                                 double (*P)[kGJ + 1]) {
+// P^-1 of a bk x bk pivot block (padded to the identity), entry (i, j) read
+// by ld(i, j), by scalar Gauss-Jordan in the registers of wave 0: lane j
+// holds column j & 31, and column k -- the multipliers of step k -- is read
+// from lane k.  No LDS or barrier per step.  The result goes to P.  Returns
+// false (in wave 0) for a pivot that is not positive and finite.  Called by
+// the whole block (one barrier at the end).
+template <typename Ld>
+__device__ __forceinline__ bool gj_pivot_inverse(int bk, Ld &&ld, double (*P)[kGJ + 1]) {
     bool ok = true;
     if (threadIdx.x < 64) {
         const int j = threadIdx.x & (kGJ - 1);
         double a[kGJ];
 #pragma unroll
-        for (int i = 0; i < kGJ; ++i)
-            a[i] = (i < bk && j < bk) ? A[(size_t)(k0 + i) * n + k0 + j] : (i == j ? 1.0 : 0.0);
+        for (int i = 0; i < kGJ; ++i) a[i] = (i < bk && j < bk) ? ld(i, j) : (i == j ? 1.0 : 0.0);
 #pragma unroll
         for (int k = 0; k < kGJ; ++k) {
             if (k < bk) {
@@ -227,11 +226,27 @@ __device__ __forceinline__ bool gj_pivot_inverse(int n, int k0, int bk, const do
 
 // Cbuf (n x kGJ) = A_:K P^-1, Rbuf (kGJ x n) = A_K:, Pbuf = P^-1; 8 rows of
 // Cbuf per block of 256 threads.
+// With Ppre, P^-1 was formed by the previous step's update kernel.  With
+// nbk > 0, block 0 also copies the next pivot block (rows and columns nk0 ..
+// nk0 + nbk) to Anext for the update kernel, which overwrites A.
 __global__ __launch_bounds__(256) void k_gj_panel(int n, int k0, int bk, const double *__restrict__ A,
                                                  double *__restrict__ Cbuf, double *__restrict__ Rbuf,
-                                                 double *__restrict__ Pbuf, int *__restrict__ flag) {
+                                                 double *__restrict__ Pbuf, const double *__restrict__ Ppre,
+                                                 int nk0, int nbk, double *__restrict__ Anext,
+                                                 int *__restrict__ flag) {
+    if (nbk > 0 && blockIdx.x == 0)
+        for (int t = threadIdx.x; t < nbk * kGJ; t += blockDim.x) {
+            const int i = t / kGJ, j = t - kGJ * (t / kGJ);
+            if (j < nbk) Anext[t] = A[(size_t)(nk0 + i) * n + nk0 + j];
+        }
     __shared__ double P[kGJ][kGJ + 1];
-    const bool ok = gj_pivot_inverse(n, k0, bk, A, P);  // valid in wave 0
+    bool ok = true;  // valid in wave 0
+    if (Ppre) {
+        for (int t = threadIdx.x; t < kGJ * kGJ; t += blockDim.x) P[t / kGJ][t - kGJ * (t / kGJ)] = Ppre[t];
+        __syncthreads();
+    } else {
+        ok = gj_pivot_inverse(bk, [&](int i, int j) { return A[(size_t)(k0 + i) * n + k0 + j]; }, P);
+    }
     if (blockIdx.x == 0) {
         for (int t = threadIdx.x; t < kGJ * kGJ; t += blockDim.x) Pbuf[t] = P[t / kGJ][t - kGJ * (t / kGJ)];
         if (threadIdx.x == 0 && !ok) flag[0] = 1;
@@ -249,13 +264,69 @@ __global__ __launch_bounds__(256) void k_gj_panel(int n, int k0, int bk, const d
     }
 }
 
-// the update of step K over 64 x 64 tiles, 4 x 4 entries per thread
+// The next step's pivot block after this step's update (rows and columns
+// nk0 .. nk0 + nbk), with the arithmetic of k_gj_update's tiles (so the same
+// values), and its inverse for the next panel (Pnext).  Run by block 0 of
+// the update, beside the tiles.
+__device__ __forceinline__ void gj_next_pivot(int n, int bk, const double *__restrict__ Anext,
+                                              const double *__restrict__ Cbuf, const double *__restrict__ Rbuf, int nk0,
+                                              int nbk, double *__restrict__ Pnext, int *__restrict__ flag,
+                                              double (*Cs)[kGJ + 1], double (*Rs)[65], double (*Ps)[kGJ + 1]) {
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    // the block before this step's update, as the panel kernel copied it (the
+    // tiles of this kernel overwrite A meanwhile)
+    double av[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int i = ty + 16 * a, j = tx + 16 * b;
+            av[a][b] = (i < nbk && j < nbk) ? Anext[i * kGJ + j] : 0.0;
+        }
+    for (int t = threadIdx.x; t < kGJ * kGJ; t += blockDim.x) {
+        const int a = t / kGJ, m = t - kGJ * (t / kGJ);
+        Cs[a][m] = (a < nbk && m < bk) ? Cbuf[(size_t)(nk0 + a) * kGJ + m] : 0.0;
+        Rs[a][m] = (a < bk && m < nbk) ? Rbuf[(size_t)a * n + nk0 + m] : 0.0;
+    }
+    __syncthreads();
+    double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+    for (int m = 0; m < bk; ++m) {
+        double cv[2], rv[2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) cv[a] = Cs[ty + 16 * a][m];
+#pragma unroll
+        for (int b = 0; b < 2; ++b) rv[b] = Rs[m][tx + 16 * b];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) acc[a][b] += cv[a] * rv[b];
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) Ps[ty + 16 * a][tx + 16 * b] = av[a][b] - acc[a][b];
+    __syncthreads();
+    const bool ok = gj_pivot_inverse(nbk, [&](int i, int j) { return Ps[i][j]; }, Cs);
+    for (int t = threadIdx.x; t < kGJ * kGJ; t += blockDim.x) Pnext[t] = Cs[t / kGJ][t - kGJ * (t / kGJ)];
+    if (threadIdx.x == 0 && !ok) flag[0] = 1;
+}
+
+// The update of step K over 64 x 64 tiles, 4 x 4 entries per thread.  With
+// nbk > 0, block 0 forms the next pivot block's inverse instead
+// (gj_next_pivot), beside the tiles and off the critical path of the step.
 __global__ __launch_bounds__(256) void k_gj_update(int n, int k0, int bk, double *__restrict__ A,
                                                   const double *__restrict__ Cbuf, const double *__restrict__ Rbuf,
-                                                  const double *__restrict__ Pbuf) {
+                                                  const double *__restrict__ Pbuf, int nk0, int nbk,
+                                                  const double *__restrict__ Anext, double *__restrict__ Pnext,
+                                                  int *__restrict__ flag) {
     __shared__ double Cs[64][kGJ + 1], Rs[kGJ][65], Ps[kGJ][kGJ + 1];
+    if (nbk > 0 && blockIdx.x == 0) {
+        gj_next_pivot(n, bk, Anext, Cbuf, Rbuf, nk0, nbk, Pnext, flag, Cs, Rs, Ps);
+        return;
+    }
     const int tiles = (n + 63) / 64;
-    const int ti = blockIdx.x / tiles, tj = blockIdx.x - tiles * (blockIdx.x / tiles);
+    const int bid = (int)blockIdx.x - (nbk > 0 ? 1 : 0);
+    const int ti = bid / tiles, tj = bid - tiles * (bid / tiles);
     const int i0 = 64 * ti, j0 = 64 * tj;
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
     // this thread's entries of A, in flight while the panels are staged
@@ -806,10 +877,16 @@ static hipError_t setup_sequence(jfm_ctx *ctx, bool tangent) {
     const int64_t nn = (int64_t)nc * nc;
     k_coarse_fix<<<nblk(nn), kBlock, 0, s>>>(nc, ctx->Ac);
     const int tiles = (nc + 63) / 64;
+    // gjP: this step's P^-1, the next step's as the update forms it, and the
+    // next pivot block before the update
+    double *Pnext = ctx->gjP + kGJ * kGJ, *Anext = ctx->gjP + 2 * kGJ * kGJ;
     for (int k0 = 0; k0 < nc; k0 += kGJ) {
         const int bk = std::min(kGJ, nc - k0);
-        k_gj_panel<<<(nc + 7) / 8, 256, 0, s>>>(nc, k0, bk, ctx->Ac, ctx->gjC, ctx->gjR, ctx->gjP, flag);
-        k_gj_update<<<tiles * tiles, 256, 0, s>>>(nc, k0, bk, ctx->Ac, ctx->gjC, ctx->gjR, ctx->gjP);
+        const int nk0 = k0 + kGJ, nbk = nk0 < nc ? std::min(kGJ, nc - nk0) : 0;
+        k_gj_panel<<<(nc + 7) / 8, 256, 0, s>>>(nc, k0, bk, ctx->Ac, ctx->gjC, ctx->gjR, ctx->gjP,
+                                                k0 > 0 ? Pnext : nullptr, nk0, nbk, Anext, flag);
+        k_gj_update<<<tiles * tiles + (nbk > 0 ? 1 : 0), 256, 0, s>>>(nc, k0, bk, ctx->Ac, ctx->gjC, ctx->gjR,
+                                                                      ctx->gjP, nk0, nbk, Anext, Pnext, flag);
     }
     k_coarse_fold<<<nblk(nc * ctx->ldc), kBlock, 0, s>>>(nc, (int)ctx->ldc, ctx->Ac, ctx->Rinv, ctx->dropped, ctx->Ahat32);
     return hipGetLastError();
@@ -864,7 +941,7 @@ int jfem_coarse_setup(jfm_ctx *ctx, int op, int coarse_nodes, double *ms) {
         CALLOC(ctx->Ac, double, nc * nc);
         CALLOC(ctx->gjC, double, nc * kGJ);
         CALLOC(ctx->gjR, double, nc * kGJ);
-        CALLOC(ctx->gjP, double, kGJ * kGJ);
+        CALLOC(ctx->gjP, double, 3 * kGJ * kGJ);
         const int64_t nslots = 3 * (int64_t)ctx->npe * ctx->n_el;
         CALLOC(ctx->pv6, double, 6 * 3 * N);
         CALLOC(ctx->qv6, double, 6 * 3 * N);
