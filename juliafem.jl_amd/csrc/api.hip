@@ -731,8 +731,10 @@ static int write_scalars(jfm_ctx *ctx) {
     return sync(ctx);
 }
 
-// One PCG iteration of parity `slot` (three or four kernels, see
-// linalg_kernels.hip); every kernel returns at once when done[slot] is set.
+// One PCG iteration of parity `slot`: the operator, then the update and the
+// direction (linalg_kernels.hip), with the coarse GEMV between them for the
+// two-level preconditioner (coarse.hip); every kernel returns at once when
+// done[slot] is set.
 template <typename T>
 static hipError_t pcg_iteration(jfm_ctx *ctx, int op, int slot, const T *Minv, bool coarse) {
     Workspace &w = ws_of<T>(ctx);
