@@ -726,9 +726,12 @@ static int read_scalars(jfm_ctx *ctx) {
     return sync(ctx);
 }
 
+// S_host is pinned and stream-ordered with the kernels that read S; the
+// caller does not write S_host again before its next read_scalars, so no
+// wait is needed here.
 static int write_scalars(jfm_ctx *ctx) {
     JFM_HIP(ctx, hipMemcpyAsync(ctx->S, ctx->S_host, sizeof(PcgScalars), hipMemcpyHostToDevice, ctx->stream));
-    return sync(ctx);
+    return JFM_OK;
 }
 
 // One PCG iteration of parity `slot`: the operator, then the update and the
@@ -849,17 +852,20 @@ static int pcg_run(jfm_ctx *ctx, int op, const T *Minv, double rtol, int maxit, 
     CHECK_LAUNCH(ctx, launch_pcg_init<T>(ctx, r, z, p, Minv, &np));
     if (coarse) CHECK_LAUNCH(ctx, launch_coarse_correct<T>(ctx, r, z, p, ctx->partials + kPartMisc, nullptr, &np));
     if (int rc = finalize_misc(ctx, np, 2, FIN_INIT)) return rc;
-    if (int rc = read_scalars(ctx)) return rc;
     // Batches of up to check_every iterations; once the residual falls, the
     // next batch is the power of two that covers the iterations its rate
-    // predicts are left, so that few launches run past convergence.
+    // predicts are left, so that few launches run past convergence.  The
+    // first batch follows the start without a host read (its kernels return
+    // at once if the start already converged); the start's r.r comes back
+    // with the first batch's scalars (rr0).
     const int bmax = std::max(2, (check_every + 1) & ~1);
     int batch = bmax;
-    double rr_prev = ctx->S_host->rr;
-    int it_prev = ctx->S_host->iter;
-    while (ctx->S_host->done[0] == DONE_RUNNING) {
+    double rr_prev = -1.0;
+    int it_prev = 0;
+    for (bool first = true; first || ctx->S_host->done[0] == DONE_RUNNING; first = false) {
         if (int rc = pcg_batch<T>(ctx, op, Minv, batch, coarse)) return rc;
         if (int rc = read_scalars(ctx)) return rc;
+        if (first) rr_prev = ctx->S_host->rr0;
         const double rr = ctx->S_host->rr, tol2 = ctx->S_host->tol2;
         const int it = ctx->S_host->iter;
         batch = bmax;

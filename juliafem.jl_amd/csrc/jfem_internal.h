@@ -121,6 +121,7 @@ struct PcgScalars {
     double tol2;        // rtol^2 * bb
     double rtol2;       // set by the host
     double tmp0, tmp1;  // results of one-off reductions
+    double rr0;         // r.r at the PCG start
     int done[2];        // 0 running, DONE_* otherwise, by parity
     int iter, maxit;
 };

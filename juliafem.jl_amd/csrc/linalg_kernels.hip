@@ -381,6 +381,7 @@ __global__ __launch_bounds__(256) void k_finalize(const double *__restrict__ par
     case FIN_INIT: {
         S->rz[0] = t[0];
         S->rr = t[1];
+        S->rr0 = t[1];
         S->iter = 0;
         const int d = (t[1] <= S->tol2) ? DONE_CONVERGED : DONE_RUNNING;
         S->done[0] = d;
