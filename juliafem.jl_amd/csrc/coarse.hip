@@ -338,13 +338,21 @@ __global__ __launch_bounds__(256) void k_gj_update(int n, int k0, int bk, double
             const int i = i0 + ty + 16 * a, j = j0 + tx + 16 * b;
             av[a][b] = (i < n && j < n) ? A[(size_t)i * n + j] : 0.0;
         }
-    for (int t = threadIdx.x; t < 64 * kGJ; t += blockDim.x) {
+    // the panels: a fixed count per thread (256 threads), so that every load
+    // is issued before the first LDS store waits on one
+#pragma unroll
+    for (int u = 0; u < 64 * kGJ / 256; ++u) {
+        const int t = (int)threadIdx.x + 256 * u;
         const int a = t / kGJ, m = t - kGJ * (t / kGJ);
         Cs[a][m] = (i0 + a < n && m < bk) ? Cbuf[(size_t)(i0 + a) * kGJ + m] : 0.0;
         const int mm = t / 64, b = t - 64 * (t / 64);
         Rs[mm][b] = (mm < bk && j0 + b < n) ? Rbuf[(size_t)mm * n + j0 + b] : 0.0;
     }
-    for (int t = threadIdx.x; t < kGJ * kGJ; t += blockDim.x) Ps[t / kGJ][t - kGJ * (t / kGJ)] = Pbuf[t];
+#pragma unroll
+    for (int u = 0; u < kGJ * kGJ / 256; ++u) {
+        const int t = (int)threadIdx.x + 256 * u;
+        Ps[t / kGJ][t - kGJ * (t / kGJ)] = Pbuf[t];
+    }
     __syncthreads();
     double acc[4][4];
 #pragma unroll
