@@ -74,7 +74,7 @@ def kernel_bytes(inf, op, prec):
     return {"element": elem, "gather": gather}
 
 
-def survey_ku(inf, prec):
+def ideal_ku(inf, prec):
     """Algorithmic work of one matrix-free K.u by the per-unit figures of the
     hot-path measurement plan: bytes = 16n + 40 n_el + 24N + 8 n_el (FP64;
     coordinates, connectivity, E, x in, y out; no scratch) and 1.6 kflop per
@@ -223,7 +223,7 @@ def main():
     else:
         dom, dom_ms, dom_bytes = f"k_gather<{tname}>", hk["gather_ms"], hk["bytes"]["gather"]
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
-    sv = survey_ku(inf, 1 if method == 1 else 0)
+    sv = ideal_ku(inf, 1 if method == 1 else 0)
     ku_ms = hk["ms"]
     peak_tf = FP32_PEAK_TFLOPS if method == 1 else FP64_PEAK_TFLOPS
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -310,7 +310,7 @@ def nonlinear(args):
         inf2 = ctx.info()
         b = kernel_bytes(inf2, "matfree", prec)
         (n1, b1), (n2, b2) = list(b.items())   # patch/combine, or element/gather
-        sv = survey_ku(inf2, prec)
+        sv = ideal_ku(inf2, prec)
         ku_large["fp64" if prec == 0 else "fp32"] = {
             "ms": tot, f"{n1}_ms": k1, f"{n2}_ms": k2,
             f"{n1}_GBps": b1 / (k1 * 1e-3) / 1e9, f"{n1}_frac": b1 / (k1 * 1e-3) / 1e9 / HBM_PEAK_GBS,
