@@ -327,16 +327,30 @@ __global__ __launch_bounds__(256) void k_pcg_dir_coarse(int N, int slot, int npa
                                                        const T *__restrict__ cd, const uint8_t *__restrict__ fixed,
                                                        const float *__restrict__ zc, const double *__restrict__ partials,
                                                        PcgScalars *S) {
-    // the first node's operands in flight before the flag test and the reductions
-    const int a0 = blockIdx.x * blockDim.x + threadIdx.x;
-    T z0[3] = {T(0), T(0), T(0)}, p0[3] = {T(0), T(0), T(0)}, d0[3] = {T(0), T(0), T(0)};
-    int g0 = 0;
-    if (a0 < N) {
-        const size_t o = 3 * (size_t)a0;
+    // the first kDN nodes' operands in flight before the flag test and the
+    // reductions
+#ifndef JFEM_DIR_NODES
+#define JFEM_DIR_NODES 2
+#endif
+    constexpr int kDN = JFEM_DIR_NODES;
+    const int a0 = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+    T zv[kDN][3], pv[kDN][3], dv[kDN][3];
+    bool fx[kDN][3];
+    int gv[kDN];
+    auto load = [&](int u, int a) {
+        const size_t o = 3 * (size_t)a;
 #pragma unroll
-        for (int k = 0; k < 3; ++k) { z0[k] = z[o + k]; p0[k] = p[o + k]; d0[k] = cd[o + k]; }
-        g0 = node_agg[a0];
-    }
+        for (int k = 0; k < 3; ++k) {
+            zv[u][k] = z[o + k];
+            pv[u][k] = p[o + k];
+            dv[u][k] = cd[o + k];
+            fx[u][k] = fixed[o + k] != 0;
+        }
+        gv[u] = node_agg[a];
+    };
+#pragma unroll
+    for (int u = 0; u < kDN; ++u)
+        if (a0 + u * stride < N) load(u, a0 + u * stride);
     const int d = S->done[slot];
     if (d) {
         if (blockIdx.x == 0 && threadIdx.x == 0) S->done[slot ^ 1] = d;
@@ -357,19 +371,22 @@ __global__ __launch_bounds__(256) void k_pcg_dir_coarse(int N, int slot, int npa
     const double rz = t[0] + t[2];
     const double beta_d = rz / S->rz[slot];
     const T beta = (T)beta_d;
-    for (int a = a0; a < N; a += gridDim.x * blockDim.x) {
+    auto update = [&](int u, int a) {
         const size_t o = 3 * (size_t)a;
-        if (a != a0) {
-#pragma unroll
-            for (int k = 0; k < 3; ++k) { z0[k] = z[o + k]; p0[k] = p[o + k]; d0[k] = cd[o + k]; }
-            g0 = node_agg[a];
-        }
-        const float *yf = zc + 6 * (size_t)g0;
+        const float *yf = zc + 6 * (size_t)gv[u];
         const T y[6] = {(T)yf[0], (T)yf[1], (T)yf[2], (T)yf[3], (T)yf[4], (T)yf[5]};
+        const T *d0 = dv[u];
         const T c[3] = {y[0] + y[4] * d0[2] - y[5] * d0[1], y[1] + y[5] * d0[0] - y[3] * d0[2],
                         y[2] + y[3] * d0[1] - y[4] * d0[0]};
 #pragma unroll
-        for (int k = 0; k < 3; ++k) p[o + k] = (z0[k] + (fixed[o + k] ? T(0) : c[k])) + beta * p0[k];
+        for (int k = 0; k < 3; ++k) p[o + k] = (zv[u][k] + (fx[u][k] ? T(0) : c[k])) + beta * pv[u][k];
+    };
+#pragma unroll
+    for (int u = 0; u < kDN; ++u)
+        if (a0 + u * stride < N) update(u, a0 + u * stride);
+    for (int a = a0 + kDN * stride; a < N; a += stride) {  // grids too small for kDN nodes per thread
+        load(0, a);
+        update(0, a);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) pcg_record(S, slot, rz, t[1], beta_d);
 }
