@@ -298,7 +298,7 @@ __global__ __launch_bounds__(256) void k_mlp(int N, const float *__restrict__ fe
                 for (int j = 0; j < 4; ++j) {
                     const int n = t * 16 + r;
                     const float z = acc[t][j] + S.b1[n];
-                    W.Z1[q + j][n] = z;
+                    if (TRAIN) W.Z1[q + j][n] = z;  // pre-activations for the backward pass
                     W.H1[q + j][n] = f2bf(gelu(z));
                 }
         }
@@ -312,7 +312,7 @@ __global__ __launch_bounds__(256) void k_mlp(int N, const float *__restrict__ fe
                 for (int j = 0; j < 4; ++j) {
                     const int n = t * 16 + r;
                     const float z = acc[t][j] + S.b2[n];
-                    W.Z2[q + j][n] = z;
+                    if (TRAIN) W.Z2[q + j][n] = z;
                     W.H2[q + j][n] = f2bf(gelu(z));
                 }
         }
