@@ -491,12 +491,12 @@ hipError_t launch_pcg_init(jfm_ctx *ctx, T *r, T *z, T *p, const T *Minv, int *n
 }
 
 #ifndef JFEM_VEC_BLOCKS
-#define JFEM_VEC_BLOCKS 512
+#define JFEM_VEC_BLOCKS 384
 #endif
 #ifndef JFEM_DIR_BLOCKS
 #define JFEM_DIR_BLOCKS 1024
 #endif
-constexpr int kVecBlocks = JFEM_VEC_BLOCKS;  // grid of the PCG update kernel
+constexpr int kVecBlocks = JFEM_VEC_BLOCKS;  // grid of the PCG update kernel (384: measured 0.4-1.3% faster solves than 512)
 constexpr int kDirBlocks = JFEM_DIR_BLOCKS;  // grid of the PCG direction kernels
 
 template <typename T>
