@@ -26,6 +26,20 @@ FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 
 
+# what each method runs, in plain words (the bench line's config.method_label)
+LABELS = {
+    "mixed-learned-2l": "FP32 two-level PCG in FP64 refinement; learned blocks probed for 16 FP32 iterations "
+                        "against block Jacobi, kept only if they win (learned_fallback = 1: rejected, block "
+                        "Jacobi used)",
+    "mixed-bj-2l": "FP32 two-level block-Jacobi PCG in FP64 refinement",
+    "fp64-bj-2l": "FP64 two-level block-Jacobi PCG",
+    "mixed-learned": "FP64 PCG to 1e-3, then FP32 learned-block PCG in FP64 refinement (probed)",
+    "mixed-bj": "FP64 PCG to 1e-3, then FP32 block-Jacobi PCG in FP64 refinement",
+    "mixed-bj-plain": "FP32 block-Jacobi PCG in FP64 refinement",
+    "fp64-bj": "FP64 block-Jacobi PCG",
+}
+REFERENCE_FIXTURE = os.path.join(ROOT, "tests", "golden", "headline_160x16x16_seed1.npz")
+
 # headline candidates: (jfm_solve method, preconditioner, extra options)
 METHODS = {
     "mixed-learned-2l": (1, 1, dict(learned_probe=16, inner_rtol=2e-3, coarse_nodes=2048)),
@@ -67,8 +81,8 @@ def kernel_bytes(inf, op, prec):
     # geometric factors (11 per factor set) or the stored K_e blocks, x once,
     # the element results (3 npe per element) written once
     elem = 8 * npe * n_el + 11 * ngf * n_el * T + 3 * N * T + 3 * npe * n_el * T
-    if op == "ebe":
-        elem = 8 * npe * n_el + 8 * 9 * (npe * (npe + 1) // 2) * n_el + 3 * N * T + 3 * npe * n_el * T
+    if op == "ebe":   # element matrices stored in the operator's precision
+        elem = 8 * npe * n_el + T * 9 * (npe * (npe + 1) // 2) * n_el + 3 * N * T + 3 * npe * n_el * T
     # gather: node pointers, the element results read once, y and the fixed mask
     gather = 4 * (N + 1) + 3 * npe * n_el * T + 3 * N * T + 3 * N
     return {"element": elem, "gather": gather}
@@ -82,6 +96,16 @@ def ideal_ku(inf, prec):
     n, N, n_el = inf["n_dofs"], inf["n_nodes"], inf["n_el"]
     scale = 1.0 if prec == 0 else 0.5
     return {"bytes": int(scale * (16 * n + 24 * N + 8 * n_el) + 40 * n_el), "flops": 1600 * n_el}
+
+
+def ebe_algorithmic(inf, prec):
+    """Per-unit figure of the EBE K.u: symmetric-packed K_e (465 values per
+    Tet10), int32 connectivity, x in and y out: FP64 3,720 n_el + 16n + 40 n_el,
+    FP32 1,860 n_el + 8n + 40 n_el (bytes)."""
+    T = 8 if prec == 0 else 4
+    npe, n_el, n = inf["npe"], inf["n_el"], inf["n_dofs"]
+    packed = (3 * npe) * (3 * npe + 1) // 2
+    return T * packed * n_el + 2 * T * n + 4 * npe * n_el
 
 
 def flops_matfree_elem(inf):
@@ -99,9 +123,14 @@ def main():
     ap.add_argument("--grid", type=int, nargs=3, default=[160, 16, 16])
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-renumbering", action="store_true",
+                    help="skip the K.u timing on shuffled and RCM-renumbered copies of the mesh")
     ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--kernels-only", action="store_true",
                     help="time the K.u kernels and known-size copies only (for counter runs)")
+    ap.add_argument("--mlp-only", action="store_true",
+                    help="run only the node features and the network inference (for MFMA counter runs: "
+                         "no PCG, no graph launches)")
     ap.add_argument("--workload", default="linear", choices=("linear", "nonlinear"),
                     help="linear: the 1.05M-DOF solve (default); nonlinear: the 8.1M-DOF neo-Hookean Newton")
     ap.add_argument("--method", default="mixed-learned-2l", choices=sorted(METHODS),
@@ -128,6 +157,17 @@ def main():
     t_upload = time.perf_counter() - t0
     inf = ctx.info()
     log(f"upload+pattern {t_upload:.2f}s nnz={inf['nnz']} affine={inf['affine']}")
+    if args.mlp_only:
+        from jfem.learned import DEFAULT_WEIGHTS
+        with open(DEFAULT_WEIGHTS, "rb") as fh:
+            ctx.set_learned_weights(fh.read())
+        ts = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            ctx.learned_outputs()       # features, network inference and blocks, copied back
+            ts.append(time.perf_counter() - t0)
+        print(json.dumps({"mlp_only": True, "n_nodes": inf["n_nodes"], "host_s": ts}), flush=True)
+        return
     t0 = time.perf_counter()
     ctx.assemble(want_vals=False, want_f=False)
     t_asm = time.perf_counter() - t0
@@ -137,8 +177,6 @@ def main():
     kernels = {}
     for op in ("csr", "matfree", "ebe"):
         for prec in (0, 1):
-            if op == "ebe" and prec == 1:
-                continue
             tot, k1, k2 = ctx.time_apply(op=op, precision=prec, reps=50)
             inf = ctx.info()  # the operator's structure exists once it has run
             b = kernel_bytes(inf, op, prec)
@@ -164,6 +202,8 @@ def main():
             kernels[name] = ent
             log(f"K.u {name}: {json.dumps({k: (round(v, 4) if isinstance(v, float) else v) for k, v in ent.items()})}")
 
+    renum = None if (args.skip_renumbering or args.kernels_only) else renumbering_ku(jfem, mesh, kernels)
+
     if args.kernels_only:
         copy = copy_calibration(ctx, inf["n_dofs"])
         print(json.dumps({"kernels": kernels, "copy": copy, "n_el": inf["n_el"], "n_dofs": inf["n_dofs"]}), flush=True)
@@ -171,8 +211,8 @@ def main():
 
     # ---- solves ----------------------------------------------------------------
     method, precond, extra = METHODS[args.method]
+    from jfem.learned import DEFAULT_WEIGHTS
     if precond == 1:
-        from jfem.learned import DEFAULT_WEIGHTS
         with open(DEFAULT_WEIGHTS, "rb") as fh:
             ctx.set_learned_weights(fh.read())
 
@@ -183,58 +223,100 @@ def main():
         st["wall_s"] = time.perf_counter() - t
         return u, st
 
-    u64, st64 = solve(0, "matfree")
-    log(f"FP64 BJ-PCG matfree: {st64}")
-    u64c, st64c = solve(0, "csr", "csr")
-    log(f"FP64 BJ-PCG csr: {st64c}")
+    def repeat(n, *a, **kw):
+        """warm once, then n runs; (last u, stats of the median run, all times)"""
+        solve(*a, **kw)
+        runs = [solve(*a, **kw) for _ in range(n)]
+        runs.sort(key=lambda r: r[1]["solve_ms"])
+        u_, st_ = runs[len(runs) // 2]
+        return u_, st_, [r[1]["solve_ms"] / 1e3 for r in runs]
+
+    u64, st64, t64 = repeat(3, 0, "matfree")
+    log(f"FP64 BJ-PCG matfree (median of 3 after a warm run): {st64}")
+    _, st64c, t64c = repeat(3, 0, "csr", "csr")
+    log(f"FP64 BJ-PCG csr (median of 3 after a warm run): {st64c}")
     others = {}
     for name in ("fp64-bj-2l", "mixed-bj-2l"):
         if name == args.method:
             continue
         m_, p_, kw_ = METHODS[name]
-        solve(m_, "matfree", precond=p_, **kw_)          # warm
-        _, st_ = solve(m_, "matfree", precond=p_, **kw_)
-        others[name] = {"s": (st_["solve_ms"]) / 1e3, "setup_s": st_["setup_ms"] / 1e3,
+        _, st_, ts_ = repeat(3, m_, "matfree", precond=p_, **kw_)
+        others[name] = {"s": st_["solve_ms"] / 1e3, "setup_s": st_["setup_ms"] / 1e3, "runs_s": ts_,
                         "iters": st_["iterations"], "relres": st_["relres"]}
         log(f"{name}: {st_}")
 
+    blob = None
+    if precond == 1:
+        with open(DEFAULT_WEIGHTS, "rb") as fh:
+            blob = fh.read()
+
+    def headline():
+        # setting the weights drops the cached learned blocks, so every timed
+        # solve recomputes the node features and the network's blocks
+        if blob is not None:
+            ctx.set_learned_weights(blob)
+        return solve(method, "matfree", precond=precond, **extra)
+
     for _ in range(args.warmup):
-        solve(method, "matfree", precond=precond, **extra)
+        headline()
     times, walls, stats = [], [], None
     u_head = None
     for _ in range(args.steps):
-        u_head, st = solve(method, "matfree", precond=precond, **extra)
+        u_head, st = headline()
         # solve_ms spans the whole call: preconditioner setup (coarse level,
-        # learned blocks and their probe) included
+        # node features, network inference and the probe) included
         times.append(st["solve_ms"] / 1e3)
         walls.append(st["wall_s"])
         stats = st
         log(f"{args.method} step: {st}")
     value = statistics.median(times)
     err_vs_fp64 = float(np.linalg.norm(u_head - u64) / np.linalg.norm(u64))
+    err_oracle = error_vs_reference(mesh, u_head) if (nx, ny, nz, args.seed) == (160, 16, 16, 1) else None
+    log(f"error vs the oracle's FP64 answer: {err_oracle}")
 
     # ---- roofline of the dominant kernel of the headline solve ---------------
     hk = kernels["matfree_fp32" if method == 1 else "matfree_fp64"]
     tname = "float" if method == 1 else "double"
+    prec = 1 if method == 1 else 0
+    sv = ideal_ku(inf, prec)
     if "patch_ms" in hk:
-        dom, dom_ms, dom_bytes = f"k_ku_patch<10,{tname},AFF>", hk["patch_ms"], hk["bytes"]["patch"]
+        dom, dom_ms, impl_bytes = f"k_ku_patch<10,{tname},AFF>", hk["patch_ms"], hk["bytes"]["patch"]
     elif hk["element_ms"] >= hk["gather_ms"]:
-        dom, dom_ms, dom_bytes = f"k_mf_elem<10,{tname},AFF>", hk["element_ms"], hk["bytes"]["element"]
+        dom, dom_ms, impl_bytes = f"k_mf_elem<10,{tname},AFF>", hk["element_ms"], hk["bytes"]["element"]
     else:
-        dom, dom_ms, dom_bytes = f"k_gather<{tname}>", hk["gather_ms"], hk["bytes"]["gather"]
-    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
-    sv = ideal_ku(inf, 1 if method == 1 else 0)
-    ku_ms = hk["ms"]
+        dom, dom_ms, impl_bytes = f"k_gather<{tname}>", hk["gather_ms"], hk["bytes"]["gather"]
+    # algorithmic bytes of the matrix-free K.u this kernel performs (every
+    # element product and the nodal sums of the patch-interior nodes), per the
+    # per-unit figure: coordinates, E, int32 connectivity, x in, y out
+    algo = sv["bytes"]
+    achieved = algo / (dom_ms * 1e-3) / 1e9
     peak_tf = FP32_PEAK_TFLOPS if method == 1 else FP64_PEAK_TFLOPS
+    traffic = measured_traffic(dom, inf)
+    ku_ms = hk["ms"]
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom, inf), "kernel": dom,
-                "bytes_per_launch": dom_bytes, "avg_launch_ms": dom_ms,
-                "ku_ideal": {"bytes": sv["bytes"], "flops": sv["flops"], "ku_ms": ku_ms,
-                             "GBps": sv["bytes"] / (ku_ms * 1e-3) / 1e9,
-                             "TFLOPs": sv["flops"] / (ku_ms * 1e-3) / 1e12,
-                             "flop_frac": sv["flops"] / (ku_ms * 1e-3) / 1e12 / peak_tf}}
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
+                "bytes_per_launch": algo, "avg_launch_ms": dom_ms,
+                "bytes_rule": "8n + 44 n_el + 12N (FP32: x in and y out 4 B per DOF, int32 connectivity 40 B "
+                              "and E 4 B per element, coordinates 12 B per node)" if method == 1 else
+                              "16n + 48 n_el + 24N (FP64)",
+                "traffic_over_algorithmic": None if traffic is None else traffic / algo,
+                "flops_per_launch": sv["flops"],
+                "flop_frac": sv["flops"] / (dom_ms * 1e-3) / 1e12 / peak_tf,
+                "implementation_bytes": impl_bytes,
+                "implementation_GBps": impl_bytes / (dom_ms * 1e-3) / 1e9,
+                "ku_whole": {"ms": ku_ms, "GBps": algo / (ku_ms * 1e-3) / 1e9,
+                             "frac": algo / (ku_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+                "ebe": {}}
+    for key, pr in (("fp64", 0), ("fp32", 1)):
+        ke = kernels.get(f"ebe_{key}")
+        if ke:
+            b = ebe_algorithmic(inf, pr)
+            roofline["ebe"][key] = {"kernel": f"k_ebe_elem<10,{'double' if pr == 0 else 'float'}>",
+                                   "bytes_per_launch": b, "element_ms": ke["element_ms"], "ku_ms": ke["ms"],
+                                   "element_frac": b / (ke["element_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                   "ku_frac": b / (ke["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
 
-    # ---- CPU baseline (oracle, bounded sample) --------------------------------
+    # ---- CPU baseline (oracle, bounded samples) -------------------------------
     cpu = None
     if not args.skip_cpu:
         cpu = cpu_baseline(mesh, st64["iterations"], args.cpu_iters)
@@ -246,21 +328,69 @@ def main():
         "data": "synthetic (seeded jittered box mesh)",
         "config": {"workload": f"Tet10 cantilever {nx}x{ny}x{nz} hexes x6, {inf['n_dofs']} DOFs, "
                                f"log-normal E sigma 0.5, rtol 1e-8", "n_dofs": inf["n_dofs"],
-                   "nnz": inf["nnz"], "method": args.method, "method_options": extra,
+                   "nnz": inf["nnz"], "method": args.method, "method_label": LABELS[args.method],
+                   "method_options": extra, "timed": "whole jfm_solve call from device-resident mesh and load: "
+                   "coarse-level setup, node features, network inference, probe, PCG and FP64 refinement",
                    "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "fp64_pcg_s": st64["solve_ms"] / 1e3, "fp64_pcg_iters": st64["iterations"],
-        "fp64_pcg_csr_s": st64c["solve_ms"] / 1e3, "fp64_pcg_csr_iters": st64c["iterations"],
+        "fp64_pcg_s": st64["solve_ms"] / 1e3, "fp64_pcg_runs_s": t64, "fp64_pcg_iters": st64["iterations"],
+        "fp64_pcg_csr_s": st64c["solve_ms"] / 1e3, "fp64_pcg_csr_runs_s": t64c,
+        "fp64_pcg_csr_iters": st64c["iterations"],
         "speedup_vs_fp64_matfree": st64["solve_ms"] / 1e3 / value,
         "speedup_vs_fp64_csr": st64c["solve_ms"] / 1e3 / value,
+        "speedup_vs_fp64_2l": (others["fp64-bj-2l"]["s"] / value) if "fp64-bj-2l" in others else None,
         "iters": stats["iterations"], "outer": stats["outer"], "relres": stats["relres"],
         "learned_fallback": stats["fallback"], "setup_ms": stats["setup_ms"], "other_methods": others,
-        "err_vs_fp64_l2": err_vs_fp64, "host_wall_s": statistics.median(walls),
+        "err_vs_oracle_l2": err_oracle, "err_vs_fp64_pcg_l2": err_vs_fp64,
+        "host_wall_s": statistics.median(walls),
         "dofs_per_s": inf["n_dofs"] / value, "upload_s": t_upload, "assembly_wall_s": t_asm,
-        "kernels": kernels,
+        "kernels": kernels, "ku_renumbering": renum,
     }
     print(json.dumps(out), flush=True)
+
+
+def renumbering_ku(jfem, mesh, kernels):
+    """Matrix-free K.u time of the same mesh with its node ids shuffled, and
+    after reverse Cuthill-McKee renumbering of the shuffled ids (the library's
+    own element order, which follows the node ids, changes with them)."""
+    import numpy as np
+    from jfem.mesh import node_bandwidth, rcm_order, renumber
+    out = {"natural": {"bandwidth": node_bandwidth(mesh),
+                       "fp32_ms": kernels["matfree_fp32"]["ms"], "fp64_ms": kernels["matfree_fp64"]["ms"]}}
+    sh = renumber(mesh, np.random.default_rng(12).permutation(mesh.n_nodes))
+    t0 = time.perf_counter()
+    perm = rcm_order(sh)
+    t_rcm = time.perf_counter() - t0
+    for name, mm in (("shuffled", sh), ("rcm", renumber(sh, perm))):
+        c = jfem.Context(0)
+        c.upload(mm)
+        ent = {"bandwidth": node_bandwidth(mm)}
+        for prec, key in ((1, "fp32_ms"), (0, "fp64_ms")):
+            ent[key] = c.time_apply(op="matfree", precision=prec, reps=50)[0]
+        c.close()
+        out[name] = ent
+    out["rcm_host_s"] = t_rcm
+    log(f"K.u by numbering: {out}")
+    return out
+
+
+def error_vs_reference(mesh, u):
+    """Relative L2 error of u against the oracle's FP64 answer for this mesh
+    (tests/golden/headline_160x16x16_seed1.npz): the larger of the error over
+    every 16th DOF and the estimate from 16 random +-1 projections."""
+    import numpy as np
+    from jfem.mesh import splitmix_uniform
+    try:
+        z = np.load(REFERENCE_FIXTURE)
+    except OSError:
+        return None
+    s = int(z["idx_stride"])
+    e_sample = np.linalg.norm(u[::s] - z["u_sample"]) / np.linalg.norm(z["u_sample"])
+    proj = np.array([np.where(splitmix_uniform(1, 40 + k, np.arange(u.size)) < 0.5, -1.0, 1.0) @ u
+                     for k in range(z["proj"].size)])
+    e_proj = np.sqrt(np.mean((proj - z["proj"]) ** 2)) / float(z["norm2"])
+    return float(max(e_sample, e_proj))
 
 
 NL_TRACTION = 52.5   # end traction giving a tip deflection of about 10% of the span
@@ -380,28 +510,91 @@ def measured_traffic(kernel, inf):
     return k["bytes_per_launch"]
 
 
+def host_cpu():
+    """(model name, usable cores) of the host running the oracle."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return model, cores
+
+
+def _median_time(fn, reps):
+    ts = []
+    out = None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out = fn()
+        ts.append(time.perf_counter() - t0)
+    return statistics.median(ts), ts, out
+
+
 def cpu_baseline(mesh, iters_needed, sample_iters):
-    """Oracle on the host: full COO->CSR assembly of the same mesh, then
-    `sample_iters` FP64 block-Jacobi PCG iterations; the time-to-solution is
-    assembly + per-iteration time x the iteration count the same method needs."""
+    """The oracle (the reference's algorithm restated in C/OpenMP, and LAPACK
+    for the direct solves) on the host, each figure the median of 3 runs:
+      * config-1 COO->CSR assembly, all cores;
+      * config-1 FP64 block-Jacobi PCG iterations (a sample of `sample_iters`
+        on all cores, 5 on one core), extrapolated to the `iters_needed`
+        iterations the method takes;
+      * the reference's own solver path, a Dirichlet-eliminated direct solve
+        (banded Cholesky, LAPACK), on the 40x4x4 mesh (all cores and one core)
+        and on the 80x8x8 jittered mesh (all cores; one run).
+    `value` is the config-1 time-to-solution on all cores: assembly + PCG."""
     import numpy as np
     from oracle import oracle as O
-    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    cores = min(cores, 16)
-    t0 = time.perf_counter()
-    csr = O.assemble_csr(mesh, nthreads=cores)
-    t_asm = time.perf_counter() - t0
+    from jfem.mesh import box_tet10
+    model, avail = host_cpu()
+    cores = min(avail, 16)
+    try:
+        from threadpoolctl import threadpool_limits
+    except ImportError:
+        threadpool_limits = None
+    t_asm, asm_runs, csr = _median_time(lambda: O.assemble_csr(mesh, nthreads=cores), 3)
     f = O.load_vector(mesh)
-    t0 = time.perf_counter()
-    _, it, rel, _ = O.pcg_bj(csr, mesh.fixed_mask(), f, rtol=1e-30, maxit=sample_iters, nthreads=cores)
-    t_it = (time.perf_counter() - t0) / max(it, 1)
+    fixed = mesh.fixed_mask()
+
+    def pcg(k, nt):
+        return O.pcg_bj(csr, fixed, f, rtol=1e-30, maxit=k, nthreads=nt)
+
+    t_pcg, _, _ = _median_time(lambda: pcg(sample_iters, cores), 3)
+    t_it = t_pcg / sample_iters
+    t_pcg1, _, _ = _median_time(lambda: pcg(5, 1), 3)
+    t_it1 = t_pcg1 / 5
     del csr
+    rows = {"cpu_model": model, "cores_available": avail,
+            "config1_assembly_s": {"cores": cores, "median_s": t_asm, "runs_s": asm_runs},
+            "config1_pcg_iteration_s": {str(cores): t_it, "1": t_it1}}
+    log(f"cpu oracle ({model}, {cores} of {avail} cores): assembly {t_asm:.2f}s, PCG {t_it * 1e3:.1f} ms/iter "
+        f"({cores} cores), {t_it1 * 1e3:.1f} ms/iter (1 core)")
+    # the direct solves
+    for name, m in (("40x4x4", box_tet10(40, 4, 4)), ("80x8x8", box_tet10(80, 8, 8, seed=7, jitter=0.1, E_sigma=0.5))):
+        c = O.assemble_csr(m, nthreads=cores)
+        fm = O.load_vector(m)
+        g = np.zeros(m.n_dofs)
+        reps = 3 if name == "40x4x4" else 1
+        td, runs, _ = _median_time(lambda: O.solve_direct_lapack(c, m.fixed_mask(), g, fm), reps)
+        ent = {"n_dofs": m.n_dofs, "median_s": td, "runs_s": runs, "cores": cores}
+        if name == "40x4x4" and threadpool_limits is not None:
+            with threadpool_limits(limits=1):
+                td1, runs1, _ = _median_time(lambda: O.solve_direct_lapack(c, m.fixed_mask(), g, fm), 3)
+            ent["one_core_median_s"] = td1
+            ent["one_core_runs_s"] = runs1
+        rows[f"direct_solve_{name}"] = ent
+        log(f"cpu oracle direct solve {name}: {td:.2f}s ({ent})")
+        del c
     value = t_asm + t_it * iters_needed
-    log(f"cpu oracle: assembly {t_asm:.1f}s, {t_it * 1e3:.1f} ms/iter x {iters_needed} iters")
     return {"value": value, "unit": "s", "cores": cores, "kind": "port",
-            "sample": f"oracle COO->CSR assembly of the full mesh ({t_asm:.1f} s) + {it} FP64 block-Jacobi "
-                      f"PCG iterations ({t_it * 1e3:.1f} ms each, OpenMP), extrapolated to the "
-                      f"{iters_needed} iterations the method needs"}
+            "sample": f"oracle COO->CSR assembly of the full mesh (median of 3: {t_asm:.1f} s) + FP64 block-Jacobi "
+                      f"PCG ({sample_iters} iterations, median of 3: {t_it * 1e3:.1f} ms each, OpenMP) "
+                      f"extrapolated to the {iters_needed} iterations the method needs; one core: "
+                      f"{t_it1 * 1e3:.1f} ms per iteration; host {model}",
+            "rows": rows}
 
 
 if __name__ == "__main__":

@@ -153,6 +153,17 @@ int jfm_csr_pattern(jfm_ctx *ctx, int64_t *nnz_out, int64_t *rowptr, int64_t *co
  * NULL to keep the results on the device only. */
 int jfm_assemble(jfm_ctx *ctx, const double *u, double *vals, double *f);
 
+/* The same for Elasticity(finite_strain = true) (src/problems_elasticity.jl:
+ * 203-451 with props.finite_strain; St. Venant-Kirchhoff S = D E of the
+ * Green-Lagrange strain, BL(F) :255-289): the material tangent Km(u) into
+ * vals, with geometric != 0 the geometric stiffness Kg(u) (:378-404, the
+ * reference's assembly.Kg) into vals_kg, and f = f_ext - f_int(u), f_int =
+ * sum w BL' S (:406-409, 432-433).  Same pattern as jfm_csr_pattern.  The
+ * context's CSR operator (JFM_OP_CSR) then holds K = Km + Kg (Km when
+ * geometric = 0), the matrix the reference solves, until the next assembly.
+ * Any output may be NULL. */
+int jfm_assemble_tl(jfm_ctx *ctx, int32_t geometric, const double *u, double *vals, double *vals_kg, double *f);
+
 /* ---- operator (compute_residual_gpu!, ext/JuliaFEMCUDAExt.jl:442-478) --
  * y = K x on the free rows, y = 0 on the rows of fixed DOFs; x is used as
  * given on every DOF.  Host pointers, FP64 in and out; precision selects the

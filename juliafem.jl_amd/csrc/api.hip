@@ -164,7 +164,7 @@ static void release_mesh(jfm_ctx *ctx) {
     free_ptr(ctx, ctx->gf64); free_ptr(ctx, ctx->gf32); free_ptr(ctx, ctx->inc_ptr); free_ptr(ctx, ctx->inc);
     free_ptr(ctx, ctx->brow); free_ptr(ctx, ctx->bcol); free_ptr(ctx, ctx->bdiag); free_ptr(ctx, ctx->blk_row);
     free_ptr(ctx, ctx->cptr); free_ptr(ctx, ctx->contrib); free_ptr(ctx, ctx->vals64); free_ptr(ctx, ctx->vals32);
-    free_ptr(ctx, ctx->Ke); free_ptr(ctx, ctx->ye64); free_ptr(ctx, ctx->ye32);
+    free_ptr(ctx, ctx->Ke); free_ptr(ctx, ctx->Ke32); free_ptr(ctx, ctx->ye64); free_ptr(ctx, ctx->ye32);
     free_ptr(ctx, ctx->vx64); free_ptr(ctx, ctx->vy64); free_ptr(ctx, ctx->vx32); free_ptr(ctx, ctx->vy32);
     free_ptr(ctx, ctx->fixed); free_ptr(ctx, ctx->g); free_ptr(ctx, ctx->face_conn); free_ptr(ctx, ctx->face_t);
     free_ptr(ctx, ctx->finc_ptr); free_ptr(ctx, ctx->finc); free_ptr(ctx, ctx->fe); free_ptr(ctx, ctx->f_ext);
@@ -176,7 +176,7 @@ static void release_mesh(jfm_ctx *ctx) {
         free_ptr(ctx, w->q); free_ptr(ctx, w->b); free_ptr(ctx, w->Minv);
         w->ok = false;
     }
-    free_ptr(ctx, ctx->Mlearn64); free_ptr(ctx, ctx->Mlearn32); free_ptr(ctx, ctx->feat); free_ptr(ctx, ctx->lout);
+    free_ptr(ctx, ctx->Mlearn64); free_ptr(ctx, ctx->Mlearn32); free_ptr(ctx, ctx->feat); free_ptr(ctx, ctx->efeat); free_ptr(ctx, ctx->lout);
     free_ptr(ctx, ctx->Muser64); free_ptr(ctx, ctx->Muser32); free_ptr(ctx, ctx->xprobe);
     free_ptr(ctx, ctx->ustate64); free_ptr(ctx, ctx->ustate32); free_ptr(ctx, ctx->Dtan);
     free_ptr(ctx, ctx->nl_u); free_ptr(ctx, ctx->nl_r); free_ptr(ctx, ctx->nl_du); free_ptr(ctx, ctx->nl_zero);
@@ -523,16 +523,18 @@ template <typename T> static T *vals_of(jfm_ctx *ctx);
 template <> double *vals_of<double>(jfm_ctx *ctx) { return ctx->vals64; }
 template <> float *vals_of<float>(jfm_ctx *ctx) { return ctx->vals32; }
 
+// the EBE operator's element matrices (FP64 and FP32, pair-major)
 static int ensure_element_matrices(jfm_ctx *ctx) {
     if (ctx->Ke) return JFM_OK;
     ALLOC(ctx, ctx->Ke, double, ctx->n_el * ctx->npair * 9);
+    ALLOC(ctx, ctx->Ke32, float, ctx->n_el * ctx->npair * 9);
     CHECK_LAUNCH(ctx, launch_element_matrices(ctx));
     return JFM_OK;
 }
 
+// CSR values, each block recomputed from the geometric factors of its elements
 static int ensure_assembled(jfm_ctx *ctx) {
     if (ctx->assembled) return JFM_OK;
-    if (int rc = ensure_element_matrices(ctx)) return rc;
     if (!ctx->vals64) ALLOC(ctx, ctx->vals64, double, 9 * ctx->nnzb);
     if (!ctx->vals32) ALLOC(ctx, ctx->vals32, float, 9 * ctx->nnzb);
     CHECK_LAUNCH(ctx, launch_assemble_values(ctx));
@@ -649,7 +651,6 @@ extern "C" int jfm_assemble(jfm_ctx *ctx, const double *u, double *vals, double 
     JFM_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t s = ctx->stream;
     ctx->assembled = false;
-    free_ptr(ctx, ctx->Ke);
     if (int rc = ensure_assembled(ctx)) return rc;
     const int64_t n = ctx->ndof;
     // f = f_ext - f_int(u)  (src/problems_elasticity.jl:432-433)
@@ -671,6 +672,50 @@ extern "C" int jfm_assemble(jfm_ctx *ctx, const double *u, double *vals, double 
     if (vals) JFM_HIP(ctx, hipMemcpyAsync(vals, ctx->vals64, sizeof(double) * 9 * ctx->nnzb, hipMemcpyDeviceToHost, s));
     if (f) JFM_HIP(ctx, hipMemcpyAsync(f, ctx->load, sizeof(double) * n, hipMemcpyDeviceToHost, s));
     return sync(ctx);
+}
+
+// Finite strain (finite_strain = true, src/problems_elasticity.jl:203-451):
+// Km(u), Kg(u) and f = f_ext - f_int(u).  The device CSR operator then holds
+// K = Km + Kg (Km alone without geometric stiffness), the K the reference
+// solves (src/solvers.jl:286-292).
+extern "C" int jfm_assemble_tl(jfm_ctx *ctx, int32_t geometric, const double *u, double *vals, double *vals_kg,
+                               double *f) {
+    if (!ctx) return JFM_EINVAL;
+    if (!ctx->has_mesh) return jfm_fail(ctx, JFM_ESTATE, "no mesh");
+    if (vals_kg && !geometric) return jfm_fail(ctx, JFM_EINVAL, "vals_kg requested without geometric stiffness");
+    JFM_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    const int64_t n = ctx->ndof;
+    const size_t vbytes = sizeof(double) * 9 * (size_t)ctx->nnzb;
+    double *Ke = nullptr;   // finite-strain element blocks, element-major, freed below
+    ALLOC(ctx, Ke, double, ctx->n_el * ctx->npair * 9);
+    if (!ctx->vals64) ALLOC(ctx, ctx->vals64, double, 9 * ctx->nnzb);
+    if (!ctx->vals32) ALLOC(ctx, ctx->vals32, float, 9 * ctx->nnzb);
+    ctx->assembled = false;
+    if (u) JFM_HIP(ctx, hipMemcpyAsync(ctx->vx64, u, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    else JFM_HIP(ctx, hipMemsetAsync(ctx->vx64, 0, sizeof(double) * n, s));
+    if (geometric && vals_kg) {
+        CHECK_LAUNCH(ctx, launch_tl_element_matrices(ctx, ctx->vx64, 1, Ke, nullptr));
+        CHECK_LAUNCH(ctx, launch_assemble_values_ke(ctx, Ke));
+        JFM_HIP(ctx, hipMemcpyAsync(vals_kg, ctx->vals64, vbytes, hipMemcpyDeviceToHost, s));
+    }
+    // Km and the element forces; f = f_ext - f_int (src/problems_elasticity.jl:432-433)
+    CHECK_LAUNCH(ctx, launch_tl_element_matrices(ctx, ctx->vx64, 0, Ke, ctx->ye64));
+    CHECK_LAUNCH(ctx, launch_assemble_values_ke(ctx, Ke));
+    if (vals) JFM_HIP(ctx, hipMemcpyAsync(vals, ctx->vals64, vbytes, hipMemcpyDeviceToHost, s));
+    CHECK_LAUNCH(ctx, launch_gather<double>(ctx, ctx->ye64, nullptr, ctx->vy64, nullptr, nullptr, nullptr, nullptr));
+    CHECK_LAUNCH(ctx, (launch_convert<double, double>(ctx, ctx->vy64, ctx->vy64, n, -1.0)));
+    CHECK_LAUNCH(ctx, launch_add(ctx, ctx->f_ext, ctx->vy64, ctx->load, n));
+    if (geometric) {
+        CHECK_LAUNCH(ctx, launch_tl_element_matrices(ctx, ctx->vx64, 2, Ke, nullptr));
+        CHECK_LAUNCH(ctx, launch_assemble_values_ke(ctx, Ke));
+    }
+    if (f) JFM_HIP(ctx, hipMemcpyAsync(f, ctx->load, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    if (int rc = sync(ctx)) return rc;
+    free_ptr(ctx, Ke);
+    ctx->assembled = true;
+    ctx->has_load = true;
+    return JFM_OK;
 }
 
 // ================================================================ solvers ==

@@ -2,7 +2,8 @@
 //
 // Reference algorithm (paths relative to the reference repository root):
 //   element stiffness      src/problems_elasticity.jl:203-451
-//                          (Km += w BL' D BL, :369-375; w = ip.weight*detJ, :248)
+//                          (Km += w BL' D BL, :369-375; w = ip.weight*detJ, :248;
+//                          finite strain BL(F), Kg, f_int, :255-289, 378-409)
 //   Jacobian and gradients src/basis/math.jl:185-212 (J = sum dN_i (x) X_i,
 //                          grad_i = invJ . dN_i, signed detJ)
 //   surface traction       src/problems_elasticity.jl:454-502
@@ -119,10 +120,13 @@ __global__ __launch_bounds__(256) void k_mf_elem(int n_el, const int32_t *__rest
 }
 
 // ------------------------------------------------------------- EBE element --
-// Phase 1 with stored element matrices: y_e = K_e x_e (upper blocks stored).
+// Phase 1 with stored element matrices: y_e = K_e x_e.  The upper blocks are
+// stored pair-major, Ke[(p * 9 + m) * n_el + e] (block p = (i <= j), entry m),
+// in the operator's precision, so the lanes of a wave read consecutive
+// addresses.
 template <int NPE, typename T>
 __global__ __launch_bounds__(256) void k_ebe_elem(int n_el, const int32_t *__restrict__ conn,
-                                                 const int32_t *__restrict__ slot, const double *__restrict__ Ke,
+                                                 const int32_t *__restrict__ slot, const T *__restrict__ Ke,
                                                  const T *__restrict__ x, T *__restrict__ ye,
                                                  const int *__restrict__ done) {
     using EL = Elem<NPE>;
@@ -140,13 +144,12 @@ __global__ __launch_bounds__(256) void k_ebe_elem(int n_el, const int32_t *__res
 #pragma unroll
         for (int c = 0; c < 3; ++c) { u[i][c] = x[3 * (size_t)nd + c] - t[c]; f[i][c] = T(0); }
     }
-    const double *K = Ke + (size_t)e * EL::NPAIR * 9;
 #pragma unroll
     for (int p = 0; p < EL::NPAIR; ++p) {
         const int i = PT.i[p], j = PT.j[p];
         T B[9];
 #pragma unroll
-        for (int m = 0; m < 9; ++m) B[m] = T(K[p * 9 + m]);
+        for (int m = 0; m < 9; ++m) B[m] = Ke[(size_t)(p * 9 + m) * n_el + e];
 #pragma unroll
         for (int c = 0; c < 3; ++c) f[i][c] += B[c * 3] * u[j][0] + B[c * 3 + 1] * u[j][1] + B[c * 3 + 2] * u[j][2];
         if (i != j) {
@@ -225,16 +228,86 @@ __global__ __launch_bounds__(256) void k_gather_simple(int nrows, const int32_t 
 }
 
 // --------------------------------------------------------- element matrices --
-// One 64-lane block per element: lanes (q, i) form grad N_i at Gauss point q
-// in LDS, lane p then accumulates block pair p = (i <= j) over the points.
+// Gradients of shape function i at Gauss point q of element e, from its
+// geometric factors: grad N_i = invJ dN_i(xi_q).
 template <int NPE>
-__global__ __launch_bounds__(64) void k_elem_mat(int n_el, int ngf, const double *__restrict__ gf,
-                                                double *__restrict__ Ke) {
+__device__ __forceinline__ void elem_grad(int n_el, int e, const double (&Ji)[9], int q, int i, double (&g)[3]) {
+    const double d0 = dn_rt<NPE>(q, i, 0), d1 = dn_rt<NPE>(q, i, 1), d2 = dn_rt<NPE>(q, i, 2);
+#pragma unroll
+    for (int b = 0; b < 3; ++b) g[b] = Ji[b * 3] * d0 + Ji[b * 3 + 1] * d1 + Ji[b * 3 + 2] * d2;
+}
+
+// Block (i, j) of K_e: sum_q cl g_i (x) g_j + cm (g_j (x) g_i + (g_i . g_j) I),
+// cl = lambda w detJ, cm = mu w detJ (the isotropic BL' D BL, Voigt D of
+// src/problems_elasticity.jl:310-317).
+template <int NPE>
+__device__ __forceinline__ void elem_block(int n_el, int ngf, const double *__restrict__ gf, int e, int i, int j,
+                                           double (&K)[9]) {
+    constexpr int NG = Elem<NPE>::NGP;
+#pragma unroll
+    for (int m = 0; m < 9; ++m) K[m] = 0.0;
+    double Ji[9], cl = 0.0, cm = 0.0;
+#pragma unroll
+    for (int q = 0; q < NG; ++q) {
+        if (q == 0 || ngf > 1) {
+            const double *G = gf + (size_t)(ngf > 1 ? q : 0) * GF * n_el + e;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) Ji[k] = G[(size_t)k * n_el];
+            cl = G[(size_t)9 * n_el];
+            cm = G[(size_t)10 * n_el];
+        }
+        double gi[3], gj[3];
+        elem_grad<NPE>(n_el, e, Ji, q, i, gi);
+        elem_grad<NPE>(n_el, e, Ji, q, j, gj);
+        const double dot = gi[0] * gj[0] + gi[1] * gj[1] + gi[2] * gj[2];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+                K[a * 3 + b] += cl * gi[a] * gj[b] + cm * (gi[b] * gj[a] + (a == b ? dot : 0.0));
+    }
+}
+
+// Element matrices for the EBE operator: thread (pair p = blockIdx.y, element
+// e), written pair-major in FP64 and FP32 so that consecutive lanes store
+// consecutive addresses.
+template <int NPE>
+__global__ __launch_bounds__(256) void k_elem_mat(int n_el, int ngf, const double *__restrict__ gf,
+                                                 double *__restrict__ Ke, float *__restrict__ Ke32) {
+    constexpr PairTable<NPE> PT = make_pairs<NPE>();
+    const int e = xcd_block() * blockDim.x + threadIdx.x, p = blockIdx.y;
+    if (e >= n_el) return;
+    double K[9];
+    elem_block<NPE>(n_el, ngf, gf, e, PT.i[p], PT.j[p], K);
+#pragma unroll
+    for (int m = 0; m < 9; ++m) {
+        Ke[(size_t)(p * 9 + m) * n_el + e] = K[m];
+        Ke32[(size_t)(p * 9 + m) * n_el + e] = (float)K[m];
+    }
+}
+
+// --------------------------------------- finite strain (total Lagrangian) --
+// Problem{Elasticity} with finite_strain = true (src/problems_elasticity.jl:
+// 203-451): at each Gauss point H = grad u, F = I + H, E = (H + H' + H'H)/2,
+// S = D E (St. Venant-Kirchhoff: lambda tr(E) I + 2 mu E).  For the isotropic
+// D the 3x3 block (i,j) of BL(F)' D BL(F) is
+//     lambda a_i (x) a_j + mu ((g_i . g_j) F F' + a_j (x) a_i),   a = F g,
+// the block of BNL' S3 BNL is (g_i' S g_j) I, and BL' S at node i is F S g_i
+// (all times w detJ in the reference configuration).  which = 0: Km, 1: Kg,
+// 2: Km + Kg, into the packed pair blocks `Ke`; f_int (when ye != null) goes
+// to the incidence slots for the nodal gather.  One 64-lane block per element.
+template <int NPE>
+__global__ __launch_bounds__(64) void k_tl_elem(int n_el, int ngf, const double *__restrict__ gf,
+                                               const int32_t *__restrict__ conn, const int32_t *__restrict__ slot,
+                                               const double *__restrict__ u, int which, double *__restrict__ Ke,
+                                               double *__restrict__ ye) {
     using EL = Elem<NPE>;
     constexpr int NG = EL::NGP, NP = EL::NPAIR;
     constexpr PairTable<NPE> PT = make_pairs<NPE>();
     __shared__ double g[NG][NPE][3];
+    __shared__ double ue[NPE][3];
     __shared__ double scl[NG], scm[NG];
+    __shared__ double Fq[NG][9], FFq[NG][9], Sq[NG][9];
     const int e = blockIdx.x, t = threadIdx.x;
     if (t < NG * NPE) {
         const int q = t / NPE, i = t % NPE, qf = (ngf == 1) ? 0 : q;
@@ -250,36 +323,145 @@ __global__ __launch_bounds__(64) void k_elem_mat(int n_el, int ngf, const double
             scm[q] = G[(size_t)10 * n_el];
         }
     }
+    if (t < NPE) {
+        const size_t a = (size_t)conn[(size_t)t * n_el + e];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) ue[t][c] = u[3 * a + c];
+    }
     __syncthreads();
-    if (t < NP) {
+    if (t < NG) {
+        const int q = t;
+        double H[9];
+#pragma unroll
+        for (int m = 0; m < 9; ++m) H[m] = 0.0;
+        for (int i = 0; i < NPE; ++i)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int A = 0; A < 3; ++A) H[c * 3 + A] += ue[i][c] * g[q][i][A];
+        double F[9], E[9];
+#pragma unroll
+        for (int m = 0; m < 9; ++m) F[m] = H[m] + ((m % 4) == 0 ? 1.0 : 0.0);
+#pragma unroll
+        for (int A = 0; A < 3; ++A)
+#pragma unroll
+            for (int B = 0; B < 3; ++B) {
+                double hh = 0.0;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) hh += H[c * 3 + A] * H[c * 3 + B];
+                E[A * 3 + B] = 0.5 * (H[A * 3 + B] + H[B * 3 + A] + hh);
+            }
+        const double tr = E[0] + E[4] + E[8];
+#pragma unroll
+        for (int m = 0; m < 9; ++m) {
+            Fq[q][m] = F[m];
+            Sq[q][m] = 2.0 * scm[q] * E[m] + ((m % 4) == 0 ? scl[q] * tr : 0.0);   // w detJ S
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                FFq[q][c * 3 + d] = F[c * 3] * F[d * 3] + F[c * 3 + 1] * F[d * 3 + 1] + F[c * 3 + 2] * F[d * 3 + 2];
+    }
+    __syncthreads();
+    if (Ke && t < NP) {
         const int i = PT.i[t], j = PT.j[t];
         double K[9];
 #pragma unroll
         for (int m = 0; m < 9; ++m) K[m] = 0.0;
 #pragma unroll
         for (int q = 0; q < NG; ++q) {
-            const double gi0 = g[q][i][0], gi1 = g[q][i][1], gi2 = g[q][i][2];
-            const double gj0 = g[q][j][0], gj1 = g[q][j][1], gj2 = g[q][j][2];
-            const double cl = scl[q], cm = scm[q];
-            const double gi[3] = {gi0, gi1, gi2}, gj[3] = {gj0, gj1, gj2};
-            const double dot = gi0 * gj0 + gi1 * gj1 + gi2 * gj2;
+            const double gi[3] = {g[q][i][0], g[q][i][1], g[q][i][2]};
+            const double gj[3] = {g[q][j][0], g[q][j][1], g[q][j][2]};
+            if (which != 1) {
+                const double cl = scl[q], cm = scm[q];
+                const double dot = gi[0] * gj[0] + gi[1] * gj[1] + gi[2] * gj[2];
+                double ai[3], aj[3];
 #pragma unroll
-            for (int a = 0; a < 3; ++a)
+                for (int c = 0; c < 3; ++c) {
+                    ai[c] = Fq[q][c * 3] * gi[0] + Fq[q][c * 3 + 1] * gi[1] + Fq[q][c * 3 + 2] * gi[2];
+                    aj[c] = Fq[q][c * 3] * gj[0] + Fq[q][c * 3 + 1] * gj[1] + Fq[q][c * 3 + 2] * gj[2];
+                }
 #pragma unroll
-                for (int b = 0; b < 3; ++b)
-                    K[a * 3 + b] += cl * gi[a] * gj[b] + cm * (gi[b] * gj[a] + (a == b ? dot : 0.0));
+                for (int c = 0; c < 3; ++c)
+#pragma unroll
+                    for (int d = 0; d < 3; ++d)
+                        K[c * 3 + d] += cl * ai[c] * aj[d] + cm * (dot * FFq[q][c * 3 + d] + aj[c] * ai[d]);
+            }
+            if (which != 0) {
+                double kg = 0.0;
+#pragma unroll
+                for (int A = 0; A < 3; ++A)
+                    kg += gi[A] * (Sq[q][A * 3] * gj[0] + Sq[q][A * 3 + 1] * gj[1] + Sq[q][A * 3 + 2] * gj[2]);
+                K[0] += kg;
+                K[4] += kg;
+                K[8] += kg;
+            }
         }
         double *out = Ke + ((size_t)e * NP + t) * 9;
 #pragma unroll
         for (int m = 0; m < 9; ++m) out[m] = K[m];
     }
+    if (ye && t < NPE) {
+        double fi[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int q = 0; q < NG; ++q) {
+            double sg[3];
+#pragma unroll
+            for (int A = 0; A < 3; ++A)
+                sg[A] = Sq[q][A * 3] * g[q][t][0] + Sq[q][A * 3 + 1] * g[q][t][1] + Sq[q][A * 3 + 2] * g[q][t][2];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) fi[c] += Fq[q][c * 3] * sg[0] + Fq[q][c * 3 + 1] * sg[1] + Fq[q][c * 3 + 2] * sg[2];
+        }
+        double *o = ye + 3 * (size_t)slot[(size_t)t * n_el + e];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) o[c] = fi[c];
+    }
 }
 
 // ------------------------------------------------- CSR values (phase 2) ----
-// One thread per 3x3 block: sums its contributions (ascending element id)
-// from the element matrices and writes the scalar-CSR-ordered values.
+// One thread per 3x3 block of K: sums its contributing element blocks in
+// ascending element id, each recomputed from the element's geometric factors
+// (no element matrices are stored or read), and writes the scalar-CSR-ordered
+// values in FP64 and FP32.
 template <int NPE>
-__global__ __launch_bounds__(256) void k_asm_vals(int nnzb, const int32_t *__restrict__ cptr,
+__global__ __launch_bounds__(256) void k_asm_vals(int nnzb, int n_el, int ngf, const double *__restrict__ gf,
+                                                 const int32_t *__restrict__ cptr, const int32_t *__restrict__ contrib,
+                                                 const int32_t *__restrict__ blk_row,
+                                                 const int32_t *__restrict__ brow, double *__restrict__ vals,
+                                                 float *__restrict__ vals32) {
+    const int s = xcd_block() * blockDim.x + threadIdx.x;
+    if (s >= nnzb) return;
+    double acc[9];
+#pragma unroll
+    for (int m = 0; m < 9; ++m) acc[m] = 0.0;
+    const int k1 = cptr[s + 1];
+    for (int k = cptr[s]; k < k1; ++k) {
+        const int c = contrib[k];
+        const int e = c / (NPE * NPE), rem = c - e * NPE * NPE;
+        const int i = rem / NPE, j = rem - i * NPE;
+        double B[9];
+        elem_block<NPE>(n_el, ngf, gf, e, i, j, B);
+#pragma unroll
+        for (int m = 0; m < 9; ++m) acc[m] += B[m];
+    }
+    const int a = blk_row[s];
+    const int r0 = brow[a], nb = brow[a + 1] - r0;
+    const size_t base = 9 * (size_t)r0 + 3 * (size_t)(s - r0);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const size_t o = base + (size_t)c * 3 * nb + d;
+            vals[o] = acc[c * 3 + d];
+            vals32[o] = (float)acc[c * 3 + d];
+        }
+}
+
+// The same from stored element matrices, element-major packed pairs
+// Ke[(e * NPAIR + p) * 9 + m] (the finite-strain blocks of k_tl_elem).
+template <int NPE>
+__global__ __launch_bounds__(256) void k_asm_vals_ke(int nnzb, const int32_t *__restrict__ cptr,
                                                  const int32_t *__restrict__ contrib,
                                                  const int32_t *__restrict__ blk_row,
                                                  const int32_t *__restrict__ brow, const double *__restrict__ Ke,
@@ -438,10 +620,23 @@ hipError_t launch_geom_factors(jfm_ctx *ctx, int *d_bad) {
 hipError_t launch_element_matrices(jfm_ctx *ctx) {
     const int n = (int)ctx->n_el;
     if (n == 0) return hipSuccess;
+    const dim3 grid(xcd_grid(nblk(n)), (unsigned)ctx->npair);
     switch (ctx->npe) {
-    case 10: k_elem_mat<10><<<n, 64, 0, ctx->stream>>>(n, ctx->ngf, ctx->gf64, ctx->Ke); break;
-    case 4: k_elem_mat<4><<<n, 64, 0, ctx->stream>>>(n, ctx->ngf, ctx->gf64, ctx->Ke); break;
-    case 8: k_elem_mat<8><<<n, 64, 0, ctx->stream>>>(n, ctx->ngf, ctx->gf64, ctx->Ke); break;
+    case 10: k_elem_mat<10><<<grid, kBlock, 0, ctx->stream>>>(n, ctx->ngf, ctx->gf64, ctx->Ke, ctx->Ke32); break;
+    case 4: k_elem_mat<4><<<grid, kBlock, 0, ctx->stream>>>(n, ctx->ngf, ctx->gf64, ctx->Ke, ctx->Ke32); break;
+    case 8: k_elem_mat<8><<<grid, kBlock, 0, ctx->stream>>>(n, ctx->ngf, ctx->gf64, ctx->Ke, ctx->Ke32); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_tl_element_matrices(jfm_ctx *ctx, const double *u, int which, double *Ke, double *ye) {
+    const int n = (int)ctx->n_el;
+    if (n == 0) return hipSuccess;
+    switch (ctx->npe) {
+    case 10: k_tl_elem<10><<<n, 64, 0, ctx->stream>>>(n, ctx->ngf, ctx->gf64, ctx->conn, ctx->inc_slot, u, which, Ke, ye); break;
+    case 4: k_tl_elem<4><<<n, 64, 0, ctx->stream>>>(n, ctx->ngf, ctx->gf64, ctx->conn, ctx->inc_slot, u, which, Ke, ye); break;
+    case 8: k_tl_elem<8><<<n, 64, 0, ctx->stream>>>(n, ctx->ngf, ctx->gf64, ctx->conn, ctx->inc_slot, u, which, Ke, ye); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -450,10 +645,24 @@ hipError_t launch_element_matrices(jfm_ctx *ctx) {
 hipError_t launch_assemble_values(jfm_ctx *ctx) {
     const int n = (int)ctx->nnzb;
     if (n == 0) return hipSuccess;
+    const int ne = (int)ctx->n_el;
+    const unsigned g = xcd_grid(nblk(n));
     switch (ctx->npe) {
-    case 10: k_asm_vals<10><<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->cptr, ctx->contrib, ctx->blk_row, ctx->brow, ctx->Ke, ctx->vals64, ctx->vals32); break;
-    case 4: k_asm_vals<4><<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->cptr, ctx->contrib, ctx->blk_row, ctx->brow, ctx->Ke, ctx->vals64, ctx->vals32); break;
-    case 8: k_asm_vals<8><<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->cptr, ctx->contrib, ctx->blk_row, ctx->brow, ctx->Ke, ctx->vals64, ctx->vals32); break;
+    case 10: k_asm_vals<10><<<g, kBlock, 0, ctx->stream>>>(n, ne, ctx->ngf, ctx->gf64, ctx->cptr, ctx->contrib, ctx->blk_row, ctx->brow, ctx->vals64, ctx->vals32); break;
+    case 4: k_asm_vals<4><<<g, kBlock, 0, ctx->stream>>>(n, ne, ctx->ngf, ctx->gf64, ctx->cptr, ctx->contrib, ctx->blk_row, ctx->brow, ctx->vals64, ctx->vals32); break;
+    case 8: k_asm_vals<8><<<g, kBlock, 0, ctx->stream>>>(n, ne, ctx->ngf, ctx->gf64, ctx->cptr, ctx->contrib, ctx->blk_row, ctx->brow, ctx->vals64, ctx->vals32); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_assemble_values_ke(jfm_ctx *ctx, const double *Ke) {
+    const int n = (int)ctx->nnzb;
+    if (n == 0) return hipSuccess;
+    switch (ctx->npe) {
+    case 10: k_asm_vals_ke<10><<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->cptr, ctx->contrib, ctx->blk_row, ctx->brow, Ke, ctx->vals64, ctx->vals32); break;
+    case 4: k_asm_vals_ke<4><<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->cptr, ctx->contrib, ctx->blk_row, ctx->brow, Ke, ctx->vals64, ctx->vals32); break;
+    case 8: k_asm_vals_ke<8><<<nblk(n), kBlock, 0, ctx->stream>>>(n, ctx->cptr, ctx->contrib, ctx->blk_row, ctx->brow, Ke, ctx->vals64, ctx->vals32); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -537,6 +746,9 @@ hipError_t launch_subset_apply(jfm_ctx *ctx, bool tangent, const double *x, doub
 template <typename T> static T *ye_of(jfm_ctx *ctx);
 template <> double *ye_of<double>(jfm_ctx *ctx) { return ctx->ye64; }
 template <> float *ye_of<float>(jfm_ctx *ctx) { return ctx->ye32; }
+template <typename T> static const T *ke_of(jfm_ctx *ctx);
+template <> const double *ke_of<double>(jfm_ctx *ctx) { return ctx->Ke; }
+template <> const float *ke_of<float>(jfm_ctx *ctx) { return ctx->Ke32; }
 template <typename T> static const T *gf_of(jfm_ctx *ctx);
 template <> const double *gf_of<double>(jfm_ctx *ctx) { return ctx->gf64; }
 template <> const float *gf_of<float>(jfm_ctx *ctx) { return ctx->gf32; }
@@ -545,7 +757,8 @@ template <int NPE, typename T>
 static void launch_elem_phase(jfm_ctx *ctx, int op, const T *x, T *ye, const int *done) {
     const int n = (int)ctx->n_el;
     if (op == JFM_OP_EBE) {
-        k_ebe_elem<NPE, T><<<xcd_grid(nblk(n)), kBlock, 0, ctx->stream>>>(n, ctx->conn, ctx->inc_slot, ctx->Ke, x, ye, done);
+        k_ebe_elem<NPE, T><<<xcd_grid(nblk(n)), kBlock, 0, ctx->stream>>>(n, ctx->conn, ctx->inc_slot, ke_of<T>(ctx), x, ye,
+                                                                        done);
     } else if (ctx->affine) {
         k_mf_elem<NPE, T, true><<<xcd_grid(nblk(n)), kBlock, 0, ctx->stream>>>(n, ctx->conn, ctx->inc_slot, gf_of<T>(ctx), x, ye, done,
                                                                               nullptr, n);

@@ -174,7 +174,8 @@ struct jfm_ctx {
     int32_t *cptr = nullptr, *contrib = nullptr;
     double *vals64 = nullptr;
     float *vals32 = nullptr;
-    double *Ke = nullptr;
+    double *Ke = nullptr;      // EBE element matrices, pair-major: [(p * 9 + m) * n_el + e]
+    float *Ke32 = nullptr;     // the same in FP32
     bool assembled = false;
 
     // boundary conditions and loads
@@ -221,6 +222,7 @@ struct jfm_ctx {
     double *xprobe = nullptr;      // the learned-selection probe's winning iterate
     float *Mlearn32 = nullptr;
     float *feat = nullptr;         // N x 24 node features
+    double *efeat = nullptr;       // 3 x n_el per-element feature inputs
     float *lout = nullptr;         // N x 6 network outputs
     // nonlinear state (tangent operator)
     double *ustate64 = nullptr;    // displacement the tangent is taken at
@@ -312,6 +314,11 @@ namespace jfem {
 hipError_t launch_geom_factors(jfm_ctx *ctx, int *d_bad);
 hipError_t launch_element_matrices(jfm_ctx *ctx);
 hipError_t launch_assemble_values(jfm_ctx *ctx);
+// finite-strain element matrices (which: 0 Km, 1 Kg, 2 Km + Kg), element-major
+// into Ke and, when ye != null, f_int(u) into the incidence slots
+hipError_t launch_tl_element_matrices(jfm_ctx *ctx, const double *u, int which, double *Ke, double *ye);
+// CSR values from element-major packed blocks Ke[(e * NPAIR + p) * 9 + m]
+hipError_t launch_assemble_values_ke(jfm_ctx *ctx, const double *Ke);
 hipError_t launch_face_loads(jfm_ctx *ctx);
 hipError_t launch_block_jacobi(jfm_ctx *ctx, double *Minv64, float *Minv32);
 // Two-phase operators; `part` (may be null) receives one partial of

@@ -34,7 +34,7 @@ struct LearnedBlob {
 };
 
 // device launchers (learned.hip)
-hipError_t launch_features(jfm_ctx *ctx, float *feat);
+hipError_t launch_features(jfm_ctx *ctx, double *ef, float *feat);
 hipError_t launch_mlp(jfm_ctx *ctx, const float *feat, const void *blob, float *out, const double *G,
                       const double *Minv, float *gpart, int *nwaves);
 hipError_t launch_mblocks(jfm_ctx *ctx, const float *out, const double *Minv, float eps, double *M64, float *M32);

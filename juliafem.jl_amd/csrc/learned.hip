@@ -4,10 +4,10 @@
 // ext/JuliaFEMCUDAExt.jl:531-577).  Pieces:
 //   k_features   24 local features per node (diagonal block, stiffness shares
 //                of the incident elements, material field, boundary flags)
-//   k_mlp        24 -> 64 -> 64 -> 6 network on 16-node tiles with
-//                v_mfma_f32_16x16x16_bf16 (BF16 operands, FP32 accumulation);
-//                with TRAIN it recomputes the forward pass and runs the
-//                backward pass, weight gradients again on MFMA
+//   k_mlp_infer  24 -> 64 -> 64 -> 6 network on 16-node tiles with
+//                v_mfma_f32_16x16x16_bf16 (BF16 operands, FP32 accumulation)
+//   k_mlp        the training pass: the same forward pass, then the backward
+//                pass, weight gradients again on MFMA
 //   k_mblocks    M_a = C_a (L L' + eps I) C_a' from the outputs
 //   k_block_apply / k_accum_dm   the loss pieces of a training step
 // The loss of one (residual r, error e = K^-1 r) pair is the energy-norm
@@ -44,13 +44,43 @@ __device__ __forceinline__ float gelu_grad(float x) {
 // ------------------------------------------------------------------ features --
 template <int NPE> constexpr int corners() { return NPE == 10 ? 4 : NPE; }
 
+// Per-element inputs of the node features, once per element instead of once
+// per incidence: ef[e] = log E, ef[n_el + e] = shape quality |invJ|_F detJ^(1/3)
+// (first Gauss point), ef[2 n_el + e] = 1 if the element touches a fixed DOF.
+template <int NPE>
+__global__ __launch_bounds__(256) void k_elem_features(int n_el, const int32_t *__restrict__ conn,
+                                                      const double *__restrict__ gf, const double *__restrict__ E,
+                                                      const double *__restrict__ nu, const uint8_t *__restrict__ fixed,
+                                                      double *__restrict__ ef) {
+    using EL = Elem<NPE>;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_el) return;
+    const double *G = gf + e;
+    double fro = 0.0;
+    for (int m = 0; m < 9; ++m) {
+        const double v = G[(size_t)m * n_el];
+        fro += v * v;
+    }
+    const double cl = G[(size_t)9 * n_el];
+    const double Ee = E[e], ne = nu[e];
+    const double la = Ee * ne / ((1.0 + ne) * (1.0 - 2.0 * ne));
+    const double det = cl / (la * EL::w(0));
+    bool fx = false;
+    for (int j = 0; j < NPE; ++j) {
+        const size_t nd = 3 * (size_t)conn[(size_t)j * n_el + e];
+        fx = fx || fixed[nd] || fixed[nd + 1] || fixed[nd + 2];
+    }
+    ef[e] = log(Ee);
+    ef[(size_t)n_el + e] = sqrt(fro) * cbrt(det);
+    ef[2 * (size_t)n_el + e] = fx ? 1.0 : 0.0;
+}
+
 template <int NPE>
 __global__ __launch_bounds__(256) void k_features(int N, int n_el, int ngf, const int32_t *__restrict__ ptr,
-                                                 const int32_t *__restrict__ inc, const int32_t *__restrict__ conn,
+                                                 const int32_t *__restrict__ inc, const double *__restrict__ ef,
                                                  const double *__restrict__ gf, const double *__restrict__ E,
-                                                 const double *__restrict__ nu, const uint8_t *__restrict__ fixed,
-                                                 const int32_t *__restrict__ fptr, const double *__restrict__ Dover,
-                                                 float *__restrict__ feat) {
+                                                 const uint8_t *__restrict__ fixed, const int32_t *__restrict__ fptr,
+                                                 const double *__restrict__ Dover, float *__restrict__ feat) {
     using EL = Elem<NPE>;
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= N) return;
@@ -64,12 +94,15 @@ __global__ __launch_bounds__(256) void k_features(int N, int n_el, int ngf, cons
     for (int k = k0; k < k1; ++k) {
         const int idx = inc[k];
         const int e = idx / NPE, i = idx - e * NPE;
-        double te = 0.0, q = 0.0;
+        double te = 0.0;
+        double Ji[9], cl = 0.0, cm = 0.0;
         for (int qp = 0; qp < EL::NGP; ++qp) {
-            const double *G = gf + (size_t)(ngf == 1 ? 0 : qp) * GF * n_el + e;
-            double Ji[9];
-            for (int m = 0; m < 9; ++m) Ji[m] = G[(size_t)m * n_el];
-            const double cl = G[(size_t)9 * n_el], cm = G[(size_t)10 * n_el];
+            if (qp == 0 || ngf > 1) {   // affine elements: one set of factors
+                const double *G = gf + (size_t)(ngf == 1 ? 0 : qp) * GF * n_el + e;
+                for (int m = 0; m < 9; ++m) Ji[m] = G[(size_t)m * n_el];
+                cl = G[(size_t)9 * n_el];
+                cm = G[(size_t)10 * n_el];
+            }
             double g[3];
             const double d0 = dn_rt<NPE>(qp, i, 0), d1 = dn_rt<NPE>(qp, i, 1), d2 = dn_rt<NPE>(qp, i, 2);
             for (int b = 0; b < 3; ++b) g[b] = Ji[b * 3] * d0 + Ji[b * 3 + 1] * d1 + Ji[b * 3 + 2] * d2;
@@ -77,26 +110,13 @@ __global__ __launch_bounds__(256) void k_features(int N, int n_el, int ngf, cons
             for (int r = 0; r < 3; ++r)
                 for (int c = 0; c < 3; ++c) D[r * 3 + c] += (cl + cm) * g[r] * g[c] + (r == c ? cm * gg : 0.0);
             te += (cl + 4.0 * cm) * gg;
-            if (qp == 0) {
-                double fro = 0.0;
-                for (int m = 0; m < 9; ++m) fro += Ji[m] * Ji[m];
-                const double Ee = E[e], ne = nu[e];
-                const double la = Ee * ne / ((1.0 + ne) * (1.0 - 2.0 * ne));
-                const double det = cl / (la * EL::w(0));
-                q = sqrt(fro) * cbrt(det);
-            }
         }
-        const double Ee = E[e];
+        const double Ee = E[e], lE = ef[e], q = ef[(size_t)n_el + e];
         tmin = fmin(tmin, te); tmax = fmax(tmax, te); tsum += te; tsq += te * te;
-        emin = fmin(emin, Ee); emax = fmax(emax, Ee); esum += Ee; lesum += log(Ee); tle += te * log(Ee);
+        emin = fmin(emin, Ee); emax = fmax(emax, Ee); esum += Ee; lesum += lE; tle += te * lE;
         qsum += q; qmin = fmin(qmin, q); qmax = fmax(qmax, q);
         if (i < corners<NPE>()) ++ncorner;
-        bool fx = false;
-        for (int j = 0; j < NPE; ++j) {
-            const size_t nd = 3 * (size_t)conn[(size_t)j * n_el + e];
-            fx = fx || fixed[nd] || fixed[nd + 1] || fixed[nd + 2];
-        }
-        if (fx) ++nfixel;
+        if (ef[2 * (size_t)n_el + e] != 0.0) ++nfixel;
     }
     if (Dover)  // diagonal block of the tangent in place of the linear one
         for (int m = 0; m < 9; ++m) D[m] = Dover[9 * (size_t)a + m];
@@ -227,9 +247,10 @@ __device__ __forceinline__ bool node_fixed(const uint8_t *fixed, int a) {
     return fixed[o] || fixed[o + 1] || fixed[o + 2];
 }
 
-// Network over all nodes.  Inference writes `out` (N x 6).  TRAIN recomputes
-// the forward pass, forms dLoss/do from G = dLoss/dM (N x 9) and writes one
-// set of weight-gradient partials per wave to gpart.
+// Training pass over all nodes: recomputes the forward pass, forms dLoss/do
+// from G = dLoss/dM (N x 9) and writes one set of weight-gradient partials per
+// wave to gpart.  (TRAIN = true is the only instantiation; inference is
+// k_mlp_infer below.)
 template <bool TRAIN>
 __global__ __launch_bounds__(256) void k_mlp(int N, const float *__restrict__ feat, const uint8_t *__restrict__ blob,
                                             float *__restrict__ out, const double *__restrict__ G,
@@ -324,14 +345,6 @@ __global__ __launch_bounds__(256) void k_mlp(int N, const float *__restrict__ fe
             for (int j = 0; j < 4; ++j) W.O[q + j][r] = acc[0][j] + S.b3[r];
         }
         __syncthreads();
-        if (!TRAIN) {
-            for (int i = l; i < kTile * kOut; i += 64) {
-                const int m = i / kOut, c = i % kOut, node = n0 + m;
-                if (node < N) out[(size_t)node * kOut + c] = W.O[m][c];
-            }
-            __syncthreads();
-            continue;
-        }
         // dLoss/do per node (lane m < 16)
         if (l < kTile) {
             const int node = n0 + l;
@@ -498,16 +511,133 @@ __global__ __launch_bounds__(256) void k_grad_reduce(int nparts, const float *__
 static inline unsigned nblk(int64_t n, int b = kBlock) { return (unsigned)((n + b - 1) / b); }
 constexpr int kTrainBlocks = 256;
 
-hipError_t launch_features(jfm_ctx *ctx, float *feat) {
-    const int N = (int)ctx->N;
+// ef: 3 n_el doubles of per-element inputs (k_elem_features)
+hipError_t launch_features(jfm_ctx *ctx, double *ef, float *feat) {
+    const int N = (int)ctx->N, ne = (int)ctx->n_el;
     const double *Dover = ctx->minv_tangent ? ctx->Dtan : nullptr;
     switch (ctx->npe) {
-    case 10: k_features<10><<<nblk(N), kBlock, 0, ctx->stream>>>(N, (int)ctx->n_el, ctx->ngf, ctx->inc_ptr, ctx->inc, ctx->conn, ctx->gf64, ctx->E, ctx->nu, ctx->fixed, ctx->finc_ptr, Dover, feat); break;
-    case 4: k_features<4><<<nblk(N), kBlock, 0, ctx->stream>>>(N, (int)ctx->n_el, ctx->ngf, ctx->inc_ptr, ctx->inc, ctx->conn, ctx->gf64, ctx->E, ctx->nu, ctx->fixed, ctx->finc_ptr, Dover, feat); break;
-    case 8: k_features<8><<<nblk(N), kBlock, 0, ctx->stream>>>(N, (int)ctx->n_el, ctx->ngf, ctx->inc_ptr, ctx->inc, ctx->conn, ctx->gf64, ctx->E, ctx->nu, ctx->fixed, ctx->finc_ptr, Dover, feat); break;
+    case 10:
+        if (ne) k_elem_features<10><<<nblk(ne), kBlock, 0, ctx->stream>>>(ne, ctx->conn, ctx->gf64, ctx->E, ctx->nu, ctx->fixed, ef);
+        k_features<10><<<nblk(N), kBlock, 0, ctx->stream>>>(N, ne, ctx->ngf, ctx->inc_ptr, ctx->inc, ef, ctx->gf64, ctx->E, ctx->fixed, ctx->finc_ptr, Dover, feat);
+        break;
+    case 4:
+        if (ne) k_elem_features<4><<<nblk(ne), kBlock, 0, ctx->stream>>>(ne, ctx->conn, ctx->gf64, ctx->E, ctx->nu, ctx->fixed, ef);
+        k_features<4><<<nblk(N), kBlock, 0, ctx->stream>>>(N, ne, ctx->ngf, ctx->inc_ptr, ctx->inc, ef, ctx->gf64, ctx->E, ctx->fixed, ctx->finc_ptr, Dover, feat);
+        break;
+    case 8:
+        if (ne) k_elem_features<8><<<nblk(ne), kBlock, 0, ctx->stream>>>(ne, ctx->conn, ctx->gf64, ctx->E, ctx->nu, ctx->fixed, ef);
+        k_features<8><<<nblk(N), kBlock, 0, ctx->stream>>>(N, ne, ctx->ngf, ctx->inc_ptr, ctx->inc, ef, ctx->gf64, ctx->E, ctx->fixed, ctx->finc_ptr, Dover, feat);
+        break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+// Wave-local barrier: the LDS a wave reads below was written by other lanes of
+// the same wave only.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Inference: the same forward pass as k_mlp (same MFMA tiles, roundings and
+// order), in persistent 4-wave blocks that stage the weights once; each wave
+// walks its own 16-node tiles with wave-local barriers only, and its LDS holds
+// no training buffers (about 39 KiB per block, four blocks per CU).
+constexpr int kInfWaves = 4;
+struct MlpInfLds {
+    uint16_t W1[kHid][kXPad];
+    uint16_t W2[kHid][kHid];
+    uint16_t W3[16][kHid];
+    float b1[kHid], b2[kHid], b3[16];
+    float shift[kFeat], scale[kFeat];
+    struct Wave {
+        uint16_t X[kTile][kXPad];
+        uint16_t H1[kTile][kHid], H2[kTile][kHid];
+        float O[kTile][16];
+    } w[kInfWaves];
+};
+
+__global__ __launch_bounds__(kInfWaves * 64) void k_mlp_infer(int N, const float *__restrict__ feat,
+                                                             const uint8_t *__restrict__ blob,
+                                                             float *__restrict__ out) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    MlpInfLds &S = *reinterpret_cast<MlpInfLds *>(lds_raw);
+    const LearnedBlob *B = reinterpret_cast<const LearnedBlob *>(blob);
+    const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63, r = l & 15, q = (l >> 4) * 4;
+    for (int i = tid; i < kHid * kXPad; i += blockDim.x) {
+        const int o = i / kXPad, c = i % kXPad;
+        S.W1[o][c] = c < kFeat ? B->W1[o * kFeat + c] : (uint16_t)0;
+    }
+    for (int i = tid; i < kHid * kHid; i += blockDim.x) S.W2[i / kHid][i % kHid] = B->W2[i];
+    for (int i = tid; i < 16 * kHid; i += blockDim.x) {
+        const int o = i / kHid, c = i % kHid;
+        S.W3[o][c] = o < kOut ? B->W3[o * kHid + c] : (uint16_t)0;
+    }
+    for (int i = tid; i < kHid; i += blockDim.x) { S.b1[i] = B->b1[i]; S.b2[i] = B->b2[i]; }
+    for (int i = tid; i < 16; i += blockDim.x) S.b3[i] = i < kOut ? B->b3[i] : 0.f;
+    for (int i = tid; i < kFeat; i += blockDim.x) { S.shift[i] = B->feat_shift[i]; S.scale[i] = B->feat_scale[i]; }
+    __syncthreads();   // the only block-wide barrier
+    MlpInfLds::Wave &W = S.w[wv];
+    constexpr int kFL = kTile * kXPad / 64;
+    const int fc = l % kXPad;
+    float fv[kFL];
+    auto fetch = [&](int n0) {
+#pragma unroll
+        for (int u = 0; u < kFL; ++u) {
+            const int node = n0 + (l + 64 * u) / kXPad;
+            fv[u] = (fc < kFeat && node < N) ? feat[(size_t)node * kFeat + fc] : 0.f;
+        }
+    };
+    const int waves = gridDim.x * kInfWaves;
+    const int first = (blockIdx.x * kInfWaves + wv) * kTile;
+    if (first < N) fetch(first);
+    for (int n0 = first; n0 < N; n0 += waves * kTile) {
+#pragma unroll
+        for (int u = 0; u < kFL; ++u) {
+            const int m = (l + 64 * u) / kXPad, node = n0 + m;
+            W.X[m][fc] = f2bf((fc < kFeat && node < N) ? (fv[u] - S.shift[fc]) * S.scale[fc] : 0.f);
+        }
+        if (n0 + waves * kTile < N) fetch(n0 + waves * kTile);   // next tile's features in flight
+        wave_sync();
+        {
+            f32x4 acc[4];
+            dense_fwd<kXPad, 4>(&W.X[0][0], kXPad, &S.W1[0][0], kXPad, acc);
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int n = t * 16 + r;
+                    W.H1[q + j][n] = f2bf(gelu(acc[t][j] + S.b1[n]));
+                }
+        }
+        wave_sync();
+        {
+            f32x4 acc[4];
+            dense_fwd<kHid, 4>(&W.H1[0][0], kHid, &S.W2[0][0], kHid, acc);
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int n = t * 16 + r;
+                    W.H2[q + j][n] = f2bf(gelu(acc[t][j] + S.b2[n]));
+                }
+        }
+        wave_sync();
+        {
+            f32x4 acc[1];
+            dense_fwd<kHid, 1>(&W.H2[0][0], kHid, &S.W3[0][0], kHid, acc);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) W.O[q + j][r] = acc[0][j] + S.b3[r];
+        }
+        wave_sync();
+        for (int i = l; i < kTile * kOut; i += 64) {
+            const int m = i / kOut, c = i % kOut, node = n0 + m;
+            if (node < N) out[(size_t)node * kOut + c] = W.O[m][c];
+        }
+        wave_sync();
+    }
 }
 
 hipError_t launch_mlp(jfm_ctx *ctx, const float *feat, const void *blob, float *out, const double *G,
@@ -520,8 +650,10 @@ hipError_t launch_mlp(jfm_ctx *ctx, const float *feat, const void *blob, float *
         *nwaves = (int)g * kWaves;
         k_mlp<true><<<g, kWaves * 64, lds, ctx->stream>>>(N, feat, (const uint8_t *)blob, out, G, Minv, ctx->fixed, gpart);
     } else {
-        k_mlp<false><<<std::min<unsigned>(tiles, 2048), kWaves * 64, lds, ctx->stream>>>(
-            N, feat, (const uint8_t *)blob, out, nullptr, Minv, ctx->fixed, nullptr);
+        // four resident blocks per CU on 256 CUs; each wave takes every 4096th tile
+        const unsigned tiles16 = (unsigned)((N + kTile - 1) / kTile);
+        const unsigned g = std::max(1u, std::min<unsigned>((tiles16 + kInfWaves - 1) / kInfWaves, 1024));
+        k_mlp_infer<<<g, kInfWaves * 64, sizeof(MlpInfLds), ctx->stream>>>(N, feat, (const uint8_t *)blob, out);
     }
     return hipGetLastError();
 }
@@ -553,6 +685,7 @@ extern "C" int32_t jfm_learned_n_params(void) { return kNParam; }
 static int ensure_buffers(jfm_ctx *ctx) {
     const int64_t N = ctx->N;
     LALLOC(ctx, ctx->feat, float, N * kFeat);
+    LALLOC(ctx, ctx->efeat, double, 3 * std::max<int64_t>(ctx->n_el, 1));
     LALLOC(ctx, ctx->lout, float, N * kOut);
     LALLOC(ctx, ctx->Mlearn64, double, N * 9);
     LALLOC(ctx, ctx->Mlearn32, float, N * 9);
@@ -585,7 +718,7 @@ extern "C" int jfm_learned_get_weights(jfm_ctx *ctx, void *blob, size_t bytes) {
 static int compute_features(jfm_ctx *ctx) {
     if (int rc = jfem_ensure_workspaces(ctx)) return rc;
     if (int rc = ensure_buffers(ctx)) return rc;
-    hipError_t e = launch_features(ctx, ctx->feat);
+    hipError_t e = launch_features(ctx, ctx->efeat, ctx->feat);
     if (e != hipSuccess) return jfm_hip_fail(ctx, e, "launch_features");
     return JFM_OK;
 }

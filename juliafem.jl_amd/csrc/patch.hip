@@ -69,12 +69,15 @@ __global__ __launch_bounds__(kPatch) void k_ku_patch(int n_el, int npatch, int n
     const int lb = pn_ptr[p], le = pn_ptr[p + 1], q0 = rstart[p];
     constexpr int kPre = 4, kPreR = sizeof(T) == 4 ? 3 : 1;  // FP64: registers are short
     int cnt[kPreR], oo[kPreR], mpre[kPreR][kPre];
+    // every index below fits 32 bits: byte offsets from the uniform base
+    // pointers keep the per-lane address arithmetic 32-bit
     auto fetch = [&](int r, int l, int &c, int &o, int (&mp)[kPre]) {
-        c = pn_cnt[l];
-        o = pn_out[l];
+        c = load_at<uint16_t>(pn_cnt, 2u * (uint32_t)l);
+        o = load_at<int32_t>(pn_out, 4u * (uint32_t)l);
         const int base = rbase[q0 + r];
 #pragma unroll
-        for (int j = 0; j < kPre; ++j) mp[j] = j < c ? (int)linT[base + j * kPatch + t] : 0;
+        for (int j = 0; j < kPre; ++j)
+            mp[j] = j < c ? (int)load_at<uint16_t>(linT, 2u * (uint32_t)(base + j * kPatch + t)) : 0;
     };
 #pragma unroll
     for (int r = 0; r < kPreR; ++r) {
@@ -108,28 +111,26 @@ __global__ __launch_bounds__(kPatch) void k_ku_patch(int n_el, int npatch, int n
         if (c > kPre) {
             const int base = rbase[q0 + r];
             for (int j = kPre; j < c; ++j) {
-                const int m = linT[base + j * kPatch + t];
+                const int m = load_at<uint16_t>(linT, 2u * (uint32_t)(base + j * kPatch + t));
                 s0 += ys[m];
                 s1 += ys[m + kPatch];
                 s2 += ys[m + 2 * kPatch];
             }
         }
         if (o >= 0) {
-            const size_t b = 3 * (size_t)o;
+            const uint32_t b = 3u * (uint32_t)o;
             if (fixed) {
-                if (fixed[b]) s0 = T(0);
-                if (fixed[b + 1]) s1 = T(0);
-                if (fixed[b + 2]) s2 = T(0);
+                if (load_at<uint8_t>(fixed, b)) s0 = T(0);
+                if (load_at<uint8_t>(fixed, b + 1)) s1 = T(0);
+                if (load_at<uint8_t>(fixed, b + 2)) s2 = T(0);
             }
-            y[b] = s0;
-            y[b + 1] = s1;
-            y[b + 2] = s2;
-            if (dotv) dv += (double)s0 * (double)dotv[b] + (double)s1 * (double)dotv[b + 1] + (double)s2 * (double)dotv[b + 2];
+            store_at<Trip<T>>(y, (uint32_t)sizeof(T) * b, Trip<T>{s0, s1, s2});
+            if (dotv) {
+                const Trip<T> d = load_at<Trip<T>>(dotv, (uint32_t)sizeof(T) * b);
+                dv += (double)s0 * (double)d.a + (double)s1 * (double)d.b + (double)s2 * (double)d.c;
+            }
         } else {
-            const size_t b = 3 * (size_t)(-o - 1);
-            bpart[b] = s0;
-            bpart[b + 1] = s1;
-            bpart[b + 2] = s2;
+            store_at<Trip<T>>(bpart, (uint32_t)sizeof(T) * 3u * (uint32_t)(-o - 1), Trip<T>{s0, s1, s2});
         }
     };
 #pragma unroll
@@ -472,9 +473,11 @@ hipError_t launch_patch_phase(jfm_ctx *ctx, const T *x, T *y, const T *dotv, con
 
 template <typename T>
 hipError_t launch_patch_combine(jfm_ctx *ctx, T *y, const T *dotv, double *part, const int *done, int *npart) {
-    // a multiple of 8 blocks (XCD mapping), at most 512 partials
+    // a multiple of 8 blocks (XCD mapping), one shared node per thread up to
+    // kMaxPart blocks (544 at 1.05M DOFs: a second node per thread would
+    // double the dependent-load chain of the kernel)
     const int nb = ctx->nbnode;
-    const unsigned g = std::min<unsigned>(xcd_grid((unsigned)((std::max(nb, 1) + kBlock - 1) / kBlock)), 512);
+    const unsigned g = std::min<unsigned>(xcd_grid((unsigned)((std::max(nb, 1) + kBlock - 1) / kBlock)), kMaxPart);
     const int chunk = (int)((nb + g - 1) / g);
     if (npart) *npart = (int)g;
     k_ku_combine<T><<<g, kBlock, 0, ctx->stream>>>(nb, chunk, ctx->bptr, ctx->bnode, (const T *)ctx->bpart,

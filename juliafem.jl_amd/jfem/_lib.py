@@ -24,7 +24,7 @@ ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ENOTSPD", -4: "ENOCONV", -5: "ESTATE", 
 
 # Exported symbols (must match include/jfem.h)
 SYMBOLS = ("jfm_create", "jfm_destroy", "jfm_last_error", "jfm_get_info", "jfm_upload_mesh",
-           "jfm_upload_bcs", "jfm_csr_pattern", "jfm_assemble", "jfm_apply", "jfm_time_apply",
+           "jfm_upload_bcs", "jfm_csr_pattern", "jfm_assemble", "jfm_assemble_tl", "jfm_apply", "jfm_time_apply",
            "jfm_solve", "jfm_solve_history", "jfm_precond_set_blocks", "jfm_learned_blob_bytes", "jfm_learned_n_params",
            "jfm_learned_set_weights", "jfm_learned_get_weights", "jfm_learned_features", "jfm_learned_outputs",
            "jfm_learned_grad", "jfm_nl_residual", "jfm_solve_nonlinear", "jfm_postprocess", "jfm_heat_solve")
@@ -97,6 +97,7 @@ def load():
     lib.jfm_upload_bcs.argtypes = [_P, _P, _P, C.c_int64, C.c_int32, _P, _P, C.c_int32]
     lib.jfm_csr_pattern.argtypes = [_P, C.POINTER(C.c_int64), _P, _P, C.c_int32]
     lib.jfm_assemble.argtypes = [_P, _P, _P, _P]
+    lib.jfm_assemble_tl.argtypes = [_P, C.c_int32, _P, _P, _P, _P]
     lib.jfm_apply.argtypes = [_P, C.c_int32, C.c_int32, _P, _P]
     lib.jfm_time_apply.argtypes = [_P, C.c_int32, C.c_int32, C.c_int32, _P]
     lib.jfm_solve.argtypes = [_P, C.POINTER(SolveOpts), _P, _P, _P, C.POINTER(SolveStats)]
@@ -204,6 +205,17 @@ class Context:
         uu = None if u is None else np.ascontiguousarray(u, dtype=np.float64)
         self._check(self._lib.jfm_assemble(self._h, _p(uu), _p(vals), _p(f)))
         return vals, f
+
+    def assemble_tl(self, u=None, geometric=True, want_vals=True, want_f=True):
+        """Finite-strain assembly: (Km values, Kg values or None, f = f_ext - f_int(u));
+        the CSR operator then holds Km + Kg."""
+        inf = self.info()
+        vals = np.empty(inf["nnz"]) if want_vals else None
+        kg = np.empty(inf["nnz"]) if (want_vals and geometric) else None
+        f = np.empty(inf["n_dofs"]) if want_f else None
+        uu = None if u is None else np.ascontiguousarray(u, dtype=np.float64)
+        self._check(self._lib.jfm_assemble_tl(self._h, int(geometric), _p(uu), _p(vals), _p(kg), _p(f)))
+        return vals, kg, f
 
     def apply(self, x, op="matfree", precision=FP64):
         x = np.ascontiguousarray(x, dtype=np.float64)

@@ -455,17 +455,121 @@ def mesh_from_groups(X, groups, volume: str, clamp: str, load: str | None = None
 
 
 # ------------------------------------------------------------ renumbering ----
-def rcm_order(mesh: Mesh) -> np.ndarray:
-    """Reverse Cuthill-McKee node order of the mesh's node adjacency
-    (src/graph/graph_ordering.jl:38-90): perm[new] = old."""
-    import scipy.sparse as sp
-    from scipy.sparse.csgraph import reverse_cuthill_mckee
+def node_graph(mesh: Mesh):
+    """Node adjacency (nodes sharing an element, self excluded) in CSR form
+    (ptr, idx), each list sorted by node id: the graph G of
+    src/graph/graph_ordering.jl."""
     conn = np.asarray(mesh.conn, np.int64)
     npe = conn.shape[1]
-    rows = np.repeat(conn, npe, axis=1).ravel()
-    cols = np.tile(conn, (1, npe)).ravel()
-    A = sp.csr_matrix((np.ones(rows.size, np.int8), (rows, cols)), shape=(mesh.n_nodes,) * 2)
-    return np.asarray(reverse_cuthill_mckee(A, symmetric_mode=True), np.int64)
+    a = np.repeat(conn, npe, axis=1).ravel()
+    b = np.tile(conn, (1, npe)).ravel()
+    keep = a != b
+    key = np.unique(a[keep] * mesh.n_nodes + b[keep])
+    src, dst = key // mesh.n_nodes, key % mesh.n_nodes
+    ptr = np.searchsorted(src, np.arange(mesh.n_nodes + 1))
+    return ptr, dst
+
+
+def symrcm(ptr, idx, v: int):
+    """Reverse Cuthill-McKee from start vertex v, restating symrcm of
+    src/graph/graph_ordering.jl:38-90 (0-based, graph in CSR form): breadth
+    first from v; each visited vertex appends its unvisited neighbours in
+    ascending degree order (a stable sort, as the reference's InsertionSort
+    over its adjacency list); the order is reversed.  Returns (perm, invperm,
+    degrees, edge, dist) with perm[new] = old.  As in the reference, every
+    vertex must be reachable from v."""
+    n = ptr.size - 1
+    degrees = np.diff(ptr)
+    perm = np.empty(n, np.int64)
+    perm[0] = v
+    visited = np.zeros(n, bool)
+    visited[v] = True
+    edge = np.zeros(n, np.int64)
+    dist = np.zeros(n, np.int64)
+    k = 1
+    for i in range(n):
+        if i >= k:
+            raise ValueError("graph is not connected from the start vertex")
+        w0 = perm[i]
+        adj = idx[ptr[w0]:ptr[w0 + 1]]
+        aux = adj[np.argsort(degrees[adj], kind="stable")]
+        new = aux[~visited[aux]]
+        visited[new] = True
+        perm[k:k + new.size] = new
+        edge[new] = w0
+        dist[new] = dist[w0] + 1
+        k += new.size
+    perm = perm[::-1].copy()
+    inv = np.empty(n, np.int64)
+    inv[perm] = np.arange(n)
+    return perm, inv, degrees, edge, dist
+
+
+def bfs_levels(ptr, idx, v: int):
+    """Breadth-first level of every vertex from v (-1: unreachable)."""
+    level = -np.ones(ptr.size - 1, np.int64)
+    level[v] = 0
+    frontier = np.array([v], np.int64)
+    d = 0
+    while frontier.size:
+        d += 1
+        starts, ends = ptr[frontier], ptr[frontier + 1]
+        nb = idx[np.repeat(starts - np.cumsum(np.r_[0, (ends - starts)[:-1]]), ends - starts)
+                 + np.arange((ends - starts).sum())]
+        nb = np.unique(nb)
+        frontier = nb[level[nb] < 0]
+        level[frontier] = d
+    return level
+
+
+def pseudo_peripheral_node(ptr, idx, start: int) -> int:
+    """A start vertex of large eccentricity (George and Liu, 1979): from
+    `start`, repeatedly move to a minimum-degree vertex of the last BFS level
+    while the eccentricity grows."""
+    deg = np.diff(ptr)
+    v = int(start)
+    lv = bfs_levels(ptr, idx, v)
+    ecc = lv.max()
+    while True:
+        last = np.flatnonzero(lv == ecc)
+        u = int(last[np.lexsort((last, deg[last]))[0]])
+        lu = bfs_levels(ptr, idx, u)
+        if lu.max() <= ecc:
+            return v
+        v, ecc, lv = u, lu.max(), lu
+
+
+def rcm_order(mesh: Mesh) -> np.ndarray:
+    """Reverse Cuthill-McKee node order of the mesh (perm[new] = old): symrcm
+    of src/graph/graph_ordering.jl:38-90 from a pseudo-peripheral start vertex.
+    The reference takes the start vertex from its caller and needs a
+    connected graph; here each connected component (in order of its lowest
+    node id) is ordered from its own pseudo-peripheral vertex, found from the
+    component's lowest-degree node."""
+    ptr, idx = node_graph(mesh)
+    n = ptr.size - 1
+    deg = np.diff(ptr)
+    done = np.zeros(n, bool)
+    parts = []
+    for s0 in range(n):
+        if done[s0]:
+            continue
+        nodes = np.flatnonzero(bfs_levels(ptr, idx, s0) >= 0)
+        done[nodes] = True
+        if nodes.size == n:
+            sptr, sidx, local = ptr, idx, nodes
+        else:                                   # the component's own CSR graph
+            remap = -np.ones(n, np.int64)
+            remap[nodes] = np.arange(nodes.size)
+            cnt = deg[nodes]
+            sptr = np.r_[0, np.cumsum(cnt)]
+            sidx = remap[np.concatenate([idx[ptr[a]:ptr[a + 1]] for a in nodes])]
+            local = nodes
+        start = int(np.argmin(np.diff(sptr)))
+        v = pseudo_peripheral_node(sptr, sidx, start)
+        perm, _, _, _, _ = symrcm(sptr, sidx, v)
+        parts.append(local[perm])
+    return np.concatenate(parts)
 
 
 def renumber(mesh: Mesh, perm: np.ndarray) -> Mesh:

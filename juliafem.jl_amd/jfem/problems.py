@@ -71,6 +71,7 @@ class Assembly:
 
     def empty(self):
         self.K = None
+        self.Kg = None
         self.f = None
         self.C1 = None
         self.C2 = None
@@ -165,7 +166,10 @@ def _elasticity_arrays(problem: Problem):
 
 
 def assemble(problem: Problem, time: float = 0.0):
-    """assemble!(problem, time): K (CSR, sparse(COO) pattern) and f = f_ext - f_int(u)."""
+    """assemble!(problem, time): K (CSR, sparse(COO) pattern) and f = f_ext - f_int(u).
+    With finite_strain the K is the material tangent Km(u) and, with
+    geometric_stiffness, assembly.Kg receives Kg(u) (src/problems_elasticity.jl:
+    203-451, 441-447)."""
     if not problem.assembly.isempty():
         warnings.warn("Problem assembly is not empty before assembling. This is probably causing "
                       "unexpected results. To remove old assembly, use `empty!(problem.assembly)`")
@@ -176,15 +180,26 @@ def assemble(problem: Problem, time: float = 0.0):
         return None
     if isinstance(problem.properties, Dirichlet):
         return _assemble_dirichlet(problem, time)
+    props = problem.properties
+    if props.formulation != "continuum":
+        raise RuntimeError(f"formulation {props.formulation} is not on the GPU path (3-D continuum only)")
+    if props.geometric_stiffness and not props.finite_strain:
+        raise RuntimeError("geometric stiffness of the small-strain model is not on the GPU path")
     etype, conn, E, nu, faces, ftype, tr = _elasticity_arrays(problem)
     ctx = problem.context()
     ctx.upload_mesh(problem.X, conn, etype, E, nu)
     n = 3 * problem.X.shape[0]
     ctx.upload_bcs(np.zeros(n, np.uint8), None, faces, ftype, tr)
-    vals, f = ctx.assemble(u=problem.u)
+    kg = None
+    if props.finite_strain:
+        vals, kg, f = ctx.assemble_tl(u=problem.u, geometric=props.geometric_stiffness)
+    else:
+        vals, f = ctx.assemble(u=problem.u)
     rowptr, colind = ctx.csr_pattern()
     import scipy.sparse as sp
     problem.assembly.K = sp.csr_matrix((vals, colind, rowptr), shape=(n, n))
+    if kg is not None:
+        problem.assembly.Kg = sp.csr_matrix((kg, colind, rowptr), shape=(n, n))
     problem.assembly.f = f
     return None
 
@@ -257,8 +272,12 @@ def solve(solver: LinearSolver, **options):
     ctx.upload_bcs(fixed, g, faces, ftype, tr)
     method = {"pcg": _lib.SOLVE_PCG, "mixed": _lib.SOLVE_MIXED_IR}[opts["method"]]
     prec = {"block_jacobi": _lib.PREC_BLOCK_JACOBI, "learned": _lib.PREC_LEARNED}[opts["precond"]]
+    op, op_outer = opts["op"], opts.get("op_outer", opts["op"])
+    if body.properties.finite_strain:
+        # K = Km + Kg lives in the context's CSR operator (src/solvers.jl:286-292)
+        op = op_outer = "csr"
     try:
-        u, la, st = ctx.solve(f=body.assembly.f, method=method, op=opts["op"], op_outer=opts.get("op_outer", opts["op"]),
+        u, la, st = ctx.solve(f=body.assembly.f, method=method, op=op, op_outer=op_outer,
                               precond=prec, rtol=opts["rtol"], inner_rtol=opts.get("inner_rtol", 1e-4),
                               want_la=True)
     except _lib.JfemError as e:
@@ -277,3 +296,43 @@ def run(solver: LinearSolver, time: float = 0.0, **options):
     for p in solver.field_problems():
         p.u = u.copy()
     return u, la
+
+
+class NonlinearSolver(LinearSolver):
+    """Solver{Nonlinear} (src/solvers.jl:536-552): min/max iterations and the
+    convergence tolerance on the norm of the solution change."""
+
+    def __init__(self, name: str = "solver", *problems: Problem):
+        super().__init__(name, *problems)
+        self.time = 0.0
+        self.iteration = 0
+        self.min_iterations = 1
+        self.max_iterations = 10
+        self.convergence_tolerance = 5.0e-5
+        self.error_if_no_convergence = True
+        self.u_norm_change = None
+
+
+def run_nonlinear(solver: NonlinearSolver, time: float = 0.0, **options):
+    """run!(::Solver{Nonlinear}) (src/solvers.jl:575-620): per iteration empty
+    and re-assemble every problem, solve the linearised system, add the
+    increment to the displacement (incremental formulation,
+    src/assembly/problems.jl:288-310) and stop when |du| falls below the
+    tolerance (has_converged, :558-572).  Returns True on convergence."""
+    for solver.iteration in range(1, solver.max_iterations + 1):
+        for p in solver.problems:
+            p.assembly.empty()
+            assemble(p, time)
+        du, la = solve(solver, **options)
+        change = 0.0
+        for p in solver.field_problems():
+            prev = np.zeros_like(du) if p.u is None else p.u
+            p.u = prev + du
+            change = max(change, float(np.linalg.norm(du)))
+            trivial = np.isclose(np.linalg.norm(p.u), 0.0)
+        solver.u_norm_change = change
+        if solver.iteration >= solver.min_iterations and (change < solver.convergence_tolerance or trivial):
+            return True
+    if solver.error_if_no_convergence:
+        raise RuntimeError(f"nonlinear iteration did not converge in {solver.iteration} iterations!")
+    return False

@@ -435,6 +435,139 @@ def nh_newton(mesh, load_steps: int = 5, rtol: float = 1e-10, max_newton: int = 
     return u, counts
 
 
+# ---------------------------------------------- total Lagrangian (SVK) --
+def tl_element(etype: str, Xe, ue, E: float, nu: float, geometric: bool = True):
+    """Finite-strain element of Problem{Elasticity} with finite_strain = true,
+    restated literally from src/problems_elasticity.jl:203-451: at each Gauss
+    point gradu = sum u_k dN_k' (grad!), F = I + gradu, the Green-Lagrange
+    strain 1/2 (gradu + gradu' + gradu' gradu) in Voigt order with engineering
+    shear (to_voigt!, :189-197), BL(F) (:270-289), the isotropic D (:310-317),
+    stress_vec = D strain_vec, Km += w BL' D BL (:369-375), with
+    geometric_stiffness Kg += w BNL' S3 BNL (:378-404), f_int += w BL' S
+    (:406-409); w = weight * detJ in the reference configuration (:248).
+    Returns (Km, Kg, f_int)."""
+    Xe = np.asarray(Xe, np.float64)
+    ue = np.asarray(ue, np.float64).reshape(-1, 3)
+    npe = Xe.shape[0]
+    nd = 3 * npe
+    w_all, xi_all = volume_rule(etype)
+    la = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+    mu = E / (2.0 * (1.0 + nu))
+    D = np.zeros((6, 6))
+    D[:3, :3] = la
+    D[0, 0] = D[1, 1] = D[2, 2] = 2 * mu + la
+    D[3, 3] = D[4, 4] = D[5, 5] = mu
+    Km = np.zeros((nd, nd))
+    Kg = np.zeros((nd, nd))
+    fint = np.zeros(nd)
+    for w0, xi in zip(w_all, xi_all):
+        _, dNr = basis(etype, xi)                  # npe x 3, d/dxi
+        J = dNr.T @ Xe                             # J[a][b] = sum_i dN_i/dxi_a X_i,b
+        detJ = np.linalg.det(J)
+        dN = np.linalg.solve(J, dNr.T)             # 3 x npe, dN[k, i] = dN_i/dX_k
+        w = w0 * detJ
+        gradu = ue.T @ dN.T                        # gradu[c, k] = sum_i u_i,c dN_i/dX_k
+        F = np.eye(3) + gradu
+        strain = 0.5 * (gradu + gradu.T + gradu.T @ gradu)
+        sv = np.array([strain[0, 0], strain[1, 1], strain[2, 2],
+                       2 * strain[0, 1], 2 * strain[1, 2], 2 * strain[0, 2]])
+        BL = np.zeros((6, nd))
+        for i in range(npe):
+            d1, d2, d3 = dN[0, i], dN[1, i], dN[2, i]
+            for c in range(3):
+                BL[0, 3 * i + c] = F[c, 0] * d1
+                BL[1, 3 * i + c] = F[c, 1] * d2
+                BL[2, 3 * i + c] = F[c, 2] * d3
+                BL[3, 3 * i + c] = F[c, 0] * d2 + F[c, 1] * d1
+                BL[4, 3 * i + c] = F[c, 1] * d3 + F[c, 2] * d2
+                BL[5, 3 * i + c] = F[c, 2] * d1 + F[c, 0] * d3
+        stress = D @ sv
+        Km += w * (BL.T @ D @ BL)
+        if geometric:
+            BNL = np.zeros((9, nd))
+            for i in range(npe):
+                for c in range(3):
+                    BNL[3 * c:3 * c + 3, 3 * i + c] = dN[:, i]
+            S = np.array([[stress[0], stress[3], stress[5]],
+                          [stress[3], stress[1], stress[4]],
+                          [stress[5], stress[4], stress[2]]])
+            S3 = np.kron(np.eye(3), S)
+            Kg += w * (BNL.T @ S3 @ BNL)
+        fint += w * (BL.T @ stress)
+    return Km, Kg, fint
+
+
+def _coo_csr(n, I, J, V):
+    m = I.size
+    rowptr = np.empty(n + 1, np.int64)
+    colind = np.empty(m, np.int64)
+    vals = np.empty(m)
+    perm = np.empty(m, np.int64)
+    nnz = int(lib().orc_coo_to_csr(n, m, _p(_i64(I)), _p(_i64(J)), _p(_f64(V)), _p(rowptr), _p(colind),
+                                   _p(vals), _p(perm)))
+    return rowptr, colind[:nnz].copy(), vals[:nnz].copy()
+
+
+def tl_assemble(mesh, u, geometric: bool = True):
+    """(Km CSR, Kg CSR or None, f = -f_int(u)) of the whole mesh: the element
+    loop above, COO triplets in ascending element order and sparse(COO)
+    (src/sparse/sparse.jl:54, 121-132); f_ext is zero (no volume load)."""
+    conn = np.asarray(mesh.conn, np.int64)
+    n_el, npe = conn.shape
+    n = mesh.n_dofs
+    U = np.asarray(u, np.float64).reshape(-1, 3)
+    dofs = (3 * conn[:, :, None] + np.arange(3)[None, None, :]).reshape(n_el, -1)
+    nd = dofs.shape[1]
+    I = np.repeat(dofs, nd, axis=1).ravel()
+    J = np.tile(dofs, (1, nd)).ravel()
+    Vm = np.empty((n_el, nd * nd))
+    Vg = np.empty((n_el, nd * nd)) if geometric else None
+    f = np.zeros(n)
+    for e in range(n_el):
+        Km, Kg, fi = tl_element(mesh.etype, mesh.X[conn[e]], U[conn[e]], float(mesh.E[e]), float(mesh.nu[e]),
+                                geometric)
+        Vm[e] = Km.ravel()
+        if geometric:
+            Vg[e] = Kg.ravel()
+        np.add.at(f, dofs[e], -fi)
+    Kmc = _coo_csr(n, I, J, Vm.ravel())
+    Kgc = _coo_csr(n, I, J, Vg.ravel()) if geometric else None
+    return Kmc, Kgc, f
+
+
+def tl_newton(mesh, load_steps: int = 1, rtol: float = 1e-10, max_newton: int = 30, geometric: bool = True):
+    """run!(::Solver{Nonlinear}) on the assembled TL system (src/solvers.jl:
+    575-620): per load step, assemble K = Km + Kg and f = lf f_ext - f_int(u),
+    solve the eliminated system directly (SuperLU), u += du, until
+    |f| <= rtol |lf f_ext| on the free DOFs.  Returns (u, Newton counts)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+    fixed = mesh.fixed_mask().astype(bool)
+    free = ~fixed
+    fext = load_vector(mesh)
+    u = np.zeros(mesh.n_dofs)
+    counts = []
+    n = mesh.n_dofs
+    for k in range(1, load_steps + 1):
+        lf = k / load_steps
+        ref = np.linalg.norm(lf * fext[free])
+        for it in range(max_newton + 1):
+            Km, Kg, f = tl_assemble(mesh, u, geometric)
+            R = lf * fext + f
+            if np.linalg.norm(R[free]) <= rtol * ref:
+                break
+            if it == max_newton:
+                raise RuntimeError("oracle TL Newton did not converge")
+            K = sp.csr_matrix((Km[2], Km[1], Km[0]), shape=(n, n))
+            if Kg is not None:
+                K = K + sp.csr_matrix((Kg[2], Kg[1], Kg[0]), shape=(n, n))
+            du = np.zeros(n)
+            du[free] = sla.spsolve(K[free][:, free].tocsc(), R[free])
+            u += du
+        counts.append(it)
+    return u, counts
+
+
 # ------------------------------------------------------------- two-level PCG --
 def coarse_aggregates(mesh, coarse_nodes: int):
     """Node aggregates as the library forms them: a coordinate grid with about

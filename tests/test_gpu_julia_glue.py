@@ -1,0 +1,159 @@
+"""Replays, through ctypes, the exact C-ABI call sequence of the Julia glue
+(juliafem.jl_amd/julia/JuliaFEMHIPExt.jl), with its arguments laid out as
+Julia lays them out (3 x N column-major coordinates, npe x n_el column-major
+1-based connectivity, 1-based CSR), and checks every result against the
+oracle:
+
+* assemble_elements! with a nonzero displacement field: the CSR triplets
+  and f = f_ext - f_int(u) = -K u (src/problems_elasticity.jl:239-242,
+  432-433);
+* solve!(..., Val{1}): the K x probe that decides whether the device K is
+  the K handed in, then the eliminated solve with prescribed values and the
+  reactions (src/solvers.jl:192-216);
+* initialize_backend / solve_backend! of the new API on a Tet4 body with
+  Tri3 load faces and node ids that are not contiguous (renumbered in
+  ascending id order, ext/JuliaFEMCUDAExt.jl:86-217).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from jfem.mesh import Mesh, box_tet4, box_tet10
+
+pytestmark = pytest.mark.gpu
+
+OP_CSR, OP_MATFREE, FP64 = 0, 2, 0
+
+
+@pytest.fixture(scope="module")
+def glue():
+    import jfem
+    from jfem import _lib
+    lib = _lib.load()
+    return jfem, _lib, lib
+
+
+def p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def new_ctx(lib):
+    h = C.c_void_p()
+    assert lib.jfm_create(0, C.byref(h)) == 0
+    return h
+
+
+def default_opts(_lib, rtol):
+    # JuliaFEMHIPExt.default_opts
+    return _lib.SolveOpts(1, OP_MATFREE, OP_MATFREE, 0, rtol, 1e-2, 200000, 50, 32, 0, 0, 0.0, 2048)
+
+
+def test_assemble_elements_and_solve(glue, oracle):
+    jfem, _lib, lib = glue
+    m = box_tet10(20, 3, 3, seed=11, jitter=0.1, E_sigma=0.4)
+    N, n = m.n_nodes, m.n_dofs
+    X = np.asfortranarray(m.X.T)                     # Julia 3 x N
+    conn = np.asfortranarray((m.conn + 1).T)         # Julia 10 x n_el, 1-based
+    u0 = 1e-4 * np.random.default_rng(1).standard_normal(n)   # the problem's displacement field
+    c = new_ctx(lib)
+    try:
+        assert lib.jfm_upload_mesh(c, N, p(X), m.n_el, 10, p(conn), p(m.E), p(m.nu), 1) == 0
+        nnz = C.c_int64(0)
+        assert lib.jfm_csr_pattern(c, C.byref(nnz), None, None, 1) == 0
+        rowptr = np.empty(n + 1, np.int64)
+        colind = np.empty(nnz.value, np.int64)
+        vals = np.empty(nnz.value)
+        f = np.empty(n)
+        assert lib.jfm_csr_pattern(c, C.byref(nnz), p(rowptr), p(colind), 1) == 0
+        assert lib.jfm_assemble(c, p(u0), p(vals), p(f)) == 0
+        csr = oracle.assemble_csr(m)
+        assert np.array_equal(rowptr - 1, csr[0]) and np.array_equal(colind - 1, csr[1])
+        s = np.abs(csr[2]).max()
+        assert np.abs(vals - csr[2]).max() <= 1e-12 * s
+        fint = oracle.spmv(csr, u0)
+        assert np.abs(f + fint).max() <= 1e-12 * np.abs(fint).max()
+
+        # solve!: the probe (no fixed DOFs, CSR operator) against K x
+        rng = np.random.default_rng(20)
+        for _ in range(2):
+            x = rng.standard_normal(n)
+            y = np.empty(n)
+            assert lib.jfm_upload_bcs(c, p(np.zeros(n, np.uint8)), None, 0, 6, None, None, 1) == 0
+            assert lib.jfm_apply(c, OP_CSR, FP64, p(x), p(y)) == 0
+            Kx = oracle.spmv(csr, x)
+            assert np.linalg.norm(Kx - y) <= 1e-12 * np.linalg.norm(Kx)
+        # a K with an extra term (e.g. Kg) fails the probe
+        Kx_mod = Kx + 1e-6 * np.abs(Kx).max() * x
+        assert np.linalg.norm(Kx_mod - y) > 1e-12 * np.linalg.norm(Kx_mod)
+
+        # the solve with prescribed values on the clamp: u_x = 1e-3 there
+        fixed = m.fixed_mask()
+        g = np.where((fixed == 1) & (np.arange(n) % 3 == 0), 1e-3, 0.0)
+        f_solver = oracle.load_vector(m) - fint      # the solver's f: surface load + volume f
+        assert lib.jfm_upload_bcs(c, p(fixed), p(g), 0, 6, None, None, 1) == 0
+        # solve!(...; rtol = 1e-10): with prescribed values this small problem's
+        # error exceeds its residual (4e-8 at rtol 1e-8), so the 1e-8 bar
+        # against the direct solve needs the tighter residual
+        opts = default_opts(_lib, 1e-10)
+        st = _lib.SolveStats()
+        u = np.empty(n)
+        la = np.empty(n)
+        assert lib.jfm_solve(c, C.byref(opts), p(f_solver), p(u), p(la), C.byref(st)) == 0, \
+            lib.jfm_last_error(c)
+        ud, lad, _ = oracle.solve_direct_lapack(csr, fixed, g, f_solver)
+        assert st.converged == 1 and st.relres <= 1e-10
+        assert np.linalg.norm(u - ud) / np.linalg.norm(ud) <= 1e-8
+        assert np.array_equal(u[fixed == 1], g[fixed == 1])
+        fx = fixed == 1
+        assert np.abs(la[fx] - lad[fx]).max() <= 1e-6 * np.abs(lad[fx]).max()
+    finally:
+        lib.jfm_destroy(c)
+
+
+def test_backend_initialize_and_solve(glue, oracle):
+    """initialize_backend + solve_backend!: nodes renumbered in ascending id
+    order; Tri3 faces carry the traction; the residual is absolute."""
+    jfem, _lib, lib = glue
+    m = box_tet4(16, 3, 3, seed=4, jitter=0.1, E_sigma=0.3)
+    ids = 7 + 3 * np.arange(m.n_nodes)[::-1]         # the physics' node ids: sparse, descending
+    # initialize_backend: node_list = sort(unique(ids)), node_map
+    node_list = np.sort(ids)
+    node_map = {int(nd): k for k, nd in enumerate(node_list)}
+    new = np.array([node_map[int(nd)] for nd in ids])  # old index -> renumbered
+    N = m.n_nodes
+    X = np.zeros((3, N), order="F")
+    X[:, new] = m.X.T
+    conn = np.asfortranarray((new[m.conn] + 1).T)
+    fixed = np.zeros(3 * N, np.uint8)
+    for nd in m.clamp_nodes:
+        fixed[3 * new[nd]:3 * new[nd] + 3] = 1
+    fconn = np.asfortranarray((new[m.faces] + 1).T)   # 3 x n_faces
+    tr = np.asfortranarray(m.face_traction().T)       # 3 x n_faces
+    c = new_ctx(lib)
+    try:
+        assert lib.jfm_upload_mesh(c, N, p(X), m.n_el, 4, p(conn), p(m.E), p(m.nu), 1) == 0
+        assert lib.jfm_upload_bcs(c, p(fixed), p(np.zeros(3 * N)), m.faces.shape[0], 3, p(fconn), p(tr), 1) == 0
+        # solve_backend!
+        f = np.empty(3 * N)
+        assert lib.jfm_assemble(c, None, None, p(f)) == 0
+        R0 = np.linalg.norm(f[fixed == 0])
+        newton_tol = 1e-10
+        rtol = min(max(newton_tol / R0, 1e-13), 0.5)
+        opts = default_opts(_lib, rtol)
+        st = _lib.SolveStats()
+        u = np.zeros(3 * N)
+        assert lib.jfm_solve(c, C.byref(opts), None, p(u), None, C.byref(st)) == 0
+        # the oracle on the renumbered mesh
+        mr = Mesh(X=np.ascontiguousarray(X.T), conn=new[m.conn], etype="Tet4", E=m.E, nu=m.nu,
+                  faces=new[m.faces], ftype="Tri3", clamp_nodes=np.sort(new[m.clamp_nodes]))
+        fr = oracle.load_vector(mr)
+        assert np.abs(f - fr).max() <= 1e-14 * np.abs(fr).max()
+        # Tri3 under constant traction: each corner carries a third of its face's force
+        assert np.allclose(fr.reshape(-1, 3).sum(0), [0, 0, -1.0], atol=1e-12)
+        csr = oracle.assemble_csr(mr)
+        ud, _, _ = oracle.solve_direct_lapack(csr, fixed, np.zeros(3 * N), fr)
+        assert st.relres * R0 <= newton_tol
+        assert np.linalg.norm(u - ud) / np.linalg.norm(ud) <= 1e-8
+    finally:
+        lib.jfm_destroy(c)

@@ -11,8 +11,8 @@ import os
 import numpy as np
 import pytest
 
-from jfem.mesh import (TET10_EDGES, box_tet10, mesh_from_groups, node_bandwidth, rcm_order, read_gmsh,
-                       renumber)
+from jfem.mesh import (TET10_EDGES, Mesh, box_tet10, mesh_from_groups, node_bandwidth, node_graph, rcm_order,
+                       read_gmsh, renumber, symrcm)
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
@@ -88,3 +88,41 @@ def test_rcm_renumbering(oracle):
                                            oracle.load_vector(mm))[0].reshape(-1, 3)
     u0, ur = solve(shuffled), solve(r)
     assert np.abs(ur - u0[perm]).max() <= 1e-8 * np.abs(u0).max()
+
+
+def test_symrcm_known_answer():
+    """symrcm (src/graph/graph_ordering.jl:38-90) worked by hand on a small
+    graph: from vertex 0 the neighbours are visited in ascending degree, ties
+    in the adjacency list's order, and the order is reversed."""
+    edges = [(0, 1), (0, 2), (0, 3), (1, 4), (1, 5), (3, 4)]
+    n = 6
+    adj = [sorted({b for a_, b in edges if a_ == a} | {a_ for a_, b in edges if b == a}) for a in range(n)]
+    ptr = np.r_[0, np.cumsum([len(a) for a in adj])]
+    idx = np.concatenate([np.array(a, np.int64) for a in adj])
+    perm, inv, deg, edge, dist = symrcm(ptr, idx, 0)
+    # degrees 3,3,1,2,2,1: from 0 the order is 2 (1), 3 (2), 1 (3); then 3 adds 4, 1 adds 5
+    assert perm.tolist() == [5, 4, 1, 3, 2, 0]
+    assert np.array_equal(inv[perm], np.arange(n))
+    assert deg.tolist() == [3, 3, 1, 2, 2, 1]
+    assert dist.tolist() == [0, 1, 1, 1, 2, 2]
+    assert [edge[k] for k in (1, 2, 3, 4, 5)] == [0, 0, 0, 3, 1]
+    with pytest.raises(ValueError):
+        symrcm(np.array([0, 1, 2, 2]), np.array([1, 0]), 0)   # vertex 2 unreachable
+
+
+def test_rcm_is_a_permutation_and_handles_components():
+    a = box_tet10(4, 2, 2, seed=1)
+    b = box_tet10(3, 1, 1, seed=2)
+    two = Mesh(X=np.vstack([a.X, b.X + 20.0]), conn=np.vstack([a.conn, b.conn + a.n_nodes]), etype="Tet10",
+               E=np.r_[a.E, b.E], nu=np.r_[a.nu, b.nu], faces=a.faces, ftype="Tri6", clamp_nodes=a.clamp_nodes)
+    shuffled = renumber(two, np.random.default_rng(4).permutation(two.n_nodes))
+    perm = rcm_order(shuffled)
+    assert np.array_equal(np.sort(perm), np.arange(two.n_nodes))
+    r = renumber(shuffled, perm)
+    assert node_bandwidth(r) < node_bandwidth(shuffled) / 3
+    # each component occupies one contiguous range of the new ids
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(perm.size)
+    first = np.isin(np.arange(two.n_nodes), np.unique(shuffled.conn[:a.n_el]))
+    ids = np.sort(inv[np.flatnonzero(first)])
+    assert ids[-1] - ids[0] + 1 == ids.size
