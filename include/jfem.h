@@ -170,6 +170,20 @@ int jfm_assemble_tl(jfm_ctx *ctx, int32_t geometric, const double *u, double *va
  * arithmetic of the operator. */
 int jfm_apply(jfm_ctx *ctx, int32_t op, int32_t precision, const double *x, double *y);
 
+/* K.u that allocates nothing: no counterpart in the reference.  For callers
+ * that cannot allocate device memory (code under HIP graph capture, serving
+ * loops with fixed memory).  x and y are DEVICE pointers (n_dofs doubles);
+ * the work is enqueued on `stream` (a hipStream_t; NULL = the context's
+ * stream) and the call returns without synchronising.  op is
+ * JFM_OP_MATFREE (two-phase form) or JFM_OP_EBE (after the element matrices
+ * exist, i.e. after one jfm_apply or jfm_assemble with it).  The element
+ * results (3 npe n_el values) and, in FP32, the converted vectors live in
+ * one 64 MiB device array per GPU held until the process ends: meshes that
+ * need more (e.g. more than 279,620 Tet10 elements in FP64) get JFM_EINVAL.
+ * Every call on a GPU shares that array, so calls that overlap on one GPU
+ * race.  Same result as jfm_apply up to the order of the nodal sums. */
+int jfm_apply_noalloc(jfm_ctx *ctx, int32_t op, int32_t precision, const double *x, double *y, void *stream);
+
 /* Average device times (ms) over `reps` back-to-back applications on
  * device-resident vectors, timed with HIP events on the context's stream:
  * avg_ms[0] = whole K.u, avg_ms[1] = its first kernel (SpMV; the element

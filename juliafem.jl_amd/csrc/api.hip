@@ -108,6 +108,10 @@ extern "C" int jfm_create(int device, jfm_ctx **out) {
         return JFM_ENOMEM;
     }
     std::memset(ctx->S_host, 0, sizeof(PcgScalars));
+    if (int rc = jfem_noalloc_init(ctx)) {  // device address of the no-allocation entry's scratch
+        jfm_destroy(ctx);
+        return rc;
+    }
     *out = ctx;
     return JFM_OK;
 }

@@ -788,6 +788,22 @@ hipError_t launch_elem_only(jfm_ctx *ctx, int op, const T *x, const int *done) {
     return hipGetLastError();
 }
 
+// Element phase of the linear operator (JFM_OP_MATFREE or JFM_OP_EBE) into the
+// incidence slots `ye` (3 npe n_el values), a buffer the caller provides.
+template <typename T>
+hipError_t launch_elem_into(jfm_ctx *ctx, int op, const T *x, T *ye) {
+    if (op != JFM_OP_MATFREE && op != JFM_OP_EBE) return hipErrorInvalidValue;
+    if (op == JFM_OP_EBE && !ctx->Ke) return hipErrorInvalidValue;
+    if (ctx->n_el == 0) return hipSuccess;
+    switch (ctx->npe) {
+    case 10: launch_elem_phase<10, T>(ctx, op, x, ye, nullptr); break;
+    case 4: launch_elem_phase<4, T>(ctx, op, x, ye, nullptr); break;
+    case 8: launch_elem_phase<8, T>(ctx, op, x, ye, nullptr); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_two_phase(jfm_ctx *ctx, int op, const T *x, T *y, const T *dotv, double *part, const int *done,
                             int *npart) {
@@ -798,6 +814,8 @@ hipError_t launch_two_phase(jfm_ctx *ctx, int op, const T *x, T *y, const T *dot
 }
 
 template hipError_t launch_elem_only<double>(jfm_ctx *, int, const double *, const int *);
+template hipError_t launch_elem_into<double>(jfm_ctx *, int, const double *, double *);
+template hipError_t launch_elem_into<float>(jfm_ctx *, int, const float *, float *);
 template hipError_t launch_elem_only<float>(jfm_ctx *, int, const float *, const int *);
 template hipError_t launch_two_phase<double>(jfm_ctx *, int, const double *, double *, const double *, double *, const int *, int *);
 template hipError_t launch_two_phase<float>(jfm_ctx *, int, const float *, float *, const float *, double *, const int *, int *);

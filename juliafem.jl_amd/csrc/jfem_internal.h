@@ -295,6 +295,8 @@ struct jfm_ctx {
 int jfem_ensure_workspaces(jfm_ctx *ctx);                                   // vectors + block Jacobi
 int jfem_apply(jfm_ctx *ctx, int op, const double *x, double *y);          // masked FP64 K.x (device)
 int jfem_dot(jfm_ctx *ctx, const double *a, const double *b, int64_t n, double *out);
+// noalloc.hip: looks up the scratch array's device address (once per device)
+int jfem_noalloc_init(jfm_ctx *ctx);
 
 // -------------------------------------------------------------- helpers ----
 int jfm_fail(jfm_ctx *ctx, int code, const std::string &msg);
@@ -325,6 +327,8 @@ hipError_t launch_block_jacobi(jfm_ctx *ctx, double *Minv64, float *Minv32);
 // dot(y, dotv) per block of the gather; returns the partial count in *npart.
 template <typename T>
 hipError_t launch_elem_only(jfm_ctx *ctx, int op, const T *x, const int *done);
+template <typename T>
+hipError_t launch_elem_into(jfm_ctx *ctx, int op, const T *x, T *ye);
 template <typename T>
 hipError_t launch_gather(jfm_ctx *ctx, const T *ye, const uint8_t *fixed, T *y, const T *dotv, double *part,
                          const int *done, int *npart);

@@ -24,7 +24,7 @@ ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ENOTSPD", -4: "ENOCONV", -5: "ESTATE", 
 
 # Exported symbols (must match include/jfem.h)
 SYMBOLS = ("jfm_create", "jfm_destroy", "jfm_last_error", "jfm_get_info", "jfm_upload_mesh",
-           "jfm_upload_bcs", "jfm_csr_pattern", "jfm_assemble", "jfm_assemble_tl", "jfm_apply", "jfm_time_apply",
+           "jfm_upload_bcs", "jfm_csr_pattern", "jfm_assemble", "jfm_assemble_tl", "jfm_apply", "jfm_apply_noalloc", "jfm_time_apply",
            "jfm_solve", "jfm_solve_history", "jfm_precond_set_blocks", "jfm_learned_blob_bytes", "jfm_learned_n_params",
            "jfm_learned_set_weights", "jfm_learned_get_weights", "jfm_learned_features", "jfm_learned_outputs",
            "jfm_learned_grad", "jfm_nl_residual", "jfm_solve_nonlinear", "jfm_postprocess", "jfm_heat_solve")
@@ -99,6 +99,7 @@ def load():
     lib.jfm_assemble.argtypes = [_P, _P, _P, _P]
     lib.jfm_assemble_tl.argtypes = [_P, C.c_int32, _P, _P, _P, _P]
     lib.jfm_apply.argtypes = [_P, C.c_int32, C.c_int32, _P, _P]
+    lib.jfm_apply_noalloc.argtypes = [_P, C.c_int32, C.c_int32, _P, _P, _P]
     lib.jfm_time_apply.argtypes = [_P, C.c_int32, C.c_int32, C.c_int32, _P]
     lib.jfm_solve.argtypes = [_P, C.POINTER(SolveOpts), _P, _P, _P, C.POINTER(SolveStats)]
     lib.jfm_solve_history.argtypes = [_P, _P, _P, C.c_int32]
