@@ -109,7 +109,7 @@ typedef struct {
     int32_t npe, affine;
     int64_t device_bytes;  /* bytes currently allocated by the context */
     /* patch form of the matrix-free operator (0 until it is first used):
-     * patches of 512 elements, their (patch, node) entries, the nodes shared
+     * patches of 480 elements, their (patch, node) entries, the nodes shared
      * between patches and their partial sums */
     int64_t n_patches, patch_nodes, shared_nodes, shared_slots;
 } jfm_info;
@@ -191,6 +191,17 @@ int jfm_apply_noalloc(jfm_ctx *ctx, int32_t op, int32_t precision, const double 
  * nodal gather of EBE; the shared-node kernel of MATFREE; 0 for CSR); each as
  * reps back-to-back launches between two events. */
 int jfm_time_apply(jfm_ctx *ctx, int32_t op, int32_t precision, int32_t reps, double *avg_ms);
+
+/* The coarse matrix of the two-level preconditioner, before inversion (no
+ * reference counterpart; for checking the coarse level against Q' K Q): the
+ * node aggregates for coarse_nodes and A_c = Q' K Q (op JFM_OP_MATFREE or
+ * JFM_OP_CSR: the linear K; JFM_OP_TANGENT: the neo-Hookean tangent at the
+ * stored state), with Q the orthonormalised rigid-body modes of each
+ * aggregate (six columns per aggregate, in aggregate order; a dropped mode
+ * has a zero row and column and a unit diagonal).  *nc receives the order;
+ * Ac (nc x nc, row major) and node_agg (n_nodes) are written when not NULL.
+ * The next solve rebuilds the coarse level. */
+int jfm_coarse_matrix(jfm_ctx *ctx, int32_t op, int32_t coarse_nodes, int64_t *nc, double *Ac, int32_t *node_agg);
 
 /* ---- solve (solve!(solver), src/solvers.jl:192-216, 273-344; and
  * solve_backend!, ext/JuliaFEMCUDAExt.jl:886-916) ---------------------

@@ -600,6 +600,16 @@ extern "C" int jfm_apply(jfm_ctx *ctx, int32_t op, int32_t precision, const doub
     return sync(ctx);
 }
 
+extern "C" int jfm_coarse_matrix(jfm_ctx *ctx, int32_t op, int32_t coarse_nodes, int64_t *nc, double *Ac,
+                                 int32_t *node_agg) {
+    if (!ctx || !nc || coarse_nodes <= 0) return JFM_EINVAL;
+    if (!ctx->has_mesh) return jfm_fail(ctx, JFM_ESTATE, "no mesh");
+    if (op != JFM_OP_MATFREE && op != JFM_OP_CSR && op != JFM_OP_TANGENT)
+        return jfm_fail(ctx, JFM_EINVAL, "op must be MATFREE, CSR or TANGENT");
+    JFM_HIP(ctx, hipSetDevice(ctx->device));
+    return jfem_coarse_matrix(ctx, op, coarse_nodes, nc, Ac, node_agg);
+}
+
 extern "C" int jfm_time_apply(jfm_ctx *ctx, int32_t op, int32_t precision, int32_t reps, double *avg_ms) {
     if (!ctx || !avg_ms || reps <= 0) return JFM_EINVAL;
     if (!ctx->has_mesh) return jfm_fail(ctx, JFM_ESTATE, "no mesh");

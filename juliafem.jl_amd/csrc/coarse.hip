@@ -50,6 +50,7 @@
 #include <vector>
 
 #include "jfem_device.h"
+#include "mf_element.h"
 
 using namespace jfem;
 
@@ -170,6 +171,139 @@ __global__ __launch_bounds__(256) void k_coarse_fix(int nc, double *__restrict__
     Ac[(size_t)j * nc + i] = s;
 }
 
+// ---- A_c = Q' K Q element by element (the linear operator) ----
+// Element e, listed as m, holds the distinct aggregates h_0 < ... < h_{k-1}
+// of its nodes (glist[gptr[m] ..]).  Item (m, a), mode i (thread 6 item + i,
+// so that the six threads of an item load the same element data in one
+// request per wave):
+// w = K_e Q_{g,i}|_e with g = h_a (the mode on e's nodes in g, fixed DOFs
+// zero), then for every b the six numbers
+//     c_j = Q_{h_b,j}|_e' w = sum_k Rinv_{h_b}[k][j] t_k,
+//     t = sum over e's nodes in h_b of [w ; d x w]   (fixed DOFs zero),
+// the contribution of e to A_c[(h_b, j), (g, i)].  They go to entry
+// gpos[gq[m] + a k + b] of the pair (h_b, g), whose entries k_gal_sum adds
+// in ascending element order: deterministic, no atomics.
+// Elements whose nodes all lie in one aggregate and hold no fixed DOF are
+// not listed: the mode is a rigid motion of the whole element, which K_e
+// annihilates (zero strain), so their contribution is zero.
+template <int NPE, bool AFF>
+__global__ __launch_bounds__(256) void k_gal_elem(int n_items, const int32_t *__restrict__ item,
+                                                 const int32_t *__restrict__ gel, const int32_t *__restrict__ gptr,
+                                                 const int32_t *__restrict__ glist, const int32_t *__restrict__ gq,
+                                                 const int32_t *__restrict__ gpos, int n_el,
+                                                 const int32_t *__restrict__ conn, const double *__restrict__ gf,
+                                                 const int32_t *__restrict__ node_agg, const double *__restrict__ cd,
+                                                 const double *__restrict__ Rinv, const uint8_t *__restrict__ fixed,
+                                                 double *__restrict__ C) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= 6 * n_items) return;
+    const int k = tid / 6, i = tid - 6 * k;
+    const int it = item[k], m = it >> 4, a = it & 15;
+    const int e = gel[m], g0 = gptr[m], ke = gptr[m + 1] - g0;
+    const int g = glist[g0 + a];
+    double q[6];
+#pragma unroll
+    for (int kk = 0; kk < 6; ++kk) q[kk] = Rinv[36 * (size_t)g + kk * 6 + i];
+    int nd[NPE], na[NPE];
+    double w[NPE][3];
+    mf_element_x<NPE, double, AFF>(
+        n_el, e, gf,
+        [&](int l, double (&v)[3]) {
+            const int an = conn[(size_t)l * n_el + e];
+            nd[l] = an;
+            na[l] = node_agg[an];
+            const double *d = cd + 3 * (size_t)an;
+            const double o[3] = {q[0] + q[4] * d[2] - q[5] * d[1], q[1] + q[5] * d[0] - q[3] * d[2],
+                                 q[2] + q[3] * d[1] - q[4] * d[0]};
+#pragma unroll
+            for (int c = 0; c < 3; ++c) v[c] = (na[l] == g && !fixed[3 * (size_t)an + c]) ? o[c] : 0.0;
+        },
+        [&](int l, double f0, double f1, double f2) {
+            w[l][0] = f0;
+            w[l][1] = f1;
+            w[l][2] = f2;
+        });
+#pragma unroll
+    for (int l = 0; l < NPE; ++l)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            if (fixed[3 * (size_t)nd[l] + c]) w[l][c] = 0.0;
+    for (int b = 0; b < ke; ++b) {
+        const int h = glist[g0 + b];
+        double t[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int l = 0; l < NPE; ++l) {
+            if (na[l] != h) continue;
+            const double *d = cd + 3 * (size_t)nd[l];
+            const double y0 = w[l][0], y1 = w[l][1], y2 = w[l][2];
+            t[0] += y0;
+            t[1] += y1;
+            t[2] += y2;
+            t[3] += d[1] * y2 - d[2] * y1;
+            t[4] += d[2] * y0 - d[0] * y2;
+            t[5] += d[0] * y1 - d[1] * y0;
+        }
+        const double *R = Rinv + 36 * (size_t)h;
+        double *o = C + 36 * (size_t)gpos[gq[m] + a * ke + b] + 6 * i;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            double s = 0.0;
+#pragma unroll
+            for (int kk = 0; kk < 6; ++kk) s += R[kk * 6 + j] * t[kk];
+            o[j] = s;
+        }
+    }
+}
+
+// One block per unordered aggregate pair {h, g} (h <= g), with its two
+// ordered pairs' entry ranges.  Entry value l (i = l / 6, j = l % 6) of the
+// pair (h, g) is A_c[(h, j), (g, i)]; the pair (g, h) holds the transposed
+// value at 6 j + i.  Thread t < 1008 takes value l = t % 36 of every 28th
+// entry from t / 36 on (eight loads in flight), the 28 group sums are added
+// in group order, and the mean of the two orientations goes to both symmetric
+// places, so A_c is symmetric by construction.  A zero diagonal (a mode
+// dropped from its aggregate) becomes 1.
+constexpr int kGalGroups = 28;
+__device__ __forceinline__ double gal_range_sum(const double *__restrict__ C, int r0, int r1, int grp, int l) {
+    double s = 0.0;
+    int r = r0 + grp;
+    for (; r + 7 * kGalGroups < r1; r += 8 * kGalGroups) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = C[36 * (size_t)(r + u * kGalGroups) + l];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += v[u];
+    }
+    for (; r < r1; r += kGalGroups) s += C[36 * (size_t)r + l];
+    return s;
+}
+
+__global__ __launch_bounds__(1024) void k_gal_sum(const int32_t *__restrict__ upair, const int32_t *__restrict__ pptr,
+                                                const double *__restrict__ C, double *__restrict__ Ac, int nc) {
+    __shared__ double sh[2][kGalGroups][36];
+    const int u = blockIdx.x, t = threadIdx.x;
+    const int P = upair[4 * u], Pt = upair[4 * u + 1], h = upair[4 * u + 2], g = upair[4 * u + 3];
+    if (t < 36 * kGalGroups) {
+        const int grp = t / 36, l = t - 36 * grp;
+        const int i = l / 6, j = l - 6 * (l / 6);
+        sh[0][grp][l] = gal_range_sum(C, pptr[P], pptr[P + 1], grp, l);
+        sh[1][grp][l] = gal_range_sum(C, pptr[Pt], pptr[Pt + 1], grp, 6 * j + i);
+    }
+    __syncthreads();
+    if (t >= 36) return;
+    const int i = t / 6, j = t - 6 * (t / 6);
+    double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+    for (int grp = 0; grp < kGalGroups; ++grp) {
+        s1 += sh[0][grp][t];
+        s2 += sh[1][grp][t];
+    }
+    double v = 0.5 * (s1 + s2);
+    if (h == g && i == j && v == 0.0) v = 1.0;
+    Ac[(size_t)(6 * h + j) * nc + 6 * g + i] = v;
+    Ac[(size_t)(6 * g + i) * nc + 6 * h + j] = v;
+}
+
 // ---- inverse of A_c by block Gauss-Jordan (in place, no pivoting) ----
 // A_c is SPD, so every pivot block -- a Schur complement of A_c -- is SPD and
 // needs no pivoting.  Step K (rows/columns k0..k0+bk-1), with P = A_KK:
@@ -179,47 +313,51 @@ __global__ __launch_bounds__(256) void k_coarse_fix(int nc, double *__restrict__
 // is not positive and finite flags the matrix as not positive definite.
 constexpr int kGJ = 32;  // pivot block
 
-__device__ __forceinline__ double readlane_f64(double v, int lane) {
-    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
-    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
-    return __hiloint2double(hi, lo);
-}
-
 // P^-1 of a bk x bk pivot block (padded to the identity), entry (i, j) read
-// by ld(i, j), by scalar Gauss-Jordan in the registers of wave 0: lane j
-// holds column j & 31, and column k -- the multipliers of step k -- is read
-// from lane k.  No LDS or barrier per step.  The result goes to P.  Returns
-// false (in wave 0) for a pivot that is not positive and finite.  Called by
-// the whole block (one barrier at the end).
+// by ld(i, j), by scalar Gauss-Jordan over a block of 256 threads: thread t
+// holds column j = t % kGJ of rows t / kGJ + 8 u (u < 4).  At step k the
+// threads holding column k (the multipliers) and row k (the pivot row)
+// write them to LDS (double-buffered by step parity: one barrier per step),
+// and every thread updates its four entries.  The result goes to P.
+// Returns false for a pivot that is not positive and finite (every thread
+// sees the same pivots).  Called by the whole block.
 template <typename Ld>
 __device__ __forceinline__ bool gj_pivot_inverse(int bk, Ld &&ld, double (*P)[kGJ + 1]) {
+    constexpr int RG = 256 / kGJ, R = kGJ / RG;  // row groups, rows per thread
+    __shared__ double colk[2][kGJ], rowk[2][kGJ];
+    const int t = threadIdx.x, j = t % kGJ, i0 = t / kGJ;
+    double a[R];
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+        const int i = i0 + RG * u;
+        a[u] = (i < bk && j < bk) ? ld(i, j) : (i == j ? 1.0 : 0.0);
+    }
     bool ok = true;
-    if (threadIdx.x < 64) {
-        const int j = threadIdx.x & (kGJ - 1);
-        double a[kGJ];
 #pragma unroll
-        for (int i = 0; i < kGJ; ++i) a[i] = (i < bk && j < bk) ? ld(i, j) : (i == j ? 1.0 : 0.0);
+    for (int k = 0; k < kGJ; ++k) {
+        if (k < bk) {  // uniform
+            double *ck = colk[k & 1], *rw = rowk[k & 1];
+            const int uk = k / RG, ik = k - RG * (k / RG);  // row k is row group ik, entry uk
+            if (j == k)
 #pragma unroll
-        for (int k = 0; k < kGJ; ++k) {
-            if (k < bk) {
-                const double p = readlane_f64(a[k], k);
-                ok = ok && p > 0.0 && p < INFINITY;
-                const double ip = 1.0 / p;
-                const double rk = a[k] * ip;  // a_kj / p
-                const bool piv = j == k;
+                for (int u = 0; u < R; ++u) ck[i0 + RG * u] = a[u];
+            if (i0 == ik) rw[j] = a[uk];
+            __syncthreads();
+            const double p = ck[k];
+            ok = ok && p > 0.0 && p < INFINITY;
+            const double ip = 1.0 / p;
+            const double r = rw[j] * ip;  // a_kj / p
+            const bool piv = j == k;
 #pragma unroll
-                for (int i = 0; i < kGJ; ++i) {
-                    if (i == k) continue;
-                    const double cik = readlane_f64(a[i], k);  // a_ik, read before lane k updates it
-                    a[i] = piv ? -cik * ip : a[i] - cik * rk;
-                }
-                a[k] = piv ? ip : rk;
+            for (int u = 0; u < R; ++u) {
+                const double c = ck[i0 + RG * u];
+                const double v = piv ? -c * ip : a[u] - c * r;
+                a[u] = (u == uk && i0 == ik) ? (piv ? ip : r) : v;
             }
         }
-        if (threadIdx.x < kGJ)
-#pragma unroll
-            for (int i = 0; i < kGJ; ++i) P[i][j] = a[i];
     }
+#pragma unroll
+    for (int u = 0; u < R; ++u) P[i0 + RG * u][j] = a[u];
     __syncthreads();
     return ok;
 }
@@ -240,7 +378,7 @@ __global__ __launch_bounds__(256) void k_gj_panel(int n, int k0, int bk, const d
             if (j < nbk) Anext[t] = A[(size_t)(nk0 + i) * n + nk0 + j];
         }
     __shared__ double P[kGJ][kGJ + 1];
-    bool ok = true;  // valid in wave 0
+    bool ok = true;
     if (Ppre) {
         for (int t = threadIdx.x; t < kGJ * kGJ; t += blockDim.x) P[t / kGJ][t - kGJ * (t / kGJ)] = Ppre[t];
         __syncthreads();
@@ -489,6 +627,7 @@ __global__ __launch_bounds__(256) void k_coarse_gemv(int nc, int ld4, const floa
     for (int row = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; row < nc; row += gridDim.x * (blockDim.x / 64)) {
         const float4 *Ar = A + (size_t)row * ld4;
         double s = 0.0;
+#pragma unroll 8
         for (int k = lane; k < ld4; k += 64) {
             const float4 a = Ar[k], b = w[k];
             s += (double)a.x * b.x + (double)a.y * b.y + (double)a.z * b.z + (double)a.w * b.w;
@@ -550,6 +689,10 @@ struct Aggregation {
     // per listed node (ncol order): its incidence slots in the colour's
     // elements, ascending element id: csub[csub_ptr[k]..csub_ptr[k+1])
     std::vector<int32_t> csub_ptr, csub;
+    // element-level Galerkin lists (k_gal_elem, k_gal_sum): listed elements,
+    // their aggregates, entry positions, items (m << 4 | a), entry ranges of
+    // the ordered aggregate pairs, unordered pairs (P, P', h, g)
+    std::vector<int32_t> gal_el, gal_gptr, gal_glist, gal_gq, gal_gpos, gal_item, gal_pptr, gal_upair;
 };
 
 // grid binning of the coordinates, about `target` nodes per bin
@@ -608,6 +751,75 @@ std::vector<std::vector<int32_t>> agg_adjacency(const std::vector<int32_t> &conn
         v.erase(std::unique(v.begin(), v.end()), v.end());
     }
     return adj;
+}
+
+// The lists of the element-level Galerkin product.  Ordered pair (h, g) is
+// pair_base[h] + (position of g in nbr[h]), nbr[h] = {h} and h's neighbours,
+// ascending.  Each pair's entries are numbered in ascending element order.
+void galerkin_lists(const jfm_ctx *ctx, const std::vector<int32_t> &agg, int n_agg,
+                    const std::vector<std::vector<int32_t>> &adj, const std::vector<uint8_t> &fixed, Aggregation &A) {
+    const int64_t n_el = ctx->n_el;
+    const int npe = ctx->npe;
+    std::vector<std::vector<int32_t>> nbr(n_agg);
+    std::vector<int64_t> pair_base(n_agg + 1, 0);
+    for (int h = 0; h < n_agg; ++h) {
+        nbr[h] = adj[h];
+        nbr[h].insert(std::lower_bound(nbr[h].begin(), nbr[h].end(), h), h);
+        pair_base[h + 1] = pair_base[h] + (int64_t)nbr[h].size();
+    }
+    auto pair_of = [&](int h, int g) -> int64_t {
+        return pair_base[h] + (std::lower_bound(nbr[h].begin(), nbr[h].end(), g) - nbr[h].begin());
+    };
+    const int64_t n_pairs = pair_base[n_agg];
+    std::vector<int32_t> cnt(n_pairs + 1, 0);
+    A.gal_gptr.assign(1, 0);
+    A.gal_gq.clear();
+    int32_t loc[32];
+    int64_t q2 = 0;
+    for (int64_t e = 0; e < n_el; ++e) {
+        int k = 0;
+        bool has_fixed = false;
+        for (int l = 0; l < npe; ++l) {
+            const int32_t a = ctx->h_conn[e * npe + l];
+            has_fixed = has_fixed || fixed[3 * (size_t)a] || fixed[3 * (size_t)a + 1] || fixed[3 * (size_t)a + 2];
+            const int32_t g = agg[a];
+            bool seen = false;
+            for (int t = 0; t < k; ++t) seen = seen || loc[t] == g;
+            if (!seen) loc[k++] = g;
+        }
+        if (k < 2 && !has_fixed) continue;  // a rigid motion of the element: no contribution
+        std::sort(loc, loc + k);
+        A.gal_el.push_back((int32_t)e);
+        A.gal_gq.push_back((int32_t)q2);
+        q2 += (int64_t)k * k;
+        for (int t = 0; t < k; ++t) A.gal_glist.push_back(loc[t]);
+        A.gal_gptr.push_back((int32_t)A.gal_glist.size());
+        for (int a = 0; a < k; ++a)
+            for (int b = 0; b < k; ++b) ++cnt[pair_of(loc[b], loc[a]) + 1];
+    }
+    for (int64_t P = 0; P < n_pairs; ++P) cnt[P + 1] += cnt[P];
+    A.gal_pptr.assign(cnt.begin(), cnt.end());
+    std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1);
+    A.gal_gpos.resize(q2);
+    A.gal_item.clear();
+    const int64_t M = (int64_t)A.gal_el.size();
+    for (int64_t m = 0; m < M; ++m) {
+        const int g0 = A.gal_gptr[m], k = A.gal_gptr[m + 1] - g0;
+        for (int a = 0; a < k; ++a) {
+            A.gal_item.push_back((int32_t)(m * 16 + a));
+            for (int b = 0; b < k; ++b)
+                A.gal_gpos[A.gal_gq[m] + a * k + b] = fill[pair_of(A.gal_glist[g0 + b], A.gal_glist[g0 + a])]++;
+        }
+    }
+    A.gal_upair.clear();
+    for (int h = 0; h < n_agg; ++h)
+        for (int32_t g : nbr[h])
+            if (g >= h) {
+                A.gal_upair.push_back((int32_t)pair_of(h, g));
+                A.gal_upair.push_back((int32_t)pair_of(g, h));
+                A.gal_upair.push_back(h);
+                A.gal_upair.push_back(g);
+            }
 }
 
 }  // namespace
@@ -731,6 +943,7 @@ static int coarse_host_setup(jfm_ctx *ctx, int coarse_nodes, const std::vector<u
         ncolor = std::max(ncolor, c + 1);
     }
     A.ncolor = ncolor;
+    galerkin_lists(ctx, agg, n_agg, adj, fixed, A);
     A.partner.assign((size_t)ncolor * n_agg, -1);
     for (int g = 0; g < n_agg; ++g) {
         A.partner[(size_t)A.color[g] * n_agg + g] = g;
@@ -825,6 +1038,10 @@ void jfem_coarse_release(jfm_ctx *ctx) {
     free_dev(ctx, ctx->cdA64); free_dev(ctx, ctx->cdA32); free_dev(ctx, ctx->apos); free_dev(ctx, ctx->rA);
     free_dev(ctx, ctx->gjC); free_dev(ctx, ctx->gjR); free_dev(ctx, ctx->gjP);
     free_dev(ctx, ctx->pv6); free_dev(ctx, ctx->qv6); free_dev(ctx, ctx->ye6);
+    free_dev(ctx, ctx->gal_el); free_dev(ctx, ctx->gal_gptr); free_dev(ctx, ctx->gal_glist);
+    free_dev(ctx, ctx->gal_gq); free_dev(ctx, ctx->gal_gpos); free_dev(ctx, ctx->gal_item);
+    free_dev(ctx, ctx->gal_pptr); free_dev(ctx, ctx->gal_upair); free_dev(ctx, ctx->gal_C);
+    ctx->gal_nitems = ctx->gal_nupair = 0;
     free_dev(ctx, ctx->dropped); free_dev(ctx, ctx->ecol); free_dev(ctx, ctx->ncol);
     ctx->h_ecol_ptr.clear(); ctx->h_ncol_ptr.clear(); ctx->h_cseg.clear();
     free_dev(ctx, ctx->segb); free_dev(ctx, ctx->sega); free_dev(ctx, ctx->csub_ptr); free_dev(ctx, ctx->csub);
@@ -873,17 +1090,68 @@ static hipError_t probe_sequence(jfm_ctx *ctx, bool tangent) {
     return e;
 }
 
-// A_c by probing, its inverse and the fold into Ahat32; a non-SPD pivot sets
-// the flag at ctx->counter + 48.
+// Zero A_c, the element entries, and their sums per aggregate pair (written
+// symmetric, unit diagonal for dropped modes).
+template <int NPE>
+static void gal_elem(jfm_ctx *ctx) {
+    const unsigned grid = nblk(6 * (int64_t)ctx->gal_nitems);
+    if (ctx->affine)
+        k_gal_elem<NPE, true><<<grid, kBlock, 0, ctx->stream>>>(
+            ctx->gal_nitems, ctx->gal_item, ctx->gal_el, ctx->gal_gptr, ctx->gal_glist, ctx->gal_gq, ctx->gal_gpos,
+            (int)ctx->n_el, ctx->conn, ctx->gf64, ctx->node_agg, ctx->cd64, ctx->Rinv, ctx->fixed, ctx->gal_C);
+    else
+        k_gal_elem<NPE, false><<<grid, kBlock, 0, ctx->stream>>>(
+            ctx->gal_nitems, ctx->gal_item, ctx->gal_el, ctx->gal_gptr, ctx->gal_glist, ctx->gal_gq, ctx->gal_gpos,
+            (int)ctx->n_el, ctx->conn, ctx->gf64, ctx->node_agg, ctx->cd64, ctx->Rinv, ctx->fixed, ctx->gal_C);
+}
+
+static hipError_t galerkin_sequence(jfm_ctx *ctx) {
+    hipStream_t s = ctx->stream;
+    const int nc = (int)ctx->nc;
+    k_fill_zero<<<1024, kBlock, 0, s>>>(ctx->Ac, (int64_t)nc * nc);
+    if (ctx->gal_nitems > 0) {
+        switch (ctx->npe) {
+        case 10: gal_elem<10>(ctx); break;
+        case 4: gal_elem<4>(ctx); break;
+        case 8: gal_elem<8>(ctx); break;
+        default: return hipErrorInvalidValue;
+        }
+    }
+    if (ctx->gal_nupair > 0)
+        k_gal_sum<<<ctx->gal_nupair, 1024, 0, s>>>(ctx->gal_upair, ctx->gal_pptr, ctx->gal_C, ctx->Ac, nc);
+    return hipGetLastError();
+}
+
+// A_c by the element-level product for K, by probing for the tangent (whose
+// element matrices do not annihilate rigid rotations), or by probing for
+// both with JFEM_COARSE_PROBE=1.
+static bool coarse_probe(bool tangent) {
+    static const bool env = [] {
+        const char *v = std::getenv("JFEM_COARSE_PROBE");
+        return v && v[0] == '1';
+    }();
+    return tangent || env;
+}
+
+// A_c (symmetric; unit diagonal for dropped modes) into ctx->Ac
+static hipError_t coarse_matrix_sequence(jfm_ctx *ctx, bool tangent) {
+    if (!coarse_probe(tangent)) return galerkin_sequence(ctx);
+    hipError_t e = probe_sequence(ctx, tangent);
+    if (e != hipSuccess) return e;
+    const int nc = (int)ctx->nc;
+    k_coarse_fix<<<nblk((int64_t)nc * nc), kBlock, 0, ctx->stream>>>(nc, ctx->Ac);
+    return hipGetLastError();
+}
+
+// A_c, its inverse and the fold into Ahat32; a non-SPD pivot sets the flag
+// at ctx->counter + 48.
 static hipError_t setup_sequence(jfm_ctx *ctx, bool tangent) {
     hipStream_t s = ctx->stream;
     const int nc = (int)ctx->nc;
     int *flag = (int *)(ctx->counter + 48);
     k_flag_clear<<<1, 1, 0, s>>>(flag);
-    hipError_t e = probe_sequence(ctx, tangent);
+    hipError_t e = coarse_matrix_sequence(ctx, tangent);
     if (e != hipSuccess) return e;
-    const int64_t nn = (int64_t)nc * nc;
-    k_coarse_fix<<<nblk(nn), kBlock, 0, s>>>(nc, ctx->Ac);
     const int tiles = (nc + 63) / 64;
     // gjP: this step's P^-1, the next step's as the update forms it, and the
     // next pivot block before the update
@@ -900,15 +1168,10 @@ static hipError_t setup_sequence(jfm_ctx *ctx, bool tangent) {
     return hipGetLastError();
 }
 
-// Builds the coarse level for operator `op` (FP64, masked).  The aggregation
-// is kept while the mesh, the fixed set and coarse_nodes stay the same; A_c is
-// rebuilt on every call.
-int jfem_coarse_setup(jfm_ctx *ctx, int op, int coarse_nodes, double *ms) {
+// The aggregation for coarse_nodes (kept while the mesh, the fixed set and
+// coarse_nodes stay the same) and, for a probed operator, the probe vectors.
+static int coarse_prepare(jfm_ctx *ctx, int op, int coarse_nodes) {
     hipStream_t s = ctx->stream;
-    hipEvent_t e0, e1;
-    JFM_HIP(ctx, hipEventCreate(&e0));
-    JFM_HIP(ctx, hipEventCreate(&e1));
-    JFM_HIP(ctx, hipEventRecord(e0, s));
     const int64_t N = ctx->N, n = ctx->ndof;
     if (ctx->coarse_built_nodes != coarse_nodes || !ctx->node_agg) {
         jfem_coarse_release(ctx);
@@ -950,13 +1213,31 @@ int jfem_coarse_setup(jfm_ctx *ctx, int op, int coarse_nodes, double *ms) {
         CALLOC(ctx->gjC, double, nc * kGJ);
         CALLOC(ctx->gjR, double, nc * kGJ);
         CALLOC(ctx->gjP, double, 3 * kGJ * kGJ);
-        const int64_t nslots = 3 * (int64_t)ctx->npe * ctx->n_el;
-        CALLOC(ctx->pv6, double, 6 * 3 * N);
-        CALLOC(ctx->qv6, double, 6 * 3 * N);
-        CALLOC(ctx->ye6, double, 6 * nslots);
-        JFM_HIP(ctx, hipMemsetAsync(ctx->pv6, 0, sizeof(double) * 18 * N, s));
-        JFM_HIP(ctx, hipMemsetAsync(ctx->qv6, 0, sizeof(double) * 18 * N, s));
-        JFM_HIP(ctx, hipMemsetAsync(ctx->ye6, 0, sizeof(double) * 6 * nslots, s));
+        if ((int64_t)A.gal_el.size() >= (int64_t(1) << 27))
+            return jfm_fail(ctx, JFM_EINVAL, "too many elements for the coarse level");
+        ctx->gal_nitems = (int)A.gal_item.size();
+        ctx->gal_nupair = (int)(A.gal_upair.size() / 4);
+        const int64_t n_entries = A.gal_pptr.back();
+        CALLOC(ctx->gal_el, int32_t, std::max<int64_t>(1, (int64_t)A.gal_el.size()));
+        CALLOC(ctx->gal_gptr, int32_t, (int64_t)A.gal_gptr.size());
+        CALLOC(ctx->gal_glist, int32_t, std::max<int64_t>(1, (int64_t)A.gal_glist.size()));
+        CALLOC(ctx->gal_gq, int32_t, std::max<int64_t>(1, (int64_t)A.gal_gq.size()));
+        CALLOC(ctx->gal_gpos, int32_t, std::max<int64_t>(1, (int64_t)A.gal_gpos.size()));
+        CALLOC(ctx->gal_item, int32_t, std::max<int64_t>(1, (int64_t)A.gal_item.size()));
+        CALLOC(ctx->gal_pptr, int32_t, (int64_t)A.gal_pptr.size());
+        CALLOC(ctx->gal_upair, int32_t, std::max<int64_t>(4, (int64_t)A.gal_upair.size()));
+        CALLOC(ctx->gal_C, double, std::max<int64_t>(36, 36 * n_entries));
+        auto up = [&](int32_t *d, const std::vector<int32_t> &v) -> hipError_t {
+            return v.empty() ? hipSuccess : hipMemcpyAsync(d, v.data(), 4 * v.size(), hipMemcpyHostToDevice, s);
+        };
+        JFM_HIP(ctx, up(ctx->gal_el, A.gal_el));
+        JFM_HIP(ctx, up(ctx->gal_gptr, A.gal_gptr));
+        JFM_HIP(ctx, up(ctx->gal_glist, A.gal_glist));
+        JFM_HIP(ctx, up(ctx->gal_gq, A.gal_gq));
+        JFM_HIP(ctx, up(ctx->gal_gpos, A.gal_gpos));
+        JFM_HIP(ctx, up(ctx->gal_item, A.gal_item));
+        JFM_HIP(ctx, up(ctx->gal_pptr, A.gal_pptr));
+        JFM_HIP(ctx, up(ctx->gal_upair, A.gal_upair));
         const int64_t ldc = (nc + 255) / 256 * 256;
         ctx->ldc = ldc;
         CALLOC(ctx->Ahat32, float, nc * ldc);
@@ -1005,10 +1286,41 @@ int jfem_coarse_setup(jfm_ctx *ctx, int op, int coarse_nodes, double *ms) {
         ctx->coarse_built_nodes = coarse_nodes;
         ++ctx->coarse_gen;
     }
+    const bool tangent = op == JFM_OP_TANGENT;
+    if (tangent && !ctx->has_state) return jfm_fail(ctx, JFM_ESTATE, "tangent coarse level needs a state");
+    if (coarse_probe(tangent) && !ctx->pv6) {
+        // probe vectors: zero outside the rows and slots each colour writes
+        const int64_t nslots = 3 * (int64_t)ctx->npe * ctx->n_el;
+        ctx->pv6 = jfm_alloc<double>(ctx, 6 * 3 * N);
+        ctx->qv6 = jfm_alloc<double>(ctx, 6 * 3 * N);
+        ctx->ye6 = jfm_alloc<double>(ctx, 6 * nslots);
+        if (!ctx->pv6 || !ctx->qv6 || !ctx->ye6) {
+            jfem_coarse_release(ctx);
+            return jfm_fail(ctx, JFM_ENOMEM, "coarse level allocation failed");
+        }
+        JFM_HIP(ctx, hipMemsetAsync(ctx->pv6, 0, sizeof(double) * 18 * N, s));
+        JFM_HIP(ctx, hipMemsetAsync(ctx->qv6, 0, sizeof(double) * 18 * N, s));
+        JFM_HIP(ctx, hipMemsetAsync(ctx->ye6, 0, sizeof(double) * 6 * nslots, s));
+    }
+    return JFM_OK;
+}
+
+// Builds the coarse level for operator `op` (FP64, masked); A_c is rebuilt on
+// every call.
+int jfem_coarse_setup(jfm_ctx *ctx, int op, int coarse_nodes, double *ms) {
+    hipStream_t s = ctx->stream;
+    hipEvent_t e0, e1;
+    JFM_HIP(ctx, hipEventCreate(&e0));
+    JFM_HIP(ctx, hipEventCreate(&e1));
+    JFM_HIP(ctx, hipEventRecord(e0, s));
+    if (int rc = coarse_prepare(ctx, op, coarse_nodes)) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return rc;
+    }
     // the same launch sequence every time for a given aggregation and
     // operator, so it is captured once as a HIP graph
     const bool tangent = op == JFM_OP_TANGENT;
-    if (tangent && !ctx->has_state) return jfm_fail(ctx, JFM_ESTATE, "tangent coarse level needs a state");
     hipError_t seq_err = hipSuccess;
     if (jfem_use_graphs()) {
         hipGraphExec_t &ex = ctx->coarse_graph[tangent ? 1 : 0];
@@ -1057,6 +1369,29 @@ int jfem_coarse_setup(jfm_ctx *ctx, int op, int coarse_nodes, double *ms) {
     if (hflag != 0) return jfm_fail(ctx, JFM_ENOTSPD, "coarse matrix is not positive definite");
     ctx->coarse_ready = true;
     ctx->coarse_tag = op == JFM_OP_TANGENT ? 1 : 0;
+    return JFM_OK;
+}
+
+// A_c itself (before inversion), for checking: nc, then A_c (nc x nc, row
+// major) and the node aggregates when the pointers are given.  The coarse
+// level has to be rebuilt afterwards.
+int jfem_coarse_matrix(jfm_ctx *ctx, int op, int coarse_nodes, int64_t *nc, double *Ac, int32_t *node_agg) {
+    if (int rc = coarse_prepare(ctx, op, coarse_nodes)) return rc;
+    const bool tangent = op == JFM_OP_TANGENT;
+    hipStream_t s = ctx->stream;
+    if (nc) *nc = ctx->nc;
+    if (node_agg)
+        JFM_HIP(ctx, hipMemcpyAsync(node_agg, ctx->node_agg, 4 * ctx->N, hipMemcpyDeviceToHost, s));
+    if (Ac) {
+        ctx->coarse_ready = false;
+        hipError_t e = coarse_matrix_sequence(ctx, tangent);
+        if (e != hipSuccess) {
+            jfem_coarse_release(ctx);
+            JFM_HIP(ctx, e);
+        }
+        JFM_HIP(ctx, hipMemcpyAsync(Ac, ctx->Ac, 8 * ctx->nc * ctx->nc, hipMemcpyDeviceToHost, s));
+    }
+    JFM_HIP(ctx, hipStreamSynchronize(s));
     return JFM_OK;
 }
 

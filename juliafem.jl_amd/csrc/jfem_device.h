@@ -66,6 +66,40 @@ __device__ __forceinline__ void sum_partials(const double *__restrict__ part, in
 
 __device__ __forceinline__ bool finite(double x) { return x - x == 0.0; }
 
+// Loads at a 32-bit byte offset from a uniform base pointer, so that the
+// compiler addresses them as SGPR base + VGPR offset with no 64-bit address
+// arithmetic per lane (every array indexed this way is below 4 GiB).
+template <typename T> struct Trip { T a, b, c; };
+template <typename V, typename T>
+__device__ __forceinline__ V load_at(const T *__restrict__ base, uint32_t byte_off) {
+    return *reinterpret_cast<const V *>(reinterpret_cast<const char *>(base) + byte_off);
+}
+template <typename V, typename T>
+__device__ __forceinline__ void store_at(T *__restrict__ base, uint32_t byte_off, const V &v) {
+    *reinterpret_cast<V *>(reinterpret_cast<char *>(base) + byte_off) = v;
+}
+
+// A node's three values (and a node's 3 x 3 block) as one vector access
+// each, instead of one per component.
+template <typename T> struct Blk9 { T v[9]; };
+template <typename T> __device__ __forceinline__ Trip<T> ld3(const T *__restrict__ p, size_t node) {
+    return reinterpret_cast<const Trip<T> *>(p)[node];
+}
+template <typename T> __device__ __forceinline__ void st3(T *__restrict__ p, size_t node, T a, T b, T c) {
+    reinterpret_cast<Trip<T> *>(p)[node] = Trip<T>{a, b, c};
+}
+template <typename T> __device__ __forceinline__ Blk9<T> ld9(const T *__restrict__ p, size_t node) {
+    return reinterpret_cast<const Blk9<T> *>(p)[node];
+}
+
+// Wave-local barrier: the LDS a wave reads after it was written by other
+// lanes of the same wave only.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // dN/dxi at the Gauss points as constant tables for runtime (q, i) indexing
 // (compile-time indexing uses Elem<NPE>::dN directly).
 template <int NPE> struct DNTable { double v[Elem<NPE>::NGP][NPE][3]; };

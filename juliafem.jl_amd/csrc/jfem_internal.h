@@ -261,6 +261,13 @@ struct jfm_ctx {
     float *wc32 = nullptr, *zc32 = nullptr;  // wc32 has ldc entries, zero past nc
     double *wc64 = nullptr;        // 2 x nc: Q' m r by iteration parity (slot 0 at the PCG start)
     double *gjC = nullptr, *gjR = nullptr, *gjP = nullptr;  // Gauss-Jordan panels
+    // A_c element by element (linear operator): listed elements, their
+    // aggregates and entry positions, items, pair ranges, unordered pairs,
+    // and the entries (36 doubles each)
+    int32_t *gal_el = nullptr, *gal_gptr = nullptr, *gal_glist = nullptr, *gal_gq = nullptr, *gal_gpos = nullptr;
+    int32_t *gal_item = nullptr, *gal_pptr = nullptr, *gal_upair = nullptr;
+    double *gal_C = nullptr;
+    int gal_nitems = 0, gal_nupair = 0;
     // patch operator (patch.hip): elements in Morton order, 480 per patch
     bool patch_ready = false;
     int npatch = 0, npn = 0, nbnode = 0, nslot = 0;
@@ -358,6 +365,7 @@ template <typename T>
 hipError_t launch_patch_apply(jfm_ctx *ctx, const T *x, T *y, const T *dotv, double *part, const int *done, int *npart);
 // coarse.hip
 int jfem_coarse_setup(jfm_ctx *ctx, int op, int coarse_nodes, double *ms);
+int jfem_coarse_matrix(jfm_ctx *ctx, int op, int coarse_nodes, int64_t *nc, double *Ac, int32_t *node_agg);
 bool jfem_use_graphs();  // false when JFEM_GRAPHS=0 (profilers)
 void jfem_coarse_release(jfm_ctx *ctx);
 template <typename T>

@@ -533,14 +533,6 @@ hipError_t launch_features(jfm_ctx *ctx, double *ef, float *feat) {
     return hipGetLastError();
 }
 
-// Wave-local barrier: the LDS a wave reads below was written by other lanes of
-// the same wave only.
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // Inference: the same forward pass as k_mlp (same MFMA tiles, roundings and
 // order), in persistent 4-wave blocks that stage the weights once; each wave
 // walks its own 16-node tiles with wave-local barriers only, and its LDS holds

@@ -153,12 +153,15 @@ __device__ __forceinline__ void restrict_q(int g, int slot, int npart_pq, const 
         for (int u = 0; u < kR; ++u) {
             const int kk = k0 + u * (int)blockDim.x;
             if (kk < k1) {
-                const size_t o = 3 * (size_t)agg_nodes[kk];
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    y[u][c] = fixed[o + c] ? T(0) : q[o + c];
-                    d[u][c] = cdA[3 * (size_t)kk + c];
-                }
+                const int a = agg_nodes[kk];
+                const Trip<T> qa = ld3(q, a), da = ld3(cdA, kk);
+                const Trip<uint8_t> fa = ld3(fixed, a);
+                y[u][0] = fa.a ? T(0) : qa.a;
+                y[u][1] = fa.b ? T(0) : qa.b;
+                y[u][2] = fa.c ? T(0) : qa.c;
+                d[u][0] = da.a;
+                d[u][1] = da.b;
+                d[u][2] = da.c;
             } else {
 #pragma unroll
                 for (int c = 0; c < 3; ++c) y[u][c] = d[u][c] = T(0);
@@ -201,9 +204,10 @@ __device__ __forceinline__ void restrict_q(int g, int slot, int npart_pq, const 
     }
 }
 
-// x += alpha p, r -= alpha q, z = Minv r; partials (r.z, r.r).  The first
-// node's operands are loaded before the alpha reduction so that their
-// latency overlaps it.
+// x += alpha p, r -= alpha q, z = Minv r; partials (r.z, r.r).  A thread's
+// first kU nodes (4 in FP32, 2 in FP64: every node at the usual grid) are
+// all loaded before the alpha reduction, so that their latencies overlap
+// each other and the reduction.
 // With rA, the masked residual also goes to rA in aggregate order (node a at
 // position apos[a]) for the coarse restriction, which then reads it
 // contiguously.
@@ -241,44 +245,51 @@ __global__ __launch_bounds__(512) void k_pcg_update(int N, int slot, int npart_p
         nb -= na;  // a multiple of kXcds
         b = (b % kXcds) * (nb / kXcds) + b / kXcds;
     }
+    constexpr int kU = sizeof(T) == 4 ? 4 : 2;
     const int stride = nb * blockDim.x;
-    int a = b * blockDim.x + threadIdx.x;
-    T xv[3], pv[3], rv[3], qv[3], M[9];
-    auto load = [&](int aa) {
-        const size_t o = 3 * (size_t)aa;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) { xv[c] = x[o + c]; pv[c] = p[o + c]; rv[c] = r[o + c]; qv[c] = q[o + c]; }
-#pragma unroll
-        for (int m = 0; m < 9; ++m) M[m] = Minv[9 * (size_t)aa + m];
+    const int a0 = b * blockDim.x + threadIdx.x;
+    Trip<T> xv[kU], pv[kU], rv[kU], qv[kU];
+    Blk9<T> M[kU];
+    auto load = [&](int u, int aa) {
+        xv[u] = ld3(x, aa);
+        pv[u] = ld3(p, aa);
+        rv[u] = ld3(r, aa);
+        qv[u] = ld3(q, aa);
+        M[u] = ld9(Minv, aa);
     };
-    if (a < N) load(a);  // in flight while the flag is read
+#pragma unroll
+    for (int u = 0; u < kU; ++u)
+        if (a0 + u * stride < N) load(u, a0 + u * stride);  // in flight while the flag is read
     if (S->done[slot]) return;
     double pq[1];
     sum_partials<1>(partials + part_pq(slot), npart_pq, pq);
     const double alpha_d = (pq[0] > 0.0 && finite(pq[0])) ? S->rz[slot] / pq[0] : 0.0;
     const T alpha = (T)alpha_d;
     double v[2] = {0.0, 0.0};
-    while (a < N) {
-        const size_t o = 3 * (size_t)a;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            x[o + c] = xv[c] + alpha * pv[c];
-            rv[c] = rv[c] - alpha * qv[c];
-            r[o + c] = rv[c];
-        }
-        const T z0 = M[0] * rv[0] + M[1] * rv[1] + M[2] * rv[2];
-        const T z1 = M[3] * rv[0] + M[4] * rv[1] + M[5] * rv[2];
-        const T z2 = M[6] * rv[0] + M[7] * rv[1] + M[8] * rv[2];
-        z[o] = z0; z[o + 1] = z1; z[o + 2] = z2;
+    auto update = [&](int u, int aa) {
+        const size_t o = 3 * (size_t)aa;
+        const T rn[3] = {rv[u].a - alpha * qv[u].a, rv[u].b - alpha * qv[u].b, rv[u].c - alpha * qv[u].c};
+        st3(x, aa, xv[u].a + alpha * pv[u].a, xv[u].b + alpha * pv[u].b, xv[u].c + alpha * pv[u].c);
+        st3(r, aa, rn[0], rn[1], rn[2]);
+        const T *Mu = M[u].v;
+        const T z0 = Mu[0] * rn[0] + Mu[1] * rn[1] + Mu[2] * rn[2];
+        const T z1 = Mu[3] * rn[0] + Mu[4] * rn[1] + Mu[5] * rn[2];
+        const T z2 = Mu[6] * rn[0] + Mu[7] * rn[1] + Mu[8] * rn[2];
+        st3(z, aa, z0, z1, z2);
         if (rA) {
-            const size_t oa = 3 * (size_t)apos[a];
+            const size_t oa = 3 * (size_t)apos[aa];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) rA[oa + c] = fixed[o + c] ? T(0) : rv[c];
+            for (int c = 0; c < 3; ++c) rA[oa + c] = fixed[o + c] ? T(0) : rn[c];
         }
-        v[0] += (double)rv[0] * z0 + (double)rv[1] * z1 + (double)rv[2] * z2;
-        v[1] += (double)rv[0] * rv[0] + (double)rv[1] * rv[1] + (double)rv[2] * rv[2];
-        a += stride;
-        if (a < N) load(a);
+        v[0] += (double)rn[0] * z0 + (double)rn[1] * z1 + (double)rn[2] * z2;
+        v[1] += (double)rn[0] * rn[0] + (double)rn[1] * rn[1] + (double)rn[2] * rn[2];
+    };
+#pragma unroll
+    for (int u = 0; u < kU; ++u)
+        if (a0 + u * stride < N) update(u, a0 + u * stride);
+    for (int a = a0 + kU * stride; a < N; a += stride) {  // grids too small for kU nodes per thread
+        load(0, a);
+        update(0, a);
     }
     write_partials<2>(v, partials + part_rz(slot), b);
     if (b == 0 && threadIdx.x == 0) {
@@ -338,18 +349,14 @@ __global__ __launch_bounds__(256) void k_pcg_dir_coarse(int N, int slot, int npa
     // reductions
     constexpr int kDN = kDirNodes;
     const int a0 = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
-    T zv[kDN][3], pv[kDN][3], dv[kDN][3];
-    bool fx[kDN][3];
+    Trip<T> zv[kDN], pv[kDN], dv[kDN];
+    Trip<uint8_t> fx[kDN];
     int gv[kDN];
     auto load = [&](int u, int a) {
-        const size_t o = 3 * (size_t)a;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            zv[u][k] = z[o + k];
-            pv[u][k] = p[o + k];
-            dv[u][k] = cd[o + k];
-            fx[u][k] = fixed[o + k] != 0;
-        }
+        zv[u] = ld3(z, a);
+        pv[u] = ld3(p, a);
+        dv[u] = ld3(cd, a);
+        fx[u] = ld3(fixed, a);
         gv[u] = node_agg[a];
     };
 #pragma unroll
@@ -376,14 +383,13 @@ __global__ __launch_bounds__(256) void k_pcg_dir_coarse(int N, int slot, int npa
     const double beta_d = rz / S->rz[slot];
     const T beta = (T)beta_d;
     auto update = [&](int u, int a) {
-        const size_t o = 3 * (size_t)a;
-        const float *yf = zc + 6 * (size_t)gv[u];
-        const T y[6] = {(T)yf[0], (T)yf[1], (T)yf[2], (T)yf[3], (T)yf[4], (T)yf[5]};
-        const T *d0 = dv[u];
+        const Trip<float> y0 = ld3(zc, 2 * (size_t)gv[u]), y1 = ld3(zc, 2 * (size_t)gv[u] + 1);
+        const T y[6] = {(T)y0.a, (T)y0.b, (T)y0.c, (T)y1.a, (T)y1.b, (T)y1.c};
+        const T d0[3] = {dv[u].a, dv[u].b, dv[u].c};
         const T c[3] = {y[0] + y[4] * d0[2] - y[5] * d0[1], y[1] + y[5] * d0[0] - y[3] * d0[2],
                         y[2] + y[3] * d0[1] - y[4] * d0[0]};
-#pragma unroll
-        for (int k = 0; k < 3; ++k) p[o + k] = (zv[u][k] + (fx[u][k] ? T(0) : c[k])) + beta * pv[u][k];
+        st3(p, a, (zv[u].a + (fx[u].a ? T(0) : c[0])) + beta * pv[u].a,
+            (zv[u].b + (fx[u].b ? T(0) : c[1])) + beta * pv[u].b, (zv[u].c + (fx[u].c ? T(0) : c[2])) + beta * pv[u].c);
     };
 #pragma unroll
     for (int u = 0; u < kDN; ++u)

@@ -200,18 +200,6 @@ __host__ __device__ __forceinline__ void mf_element_gp(int n_el, int e, const T 
     }
 }
 
-// Loads at a 32-bit byte offset from a uniform base pointer, so that the
-// compiler addresses them as SGPR base + VGPR offset with no 64-bit address
-// arithmetic per lane (every array indexed this way is below 4 GiB).
-template <typename T> struct Trip { T a, b, c; };
-template <typename V, typename T>
-__device__ __forceinline__ V load_at(const T *__restrict__ base, uint32_t byte_off) {
-    return *reinterpret_cast<const V *>(reinterpret_cast<const char *>(base) + byte_off);
-}
-template <typename V, typename T>
-__device__ __forceinline__ void store_at(T *__restrict__ base, uint32_t byte_off, const V &v) {
-    *reinterpret_cast<V *>(reinterpret_cast<char *>(base) + byte_off) = v;
-}
 
 // Element e with SoA connectivity conn[i * n_el + e] and x gathered from HBM
 // (one three-component load per node).

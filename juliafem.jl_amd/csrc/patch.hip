@@ -183,20 +183,23 @@ __global__ __launch_bounds__(256) void k_ku_combine(int nb, int chunk, const int
             rb = bptr[b + 1];
         }
         for (int r = ra; r < rb; ++r) {
-            s0 += bpart[3 * (size_t)r];
-            s1 += bpart[3 * (size_t)r + 1];
-            s2 += bpart[3 * (size_t)r + 2];
+            const Trip<T> v = ld3(bpart, r);
+            s0 += v.a;
+            s1 += v.b;
+            s2 += v.c;
         }
-        const size_t o = 3 * (size_t)bnode[b];
+        const int a = bnode[b];
         if (fixed) {
-            if (fixed[o]) s0 = T(0);
-            if (fixed[o + 1]) s1 = T(0);
-            if (fixed[o + 2]) s2 = T(0);
+            const Trip<uint8_t> f = ld3(fixed, a);
+            if (f.a) s0 = T(0);
+            if (f.b) s1 = T(0);
+            if (f.c) s2 = T(0);
         }
-        y[o] = s0;
-        y[o + 1] = s1;
-        y[o + 2] = s2;
-        if (dotv) dv += (double)s0 * (double)dotv[o] + (double)s1 * (double)dotv[o + 1] + (double)s2 * (double)dotv[o + 2];
+        st3(y, a, s0, s1, s2);
+        if (dotv) {
+            const Trip<T> d = ld3(dotv, a);
+            dv += (double)s0 * (double)d.a + (double)s1 * (double)d.b + (double)s2 * (double)d.c;
+        }
     }
     if (part) {
         double v[1] = {dv};

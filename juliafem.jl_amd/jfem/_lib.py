@@ -24,7 +24,7 @@ ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ENOTSPD", -4: "ENOCONV", -5: "ESTATE", 
 
 # Exported symbols (must match include/jfem.h)
 SYMBOLS = ("jfm_create", "jfm_destroy", "jfm_last_error", "jfm_get_info", "jfm_upload_mesh",
-           "jfm_upload_bcs", "jfm_csr_pattern", "jfm_assemble", "jfm_assemble_tl", "jfm_apply", "jfm_apply_noalloc", "jfm_time_apply",
+           "jfm_upload_bcs", "jfm_csr_pattern", "jfm_assemble", "jfm_assemble_tl", "jfm_apply", "jfm_apply_noalloc", "jfm_time_apply", "jfm_coarse_matrix",
            "jfm_solve", "jfm_solve_history", "jfm_precond_set_blocks", "jfm_learned_blob_bytes", "jfm_learned_n_params",
            "jfm_learned_set_weights", "jfm_learned_get_weights", "jfm_learned_features", "jfm_learned_outputs",
            "jfm_learned_grad", "jfm_nl_residual", "jfm_solve_nonlinear", "jfm_postprocess", "jfm_heat_solve")
@@ -101,6 +101,7 @@ def load():
     lib.jfm_apply.argtypes = [_P, C.c_int32, C.c_int32, _P, _P]
     lib.jfm_apply_noalloc.argtypes = [_P, C.c_int32, C.c_int32, _P, _P, _P]
     lib.jfm_time_apply.argtypes = [_P, C.c_int32, C.c_int32, C.c_int32, _P]
+    lib.jfm_coarse_matrix.argtypes = [_P, C.c_int32, C.c_int32, _P, _P, _P]
     lib.jfm_solve.argtypes = [_P, C.POINTER(SolveOpts), _P, _P, _P, C.POINTER(SolveStats)]
     lib.jfm_solve_history.argtypes = [_P, _P, _P, C.c_int32]
     lib.jfm_precond_set_blocks.argtypes = [_P, _P]
@@ -229,6 +230,17 @@ class Context:
         ms = (C.c_double * 3)()
         self._check(self._lib.jfm_time_apply(self._h, OPS[op], precision, reps, ms))
         return tuple(ms)
+
+    def coarse_matrix(self, coarse_nodes: int, op="matfree"):
+        """(A_c = Q' K Q before inversion, node aggregates) of the two-level
+        preconditioner for coarse_nodes (op "tangent": the neo-Hookean
+        tangent at the stored state)."""
+        nc = C.c_int64(0)
+        self._check(self._lib.jfm_coarse_matrix(self._h, OPS[op], coarse_nodes, C.byref(nc), None, None))
+        A = np.empty((nc.value, nc.value))
+        agg = np.empty(self.info()["n_nodes"], np.int32)
+        self._check(self._lib.jfm_coarse_matrix(self._h, OPS[op], coarse_nodes, C.byref(nc), _p(A), _p(agg)))
+        return A, agg
 
     def solve(self, f=None, method=SOLVE_PCG, op="matfree", op_outer="matfree", precond=PREC_BLOCK_JACOBI,
               rtol=1e-8, inner_rtol=1e-4, maxit=100000, max_outer=20, check_every=16, learned_probe=0,

@@ -170,3 +170,73 @@ def test_restriction_recurrence_matches_direct_restriction(cfg0, tmp_path):
         assert abs(it - it_d) <= 2 + 0.03 * it_d
         assert np.linalg.norm(u - r["u%d" % meth]) <= 1e-8 * np.linalg.norm(u)
         assert np.linalg.norm(u - ud) <= 1e-8 * np.linalg.norm(ud)
+
+
+_CHILD_AC = r"""
+import sys
+import numpy as np
+sys.path[:0] = [{root!r}, {pkg!r}]
+import jfem
+from jfem import mesh as M
+m = M.{gen}({dims}, seed={seed}, jitter=0.1, E_sigma={sig})
+c = jfem.Context(0)
+c.upload(m)
+A, agg = c.coarse_matrix({cn})
+np.savez({out!r}, A=A, agg=agg)
+"""
+
+
+def _block_norms(A, nag):
+    B = A.reshape(nag, 6, nag, 6)
+    return np.sqrt((B ** 2).sum(axis=(1, 3)))
+
+
+@pytest.mark.parametrize("gen,dims,seed,sig,cn", [("box_tet10", "40, 4, 4", 1, 0.5, 32),
+                                                  ("box_tet10", "40, 4, 4", 1, 0.5, 256),
+                                                  ("box_hex8", "16, 4, 4", 2, 0.3, 64)])
+def test_coarse_matrix_element_products(oracle, tmp_path, gen, dims, seed, sig, cn):
+    """A_c = Q' K Q formed element by element (elements inside one aggregate
+    skipped: K_e annihilates the rigid modes) against (1) the same matrix
+    by probing K with the modes of distance-2 coloured aggregates
+    (JFEM_COARSE_PROBE=1, a child process; the same basis, so entry by
+    entry) and (2) the oracle's P' K P with P from a QR of the masked modes
+    (another orthonormal basis of each aggregate's span, so the invariants:
+    the aggregation bit for bit, the eigenvalues and the Frobenius norm of
+    every aggregate-pair block)."""
+    import jfem
+    m = getattr(M, gen)(*[int(v) for v in dims.split(",")], seed=seed, jitter=0.1, E_sigma=sig)
+    ctx = jfem.Context(0)
+    ctx.upload(m)
+    say(f"coarse matrix {gen} {dims} coarse_nodes={cn}")
+    A, agg = ctx.coarse_matrix(cn)
+    ctx.close()
+    nag = int(agg.max()) + 1
+    assert A.shape == (6 * nag, 6 * nag)
+    assert np.array_equal(A, A.T)
+    s = np.abs(A).max()
+    # (1) probing, in a child process
+    out = str(tmp_path / "probe.npz")
+    code = _CHILD_AC.format(root=ROOT, pkg=os.path.join(ROOT, "juliafem.jl_amd"), gen=gen, dims=dims, seed=seed,
+                            sig=sig, cn=cn, out=out)
+    subprocess.run([sys.executable, "-c", code], env=dict(os.environ, JFEM_COARSE_PROBE="1"), check=True,
+                   timeout=120)
+    r = np.load(out)
+    assert np.array_equal(r["agg"], agg)
+    dp = np.abs(A - r["A"]).max()
+    say(f"  n_agg={nag} max|A|={s:.3e} max|A_elem - A_probe|={dp:.2e}")
+    assert dp <= 1e-11 * s
+    # (2) the oracle
+    import scipy.sparse as sp
+    assert np.array_equal(agg, oracle.coarse_aggregates(m, cn))
+    csr = oracle.assemble_csr(m)
+    P, _, _ = oracle.twolevel_operator(m, csr, cn)
+    assert P.shape[1] == 6 * nag          # no mode dropped on these meshes
+    n = m.n_dofs
+    K = sp.csr_matrix((csr[2], csr[1], csr[0]), shape=(n, n))
+    D = sp.diags((m.fixed_mask() == 0).astype(np.float64))
+    Ao = (P.T @ (D @ K @ D) @ P).toarray()
+    ev, evo = np.linalg.eigvalsh(A), np.linalg.eigvalsh(Ao)
+    de = np.abs(ev - evo).max() / evo.max()
+    db = np.abs(_block_norms(A, nag) - _block_norms(Ao, nag)).max() / s
+    say(f"  eigenvalues {evo.min():.3e}..{evo.max():.3e}: rel diff {de:.2e}; block norms rel diff {db:.2e}")
+    assert de <= 1e-11 and db <= 1e-11
