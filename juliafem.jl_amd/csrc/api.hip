@@ -530,8 +530,8 @@ template <> float *vals_of<float>(jfm_ctx *ctx) { return ctx->vals32; }
 // the EBE operator's element matrices (FP64 and FP32, pair-major)
 static int ensure_element_matrices(jfm_ctx *ctx) {
     if (ctx->Ke) return JFM_OK;
-    ALLOC(ctx, ctx->Ke, double, ctx->n_el * ctx->npair * 9);
-    ALLOC(ctx, ctx->Ke32, float, ctx->n_el * ctx->npair * 9);
+    ALLOC(ctx, ctx->Ke, double, ctx->n_el * ke_stride(ctx->npair, 2));
+    ALLOC(ctx, ctx->Ke32, float, ctx->n_el * ke_stride(ctx->npair, 4));
     CHECK_LAUNCH(ctx, launch_element_matrices(ctx));
     return JFM_OK;
 }

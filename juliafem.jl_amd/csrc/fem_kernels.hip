@@ -121,9 +121,19 @@ __global__ __launch_bounds__(256) void k_mf_elem(int n_el, const int32_t *__rest
 
 // ------------------------------------------------------------- EBE element --
 // Phase 1 with stored element matrices: y_e = K_e x_e.  The upper blocks are
-// stored pair-major, Ke[(p * 9 + m) * n_el + e] (block p = (i <= j), entry m),
-// in the operator's precision, so the lanes of a wave read consecutive
-// addresses.
+// stored in the operator's precision as chunks of V = 16 / sizeof(T) values
+// (two doubles, four floats) interleaved by element: entry q = 9 p + m
+// (block p = (i <= j), entry m) of element e at Ke[((q / V) n_el + e) V +
+// q % V], so that a lane loads 16 bytes per instruction and the lanes of a
+// wave read consecutive addresses (ke_stride rounds 9 NPAIR up to a
+// multiple of V).
+// The element matrices are read once per product: non-temporal loads keep
+// them from displacing x and the connectivity in the caches (at 1.05M DOFs
+// the FP64 element phase went from 0.214 to 0.185 ms, FP32 from 0.105 to
+// 0.090 ms).
+#ifndef JFEM_EBE_NT
+#define JFEM_EBE_NT 1
+#endif
 template <int NPE, typename T>
 __global__ __launch_bounds__(256) void k_ebe_elem(int n_el, const int32_t *__restrict__ conn,
                                                  const int32_t *__restrict__ slot, const T *__restrict__ Ke,
@@ -144,17 +154,35 @@ __global__ __launch_bounds__(256) void k_ebe_elem(int n_el, const int32_t *__res
 #pragma unroll
         for (int c = 0; c < 3; ++c) { u[i][c] = x[3 * (size_t)nd + c] - t[c]; f[i][c] = T(0); }
     }
+    constexpr int V = 16 / sizeof(T), NQ = 9 * EL::NPAIR;
+    struct alignas(16) Chunk { T v[V]; };
+    const Chunk *Kc = reinterpret_cast<const Chunk *>(Ke);
+    T B[9];
+#pragma clang loop unroll(full)
+    for (int ch = 0; ch < (NQ + V - 1) / V; ++ch) {
+#if JFEM_EBE_NT
+        Chunk kv;
 #pragma unroll
-    for (int p = 0; p < EL::NPAIR; ++p) {
-        const int i = PT.i[p], j = PT.j[p];
-        T B[9];
+        for (int v = 0; v < V; ++v) kv.v[v] = __builtin_nontemporal_load(&Kc[(size_t)ch * n_el + e].v[v]);
+#else
+        const Chunk kv = Kc[(size_t)ch * n_el + e];
+#endif
 #pragma unroll
-        for (int m = 0; m < 9; ++m) B[m] = Ke[(size_t)(p * 9 + m) * n_el + e];
+        for (int v = 0; v < V; ++v) {
+            const int q = ch * V + v;
+            if (q < NQ) {
+                B[q % 9] = kv.v[v];
+                if (q % 9 == 8) {  // block p complete
+                    const int p = q / 9, i = PT.i[p], j = PT.j[p];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) f[i][c] += B[c * 3] * u[j][0] + B[c * 3 + 1] * u[j][1] + B[c * 3 + 2] * u[j][2];
-        if (i != j) {
+                    for (int c = 0; c < 3; ++c)
+                        f[i][c] += B[c * 3] * u[j][0] + B[c * 3 + 1] * u[j][1] + B[c * 3 + 2] * u[j][2];
+                    if (i != j) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) f[j][c] += B[c] * u[i][0] + B[3 + c] * u[i][1] + B[6 + c] * u[i][2];
+                        for (int c = 0; c < 3; ++c) f[j][c] += B[c] * u[i][0] + B[3 + c] * u[i][1] + B[6 + c] * u[i][2];
+                    }
+                }
+            }
         }
     }
 #pragma unroll
@@ -269,8 +297,8 @@ __device__ __forceinline__ void elem_block(int n_el, int ngf, const double *__re
 }
 
 // Element matrices for the EBE operator: thread (pair p = blockIdx.y, element
-// e), written pair-major in FP64 and FP32 so that consecutive lanes store
-// consecutive addresses.
+// e), written in FP64 and FP32 in the chunked, element-interleaved layout
+// k_ebe_elem reads.
 template <int NPE>
 __global__ __launch_bounds__(256) void k_elem_mat(int n_el, int ngf, const double *__restrict__ gf,
                                                  double *__restrict__ Ke, float *__restrict__ Ke32) {
@@ -281,8 +309,9 @@ __global__ __launch_bounds__(256) void k_elem_mat(int n_el, int ngf, const doubl
     elem_block<NPE>(n_el, ngf, gf, e, PT.i[p], PT.j[p], K);
 #pragma unroll
     for (int m = 0; m < 9; ++m) {
-        Ke[(size_t)(p * 9 + m) * n_el + e] = K[m];
-        Ke32[(size_t)(p * 9 + m) * n_el + e] = (float)K[m];
+        const int q = 9 * p + m;
+        Ke[((size_t)(q / 2) * n_el + e) * 2 + q % 2] = K[m];
+        Ke32[((size_t)(q / 4) * n_el + e) * 4 + q % 4] = (float)K[m];
     }
 }
 

@@ -174,8 +174,8 @@ struct jfm_ctx {
     int32_t *cptr = nullptr, *contrib = nullptr;
     double *vals64 = nullptr;
     float *vals32 = nullptr;
-    double *Ke = nullptr;      // EBE element matrices, pair-major: [(p * 9 + m) * n_el + e]
-    float *Ke32 = nullptr;     // the same in FP32
+    double *Ke = nullptr;      // EBE element matrices: entry q = 9 p + m at [((q / 2) n_el + e) 2 + q % 2]
+    float *Ke32 = nullptr;     // the same in FP32, chunks of 4: [((q / 4) n_el + e) 4 + q % 4]
     bool assembled = false;
 
     // boundary conditions and loads
@@ -326,6 +326,9 @@ hipError_t launch_assemble_values(jfm_ctx *ctx);
 // finite-strain element matrices (which: 0 Km, 1 Kg, 2 Km + Kg), element-major
 // into Ke and, when ye != null, f_int(u) into the incidence slots
 hipError_t launch_tl_element_matrices(jfm_ctx *ctx, const double *u, int which, double *Ke, double *ye);
+// Values per element of the EBE matrices: 9 NPAIR rounded up to a multiple
+// of the chunk width V (2 doubles or 4 floats).
+inline int64_t ke_stride(int npair, int V) { return (9 * (int64_t)npair + V - 1) / V * V; }
 // CSR values from element-major packed blocks Ke[(e * NPAIR + p) * 9 + m]
 hipError_t launch_assemble_values_ke(jfm_ctx *ctx, const double *Ke);
 hipError_t launch_face_loads(jfm_ctx *ctx);
