@@ -351,13 +351,15 @@ __global__ __launch_bounds__(256) void k_pcg_dir_coarse(int N, int slot, int npa
     const int a0 = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
     Trip<T> zv[kDN], pv[kDN], dv[kDN];
     Trip<uint8_t> fx[kDN];
-    int gv[kDN];
+    Trip<float> yc[kDN][2];  // the node's aggregate's (t, omega), read as soon as its id arrives
     auto load = [&](int u, int a) {
         zv[u] = ld3(z, a);
         pv[u] = ld3(p, a);
         dv[u] = ld3(cd, a);
         fx[u] = ld3(fixed, a);
-        gv[u] = node_agg[a];
+        const int g = node_agg[a];
+        yc[u][0] = ld3(zc, 2 * (size_t)g);
+        yc[u][1] = ld3(zc, 2 * (size_t)g + 1);
     };
 #pragma unroll
     for (int u = 0; u < kDN; ++u)
@@ -383,7 +385,7 @@ __global__ __launch_bounds__(256) void k_pcg_dir_coarse(int N, int slot, int npa
     const double beta_d = rz / S->rz[slot];
     const T beta = (T)beta_d;
     auto update = [&](int u, int a) {
-        const Trip<float> y0 = ld3(zc, 2 * (size_t)gv[u]), y1 = ld3(zc, 2 * (size_t)gv[u] + 1);
+        const Trip<float> y0 = yc[u][0], y1 = yc[u][1];
         const T y[6] = {(T)y0.a, (T)y0.b, (T)y0.c, (T)y1.a, (T)y1.b, (T)y1.c};
         const T d0[3] = {dv[u].a, dv[u].b, dv[u].c};
         const T c[3] = {y[0] + y[4] * d0[2] - y[5] * d0[1], y[1] + y[5] * d0[0] - y[3] * d0[2],

@@ -120,9 +120,10 @@ __global__ __launch_bounds__(kPatch) void k_ku_patch(int n_el, int npatch, int n
         if (o >= 0) {
             const uint32_t b = 3u * (uint32_t)o;
             if (fixed) {
-                if (load_at<uint8_t>(fixed, b)) s0 = T(0);
-                if (load_at<uint8_t>(fixed, b + 1)) s1 = T(0);
-                if (load_at<uint8_t>(fixed, b + 2)) s2 = T(0);
+                const Trip<uint8_t> fx = load_at<Trip<uint8_t>>(fixed, b);
+                if (fx.a) s0 = T(0);
+                if (fx.b) s1 = T(0);
+                if (fx.c) s2 = T(0);
             }
             store_at<Trip<T>>(y, (uint32_t)sizeof(T) * b, Trip<T>{s0, s1, s2});
             if (dotv) {
@@ -165,41 +166,55 @@ __global__ __launch_bounds__(256) void k_ku_combine(int nb, int chunk, const int
                                                    const uint8_t *__restrict__ fixed, T *__restrict__ y,
                                                    const T *__restrict__ dotv, const double *__restrict__ ppart,
                                                    int npatch, double *__restrict__ part, const int *__restrict__ done) {
-    // the first node's slot range is read with the convergence flag
+    // the first node's slot range, its first kPre partials, its mask and its
+    // dotv values are all in flight before the convergence flag is tested
+    // (the grid gives most threads exactly one node)
+    constexpr int kPre = 4;
     const int b0 = xcd_block() * chunk, b1 = min(b0 + chunk, nb);
-    int ra = 0, rb = 0;
-    if (b0 + (int)threadIdx.x < b1) {
-        ra = bptr[b0 + threadIdx.x];
-        rb = bptr[b0 + threadIdx.x + 1];
-    }
+    auto sum_node = [&](int ra, int rb, const Trip<T> (&v)[kPre], int a, const Trip<uint8_t> &f, const Trip<T> &d,
+                        double &dv) {
+        T s0 = T(0), s1 = T(0), s2 = T(0);
+#pragma unroll
+        for (int u = 0; u < kPre; ++u)
+            if (ra + u < rb) {
+                s0 += v[u].a;
+                s1 += v[u].b;
+                s2 += v[u].c;
+            }
+        for (int r = ra + kPre; r < rb; ++r) {
+            const Trip<T> w = ld3(bpart, r);
+            s0 += w.a;
+            s1 += w.b;
+            s2 += w.c;
+        }
+        if (f.a) s0 = T(0);
+        if (f.b) s1 = T(0);
+        if (f.c) s2 = T(0);
+        st3(y, a, s0, s1, s2);
+        if (dotv) dv += (double)s0 * (double)d.a + (double)s1 * (double)d.b + (double)s2 * (double)d.c;
+    };
+    auto load_node = [&](int b, int &ra, int &rb, Trip<T> (&v)[kPre], int &a, Trip<uint8_t> &f, Trip<T> &d) {
+        ra = bptr[b];
+        rb = bptr[b + 1];
+        a = bnode[b];
+#pragma unroll
+        for (int u = 0; u < kPre; ++u) v[u] = ra + u < rb ? ld3(bpart, ra + u) : Trip<T>{T(0), T(0), T(0)};
+        f = fixed ? ld3(fixed, a) : Trip<uint8_t>{0, 0, 0};
+        d = dotv ? ld3(dotv, a) : Trip<T>{T(0), T(0), T(0)};
+    };
+    int ra = 0, rb = 0, a = 0;
+    Trip<T> v[kPre], d{T(0), T(0), T(0)};
+    Trip<uint8_t> f{0, 0, 0};
+    const int bf = b0 + (int)threadIdx.x;
+    if (bf < b1) load_node(bf, ra, rb, v, a, f, d);
     if (done && *done) return;
     double dv = 0.0;
     if (dotv)
         for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < npatch; q += gridDim.x * blockDim.x) dv += ppart[q];
-    for (int b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
-        T s0 = T(0), s1 = T(0), s2 = T(0);
-        if (b != b0 + (int)threadIdx.x) {
-            ra = bptr[b];
-            rb = bptr[b + 1];
-        }
-        for (int r = ra; r < rb; ++r) {
-            const Trip<T> v = ld3(bpart, r);
-            s0 += v.a;
-            s1 += v.b;
-            s2 += v.c;
-        }
-        const int a = bnode[b];
-        if (fixed) {
-            const Trip<uint8_t> f = ld3(fixed, a);
-            if (f.a) s0 = T(0);
-            if (f.b) s1 = T(0);
-            if (f.c) s2 = T(0);
-        }
-        st3(y, a, s0, s1, s2);
-        if (dotv) {
-            const Trip<T> d = ld3(dotv, a);
-            dv += (double)s0 * (double)d.a + (double)s1 * (double)d.b + (double)s2 * (double)d.c;
-        }
+    if (bf < b1) sum_node(ra, rb, v, a, f, d, dv);
+    for (int b = bf + blockDim.x; b < b1; b += blockDim.x) {  // chunks longer than the block
+        load_node(b, ra, rb, v, a, f, d);
+        sum_node(ra, rb, v, a, f, d, dv);
     }
     if (part) {
         double v[1] = {dv};
