@@ -44,7 +44,7 @@ end
 SolveStats() = SolveStats(0, 0, 0, 0, 0.0, 0.0, 0.0)
 
 # FP32 two-level PCG inside FP64 refinement (the library's fastest exact-to-rtol method)
-default_opts(rtol) = SolveOpts(1, JFM_OP_MATFREE, JFM_OP_MATFREE, 0, rtol, 1e-2, 200000, 50, 32, 0, 0, 0.0, 2048)
+default_opts(rtol) = SolveOpts(1, JFM_OP_MATFREE, JFM_OP_MATFREE, 0, rtol, 2e-3, 200000, 50, 16, 0, 0, 0.0, 1536)
 
 function new_ctx()
     out = Ref{Ptr{Cvoid}}(C_NULL)

@@ -49,14 +49,14 @@ def test_learned_held_out_seeds_config1_size():
             ctx.upload(m)
             ctx.set_learned_weights(w)
             say(f"held-out seed {seed}: {m.n_dofs} DOFs")
-            u64, _, s64 = ctx.solve(method=0, op="matfree", precond=0, rtol=1e-9, coarse_nodes=2048,
-                                    check_every=32)
-            ub, _, sb = ctx.solve(method=1, op="matfree", precond=0, rtol=1e-8, inner_rtol=1e-3,
-                                  coarse_nodes=2048, check_every=32)
-            ul, _, sl = ctx.solve(method=1, op="matfree", precond=1, rtol=1e-8, inner_rtol=1e-3,
-                                  coarse_nodes=2048, check_every=32)
+            u64, _, s64 = ctx.solve(method=0, op="matfree", precond=0, rtol=1e-9, coarse_nodes=1536,
+                                    check_every=16)
+            ub, _, sb = ctx.solve(method=1, op="matfree", precond=0, rtol=1e-8, inner_rtol=2e-3,
+                                  coarse_nodes=1536, check_every=16)
+            ul, _, sl = ctx.solve(method=1, op="matfree", precond=1, rtol=1e-8, inner_rtol=2e-3,
+                                  coarse_nodes=1536, check_every=16)
             uh, _, sh = ctx.solve(method=1, op="matfree", precond=1, learned_probe=16, rtol=1e-8,
-                                  inner_rtol=2e-3, coarse_nodes=2048, check_every=32)
+                                  inner_rtol=2e-3, coarse_nodes=1536, check_every=16)
         finally:
             ctx.close()
         n64 = np.linalg.norm(u64)

@@ -23,11 +23,11 @@ pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
 # the benchmark's methods: (jfm_solve method, preconditioner, options)
-HEADLINE = dict(method=1, precond=1, learned_probe=16, inner_rtol=2e-3, coarse_nodes=2048)
+HEADLINE = dict(method=1, precond=1, learned_probe=16, inner_rtol=2e-3, coarse_nodes=1536, check_every=16)
 METHODS = {
     "mixed-learned-2l": HEADLINE,
-    "mixed-bj-2l": dict(method=1, precond=0, inner_rtol=1e-2, coarse_nodes=2048),
-    "fp64-bj-2l": dict(method=0, precond=0, coarse_nodes=2048),
+    "mixed-bj-2l": dict(method=1, precond=0, inner_rtol=2e-3, coarse_nodes=1536, check_every=16),
+    "fp64-bj-2l": dict(method=0, precond=0, coarse_nodes=1536, check_every=16),
     "fp64-bj": dict(method=0, precond=0),
 }
 
@@ -79,7 +79,8 @@ def test_mid_solve_vs_direct(mid, name):
     say(f"80x8x8 solve {name}")
     u, la, st = mid["ctx"].solve(method=o["method"], precond=o["precond"], rtol=1e-8,
                                  inner_rtol=o.get("inner_rtol", 1e-4), learned_probe=o.get("learned_probe", 0),
-                                 coarse_nodes=o.get("coarse_nodes", 0), check_every=32, want_la=True)
+                                 coarse_nodes=o.get("coarse_nodes", 0), check_every=o.get("check_every", 32),
+                                 want_la=True)
     ud = mid["ud"]
     err = np.linalg.norm(u - ud) / np.linalg.norm(ud)
     say(f"  iterations={st['iterations']} relres={st['relres']:.2e} err_vs_direct={err:.2e} "
@@ -147,7 +148,7 @@ def test_config1_solve_vs_oracle_fixture(big, name):
     say(f"160x16x16 solve {name}")
     u, _, st = big["ctx"].solve(method=o["method"], precond=o["precond"], rtol=1e-8,
                                 inner_rtol=o.get("inner_rtol", 1e-4), learned_probe=o.get("learned_probe", 0),
-                                coarse_nodes=o.get("coarse_nodes", 0), check_every=32)
+                                coarse_nodes=o.get("coarse_nodes", 0), check_every=o.get("check_every", 32))
     e = _compare(big["m"], u, ref)
     say(f"  iterations={st['iterations']} relres={st['relres']:.2e} err sample={e[0]:.2e} "
         f"projections={e[1]:.2e} norm={e[2]:.2e} end={e[3]:.2e}")

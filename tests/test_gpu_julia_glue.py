@@ -46,7 +46,7 @@ def new_ctx(lib):
 
 def default_opts(_lib, rtol):
     # JuliaFEMHIPExt.default_opts
-    return _lib.SolveOpts(1, OP_MATFREE, OP_MATFREE, 0, rtol, 1e-2, 200000, 50, 32, 0, 0, 0.0, 2048)
+    return _lib.SolveOpts(1, OP_MATFREE, OP_MATFREE, 0, rtol, 2e-3, 200000, 50, 16, 0, 0, 0.0, 1536)
 
 
 def test_assemble_elements_and_solve(glue, oracle):
