@@ -389,15 +389,27 @@ __global__ __launch_bounds__(256) void k_gj_panel(int n, int k0, int bk, const d
         for (int t = threadIdx.x; t < kGJ * kGJ; t += blockDim.x) Pbuf[t] = P[t / kGJ][t - kGJ * (t / kGJ)];
         if (threadIdx.x == 0 && !ok) flag[0] = 1;
     }
+    // Only the tiles on and below the diagonal (64 x 64) are kept, diagonal
+    // tiles whole: M_iK of a row above the pivot's tile is -M_Ki' (i pivoted
+    // earlier, K not), and row K of a column right of it is M_jK' (neither
+    // pivoted).
+    const int TK = k0 / 64;
     const int row = blockIdx.x * 8 + threadIdx.x / kGJ, m = threadIdx.x & (kGJ - 1);
     if (row < n) {
-        const double *Ar = A + (size_t)row * n + k0;
         double c = 0.0;
-        for (int l = 0; l < bk; ++l) c += Ar[l] * P[l][m];
+        if (row / 64 >= TK) {
+            const double *Ar = A + (size_t)row * n + k0;
+            for (int l = 0; l < bk; ++l) c += Ar[l] * P[l][m];
+            if (row / 64 > TK && m < bk) Rbuf[(size_t)m * n + row] = Ar[m];
+        } else {
+            for (int l = 0; l < bk; ++l) c -= A[(size_t)(k0 + l) * n + row] * P[l][m];
+        }
         Cbuf[(size_t)row * kGJ + m] = c;
     }
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < (int64_t)bk * n; t += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t mm = t / n, j = t - mm * n;
+    const int jmax = min(n, 64 * (TK + 1));  // row K itself up to the end of the pivot's tile
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < (int64_t)bk * jmax;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t mm = t / jmax, j = t - mm * jmax;
         Rbuf[mm * n + j] = A[(size_t)(k0 + mm) * n + j];
     }
 }
@@ -449,9 +461,10 @@ __device__ __forceinline__ void gj_next_pivot(int n, int bk, const double *__res
     if (threadIdx.x == 0 && !ok) flag[0] = 1;
 }
 
-// The update of step K over 64 x 64 tiles, 4 x 4 entries per thread.  With
-// nbk > 0, block 0 forms the next pivot block's inverse instead
-// (gj_next_pivot), beside the tiles and off the critical path of the step.
+// The update of step K over the 64 x 64 tiles on and below the diagonal,
+// 4 x 4 entries per thread.  With nbk > 0, block 0 forms the next pivot
+// block's inverse instead (gj_next_pivot), beside the tiles and off the
+// critical path of the step.
 __global__ __launch_bounds__(256) void k_gj_update(int n, int k0, int bk, double *__restrict__ A,
                                                   const double *__restrict__ Cbuf, const double *__restrict__ Rbuf,
                                                   const double *__restrict__ Pbuf, int nk0, int nbk,
@@ -462,9 +475,12 @@ __global__ __launch_bounds__(256) void k_gj_update(int n, int k0, int bk, double
         gj_next_pivot(n, bk, Anext, Cbuf, Rbuf, nk0, nbk, Pnext, flag, Cs, Rs, Ps);
         return;
     }
-    const int tiles = (n + 63) / 64;
+    // tiles on and below the diagonal: bid = ti (ti + 1) / 2 + tj, tj <= ti
     const int bid = (int)blockIdx.x - (nbk > 0 ? 1 : 0);
-    const int ti = bid / tiles, tj = bid - tiles * (bid / tiles);
+    int ti = (int)((sqrtf(8.f * (float)bid + 1.f) - 1.f) * 0.5f);
+    while ((ti + 1) * (ti + 2) / 2 <= bid) ++ti;
+    while (ti * (ti + 1) / 2 > bid) --ti;
+    const int tj = bid - ti * (ti + 1) / 2;
     const int i0 = 64 * ti, j0 = 64 * tj;
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
     // this thread's entries of A, in flight while the panels are staged
@@ -536,8 +552,8 @@ __global__ __launch_bounds__(256) void k_gj_update(int n, int k0, int bk, double
 
 __global__ void k_flag_clear(int *flag) { flag[0] = 0; }
 
-// Ahat = R^-1 A^-1 R^-T from the upper triangle (row-major) of the inverse:
-// entry (r, s) of A^-1 with r >= s is read at L[s * nc + r].
+// Ahat = R^-1 A^-1 R^-T from the lower triangle (row-major) of the inverse:
+// entry (r, s) of A^-1 with r >= s is read at L[r * nc + s].
 // Rows have stride ld; columns nc..ld-1 are set to zero.
 __global__ __launch_bounds__(256) void k_coarse_fold(int nc, int ld, const double *__restrict__ L,
                                                     const double *__restrict__ Rinv, const int32_t *__restrict__ dropped,
@@ -559,7 +575,7 @@ __global__ __launch_bounds__(256) void k_coarse_fold(int nc, int ld, const doubl
         double inner = 0.0;
         for (int l = 0; l < 6; ++l) {
             const int ss = 6 * h + l;
-            double a = rr >= ss ? L[(size_t)ss * nc + rr] : L[(size_t)rr * nc + ss];
+            double a = rr >= ss ? L[(size_t)rr * nc + ss] : L[(size_t)ss * nc + rr];
             if (dropped[rr] || dropped[ss]) a = 0.0;
             inner += a * Rh[j * 6 + l];
         }
@@ -1161,8 +1177,8 @@ static hipError_t setup_sequence(jfm_ctx *ctx, bool tangent) {
         const int nk0 = k0 + kGJ, nbk = nk0 < nc ? std::min(kGJ, nc - nk0) : 0;
         k_gj_panel<<<(nc + 7) / 8, 256, 0, s>>>(nc, k0, bk, ctx->Ac, ctx->gjC, ctx->gjR, ctx->gjP,
                                                 k0 > 0 ? Pnext : nullptr, nk0, nbk, Anext, flag);
-        k_gj_update<<<tiles * tiles + (nbk > 0 ? 1 : 0), 256, 0, s>>>(nc, k0, bk, ctx->Ac, ctx->gjC, ctx->gjR,
-                                                                      ctx->gjP, nk0, nbk, Anext, Pnext, flag);
+        k_gj_update<<<tiles * (tiles + 1) / 2 + (nbk > 0 ? 1 : 0), 256, 0, s>>>(
+            nc, k0, bk, ctx->Ac, ctx->gjC, ctx->gjR, ctx->gjP, nk0, nbk, Anext, Pnext, flag);
     }
     k_coarse_fold<<<nblk(nc * ctx->ldc), kBlock, 0, s>>>(nc, (int)ctx->ldc, ctx->Ac, ctx->Rinv, ctx->dropped, ctx->Ahat32);
     return hipGetLastError();
@@ -1401,6 +1417,7 @@ int jfem_coarse_matrix(jfm_ctx *ctx, int op, int coarse_nodes, int64_t *nc, doub
 #define JFEM_GEMV_ROWS 4
 #endif
 constexpr int kGemvRows = JFEM_GEMV_ROWS;  // rows (waves) per block of the coarse GEMV
+static_assert(64 * kGemvRows <= 256, "k_coarse_gemv is compiled for blocks of at most 256 threads");
 static unsigned gemv_grid(int64_t nc) {
     return (unsigned)std::min<int64_t>((nc + kGemvRows - 1) / kGemvRows, kMaxPart);
 }
