@@ -343,15 +343,18 @@ __device__ __forceinline__ bool gj_pivot_inverse(int bk, Ld &&ld, double (*P)[kG
                 for (int u = 0; u < R; ++u) ck[i0 + RG * u] = a[u];
             if (i0 == ik) rw[j] = a[uk];
             __syncthreads();
-            const double p = ck[k];
+            // every LDS read of the step at once, then 1/p
+            const double p = ck[k], rwj = rw[j];
+            double c[R];
+#pragma unroll
+            for (int u = 0; u < R; ++u) c[u] = ck[i0 + RG * u];
             ok = ok && p > 0.0 && p < INFINITY;
             const double ip = 1.0 / p;
-            const double r = rw[j] * ip;  // a_kj / p
+            const double r = rwj * ip;  // a_kj / p
             const bool piv = j == k;
 #pragma unroll
             for (int u = 0; u < R; ++u) {
-                const double c = ck[i0 + RG * u];
-                const double v = piv ? -c * ip : a[u] - c * r;
+                const double v = piv ? -c[u] * ip : a[u] - c[u] * r;
                 a[u] = (u == uk && i0 == ik) ? (piv ? ip : r) : v;
             }
         }
