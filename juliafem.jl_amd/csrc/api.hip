@@ -804,12 +804,12 @@ static hipError_t pcg_iteration(jfm_ctx *ctx, int op, int slot, const T *Minv, b
     int np_pq = 0, np_rz = 0;
     hipError_t e = op_apply<T>(ctx, op, p, q, p, ctx->partials + part_pq(slot), &ctx->S->done[slot], &np_pq);
     if (e != hipSuccess) return e;
-    if (coarse && jfem_coarse_w_recurrence(ctx)) {
-        // extra blocks of the update form w = w - alpha Q' m q for the GEMV
+    if (coarse && jfem_coarse_fused_restriction(ctx)) {
+        // the update's blocks, one per aggregate, form w = Q' m r for the GEMV
         int np_wz = 0;
         e = launch_pcg_update<T>(ctx, slot, np_pq, x, r, z, p, q, Minv, &np_rz, nullptr, true);
         if (e != hipSuccess) return e;
-        e = launch_coarse_gemv_rec(ctx, slot, &np_wz);
+        e = launch_coarse_gemv_fused(ctx, slot, &np_wz);
         if (e != hipSuccess) return e;
         return launch_pcg_direction_coarse<T>(ctx, slot, np_rz, np_wz, p, z);
     }

@@ -30,15 +30,14 @@
 //    operator, captured once as a HIP graph and replayed.
 //
 // Per PCG iteration:
-//  k_pcg_update       x, r, z = B r and the partials of r.Br and r.r; extra
-//                     blocks (one per aggregate) form the restriction by
-//                     recurrence, w_g <- w_g - alpha sum_{a in g} [m_a q_a ;
-//                     d_a x m_a q_a] (q = K p), as r <- r - alpha q; w is
-//                     formed directly from r at the PCG start
+//  k_pcg_update_agg   x, r, z = B r and the partials of r.Br and r.r, one
+//                     block per aggregate over its nodes; the block also
+//                     restricts the new residual, w_g = sum_{a in g}
+//                     [m_a r_a ; d_a x m_a r_a], from the values it holds
 //  k_coarse_gemv      zc = Ahat w, partials of w.zc             (wave per row)
 //  k_pcg_dir_coarse   p_a = z_a + m_a o (t_g + omega_g x d_a) + beta p_a
-// (JFEM_W_RECURRENCE=0: the update writes m o r in aggregate order and
-// k_coarse_restrict sums it after the update instead.)
+// (JFEM_FUSED_RESTRICTION=0: the node-ordered update writes m o r in
+// aggregate order and k_coarse_restrict sums it after the update instead.)
 // The prolongation is folded into the direction update, and r.z of the full
 // preconditioned residual is r.Br + w.zc (r.(m o Q zc) = (Q' m r).zc), so the
 // corrected z is never stored.  At the start of PCG, k_coarse_prolong adds
@@ -1450,17 +1449,17 @@ hipError_t launch_coarse_product_agg(jfm_ctx *ctx, const T *rA, const int *done,
     return hipGetLastError();
 }
 
-bool jfem_coarse_w_recurrence(const jfm_ctx *ctx) {
+bool jfem_coarse_fused_restriction(const jfm_ctx *ctx) {
     static const bool on = [] {
-        const char *v = std::getenv("JFEM_W_RECURRENCE");
+        const char *v = std::getenv("JFEM_FUSED_RESTRICTION");
         return !(v && v[0] == '0');
     }();
     return on && ctx->n_agg > 0;
 }
 
-// zc = Ahat w_new for the iteration of parity `slot`, w_new as the update
-// kernel's restriction blocks wrote it (wc32, and wc64 + (slot ^ 1) nc)
-hipError_t launch_coarse_gemv_rec(jfm_ctx *ctx, int slot, int *npart_wz) {
+// zc = Ahat w for the iteration of parity `slot`, w as the aggregate-ordered
+// update kernel wrote it (wc32, and wc64 + (slot ^ 1) nc)
+hipError_t launch_coarse_gemv_fused(jfm_ctx *ctx, int slot, int *npart_wz) {
     const int nc = (int)ctx->nc;
     k_coarse_gemv<<<gemv_grid(nc), 64 * kGemvRows, 0, ctx->stream>>>(
         nc, (int)(ctx->ldc / 4), (const float4 *)ctx->Ahat32, (const float4 *)ctx->wc32,

@@ -64,6 +64,36 @@ __device__ __forceinline__ void sum_partials(const double *__restrict__ part, in
     block_sum_all<NV>(out);
 }
 
+// The same sums formed by one wave alone, with no barrier, so that a kernel
+// can take them while its other loads are in flight: lane l loads partials
+// l, l + 64, ... (all issued before the first add), adds them in that order,
+// then the xor butterfly.  Every wave of every block obtains the same bits
+// (the butterfly's additions are commutative pairs), which differ from
+// sum_partials' only in the order of the additions.
+template <int NV> struct WavePartials {
+    static constexpr int kPer = kMaxPart / 64;
+    double v[NV][kPer];
+    __device__ __forceinline__ void load(const double *__restrict__ part, int npart) {
+        const int lane = (int)threadIdx.x & 63;
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+#pragma unroll
+            for (int j = 0; j < kPer; ++j) {
+                const int i = lane + 64 * j;
+                v[k][j] = i < npart ? part[k * kMaxPart + i] : 0.0;
+            }
+    }
+    __device__ __forceinline__ void sum(double (&out)[NV]) const {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            double s = 0.0;
+#pragma unroll
+            for (int j = 0; j < kPer; ++j) s += v[k][j];
+            out[k] = wave_sum(s);
+        }
+    }
+};
+
 __device__ __forceinline__ bool finite(double x) { return x - x == 0.0; }
 
 // Loads at a 32-bit byte offset from a uniform base pointer, so that the

@@ -382,14 +382,13 @@ hipError_t launch_coarse_product(jfm_ctx *ctx, const T *r, const int *done, int 
 template <typename T>
 hipError_t launch_coarse_product_agg(jfm_ctx *ctx, const T *rA, const int *done, int *npart_wz);
 template <typename T> T *coarse_rA(jfm_ctx *ctx);
-// The iteration's coarse right-hand side by recurrence: w_{k+1} = w_k -
-// alpha Q' m q_k, formed by extra blocks of the update kernel
-// (launch_pcg_update(..., restrict_q = true)) from the previous parity's w
-// in FP64 (wc64 + slot nc; slot 0 is the exact restriction of the PCG
-// start); the GEMV then reads it as the direct restriction leaves it.  Used
-// when jfem_coarse_w_recurrence(ctx) (JFEM_W_RECURRENCE=0 turns it off).
-bool jfem_coarse_w_recurrence(const jfm_ctx *ctx);
-hipError_t launch_coarse_gemv_rec(jfm_ctx *ctx, int slot, int *npart_wz);
+// The iteration's coarse right-hand side w = Q' m r of the new residual,
+// formed inside the update kernel, one block per aggregate
+// (launch_pcg_update(..., restrict_q = true)), in FP64 at wc64 + (slot ^ 1)
+// nc and in FP32 at wc32, where the GEMV reads it.  Used when
+// jfem_coarse_fused_restriction(ctx) (JFEM_FUSED_RESTRICTION=0 turns it off).
+bool jfem_coarse_fused_restriction(const jfm_ctx *ctx);
+hipError_t launch_coarse_gemv_fused(jfm_ctx *ctx, int slot, int *npart_wz);
 // p = z + m o Q zc + beta p, beta from the (r.z, r.r) and w.zc partials
 template <typename T>
 hipError_t launch_pcg_direction_coarse(jfm_ctx *ctx, int slot, int npart_rz, int npart_wz, T *p, const T *z);

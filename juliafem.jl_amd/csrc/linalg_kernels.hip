@@ -128,82 +128,6 @@ __global__ __launch_bounds__(256) void k_pcg_init(int N, const T *__restrict__ r
     write_partials<2>(v, part);
 }
 
-// The coarse right-hand side by recurrence, one block per aggregate g:
-// w_new,g = w_old,g - alpha u_g with u_g = sum_{a in g} [m_a q_a ; d_a x
-// (m_a q_a)] over agg_nodes[agg_ptr[g] .. agg_ptr[g + 1]) (cdA: the offsets
-// d in that order), alpha as the update blocks form it (rounded to T as they
-// apply it), written in FP32 (wc, the GEMV's operand) and FP64 (w_new).  A
-// thread's nodes are loaded JFEM_RESTRICT_NODES (2) at a time: the FP64
-// solve measured 0.8% faster than with 4, the FP32 one the same.
-template <typename T>
-__device__ __forceinline__ void restrict_q(int g, int slot, int npart_pq, const int32_t *__restrict__ agg_ptr,
-                                           const int32_t *__restrict__ agg_nodes, const T *__restrict__ cdA,
-                                           const uint8_t *__restrict__ fixed, const T *__restrict__ q,
-                                           const double *__restrict__ partials, const PcgScalars *S,
-                                           const double *__restrict__ w_old, double *__restrict__ w_new,
-                                           float *__restrict__ wc) {
-#ifndef JFEM_RESTRICT_NODES
-#define JFEM_RESTRICT_NODES 2
-#endif
-    constexpr int kR = JFEM_RESTRICT_NODES;  // nodes per thread whose loads are in flight together
-    const int k1 = agg_ptr[g + 1];
-    T y[kR][3], d[kR][3];
-    auto load = [&](int k0) {
-#pragma unroll
-        for (int u = 0; u < kR; ++u) {
-            const int kk = k0 + u * (int)blockDim.x;
-            if (kk < k1) {
-                const int a = agg_nodes[kk];
-                const Trip<T> qa = ld3(q, a), da = ld3(cdA, kk);
-                const Trip<uint8_t> fa = ld3(fixed, a);
-                y[u][0] = fa.a ? T(0) : qa.a;
-                y[u][1] = fa.b ? T(0) : qa.b;
-                y[u][2] = fa.c ? T(0) : qa.c;
-                d[u][0] = da.a;
-                d[u][1] = da.b;
-                d[u][2] = da.c;
-            } else {
-#pragma unroll
-                for (int c = 0; c < 3; ++c) y[u][c] = d[u][c] = T(0);
-            }
-        }
-    };
-    int k = agg_ptr[g] + threadIdx.x;
-    load(k);
-    if (S->done[slot]) return;  // uniform
-    double w[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (;;) {
-#pragma unroll
-        for (int u = 0; u < kR; ++u) {
-            const double y0 = y[u][0], y1 = y[u][1], y2 = y[u][2];
-            const double d0 = d[u][0], d1 = d[u][1], d2 = d[u][2];
-            w[0] += y0;
-            w[1] += y1;
-            w[2] += y2;
-            w[3] += d1 * y2 - d2 * y1;
-            w[4] += d2 * y0 - d0 * y2;
-            w[5] += d0 * y1 - d1 * y0;
-        }
-        k += kR * (int)blockDim.x;
-        if (k >= k1) break;
-        load(k);
-    }
-    double pq[1];
-    sum_partials<1>(partials + part_pq(slot), npart_pq, pq);
-    const double alpha_d = (pq[0] > 0.0 && finite(pq[0])) ? S->rz[slot] / pq[0] : 0.0;
-    const double alpha = (double)(T)alpha_d;
-    block_sum_all<6>(w);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int j = 0; j < 6; ++j) {
-            const size_t o = 6 * (size_t)g + j;
-            const double wn = w_old[o] - alpha * w[j];
-            w_new[o] = wn;
-            wc[o] = (float)wn;
-        }
-    }
-}
-
 // x += alpha p, r -= alpha q, z = Minv r; partials (r.z, r.r).  A thread's
 // first kU nodes (4 in FP32, 2 in FP64: every node at the usual grid) are
 // all loaded before the alpha reduction, so that their latencies overlap
@@ -211,40 +135,14 @@ __device__ __forceinline__ void restrict_q(int g, int slot, int npart_pq, const 
 // With rA, the masked residual also goes to rA in aggregate order (node a at
 // position apos[a]) for the coarse restriction, which then reads it
 // contiguously.
-// With w_new, extra blocks form the coarse right-hand side by recurrence
-// (restrict_q) instead.
 template <typename T>
 __global__ __launch_bounds__(512) void k_pcg_update(int N, int slot, int npart_pq, T *__restrict__ x,
                                                    T *__restrict__ r, T *__restrict__ z, const T *__restrict__ p,
                                                    const T *__restrict__ q, const T *__restrict__ Minv,
                                                    double *__restrict__ partials, PcgScalars *S,
                                                    T *__restrict__ rA = nullptr, const int32_t *__restrict__ apos = nullptr,
-                                                   const uint8_t *__restrict__ fixed = nullptr, int n_agg = 0,
-                                                   const int32_t *__restrict__ agg_ptr = nullptr,
-                                                   const int32_t *__restrict__ agg_nodes = nullptr,
-                                                   const T *__restrict__ cdA = nullptr,
-                                                   const double *__restrict__ w_old = nullptr,
-                                                   double *__restrict__ w_new = nullptr, float *__restrict__ wc = nullptr) {
-    // With w_new, the first xcd_grid(n_agg) blocks (dispatched first, so that
-    // they run beside the update) form the coarse right-hand side by
-    // recurrence, and both parts give each XCD a contiguous eighth of their
-    // aggregates and of the nodes: aggregates are numbered along x as the
-    // nodes usually are, so an aggregate, its neighbours that share cache
-    // lines of q with it and the update of the same nodes meet in one L2.
-    int b = (int)blockIdx.x;
-    int nb = (int)gridDim.x;
-    if (w_new) {
-        const int na = (int)xcd_grid((unsigned)n_agg);
-        if (b < na) {
-            const int g = (b % kXcds) * (na / kXcds) + b / kXcds;
-            if (g < n_agg)
-                restrict_q<T>(g, slot, npart_pq, agg_ptr, agg_nodes, cdA, fixed, q, partials, S, w_old, w_new, wc);
-            return;
-        }
-        b -= na;
-        nb -= na;  // a multiple of kXcds
-        b = (b % kXcds) * (nb / kXcds) + b / kXcds;
-    }
+                                                   const uint8_t *__restrict__ fixed = nullptr) {
+    const int b = (int)blockIdx.x, nb = (int)gridDim.x;
     constexpr int kU = sizeof(T) == 4 ? 4 : 2;
     const int stride = nb * blockDim.x;
     const int a0 = b * blockDim.x + threadIdx.x;
@@ -293,6 +191,113 @@ __global__ __launch_bounds__(512) void k_pcg_update(int N, int slot, int npart_p
     }
     write_partials<2>(v, partials + part_rz(slot), b);
     if (b == 0 && threadIdx.x == 0) {
+        S->alpha = alpha_d;
+        S->pq = pq[0];
+    }
+}
+
+// Two-level form of the update, with the aggregates' nodes as the blocks'
+// work: block b takes aggregates b per .. (b + 1) per - 1, each one's nodes
+// agg_nodes[agg_ptr[g] .. agg_ptr[g + 1]) (ascending node id).  Besides x, r,
+// z and the partials of (r.Br, r.r) (slot b), it forms the coarse right-hand
+// side of the new residual from the values in registers, w_g = sum_{a in g}
+// [m_a r_a ; d_a x (m_a r_a)] (cdA: the offsets d in agg_nodes order), in
+// FP64 (w_new) and FP32 (wc, the GEMV's operand).  The first aggregate's
+// first kU nodes per thread are in flight before the alpha reduction.
+// 512 threads with four FP32 (three FP64) nodes each cover a 1,536-node
+// aggregate in one pass of loads; 1,024 threads with two nodes each measured
+// 4% slower solves.
+constexpr int kAggThreads = 512;
+template <typename T>
+__global__ __launch_bounds__(kAggThreads) void k_pcg_update_agg(
+    int n_agg, int nblk, int per, int slot, int npart_pq, T *__restrict__ x, T *__restrict__ r, T *__restrict__ z,
+    const T *__restrict__ p, const T *__restrict__ q, const T *__restrict__ Minv, const uint8_t *__restrict__ fixed,
+    const int32_t *__restrict__ agg_ptr, const int32_t *__restrict__ agg_nodes, const T *__restrict__ cdA,
+    double *__restrict__ partials, PcgScalars *S, double *__restrict__ w_new, float *__restrict__ wc) {
+    const int b = xcd_block();
+    if (b >= nblk) return;  // whole block (grid padded to a multiple of 8)
+    const int g0 = b * per, g1 = min(g0 + per, n_agg);
+    constexpr int kU = sizeof(T) == 4 ? 4 : 3;
+    const int t = (int)threadIdx.x, nt = (int)blockDim.x;
+    Trip<T> xv[kU], pv[kU], rv[kU], qv[kU], dv[kU];
+    Blk9<T> M[kU];
+    Trip<uint8_t> fv[kU];
+    int av[kU];
+    auto load = [&](int u, int k) {
+        const int a = agg_nodes[k];
+        av[u] = a;
+        xv[u] = ld3(x, a);
+        pv[u] = ld3(p, a);
+        rv[u] = ld3(r, a);
+        qv[u] = ld3(q, a);
+        M[u] = ld9(Minv, a);
+        fv[u] = ld3(fixed, a);
+        dv[u] = ld3(cdA, k);
+    };
+    int kb = 0, ke = 0;
+    auto prefetch = [&](int g) {
+        kb = agg_ptr[g];
+        ke = agg_ptr[g + 1];
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+            if (kb + t + u * nt < ke) load(u, kb + t + u * nt);
+    };
+    // the p.q partials are loaded first and summed by each wave alone, so
+    // that alpha is known while the node loads are still in flight and the
+    // first nodes' stores overlap the later nodes' loads (0.8 us faster than
+    // the block reduction after the loads)
+    WavePartials<1> wp;
+    wp.load(partials + part_pq(slot), npart_pq);
+    if (g0 < g1) prefetch(g0);
+    if (S->done[slot]) return;  // uniform
+    double pq[1];
+    wp.sum(pq);
+    const double alpha_d = (pq[0] > 0.0 && finite(pq[0])) ? S->rz[slot] / pq[0] : 0.0;
+    const T alpha = (T)alpha_d;
+    double v[2] = {0.0, 0.0};
+    auto update = [&](int u, double (&w)[6]) {
+        const int a = av[u];
+        const T rn[3] = {rv[u].a - alpha * qv[u].a, rv[u].b - alpha * qv[u].b, rv[u].c - alpha * qv[u].c};
+        st3(x, a, xv[u].a + alpha * pv[u].a, xv[u].b + alpha * pv[u].b, xv[u].c + alpha * pv[u].c);
+        st3(r, a, rn[0], rn[1], rn[2]);
+        const T *Mu = M[u].v;
+        const T z0 = Mu[0] * rn[0] + Mu[1] * rn[1] + Mu[2] * rn[2];
+        const T z1 = Mu[3] * rn[0] + Mu[4] * rn[1] + Mu[5] * rn[2];
+        const T z2 = Mu[6] * rn[0] + Mu[7] * rn[1] + Mu[8] * rn[2];
+        st3(z, a, z0, z1, z2);
+        v[0] += (double)rn[0] * z0 + (double)rn[1] * z1 + (double)rn[2] * z2;
+        v[1] += (double)rn[0] * rn[0] + (double)rn[1] * rn[1] + (double)rn[2] * rn[2];
+        const double y0 = fv[u].a ? 0.0 : (double)rn[0], y1 = fv[u].b ? 0.0 : (double)rn[1],
+                     y2 = fv[u].c ? 0.0 : (double)rn[2];
+        const double d0 = dv[u].a, d1 = dv[u].b, d2 = dv[u].c;
+        w[0] += y0;
+        w[1] += y1;
+        w[2] += y2;
+        w[3] += d1 * y2 - d2 * y1;
+        w[4] += d2 * y0 - d0 * y2;
+        w[5] += d0 * y1 - d1 * y0;
+    };
+    for (int g = g0; g < g1; ++g) {
+        if (g > g0) prefetch(g);
+        double w[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+            if (kb + t + u * nt < ke) update(u, w);
+        for (int k = kb + t + kU * nt; k < ke; k += nt) {  // aggregates of more than kU x nt nodes
+            load(0, k);
+            update(0, w);
+        }
+        block_sum_all<6>(w);
+        if (t == 0) {
+#pragma unroll
+            for (int j = 0; j < 6; ++j) {
+                w_new[6 * (size_t)g + j] = w[j];
+                wc[6 * (size_t)g + j] = (float)w[j];
+            }
+        }
+    }
+    write_partials<2>(v, partials + part_rz(slot), b);
+    if (b == 0 && t == 0) {
         S->alpha = alpha_d;
         S->pq = pq[0];
     }
@@ -516,17 +521,15 @@ hipError_t launch_pcg_update(jfm_ctx *ctx, int slot, int npart_pq, T *x, T *r, T
                              const T *Minv, int *npart, T *rA, bool restrict_q) {
     const unsigned g = std::min<unsigned>(capped_grid(ctx->N, kBlock), kVecBlocks);
     *npart = (int)g;
-    if (restrict_q) {  // xcd_grid(n_agg) more blocks form the coarse right-hand side (wc64 by parity)
+    if (restrict_q) {  // one block per aggregate (or per run of them), w to wc64 by parity
         const T *cdA = sizeof(T) == 8 ? (const T *)ctx->cdA64 : (const T *)ctx->cdA32;
         const int n_agg = (int)ctx->n_agg;
-        // 512-thread blocks: an aggregate's nodes in one pass of four per
-        // thread; the update part keeps the thread count of the plain update
-        const unsigned gv = xcd_grid((g + 1) / 2);
-        *npart = (int)gv;
-        k_pcg_update<T><<<gv + xcd_grid((unsigned)n_agg), 2 * kBlock, 0, ctx->stream>>>(
-            (int)ctx->N, slot, npart_pq, x, r, z, p, q, Minv, ctx->partials, ctx->S, nullptr, nullptr, ctx->fixed,
-            n_agg, ctx->agg_ptr, ctx->agg_nodes, cdA, ctx->wc64 + (size_t)slot * ctx->nc,
-            ctx->wc64 + (size_t)(slot ^ 1) * ctx->nc, ctx->wc32);
+        const int per = (n_agg + kMaxPart - 1) / kMaxPart;
+        const int nb = (n_agg + per - 1) / per;
+        *npart = nb;
+        k_pcg_update_agg<T><<<xcd_grid((unsigned)nb), kAggThreads, 0, ctx->stream>>>(
+            n_agg, nb, per, slot, npart_pq, x, r, z, p, q, Minv, ctx->fixed, ctx->agg_ptr, ctx->agg_nodes, cdA,
+            ctx->partials, ctx->S, ctx->wc64 + (size_t)(slot ^ 1) * ctx->nc, ctx->wc32);
         return hipGetLastError();
     }
     k_pcg_update<T><<<g, kBlock, 0, ctx->stream>>>((int)ctx->N, slot, npart_pq, x, r, z, p, q, Minv,

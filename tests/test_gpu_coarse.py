@@ -142,31 +142,31 @@ np.savez({out!r}, **out)
 """
 
 
-def test_restriction_recurrence_matches_direct_restriction(cfg0, tmp_path):
-    """Per iteration the coarse right-hand side follows w <- w - alpha Q' m q
-    (q = K p, restricted by extra blocks of the update kernel), and is formed
-    exactly from r at every PCG start.  Against the direct restriction of r
-    at every iteration (JFEM_W_RECURRENCE=0, a child process): the same
-    answer, iteration counts within rounding (3% + 2), in FP64 PCG and in the
-    FP32 inner solves."""
+def test_fused_restriction_matches_separate_restriction(cfg0, tmp_path):
+    """Per iteration the coarse right-hand side w = Q' m r is formed inside
+    the update kernel, one block per aggregate, from the new residual it
+    holds.  Against the restriction by a kernel of its own after a
+    node-ordered update (JFEM_FUSED_RESTRICTION=0, a child process): the
+    same answer, iteration counts within rounding (3% + 2), in FP64 PCG and
+    in the FP32 inner solves."""
     m, ctx, csr, f, ud = cfg0
     res = {}
     for meth in (0, 1):
-        say(f"restriction by recurrence, method={meth}")
+        say(f"restriction inside the update, method={meth}")
         u, _, st = ctx.solve(method=meth, op="matfree", op_outer="matfree", rtol=1e-10, inner_rtol=1e-3,
                              coarse_nodes=32)
         assert st["converged"] == 1
         res[meth] = (u, st["iterations"])
     out = str(tmp_path / "direct.npz")
     code = _CHILD.format(root=ROOT, pkg=os.path.join(ROOT, "juliafem.jl_amd"), out=out)
-    say("direct restriction in a child process (JFEM_W_RECURRENCE=0)")
-    env = dict(os.environ, JFEM_W_RECURRENCE="0")
+    say("separate restriction kernel in a child process (JFEM_FUSED_RESTRICTION=0)")
+    env = dict(os.environ, JFEM_FUSED_RESTRICTION="0")
     subprocess.run([sys.executable, "-c", code], env=env, check=True, timeout=120)
     r = np.load(out)
     for meth in (0, 1):
         u, it = res[meth]
         it_d = int(r["it%d" % meth])
-        say(f"  method={meth}: iterations recurrence={it} direct={it_d}")
+        say(f"  method={meth}: iterations fused={it} separate={it_d}")
         assert abs(it - it_d) <= 2 + 0.03 * it_d
         assert np.linalg.norm(u - r["u%d" % meth]) <= 1e-8 * np.linalg.norm(u)
         assert np.linalg.norm(u - ud) <= 1e-8 * np.linalg.norm(ud)
