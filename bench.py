@@ -68,14 +68,16 @@ def kernel_bytes(inf, op, prec):
     if op == "csr":
         spmv = 9 * nnzb * T + 4 * nnzb + 4 * (N + 1) + 2 * 3 * N * T + 3 * N
         return {"spmv": spmv}
-    if op == "matfree" and inf.get("n_patches", 0) > 0:
+    if op in ("matfree", "ebe") and inf.get("n_patches", 0) > 0:
         # patch kernel: connectivity (int32, npe per element), the geometric
-        # factors, x once, the patch-local incidences (uint16, npe per element),
-        # per patch node its count (uint16) and target (int32); the fixed mask
-        # and y of the interior nodes, the partial sums of the shared entries
+        # factors (matrix-free) or the stored upper blocks of K_e (EBE), x
+        # once, the patch-local incidences (uint16, npe per element), per
+        # patch node its count (uint16) and target (int32); the fixed mask and
+        # y of the interior nodes, the partial sums of the shared entries
         npn, nb, ns = inf["patch_nodes"], inf["shared_nodes"], inf["shared_slots"]
         interior = npn - ns
-        patch = (4 * npe * n_el + 11 * ngf * n_el * T + 3 * N * T + 2 * npe * n_el + 6 * npn
+        el = 11 * ngf * n_el * T if op == "matfree" else T * 9 * (npe * (npe + 1) // 2) * n_el
+        patch = (4 * npe * n_el + el + 3 * N * T + 2 * npe * n_el + 6 * npn
                  + 3 * interior + 3 * interior * T + 3 * ns * T)
         # shared-node kernel: slot ranges and node ids, the partials, mask, y
         combine = 8 * nb + 4 + 3 * ns * T + 3 * nb + 3 * nb * T
@@ -193,7 +195,8 @@ def main():
                 ent["patch_GBps"] = b["patch"] / (k1 * 1e-3) / 1e9
                 ent["combine_GBps"] = b["combine"] / (k2 * 1e-3) / 1e9
                 ent["Ku_GBps"] = (b["patch"] + b["combine"]) / (tot * 1e-3) / 1e9
-                ent["patch_TFLOPs"] = flops_matfree_elem(inf) / (k1 * 1e-3) / 1e12
+                if op == "matfree":
+                    ent["patch_TFLOPs"] = flops_matfree_elem(inf) / (k1 * 1e-3) / 1e12
             else:
                 ent["element_ms"], ent["gather_ms"] = k1, k2
                 ent["element_GBps"] = b["element"] / (k1 * 1e-3) / 1e9
@@ -314,9 +317,13 @@ def main():
         ke = kernels.get(f"ebe_{key}")
         if ke:
             b = ebe_algorithmic(inf, pr)
-            roofline["ebe"][key] = {"kernel": f"k_ebe_elem<10,{'double' if pr == 0 else 'float'}>",
-                                   "bytes_per_launch": b, "element_ms": ke["element_ms"], "ku_ms": ke["ms"],
-                                   "element_frac": b / (ke["element_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            tn = 'double' if pr == 0 else 'float'
+            # the patch form (stored-matrix element products summed in LDS,
+            # then the shared-node kernel), or the two-phase one without patches
+            k1 = ke.get("patch_ms", ke.get("element_ms"))
+            roofline["ebe"][key] = {"kernel": f"k_ku_patch<10,{tn},EBE>" if "patch_ms" in ke else f"k_ebe_elem<10,{tn}>",
+                                   "bytes_per_launch": b, "element_ms": k1, "ku_ms": ke["ms"],
+                                   "element_frac": b / (k1 * 1e-3) / 1e9 / HBM_PEAK_GBS,
                                    "ku_frac": b / (ke["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
 
     # ---- CPU baseline (oracle, bounded samples) -------------------------------

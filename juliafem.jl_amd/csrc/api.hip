@@ -528,10 +528,16 @@ template <> double *vals_of<double>(jfm_ctx *ctx) { return ctx->vals64; }
 template <> float *vals_of<float>(jfm_ctx *ctx) { return ctx->vals32; }
 
 // the EBE operator's element matrices (FP64 and FP32, pair-major)
+// With the patch operator, the element matrices are stored in its element
+// order and the EBE product runs in the patch form (element results summed
+// in LDS, shared nodes by the combine kernel).
 static int ensure_element_matrices(jfm_ctx *ctx) {
     if (ctx->Ke) return JFM_OK;
+    if (jfem_patch_enabled())
+        if (int rc = jfem_patch_setup(ctx)) return rc;
     ALLOC(ctx, ctx->Ke, double, ctx->n_el * ke_stride(ctx->npair, 2));
     ALLOC(ctx, ctx->Ke32, float, ctx->n_el * ke_stride(ctx->npair, 4));
+    ctx->ke_perm = ctx->patch_ready ? ctx->eperm : nullptr;
     CHECK_LAUNCH(ctx, launch_element_matrices(ctx));
     return JFM_OK;
 }
@@ -628,10 +634,10 @@ extern "C" int jfm_time_apply(jfm_ctx *ctx, int32_t op, int32_t precision, int32
                        : launch_bsr_spmv<float>(ctx, ctx->vals32, ctx->vx32, ctx->vy32, nullptr, nullptr, nullptr, nullptr);
         }
         hipError_t e = hipSuccess;
-        if (op == JFM_OP_MATFREE && ctx->patch_ready) {  // patch kernel, then the shared-node kernel
+        if (jfem_patch_form(ctx, op)) {  // patch kernel, then the shared-node kernel
             if (part != 2)
-                e = f64 ? launch_patch_phase<double>(ctx, ctx->vx64, ctx->vy64, nullptr, nullptr)
-                        : launch_patch_phase<float>(ctx, ctx->vx32, ctx->vy32, nullptr, nullptr);
+                e = f64 ? launch_patch_phase<double>(ctx, op, ctx->vx64, ctx->vy64, nullptr, nullptr)
+                        : launch_patch_phase<float>(ctx, op, ctx->vx32, ctx->vy32, nullptr, nullptr);
             if (e != hipSuccess || part == 1) return e;
             return f64 ? launch_patch_combine<double>(ctx, ctx->vy64, nullptr, nullptr, nullptr, nullptr)
                        : launch_patch_combine<float>(ctx, ctx->vy32, nullptr, nullptr, nullptr, nullptr);

@@ -19,21 +19,10 @@
 #include <algorithm>
 
 #include "jfem_device.h"
+#include "ebe_element.h"
 #include "mf_element.h"
 
 namespace jfem {
-
-template <int NPE> struct PairTable { int i[Elem<NPE>::NPAIR], j[Elem<NPE>::NPAIR]; };
-template <int NPE> constexpr PairTable<NPE> make_pairs() {
-    PairTable<NPE> t{};
-    int p = 0;
-    for (int i = 0; i < NPE; ++i)
-        for (int j = i; j < NPE; ++j) { t.i[p] = i; t.j[p] = j; ++p; }
-    return t;
-}
-template <int NPE> __host__ __device__ constexpr int pair_index(int i, int j) {
-    return i * NPE - (i * (i - 1)) / 2 + (j - i);
-}
 
 // ------------------------------------------------------- geometric factors --
 // invJ, lambda*w*detJ, mu*w*detJ per element (affine) or per Gauss point.
@@ -120,76 +109,28 @@ __global__ __launch_bounds__(256) void k_mf_elem(int n_el, const int32_t *__rest
 }
 
 // ------------------------------------------------------------- EBE element --
-// Phase 1 with stored element matrices: y_e = K_e x_e.  The upper blocks are
-// stored in the operator's precision as chunks of V = 16 / sizeof(T) values
-// (two doubles, four floats) interleaved by element: entry q = 9 p + m
-// (block p = (i <= j), entry m) of element e at Ke[((q / V) n_el + e) V +
-// q % V], so that a lane loads 16 bytes per instruction and the lanes of a
-// wave read consecutive addresses (ke_stride rounds 9 NPAIR up to a
-// multiple of V).
-// The element matrices are read once per product: non-temporal loads keep
-// them from displacing x and the connectivity in the caches (at 1.05M DOFs
-// the FP64 element phase went from 0.214 to 0.185 ms, FP32 from 0.105 to
-// 0.090 ms).
-#ifndef JFEM_EBE_NT
-#define JFEM_EBE_NT 1
-#endif
+// Phase 1 with stored element matrices: y_e = K_e x_e (ebe_element.h), one
+// thread per element in the order the matrices are stored.  With eperm the
+// matrices are in the patch operator's order: position k holds element
+// eperm[k], conn is the patch-ordered connectivity (indexed by k) and the
+// results go to element eperm[k]'s incidence slots.
 template <int NPE, typename T>
 __global__ __launch_bounds__(256) void k_ebe_elem(int n_el, const int32_t *__restrict__ conn,
-                                                 const int32_t *__restrict__ slot, const T *__restrict__ Ke,
-                                                 const T *__restrict__ x, T *__restrict__ ye,
+                                                 const int32_t *__restrict__ eperm, const int32_t *__restrict__ slot,
+                                                 const T *__restrict__ Ke, const T *__restrict__ x, T *__restrict__ ye,
                                                  const int *__restrict__ done) {
-    using EL = Elem<NPE>;
-    constexpr PairTable<NPE> PT = make_pairs<NPE>();
     if (done && *done) return;
-    const int e = xcd_block() * blockDim.x + threadIdx.x;
-    if (e >= n_el) return;
-    // K_e annihilates translations: subtracting node 0's value keeps FP32 accurate
-    T u[NPE][3], f[NPE][3], t[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) t[c] = x[3 * (size_t)conn[e] + c];
-#pragma unroll
-    for (int i = 0; i < NPE; ++i) {
-        const int nd = conn[(size_t)i * n_el + e];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) { u[i][c] = x[3 * (size_t)nd + c] - t[c]; f[i][c] = T(0); }
-    }
-    constexpr int V = 16 / sizeof(T), NQ = 9 * EL::NPAIR;
-    struct alignas(16) Chunk { T v[V]; };
-    const Chunk *Kc = reinterpret_cast<const Chunk *>(Ke);
-    T B[9];
-#pragma clang loop unroll(full)
-    for (int ch = 0; ch < (NQ + V - 1) / V; ++ch) {
-#if JFEM_EBE_NT
-        Chunk kv;
-#pragma unroll
-        for (int v = 0; v < V; ++v) kv.v[v] = __builtin_nontemporal_load(&Kc[(size_t)ch * n_el + e].v[v]);
-#else
-        const Chunk kv = Kc[(size_t)ch * n_el + e];
-#endif
-#pragma unroll
-        for (int v = 0; v < V; ++v) {
-            const int q = ch * V + v;
-            if (q < NQ) {
-                B[q % 9] = kv.v[v];
-                if (q % 9 == 8) {  // block p complete
-                    const int p = q / 9, i = PT.i[p], j = PT.j[p];
-#pragma unroll
-                    for (int c = 0; c < 3; ++c)
-                        f[i][c] += B[c * 3] * u[j][0] + B[c * 3 + 1] * u[j][1] + B[c * 3 + 2] * u[j][2];
-                    if (i != j) {
-#pragma unroll
-                        for (int c = 0; c < 3; ++c) f[j][c] += B[c] * u[i][0] + B[3 + c] * u[i][1] + B[6 + c] * u[i][2];
-                    }
-                }
-            }
-        }
-    }
+    const int k = xcd_block() * blockDim.x + threadIdx.x;
+    if (k >= n_el) return;
+    const int e = eperm ? eperm[k] : k;
+    T f[NPE][3];
+    ebe_element<NPE, T>(n_el, k, conn, Ke, x, f);
 #pragma unroll
     for (int i = 0; i < NPE; ++i) {
         T *out = ye + 3 * (size_t)slot[(size_t)i * n_el + e];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) out[c] = f[i][c];
+        out[0] = f[i][0];
+        out[1] = f[i][1];
+        out[2] = f[i][2];
     }
 }
 
@@ -650,10 +591,13 @@ hipError_t launch_element_matrices(jfm_ctx *ctx) {
     const int n = (int)ctx->n_el;
     if (n == 0) return hipSuccess;
     const dim3 grid(xcd_grid(nblk(n)), (unsigned)ctx->npair);
+    // in the patch order when the patch operator is set up (ke_perm), from the
+    // factors in that order, so that the matrices sit where it reads them
+    const double *gf = ctx->ke_perm ? ctx->gfP64 : ctx->gf64;
     switch (ctx->npe) {
-    case 10: k_elem_mat<10><<<grid, kBlock, 0, ctx->stream>>>(n, ctx->ngf, ctx->gf64, ctx->Ke, ctx->Ke32); break;
-    case 4: k_elem_mat<4><<<grid, kBlock, 0, ctx->stream>>>(n, ctx->ngf, ctx->gf64, ctx->Ke, ctx->Ke32); break;
-    case 8: k_elem_mat<8><<<grid, kBlock, 0, ctx->stream>>>(n, ctx->ngf, ctx->gf64, ctx->Ke, ctx->Ke32); break;
+    case 10: k_elem_mat<10><<<grid, kBlock, 0, ctx->stream>>>(n, ctx->ngf, gf, ctx->Ke, ctx->Ke32); break;
+    case 4: k_elem_mat<4><<<grid, kBlock, 0, ctx->stream>>>(n, ctx->ngf, gf, ctx->Ke, ctx->Ke32); break;
+    case 8: k_elem_mat<8><<<grid, kBlock, 0, ctx->stream>>>(n, ctx->ngf, gf, ctx->Ke, ctx->Ke32); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -786,8 +730,8 @@ template <int NPE, typename T>
 static void launch_elem_phase(jfm_ctx *ctx, int op, const T *x, T *ye, const int *done) {
     const int n = (int)ctx->n_el;
     if (op == JFM_OP_EBE) {
-        k_ebe_elem<NPE, T><<<xcd_grid(nblk(n)), kBlock, 0, ctx->stream>>>(n, ctx->conn, ctx->inc_slot, ke_of<T>(ctx), x, ye,
-                                                                        done);
+        k_ebe_elem<NPE, T><<<xcd_grid(nblk(n)), kBlock, 0, ctx->stream>>>(
+            n, ctx->ke_perm ? ctx->connP : ctx->conn, ctx->ke_perm, ctx->inc_slot, ke_of<T>(ctx), x, ye, done);
     } else if (ctx->affine) {
         k_mf_elem<NPE, T, true><<<xcd_grid(nblk(n)), kBlock, 0, ctx->stream>>>(n, ctx->conn, ctx->inc_slot, gf_of<T>(ctx), x, ye, done,
                                                                               nullptr, n);
@@ -836,7 +780,7 @@ hipError_t launch_elem_into(jfm_ctx *ctx, int op, const T *x, T *ye) {
 template <typename T>
 hipError_t launch_two_phase(jfm_ctx *ctx, int op, const T *x, T *y, const T *dotv, double *part, const int *done,
                             int *npart) {
-    if (op == JFM_OP_MATFREE && ctx->patch_ready) return launch_patch_apply<T>(ctx, x, y, dotv, part, done, npart);
+    if (jfem_patch_form(ctx, op)) return launch_patch_apply<T>(ctx, op, x, y, dotv, part, done, npart);
     hipError_t e = launch_elem_only<T>(ctx, op, x, done);
     if (e != hipSuccess) return e;
     return launch_gather<T>(ctx, ye_of<T>(ctx), op == kOpMass ? nullptr : ctx->fixed, y, dotv, part, done, npart);

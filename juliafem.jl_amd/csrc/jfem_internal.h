@@ -176,6 +176,9 @@ struct jfm_ctx {
     float *vals32 = nullptr;
     double *Ke = nullptr;      // EBE element matrices: entry q = 9 p + m at [((q / 2) n_el + e) 2 + q % 2]
     float *Ke32 = nullptr;     // the same in FP32, chunks of 4: [((q / 4) n_el + e) 4 + q % 4]
+    // non-null when Ke and Ke32 are stored in the patch order (position k
+    // holds element ke_perm[k]; ke_perm is the patch structure's eperm)
+    const int32_t *ke_perm = nullptr;
     bool assembled = false;
 
     // boundary conditions and loads
@@ -356,16 +359,20 @@ hipError_t launch_nh_elem(jfm_ctx *ctx, bool tangent, const T *gf, const T *u, c
 hipError_t launch_subset_apply(jfm_ctx *ctx, bool tangent, const double *x, double *y, double *ye, int nv, int64_t xs,
                                int64_t ys, const int32_t *elist, int ne, const int32_t *nlist, int nn);
 hipError_t launch_nh_block_jacobi(jfm_ctx *ctx, const double *u, double *M64, float *M32, double *D);
-// patch.hip: fused matrix-free K.u (JFM_OP_MATFREE)
+// patch.hip: fused K.u over element patches (JFM_OP_MATFREE, and JFM_OP_EBE
+// when the element matrices are stored in the patch order)
 bool jfem_patch_enabled();  // false when JFEM_PATCH=0 (two-phase operator)
 int jfem_patch_setup(jfm_ctx *ctx);
 void jfem_patch_release(jfm_ctx *ctx);
+// whether `op` runs in the patch form on this context
+inline bool jfem_patch_form(const jfm_ctx *ctx, int op);
 template <typename T>
-hipError_t launch_patch_phase(jfm_ctx *ctx, const T *x, T *y, const T *dotv, const int *done);
+hipError_t launch_patch_phase(jfm_ctx *ctx, int op, const T *x, T *y, const T *dotv, const int *done);
 template <typename T>
 hipError_t launch_patch_combine(jfm_ctx *ctx, T *y, const T *dotv, double *part, const int *done, int *npart);
 template <typename T>
-hipError_t launch_patch_apply(jfm_ctx *ctx, const T *x, T *y, const T *dotv, double *part, const int *done, int *npart);
+hipError_t launch_patch_apply(jfm_ctx *ctx, int op, const T *x, T *y, const T *dotv, double *part, const int *done,
+                              int *npart);
 // coarse.hip
 int jfem_coarse_setup(jfm_ctx *ctx, int op, int coarse_nodes, double *ms);
 int jfem_coarse_matrix(jfm_ctx *ctx, int op, int coarse_nodes, int64_t *nc, double *Ac, int32_t *node_agg);
@@ -426,4 +433,8 @@ hipError_t launch_axpy_scaled(jfm_ctx *ctx, double *u, const float *d, double sc
 hipError_t launch_add(jfm_ctx *ctx, const double *a, const double *b, double *out, int64_t n);  // out = a + b
 template <typename T>
 hipError_t launch_dot(jfm_ctx *ctx, const T *a, const T *b, int64_t n, int *npart);
+inline bool jfem_patch_form(const jfm_ctx *ctx, int op) {
+    return ctx->patch_ready && (op == JFM_OP_MATFREE || (op == JFM_OP_EBE && ctx->ke_perm));
+}
+
 }  // namespace jfem

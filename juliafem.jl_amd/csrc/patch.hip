@@ -24,6 +24,7 @@
 #include <numeric>
 #include <vector>
 
+#include "ebe_element.h"
 #include "jfem_device.h"
 #include "mf_element.h"
 
@@ -49,9 +50,12 @@ constexpr int kPatch = JFEM_PATCH_ELEMS;
 // shared node's partial sum goes to its slot s (pn_out = -(s + 1)):
 // bpart[3 s + c].
 // Block partials of y.dotv over the interior nodes go to ppart.
-template <int NPE, typename T, bool AFF>
+// The element product is the matrix-free one (mf_element.h) from the
+// patch-ordered geometric factors el, or with EBE the stored-matrix one
+// (ebe_element.h) from element matrices el stored in the patch order.
+template <int NPE, typename T, bool AFF, bool EBE>
 __global__ __launch_bounds__(kPatch) void k_ku_patch(int n_el, int npatch, int npn, const int32_t *__restrict__ connP,
-                                                    const T *__restrict__ gfP, const T *__restrict__ x,
+                                                    const T *__restrict__ el, const T *__restrict__ x,
                                                     const int32_t *__restrict__ pn_ptr, const uint16_t *__restrict__ pn_cnt,
                                                     const int32_t *__restrict__ pn_out, const int32_t *__restrict__ rstart,
                                                     const int32_t *__restrict__ rbase, const uint16_t *__restrict__ linT,
@@ -89,12 +93,20 @@ __global__ __launch_bounds__(kPatch) void k_ku_patch(int n_el, int npatch, int n
         if (l < le) fetch(r, l, cnt[r], oo[r], mpre[r]);
     }
     if (dn) return;  // uniform
-    if (k < n_el)
-        mf_element<NPE, T, AFF>(n_el, k, connP, gfP, x, [&](int i, T f0, T f1, T f2) {
+    if (k < n_el) {
+        auto to_lds = [&](int i, T f0, T f1, T f2) {
             ys[(3 * i) * kPatch + t] = f0;
             ys[(3 * i + 1) * kPatch + t] = f1;
             ys[(3 * i + 2) * kPatch + t] = f2;
-        });
+        };
+        if constexpr (EBE) {
+            T f[NPE][3];
+            ebe_element<NPE, T>(n_el, k, connP, el, x, f);
+#pragma unroll
+            for (int i = 0; i < NPE; ++i) to_lds(i, f[i][0], f[i][1], f[i][2]);
+        } else
+            mf_element<NPE, T, AFF>(n_el, k, connP, el, x, to_lds);
+    }
     __syncthreads();
     double dv = 0.0;
     // node l of round r: the sum of its incidences, to y or to its slot
@@ -266,6 +278,13 @@ void jfem_patch_release(jfm_ctx *ctx) {
     }
     ctx->patch_ready = false;
     ctx->npatch = ctx->npn = ctx->nbnode = ctx->nslot = 0;
+    if (ctx->ke_perm) {  // element matrices stored in the released patch order
+        jfm_free(ctx, ctx->Ke);
+        jfm_free(ctx, ctx->Ke32);
+        ctx->Ke = nullptr;
+        ctx->Ke32 = nullptr;
+        ctx->ke_perm = nullptr;
+    }
 }
 
 // The patch structure on the host: element order, patch-local incidences in
@@ -459,31 +478,36 @@ int jfem_patch_setup(jfm_ctx *ctx) {
 template <typename T> static const T *gfP_of(jfm_ctx *ctx);
 template <> const double *gfP_of<double>(jfm_ctx *ctx) { return ctx->gfP64; }
 template <> const float *gfP_of<float>(jfm_ctx *ctx) { return ctx->gfP32; }
+template <typename T> static const T *keP_of(jfm_ctx *ctx);
+template <> const double *keP_of<double>(jfm_ctx *ctx) { return ctx->Ke; }
+template <> const float *keP_of<float>(jfm_ctx *ctx) { return ctx->Ke32; }
 
 template <int NPE, typename T>
-static void patch_kernel(jfm_ctx *ctx, const T *x, T *y, const T *dotv, const int *done) {
+static void patch_kernel(jfm_ctx *ctx, int op, const T *x, T *y, const T *dotv, const int *done) {
     const dim3 grid(xcd_grid(ctx->npatch));
     const int n = (int)ctx->n_el;
     double *pp = dotv ? ctx->ppart : nullptr;
-    if (ctx->affine)
-        k_ku_patch<NPE, T, true><<<grid, kPatch, 0, ctx->stream>>>(n, ctx->npatch, ctx->npn, ctx->connP, gfP_of<T>(ctx), x,
-                                                                   ctx->pn_ptr, ctx->pn_cnt, ctx->pn_out, ctx->rstart,
-                                                                   ctx->rbase, ctx->linT, ctx->fixed, y, (T *)ctx->bpart,
-                                                                   dotv, pp, done);
+    auto launch = [&](auto kernel, const T *el) {
+        kernel<<<grid, kPatch, 0, ctx->stream>>>(n, ctx->npatch, ctx->npn, ctx->connP, el, x, ctx->pn_ptr, ctx->pn_cnt,
+                                                 ctx->pn_out, ctx->rstart, ctx->rbase, ctx->linT, ctx->fixed, y,
+                                                 (T *)ctx->bpart, dotv, pp, done);
+    };
+    if (op == JFM_OP_EBE)
+        launch(k_ku_patch<NPE, T, true, true>, keP_of<T>(ctx));
+    else if (ctx->affine)
+        launch(k_ku_patch<NPE, T, true, false>, gfP_of<T>(ctx));
     else
-        k_ku_patch<NPE, T, false><<<grid, kPatch, 0, ctx->stream>>>(n, ctx->npatch, ctx->npn, ctx->connP, gfP_of<T>(ctx), x,
-                                                                    ctx->pn_ptr, ctx->pn_cnt, ctx->pn_out, ctx->rstart,
-                                                                    ctx->rbase, ctx->linT, ctx->fixed, y, (T *)ctx->bpart,
-                                                                    dotv, pp, done);
+        launch(k_ku_patch<NPE, T, false, false>, gfP_of<T>(ctx));
 }
 
 template <typename T>
-hipError_t launch_patch_phase(jfm_ctx *ctx, const T *x, T *y, const T *dotv, const int *done) {
+hipError_t launch_patch_phase(jfm_ctx *ctx, int op, const T *x, T *y, const T *dotv, const int *done) {
     if (ctx->n_el == 0) return hipSuccess;
+    if (!jfem_patch_form(ctx, op)) return hipErrorInvalidValue;
     switch (ctx->npe) {
-    case 10: patch_kernel<10, T>(ctx, x, y, dotv, done); break;
-    case 4: patch_kernel<4, T>(ctx, x, y, dotv, done); break;
-    case 8: patch_kernel<8, T>(ctx, x, y, dotv, done); break;
+    case 10: patch_kernel<10, T>(ctx, op, x, y, dotv, done); break;
+    case 4: patch_kernel<4, T>(ctx, op, x, y, dotv, done); break;
+    case 8: patch_kernel<8, T>(ctx, op, x, y, dotv, done); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -505,17 +529,18 @@ hipError_t launch_patch_combine(jfm_ctx *ctx, T *y, const T *dotv, double *part,
 }
 
 template <typename T>
-hipError_t launch_patch_apply(jfm_ctx *ctx, const T *x, T *y, const T *dotv, double *part, const int *done, int *npart) {
-    hipError_t e = launch_patch_phase<T>(ctx, x, y, dotv, done);
+hipError_t launch_patch_apply(jfm_ctx *ctx, int op, const T *x, T *y, const T *dotv, double *part, const int *done,
+                              int *npart) {
+    hipError_t e = launch_patch_phase<T>(ctx, op, x, y, dotv, done);
     if (e != hipSuccess) return e;
     return launch_patch_combine<T>(ctx, y, dotv, part, done, npart);
 }
 
-template hipError_t launch_patch_phase<double>(jfm_ctx *, const double *, double *, const double *, const int *);
-template hipError_t launch_patch_phase<float>(jfm_ctx *, const float *, float *, const float *, const int *);
+template hipError_t launch_patch_phase<double>(jfm_ctx *, int, const double *, double *, const double *, const int *);
+template hipError_t launch_patch_phase<float>(jfm_ctx *, int, const float *, float *, const float *, const int *);
 template hipError_t launch_patch_combine<double>(jfm_ctx *, double *, const double *, double *, const int *, int *);
 template hipError_t launch_patch_combine<float>(jfm_ctx *, float *, const float *, double *, const int *, int *);
-template hipError_t launch_patch_apply<double>(jfm_ctx *, const double *, double *, const double *, double *, const int *, int *);
-template hipError_t launch_patch_apply<float>(jfm_ctx *, const float *, float *, const float *, double *, const int *, int *);
+template hipError_t launch_patch_apply<double>(jfm_ctx *, int, const double *, double *, const double *, double *, const int *, int *);
+template hipError_t launch_patch_apply<float>(jfm_ctx *, int, const float *, float *, const float *, double *, const int *, int *);
 
 }  // namespace jfem
