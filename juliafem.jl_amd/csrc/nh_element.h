@@ -1,7 +1,7 @@
 // nh_element.h -- the neo-Hookean total-Lagrangian element (gfx950): the
 // Gauss-point state and its directional derivative, and the element's
-// internal forces f_int,e(u) or tangent product K_T,e(u) v_e.  Shared by the
-// two-phase kernels (nonlinear.hip) and the patch operator (patch.hip).
+// internal forces f_int,e(u) or tangent product K_T,e(u) v_e, for the
+// two-phase kernels of nonlinear.hip.
 //
 // Reference: src/materials/neo_hookean.jl:129-137 (psi = mu/2 (I1 - 3) -
 // mu ln J + lambda/2 ln^2 J), :205-231 (S = 2 dpsi/dC, D = 4 d2psi/dC2 by AD),
@@ -111,10 +111,11 @@ __device__ __forceinline__ void nh_dP(const T *F, const T *A, const T *S, T lnJ,
 
 // Element e (SoA connectivity conn[i * n_el + e], geometric factors
 // gf[(q * GF + k) * n_el + e]) at the state u: f_int,e(u) (TANGENT = false)
-// or K_T,e(u) v_e (TANGENT = true); out(i, f0, f1, f2) receives node i's terms.
-template <int NPE, typename T, bool AFF, bool TANGENT, typename Out>
+// or K_T,e(u) v_e (TANGENT = true) into f (an output array rather than a
+// callback, which cost the FP32 tangent kernel 38 VGPRs).
+template <int NPE, typename T, bool AFF, bool TANGENT>
 __device__ __forceinline__ void nh_element(int n_el, int e, const int32_t *__restrict__ conn, const T *__restrict__ gf,
-                                           const T *__restrict__ u, const T *__restrict__ v, Out &&out) {
+                                           const T *__restrict__ u, const T *__restrict__ v, T (&f)[NPE][3]) {
     using EL = Elem<NPE>;
     constexpr int NG = EL::NGP;
     const int n0 = conn[e];
@@ -123,7 +124,6 @@ __device__ __forceinline__ void nh_element(int n_el, int e, const int32_t *__res
     if (TANGENT) {
         v0[0] = v[3 * (size_t)n0]; v0[1] = v[3 * (size_t)n0 + 1]; v0[2] = v[3 * (size_t)n0 + 2];
     }
-    T f[NPE][3];
 #pragma unroll
     for (int i = 0; i < NPE; ++i) f[i][0] = f[i][1] = f[i][2] = T(0);
     T Ji[9], cl, cm;
@@ -190,8 +190,6 @@ __device__ __forceinline__ void nh_element(int n_el, int e, const int32_t *__res
                 }
             }
     }
-#pragma unroll
-    for (int i = 0; i < NPE; ++i) out(i, f[i][0], f[i][1], f[i][2]);
 }
 
 }  // namespace jfem

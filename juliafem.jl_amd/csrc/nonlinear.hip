@@ -26,12 +26,15 @@ __global__ __launch_bounds__(256) void k_nh_elem(int n_el, const int32_t *__rest
     const int e = elist ? elist[idx] : idx;   // optional element subset
     if (TANGENT) v += blockIdx.y * vs;        // vector blockIdx.y of a batch
     ye += blockIdx.y * ys;
-    nh_element<NPE, T, AFF, TANGENT>(n_el, e, conn, gf, u, v, [&](int i, T f0, T f1, T f2) {
+    T f[NPE][3];
+    nh_element<NPE, T, AFF, TANGENT>(n_el, e, conn, gf, u, v, f);
+#pragma unroll
+    for (int i = 0; i < NPE; ++i) {
         T *o = ye + 3 * (size_t)slot[(size_t)i * n_el + e];
-        o[0] = f0;
-        o[1] = f1;
-        o[2] = f2;
-    });
+        o[0] = f[i][0];
+        o[1] = f[i][1];
+        o[2] = f[i][2];
+    }
 }
 
 // Diagonal 3x3 blocks of the tangent K_T(u), summed over the node's
