@@ -120,10 +120,17 @@ extern "C" void jfm_destroy(jfm_ctx *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
     drop_graphs(ctx);
     for (void *p : ctx->allocs) (void)hipFree(p);
     ctx->allocs.clear();
     if (ctx->S_host) (void)hipHostFree(ctx->S_host);
+    if (ctx->stream2) {
+        (void)hipStreamSynchronize(ctx->stream2);
+        (void)hipStreamDestroy(ctx->stream2);
+    }
+    for (hipEvent_t ev : {ctx->ev_fork, ctx->ev_l0, ctx->ev_l1})
+        if (ev) (void)hipEventDestroy(ev);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -164,6 +171,7 @@ static void release_heat(jfm_ctx *ctx) {
 }
 
 static void release_mesh(jfm_ctx *ctx) {
+    jfem_learned_drop(ctx);  // no stream2 work may still use the buffers freed here
     free_ptr(ctx, ctx->X); free_ptr(ctx, ctx->conn); free_ptr(ctx, ctx->E); free_ptr(ctx, ctx->nu);
     free_ptr(ctx, ctx->gf64); free_ptr(ctx, ctx->gf32); free_ptr(ctx, ctx->inc_ptr); free_ptr(ctx, ctx->inc);
     free_ptr(ctx, ctx->brow); free_ptr(ctx, ctx->bcol); free_ptr(ctx, ctx->bdiag); free_ptr(ctx, ctx->blk_row);
@@ -492,6 +500,7 @@ extern "C" int jfm_upload_bcs(jfm_ctx *ctx, const uint8_t *fixed, const double *
     if (int rc = sync(ctx)) return rc;
     ctx->has_bcs = true;
     ctx->has_load = true;
+    jfem_learned_drop(ctx);
     ctx->learned_ready = false;
     ctx->has_user = false;
     for (Workspace *w : {&ctx->ws64, &ctx->ws32}) w->ok = false;  // preconditioners depend on the fixed set
@@ -1030,6 +1039,11 @@ extern "C" int jfm_solve(jfm_ctx *ctx, const jfm_solve_opts *opts, const double 
     ctx->hist_it.clear();
     ctx->hist_res.clear();
     const bool coarse = opts->coarse_nodes > 0 && bnorm > 0.0;
+    // the learned blocks on a second stream beside the coarse setup, which
+    // leaves most CUs idle (both read only the mesh and the block Jacobi)
+    // (headline at config 1: 16.20-16.22 ms against 16.28-16.31 ms in sequence)
+    if (coarse && opts->precond == JFM_PREC_LEARNED)
+        if (int rc = jfem_learned_launch(ctx)) return rc;
     if (coarse)
         if (int rc = jfem_coarse_setup(ctx, op_outer, opts->coarse_nodes, &st.setup_ms)) return rc;
     const double *Minv64 = (const double *)ctx->ws64.Minv;
@@ -1337,6 +1351,7 @@ extern "C" int jfm_solve_nonlinear(jfm_ctx *ctx, const jfm_newton_opts *opts, do
         ~Restore() {
             c->ws64.ok = c->ws32.ok = false;
             c->minv_tangent = false;
+            jfem_learned_drop(c);
             c->learned_ready = false;
         }
     } restore{ctx};
@@ -1379,6 +1394,7 @@ extern "C" int jfm_solve_nonlinear(jfm_ctx *ctx, const jfm_newton_opts *opts, do
                 // coarse rebuild; later iterates of the step keep its outcome.
                 // Accepted blocks are recomputed from the new tangent diagonal at
                 // every Newton step.
+                jfem_learned_drop(ctx);
                 ctx->learned_ready = false;
                 bool use = false;
                 int its = 0;

@@ -221,6 +221,12 @@ struct jfm_ctx {
     std::vector<unsigned char> mlp_host;
     bool has_mlp = false;
     bool learned_ready = false;    // Mlearn matches the current mesh/BCs/weights
+    bool mlp_dev = false;          // mlp_w holds mlp_host
+    // the learned blocks formed on stream2 beside other work (jfem_learned_launch):
+    // pending until jfem_learned_prepare joins them (ev_l0/ev_l1 bracket the work)
+    bool learned_pending = false;
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_l0 = nullptr, ev_l1 = nullptr;
     double *Mlearn64 = nullptr;    // N x 9 learned SPD blocks
     double *xprobe = nullptr;      // the learned-selection probe's winning iterate
     float *Mlearn32 = nullptr;

@@ -41,5 +41,14 @@ hipError_t launch_mblocks(jfm_ctx *ctx, const float *out, const double *Minv, fl
 }  // namespace jfem
 
 // Builds features, runs the network and forms Mlearn64/Mlearn32 for the
-// current mesh and boundary conditions.  *ms receives the device time.
+// current mesh and boundary conditions.  *ms receives the device time.  If
+// jfem_learned_launch started that work, joins it instead (the main stream
+// waits for it).
 int jfem_learned_prepare(jfm_ctx *ctx, double *ms);
+// Starts the same work on a second stream after the work queued so far on
+// the main stream, so that it runs beside what the main stream does next
+// (the coarse setup); jfem_learned_prepare joins it.  No-op when the blocks
+// are current or no weights are loaded.
+int jfem_learned_launch(jfm_ctx *ctx);
+// Waits for a started, not yet joined, learned launch and drops it.
+void jfem_learned_drop(jfm_ctx *ctx);

@@ -692,9 +692,16 @@ extern "C" int jfm_learned_set_weights(jfm_ctx *ctx, const void *blob, size_t by
     if (b->magic != kBlobMagic || b->n_in != kFeat || b->n_h1 != kHid || b->n_h2 != kHid || b->n_out != kOut)
         return jfm_fail(ctx, JFM_EINVAL, "weight blob header does not match the 24-64-64-6 network");
     if (!(b->eps >= 0.0f)) return jfm_fail(ctx, JFM_EINVAL, "eps must be >= 0");
+    jfem_learned_drop(ctx);
     ctx->mlp_host.assign((const unsigned char *)blob, (const unsigned char *)blob + bytes);
     ctx->has_mlp = true;
     ctx->learned_ready = false;
+    // the device copy now, so that no solve waits on a host-to-device copy
+    JFM_HIP(ctx, hipSetDevice(ctx->device));
+    LALLOC(ctx, ctx->mlp_w, unsigned char, (int64_t)sizeof(LearnedBlob));
+    JFM_HIP(ctx, hipMemcpyAsync(ctx->mlp_w, ctx->mlp_host.data(), bytes, hipMemcpyHostToDevice, ctx->stream));
+    JFM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->mlp_dev = true;
     return JFM_OK;
 }
 
@@ -715,29 +722,79 @@ static int compute_features(jfm_ctx *ctx) {
     return JFM_OK;
 }
 
-int jfem_learned_prepare(jfm_ctx *ctx, double *ms) {
-    if (!ctx->has_mlp) return jfm_fail(ctx, JFM_ESTATE, "learned preconditioner requested but no weights are loaded");
-    if (!ctx->has_mesh) return jfm_fail(ctx, JFM_ESTATE, "no mesh");
-    if (ms) *ms = 0.0;
-    if (ctx->learned_ready) return JFM_OK;
-    hipEvent_t e0, e1;
-    JFM_HIP(ctx, hipEventCreate(&e0));
-    JFM_HIP(ctx, hipEventCreate(&e1));
-    JFM_HIP(ctx, hipEventRecord(e0, ctx->stream));
+// features, network and blocks, enqueued on ctx->stream
+static int learned_enqueue(jfm_ctx *ctx) {
     if (int rc = compute_features(ctx)) return rc;
-    JFM_HIP(ctx, hipMemcpyAsync(ctx->mlp_w, ctx->mlp_host.data(), sizeof(LearnedBlob), hipMemcpyHostToDevice, ctx->stream));
+    if (!ctx->mlp_dev) {
+        JFM_HIP(ctx, hipMemcpyAsync(ctx->mlp_w, ctx->mlp_host.data(), sizeof(LearnedBlob), hipMemcpyHostToDevice,
+                                    ctx->stream));
+        ctx->mlp_dev = true;
+    }
     int nw = 0;
     hipError_t e = launch_mlp(ctx, ctx->feat, ctx->mlp_w, ctx->lout, nullptr, (const double *)ctx->ws64.Minv, nullptr, &nw);
     if (e != hipSuccess) return jfm_hip_fail(ctx, e, "launch_mlp");
     const float eps = ((const LearnedBlob *)ctx->mlp_host.data())->eps;
     e = launch_mblocks(ctx, ctx->lout, (const double *)ctx->ws64.Minv, eps, ctx->Mlearn64, ctx->Mlearn32);
     if (e != hipSuccess) return jfm_hip_fail(ctx, e, "launch_mblocks");
-    JFM_HIP(ctx, hipEventRecord(e1, ctx->stream));
-    JFM_HIP(ctx, hipEventSynchronize(e1));
+    return JFM_OK;
+}
+
+static int learned_events(jfm_ctx *ctx) {
+    for (hipEvent_t *ev : {&ctx->ev_fork, &ctx->ev_l0, &ctx->ev_l1})
+        if (!*ev) JFM_HIP(ctx, hipEventCreate(ev));
+    return JFM_OK;
+}
+
+int jfem_learned_launch(jfm_ctx *ctx) {
+    if (!ctx->has_mlp || !ctx->has_mesh || ctx->learned_ready || ctx->learned_pending) return JFM_OK;
+    // buffers and block Jacobi first (they allocate, and the blocks are an input)
+    if (int rc = jfem_ensure_workspaces(ctx)) return rc;
+    if (int rc = ensure_buffers(ctx)) return rc;
+    if (!ctx->stream2) JFM_HIP(ctx, hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
+    if (int rc = learned_events(ctx)) return rc;
+    JFM_HIP(ctx, hipEventRecord(ctx->ev_fork, ctx->stream));
+    JFM_HIP(ctx, hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0));
+    // the launchers enqueue on ctx->stream: point it at stream2 meanwhile
+    hipStream_t main = ctx->stream;
+    ctx->stream = ctx->stream2;
+    int rc = JFM_OK;
+    hipError_t e = hipEventRecord(ctx->ev_l0, ctx->stream);
+    if (e == hipSuccess) rc = learned_enqueue(ctx);
+    if (e == hipSuccess && rc == JFM_OK) e = hipEventRecord(ctx->ev_l1, ctx->stream);
+    ctx->stream = main;
+    if (e != hipSuccess) return jfm_hip_fail(ctx, e, "learned launch");
+    if (rc) return rc;
+    ctx->learned_pending = true;
+    return JFM_OK;
+}
+
+void jfem_learned_drop(jfm_ctx *ctx) {
+    if (ctx->learned_pending && ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
+    ctx->learned_pending = false;
+}
+
+int jfem_learned_prepare(jfm_ctx *ctx, double *ms) {
+    if (!ctx->has_mlp) return jfm_fail(ctx, JFM_ESTATE, "learned preconditioner requested but no weights are loaded");
+    if (!ctx->has_mesh) return jfm_fail(ctx, JFM_ESTATE, "no mesh");
+    if (ms) *ms = 0.0;
+    if (ctx->learned_pending) {  // started beside earlier work: join it
+        JFM_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_l1, 0));
+        JFM_HIP(ctx, hipEventSynchronize(ctx->ev_l1));
+        float t = 0.f;
+        JFM_HIP(ctx, hipEventElapsedTime(&t, ctx->ev_l0, ctx->ev_l1));
+        if (ms) *ms = t;
+        ctx->learned_pending = false;
+        ctx->learned_ready = true;
+        return JFM_OK;
+    }
+    if (ctx->learned_ready) return JFM_OK;
+    if (int rc = learned_events(ctx)) return rc;
+    JFM_HIP(ctx, hipEventRecord(ctx->ev_l0, ctx->stream));
+    if (int rc = learned_enqueue(ctx)) return rc;
+    JFM_HIP(ctx, hipEventRecord(ctx->ev_l1, ctx->stream));
+    JFM_HIP(ctx, hipEventSynchronize(ctx->ev_l1));
     float t = 0.f;
-    JFM_HIP(ctx, hipEventElapsedTime(&t, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    JFM_HIP(ctx, hipEventElapsedTime(&t, ctx->ev_l0, ctx->ev_l1));
     if (ms) *ms = t;
     ctx->learned_ready = true;
     return JFM_OK;
@@ -756,6 +813,7 @@ extern "C" int jfm_learned_features(jfm_ctx *ctx, float *feat) {
 extern "C" int jfm_learned_outputs(jfm_ctx *ctx, float *out, double *M) {
     if (!ctx) return JFM_EINVAL;
     JFM_HIP(ctx, hipSetDevice(ctx->device));
+    jfem_learned_drop(ctx);
     ctx->learned_ready = false;
     if (int rc = jfem_learned_prepare(ctx, nullptr)) return rc;
     if (out) JFM_HIP(ctx, hipMemcpyAsync(out, ctx->lout, sizeof(float) * ctx->N * kOut, hipMemcpyDeviceToHost, ctx->stream));
@@ -768,6 +826,7 @@ extern "C" int jfm_learned_grad(jfm_ctx *ctx, int32_t n_pairs, const double *r, 
                                 double *loss, float *grad) {
     if (!ctx || n_pairs <= 0 || !r || !e || !grad || coarse_nodes < 0) return JFM_EINVAL;
     JFM_HIP(ctx, hipSetDevice(ctx->device));
+    jfem_learned_drop(ctx);
     ctx->learned_ready = false;
     if (int rc = jfem_learned_prepare(ctx, nullptr)) return rc;
     // the blocks act inside the two-level preconditioner: z = M r + Q A_c^-1 Q' r
