@@ -49,7 +49,9 @@ constexpr int kPatch = JFEM_PATCH_ELEMS;
 // first rounds' node data is loaded before the element product.  A
 // shared node's partial sum goes to its slot s (pn_out = -(s + 1)):
 // bpart[3 s + c].
-// Block partials of y.dotv over the interior nodes go to ppart.
+// Block partials of y.dotv over the interior nodes go to ppart; with ppart
+// and no dotv (affine Tet10 matrix-free form), the block partials of the
+// element energies x_e.K_e x_e instead, which sum to x.Kx.
 // The element product is the matrix-free one (mf_element.h) from the
 // patch-ordered geometric factors el, or with EBE the stored-matrix one
 // (ebe_element.h) from element matrices el stored in the patch order.
@@ -93,6 +95,11 @@ __global__ __launch_bounds__(kPatch) void k_ku_patch(int n_el, int npatch, int n
         if (l < le) fetch(r, l, cnt[r], oo[r], mpre[r]);
     }
     if (dn) return;  // uniform
+    // p.Kp of the PCG step as the sum of the elements' energies (affine Tet10
+    // matrix-free form, when partials are asked for without dotv: the caller
+    // guarantees x = 0 on fixed DOFs, so x.(masked Kx) = x.Kx)
+    constexpr bool kEnergy = !EBE && AFF && NPE == 10;
+    double en = 0.0;
     if (k < n_el) {
         auto to_lds = [&](int i, T f0, T f1, T f2) {
             ys[(3 * i) * kPatch + t] = f0;
@@ -104,11 +111,41 @@ __global__ __launch_bounds__(kPatch) void k_ku_patch(int n_el, int npatch, int n
             ebe_element<NPE, T>(n_el, k, connP, el, x, f);
 #pragma unroll
             for (int i = 0; i < NPE; ++i) to_lds(i, f[i][0], f[i][1], f[i][2]);
+        } else if constexpr (kEnergy) {
+            if (ppart && !dotv) {
+                // x.Kx of the element from the values in registers (u = x - x_0,
+                // y = K_e u: u.y = x.K_e x since K_e annihilates translations),
+                // each node's three products in T, the nodes' sums in FP64
+                T xs[NPE][3];
+#pragma unroll
+                for (int i = 0; i < NPE; ++i) {
+                    const uint32_t a = (uint32_t)load_at<int32_t>(connP, 4u * (uint32_t)(i * n_el + k));
+                    const Trip<T> w = load_at<Trip<T>>(x, (uint32_t)(3 * sizeof(T)) * a);
+                    xs[i][0] = w.a;
+                    xs[i][1] = w.b;
+                    xs[i][2] = w.c;
+                }
+                mf_element_x<NPE, T, AFF>(
+                    n_el, k, el,
+                    [&](int i, T (&v)[3]) {
+                        v[0] = xs[i][0];
+                        v[1] = xs[i][1];
+                        v[2] = xs[i][2];
+                    },
+                    [&](int i, T f0, T f1, T f2) {
+                        to_lds(i, f0, f1, f2);
+                        if (i > 0)
+                            en += (double)((xs[i][0] - xs[0][0]) * f0 + (xs[i][1] - xs[0][1]) * f1 +
+                                           (xs[i][2] - xs[0][2]) * f2);
+                    });
+            } else {
+                mf_element<NPE, T, AFF>(n_el, k, connP, el, x, to_lds);
+            }
         } else
             mf_element<NPE, T, AFF>(n_el, k, connP, el, x, to_lds);
     }
     __syncthreads();
-    double dv = 0.0;
+    double dv = en;
     // node l of round r: the sum of its incidences, to y or to its slot
     auto node = [&](int r, int l, int c, int o, const int (&mp)[kPre]) {
         T s0 = T(0), s1 = T(0), s2 = T(0);
@@ -221,7 +258,7 @@ __global__ __launch_bounds__(256) void k_ku_combine(int nb, int chunk, const int
     if (bf < b1) load_node(bf, ra, rb, v, a, f, d);
     if (done && *done) return;
     double dv = 0.0;
-    if (dotv)
+    if (ppart)
         for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < npatch; q += gridDim.x * blockDim.x) dv += ppart[q];
     if (bf < b1) sum_node(ra, rb, v, a, f, d, dv);
     for (int b = bf + blockDim.x; b < b1; b += blockDim.x) {  // chunks longer than the block
@@ -482,11 +519,21 @@ template <typename T> static const T *keP_of(jfm_ctx *ctx);
 template <> const double *keP_of<double>(jfm_ctx *ctx) { return ctx->Ke; }
 template <> const float *keP_of<float>(jfm_ctx *ctx) { return ctx->Ke32; }
 
+// Whether the patch kernel forms x.(Kx) from the element energies (no dotv
+// reads): the affine Tet10 matrix-free form, with dotv = x, which the PCG
+// passes (its directions are zero on fixed DOFs).
+// (headline at config 1: 16.10-16.14 ms against 16.26-16.27 ms with the dot
+// over the nodes)
+template <typename T> static bool energy_dot(const jfm_ctx *ctx, int op, const T *x, const T *dotv) {
+    return op == JFM_OP_MATFREE && ctx->affine && ctx->npe == 10 && dotv && dotv == x;
+}
+
 template <int NPE, typename T>
 static void patch_kernel(jfm_ctx *ctx, int op, const T *x, T *y, const T *dotv, const int *done) {
     const dim3 grid(xcd_grid(ctx->npatch));
     const int n = (int)ctx->n_el;
     double *pp = dotv ? ctx->ppart : nullptr;
+    if (energy_dot<T>(ctx, op, x, dotv)) dotv = nullptr;
     auto launch = [&](auto kernel, const T *el) {
         kernel<<<grid, kPatch, 0, ctx->stream>>>(n, ctx->npatch, ctx->npn, ctx->connP, el, x, ctx->pn_ptr, ctx->pn_cnt,
                                                  ctx->pn_out, ctx->rstart, ctx->rbase, ctx->linT, ctx->fixed, y,
@@ -523,7 +570,7 @@ hipError_t launch_patch_combine(jfm_ctx *ctx, T *y, const T *dotv, double *part,
     const int chunk = (int)((nb + g - 1) / g);
     if (npart) *npart = (int)g;
     k_ku_combine<T><<<g, kBlock, 0, ctx->stream>>>(nb, chunk, ctx->bptr, ctx->bnode, (const T *)ctx->bpart,
-                                                   ctx->fixed, y, dotv, ctx->n_el ? ctx->ppart : nullptr,
+                                                   ctx->fixed, y, dotv, (ctx->n_el && part) ? ctx->ppart : nullptr,
                                                    ctx->n_el ? ctx->npatch : 0, part, done);
     return hipGetLastError();
 }
@@ -533,7 +580,8 @@ hipError_t launch_patch_apply(jfm_ctx *ctx, int op, const T *x, T *y, const T *d
                               int *npart) {
     hipError_t e = launch_patch_phase<T>(ctx, op, x, y, dotv, done);
     if (e != hipSuccess) return e;
-    return launch_patch_combine<T>(ctx, y, dotv, part, done, npart);
+    // with the element energies the shared nodes add nothing to the dot
+    return launch_patch_combine<T>(ctx, y, energy_dot<T>(ctx, op, x, dotv) ? nullptr : dotv, part, done, npart);
 }
 
 template hipError_t launch_patch_phase<double>(jfm_ctx *, int, const double *, double *, const double *, const int *);
