@@ -1045,7 +1045,10 @@ extern "C" int jfm_solve(jfm_ctx *ctx, const jfm_solve_opts *opts, const double 
     if (coarse && opts->precond == JFM_PREC_LEARNED)
         if (int rc = jfem_learned_launch(ctx)) return rc;
     if (coarse)
-        if (int rc = jfem_coarse_setup(ctx, op_outer, opts->coarse_nodes, &st.setup_ms)) return rc;
+        if (int rc = jfem_coarse_setup(ctx, op_outer, opts->coarse_nodes, &st.setup_ms)) {
+            jfem_learned_drop(ctx);  // no learned work is left running past the call
+            return rc;
+        }
     const double *Minv64 = (const double *)ctx->ws64.Minv;
     const float *Minv32 = (const float *)ctx->ws32.Minv;
     if (opts->precond == JFM_PREC_USER) {
