@@ -34,10 +34,12 @@ struct LearnedBlob {
 };
 
 // device launchers (learned.hip)
-hipError_t launch_features(jfm_ctx *ctx, double *ef, float *feat);
+// (on stream s)
+hipError_t launch_features(jfm_ctx *ctx, double *ef, float *feat, hipStream_t s);
 hipError_t launch_mlp(jfm_ctx *ctx, const float *feat, const void *blob, float *out, const double *G,
-                      const double *Minv, float *gpart, int *nwaves);
-hipError_t launch_mblocks(jfm_ctx *ctx, const float *out, const double *Minv, float eps, double *M64, float *M32);
+                      const double *Minv, float *gpart, int *nwaves, hipStream_t s);
+hipError_t launch_mblocks(jfm_ctx *ctx, const float *out, const double *Minv, float eps, double *M64, float *M32,
+                          hipStream_t s);
 }  // namespace jfem
 
 // Builds features, runs the network and forms Mlearn64/Mlearn32 for the

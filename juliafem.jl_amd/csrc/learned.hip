@@ -512,21 +512,21 @@ static inline unsigned nblk(int64_t n, int b = kBlock) { return (unsigned)((n + 
 constexpr int kTrainBlocks = 256;
 
 // ef: 3 n_el doubles of per-element inputs (k_elem_features)
-hipError_t launch_features(jfm_ctx *ctx, double *ef, float *feat) {
+hipError_t launch_features(jfm_ctx *ctx, double *ef, float *feat, hipStream_t s) {
     const int N = (int)ctx->N, ne = (int)ctx->n_el;
     const double *Dover = ctx->minv_tangent ? ctx->Dtan : nullptr;
     switch (ctx->npe) {
     case 10:
-        if (ne) k_elem_features<10><<<nblk(ne), kBlock, 0, ctx->stream>>>(ne, ctx->conn, ctx->gf64, ctx->E, ctx->nu, ctx->fixed, ef);
-        k_features<10><<<nblk(N), kBlock, 0, ctx->stream>>>(N, ne, ctx->ngf, ctx->inc_ptr, ctx->inc, ef, ctx->gf64, ctx->E, ctx->fixed, ctx->finc_ptr, Dover, feat);
+        if (ne) k_elem_features<10><<<nblk(ne), kBlock, 0, s>>>(ne, ctx->conn, ctx->gf64, ctx->E, ctx->nu, ctx->fixed, ef);
+        k_features<10><<<nblk(N), kBlock, 0, s>>>(N, ne, ctx->ngf, ctx->inc_ptr, ctx->inc, ef, ctx->gf64, ctx->E, ctx->fixed, ctx->finc_ptr, Dover, feat);
         break;
     case 4:
-        if (ne) k_elem_features<4><<<nblk(ne), kBlock, 0, ctx->stream>>>(ne, ctx->conn, ctx->gf64, ctx->E, ctx->nu, ctx->fixed, ef);
-        k_features<4><<<nblk(N), kBlock, 0, ctx->stream>>>(N, ne, ctx->ngf, ctx->inc_ptr, ctx->inc, ef, ctx->gf64, ctx->E, ctx->fixed, ctx->finc_ptr, Dover, feat);
+        if (ne) k_elem_features<4><<<nblk(ne), kBlock, 0, s>>>(ne, ctx->conn, ctx->gf64, ctx->E, ctx->nu, ctx->fixed, ef);
+        k_features<4><<<nblk(N), kBlock, 0, s>>>(N, ne, ctx->ngf, ctx->inc_ptr, ctx->inc, ef, ctx->gf64, ctx->E, ctx->fixed, ctx->finc_ptr, Dover, feat);
         break;
     case 8:
-        if (ne) k_elem_features<8><<<nblk(ne), kBlock, 0, ctx->stream>>>(ne, ctx->conn, ctx->gf64, ctx->E, ctx->nu, ctx->fixed, ef);
-        k_features<8><<<nblk(N), kBlock, 0, ctx->stream>>>(N, ne, ctx->ngf, ctx->inc_ptr, ctx->inc, ef, ctx->gf64, ctx->E, ctx->fixed, ctx->finc_ptr, Dover, feat);
+        if (ne) k_elem_features<8><<<nblk(ne), kBlock, 0, s>>>(ne, ctx->conn, ctx->gf64, ctx->E, ctx->nu, ctx->fixed, ef);
+        k_features<8><<<nblk(N), kBlock, 0, s>>>(N, ne, ctx->ngf, ctx->inc_ptr, ctx->inc, ef, ctx->gf64, ctx->E, ctx->fixed, ctx->finc_ptr, Dover, feat);
         break;
     default: return hipErrorInvalidValue;
     }
@@ -633,26 +633,27 @@ __global__ __launch_bounds__(kInfWaves * 64) void k_mlp_infer(int N, const float
 }
 
 hipError_t launch_mlp(jfm_ctx *ctx, const float *feat, const void *blob, float *out, const double *G,
-                      const double *Minv, float *gpart, int *nwaves) {
+                      const double *Minv, float *gpart, int *nwaves, hipStream_t s) {
     const int N = (int)ctx->N;
     const size_t lds = sizeof(MlpLds);
     const unsigned tiles = (unsigned)((N + kWaves * kTile - 1) / (kWaves * kTile));
     if (G) {
         const unsigned g = std::min<unsigned>(tiles, kTrainBlocks);
         *nwaves = (int)g * kWaves;
-        k_mlp<true><<<g, kWaves * 64, lds, ctx->stream>>>(N, feat, (const uint8_t *)blob, out, G, Minv, ctx->fixed, gpart);
+        k_mlp<true><<<g, kWaves * 64, lds, s>>>(N, feat, (const uint8_t *)blob, out, G, Minv, ctx->fixed, gpart);
     } else {
         // four resident blocks per CU on 256 CUs; each wave takes every 4096th tile
         const unsigned tiles16 = (unsigned)((N + kTile - 1) / kTile);
         const unsigned g = std::max(1u, std::min<unsigned>((tiles16 + kInfWaves - 1) / kInfWaves, 1024));
-        k_mlp_infer<<<g, kInfWaves * 64, sizeof(MlpInfLds), ctx->stream>>>(N, feat, (const uint8_t *)blob, out);
+        k_mlp_infer<<<g, kInfWaves * 64, sizeof(MlpInfLds), s>>>(N, feat, (const uint8_t *)blob, out);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_mblocks(jfm_ctx *ctx, const float *out, const double *Minv, float eps, double *M64, float *M32) {
+hipError_t launch_mblocks(jfm_ctx *ctx, const float *out, const double *Minv, float eps, double *M64, float *M32,
+                          hipStream_t s) {
     const int N = (int)ctx->N;
-    k_mblocks<<<nblk(N), kBlock, 0, ctx->stream>>>(N, out, Minv, ctx->fixed, eps, M64, M32);
+    k_mblocks<<<nblk(N), kBlock, 0, s>>>(N, out, Minv, ctx->fixed, eps, M64, M32);
     return hipGetLastError();
 }
 
@@ -714,27 +715,27 @@ extern "C" int jfm_learned_get_weights(jfm_ctx *ctx, void *blob, size_t bytes) {
 }
 
 // features for the current mesh and boundary conditions
-static int compute_features(jfm_ctx *ctx) {
+static int compute_features(jfm_ctx *ctx, hipStream_t s) {
     if (int rc = jfem_ensure_workspaces(ctx)) return rc;
     if (int rc = ensure_buffers(ctx)) return rc;
-    hipError_t e = launch_features(ctx, ctx->efeat, ctx->feat);
+    hipError_t e = launch_features(ctx, ctx->efeat, ctx->feat, s);
     if (e != hipSuccess) return jfm_hip_fail(ctx, e, "launch_features");
     return JFM_OK;
 }
 
-// features, network and blocks, enqueued on ctx->stream
-static int learned_enqueue(jfm_ctx *ctx) {
-    if (int rc = compute_features(ctx)) return rc;
+// features, network and blocks, enqueued on stream s
+static int learned_enqueue(jfm_ctx *ctx, hipStream_t s) {
+    if (int rc = compute_features(ctx, s)) return rc;
     if (!ctx->mlp_dev) {
-        JFM_HIP(ctx, hipMemcpyAsync(ctx->mlp_w, ctx->mlp_host.data(), sizeof(LearnedBlob), hipMemcpyHostToDevice,
-                                    ctx->stream));
+        JFM_HIP(ctx, hipMemcpyAsync(ctx->mlp_w, ctx->mlp_host.data(), sizeof(LearnedBlob), hipMemcpyHostToDevice, s));
         ctx->mlp_dev = true;
     }
     int nw = 0;
-    hipError_t e = launch_mlp(ctx, ctx->feat, ctx->mlp_w, ctx->lout, nullptr, (const double *)ctx->ws64.Minv, nullptr, &nw);
+    hipError_t e = launch_mlp(ctx, ctx->feat, ctx->mlp_w, ctx->lout, nullptr, (const double *)ctx->ws64.Minv, nullptr,
+                              &nw, s);
     if (e != hipSuccess) return jfm_hip_fail(ctx, e, "launch_mlp");
     const float eps = ((const LearnedBlob *)ctx->mlp_host.data())->eps;
-    e = launch_mblocks(ctx, ctx->lout, (const double *)ctx->ws64.Minv, eps, ctx->Mlearn64, ctx->Mlearn32);
+    e = launch_mblocks(ctx, ctx->lout, (const double *)ctx->ws64.Minv, eps, ctx->Mlearn64, ctx->Mlearn32, s);
     if (e != hipSuccess) return jfm_hip_fail(ctx, e, "launch_mblocks");
     return JFM_OK;
 }
@@ -754,16 +755,14 @@ int jfem_learned_launch(jfm_ctx *ctx) {
     if (int rc = learned_events(ctx)) return rc;
     JFM_HIP(ctx, hipEventRecord(ctx->ev_fork, ctx->stream));
     JFM_HIP(ctx, hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0));
-    // the launchers enqueue on ctx->stream: point it at stream2 meanwhile
-    hipStream_t main = ctx->stream;
-    ctx->stream = ctx->stream2;
-    int rc = JFM_OK;
-    hipError_t e = hipEventRecord(ctx->ev_l0, ctx->stream);
-    if (e == hipSuccess) rc = learned_enqueue(ctx);
-    if (e == hipSuccess && rc == JFM_OK) e = hipEventRecord(ctx->ev_l1, ctx->stream);
-    ctx->stream = main;
-    if (e != hipSuccess) return jfm_hip_fail(ctx, e, "learned launch");
-    if (rc) return rc;
+    JFM_HIP(ctx, hipEventRecord(ctx->ev_l0, ctx->stream2));
+    int rc = learned_enqueue(ctx, ctx->stream2);
+    if (rc == JFM_OK)
+        if (hipError_t e = hipEventRecord(ctx->ev_l1, ctx->stream2)) rc = jfm_hip_fail(ctx, e, "learned launch");
+    if (rc) {  // nothing half-launched is left to be joined later
+        (void)hipStreamSynchronize(ctx->stream2);
+        return rc;
+    }
     ctx->learned_pending = true;
     return JFM_OK;
 }
@@ -790,7 +789,7 @@ int jfem_learned_prepare(jfm_ctx *ctx, double *ms) {
     if (ctx->learned_ready) return JFM_OK;
     if (int rc = learned_events(ctx)) return rc;
     JFM_HIP(ctx, hipEventRecord(ctx->ev_l0, ctx->stream));
-    if (int rc = learned_enqueue(ctx)) return rc;
+    if (int rc = learned_enqueue(ctx, ctx->stream)) return rc;
     JFM_HIP(ctx, hipEventRecord(ctx->ev_l1, ctx->stream));
     JFM_HIP(ctx, hipEventSynchronize(ctx->ev_l1));
     float t = 0.f;
@@ -804,7 +803,7 @@ extern "C" int jfm_learned_features(jfm_ctx *ctx, float *feat) {
     if (!ctx) return JFM_EINVAL;
     if (!ctx->has_mesh) return jfm_fail(ctx, JFM_ESTATE, "no mesh");
     JFM_HIP(ctx, hipSetDevice(ctx->device));
-    if (int rc = compute_features(ctx)) return rc;
+    if (int rc = compute_features(ctx, ctx->stream)) return rc;
     if (feat) JFM_HIP(ctx, hipMemcpyAsync(feat, ctx->feat, sizeof(float) * ctx->N * kFeat, hipMemcpyDeviceToHost, ctx->stream));
     JFM_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return JFM_OK;
@@ -871,7 +870,8 @@ extern "C" int jfm_learned_grad(jfm_ctx *ctx, int32_t n_pairs, const double *r, 
     }
     if (rc == JFM_OK) {
         int nw = 0;
-        hipError_t err = launch_mlp(ctx, ctx->feat, ctx->mlp_w, ctx->lout, G, (const double *)ctx->ws64.Minv, gpart, &nw);
+        hipError_t err = launch_mlp(ctx, ctx->feat, ctx->mlp_w, ctx->lout, G, (const double *)ctx->ws64.Minv, gpart, &nw,
+                                    ctx->stream);
         if (err == hipSuccess) {
             k_grad_reduce<<<nblk(kNParam), kBlock, 0, s>>>(nw, gpart, dgrad);
             err = hipGetLastError();
