@@ -1039,13 +1039,11 @@ extern "C" int jfm_solve(jfm_ctx *ctx, const jfm_solve_opts *opts, const double 
     ctx->hist_it.clear();
     ctx->hist_res.clear();
     const bool coarse = opts->coarse_nodes > 0 && bnorm > 0.0;
-    // the learned blocks on a second stream beside the coarse setup, which
-    // leaves most CUs idle (both read only the mesh and the block Jacobi)
-    // (headline at config 1: 16.20-16.22 ms against 16.28-16.31 ms in sequence)
-    if (coarse && opts->precond == JFM_PREC_LEARNED)
-        if (int rc = jfem_learned_launch(ctx)) return rc;
+    // the learned blocks on a second stream beside the coarse level's
+    // inversion (both read only the mesh and the block Jacobi)
     if (coarse)
-        if (int rc = jfem_coarse_setup(ctx, op_outer, opts->coarse_nodes, &st.setup_ms)) {
+        if (int rc = jfem_coarse_setup(ctx, op_outer, opts->coarse_nodes, &st.setup_ms,
+                                       opts->precond == JFM_PREC_LEARNED)) {
             jfem_learned_drop(ctx);  // no learned work is left running past the call
             return rc;
         }

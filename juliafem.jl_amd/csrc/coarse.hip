@@ -49,6 +49,7 @@
 #include <vector>
 
 #include "jfem_device.h"
+#include "jfem_learned.h"
 #include "mf_element.h"
 
 using namespace jfem;
@@ -1068,6 +1069,10 @@ void jfem_coarse_release(jfm_ctx *ctx) {
             (void)hipGraphExecDestroy(ctx->coarse_graph[k]);
             ctx->coarse_graph[k] = nullptr;
         }
+    if (ctx->coarse_inv_graph) {
+        (void)hipGraphExecDestroy(ctx->coarse_inv_graph);
+        ctx->coarse_inv_graph = nullptr;
+    }
     ctx->coarse_ready = false;
     ctx->coarse_tag = -1;
     ctx->coarse_built_nodes = 0;
@@ -1163,13 +1168,17 @@ static hipError_t coarse_matrix_sequence(jfm_ctx *ctx, bool tangent) {
 
 // A_c, its inverse and the fold into Ahat32; a non-SPD pivot sets the flag
 // at ctx->counter + 48.
-static hipError_t setup_sequence(jfm_ctx *ctx, bool tangent) {
+// A_c (the not-SPD flag cleared first) ...
+static hipError_t setup_matrix_sequence(jfm_ctx *ctx, bool tangent) {
+    k_flag_clear<<<1, 1, 0, ctx->stream>>>((int *)(ctx->counter + 48));
+    return coarse_matrix_sequence(ctx, tangent);
+}
+
+// ... then its inverse and the fold into Ahat
+static hipError_t setup_inverse_sequence(jfm_ctx *ctx) {
     hipStream_t s = ctx->stream;
     const int nc = (int)ctx->nc;
     int *flag = (int *)(ctx->counter + 48);
-    k_flag_clear<<<1, 1, 0, s>>>(flag);
-    hipError_t e = coarse_matrix_sequence(ctx, tangent);
-    if (e != hipSuccess) return e;
     const int tiles = (nc + 63) / 64;
     // gjP: this step's P^-1, the next step's as the update forms it, and the
     // next pivot block before the update
@@ -1325,7 +1334,7 @@ static int coarse_prepare(jfm_ctx *ctx, int op, int coarse_nodes) {
 
 // Builds the coarse level for operator `op` (FP64, masked); A_c is rebuilt on
 // every call.
-int jfem_coarse_setup(jfm_ctx *ctx, int op, int coarse_nodes, double *ms) {
+int jfem_coarse_setup(jfm_ctx *ctx, int op, int coarse_nodes, double *ms, bool learned_beside) {
     hipStream_t s = ctx->stream;
     hipEvent_t e0, e1;
     JFM_HIP(ctx, hipEventCreate(&e0));
@@ -1336,13 +1345,11 @@ int jfem_coarse_setup(jfm_ctx *ctx, int op, int coarse_nodes, double *ms) {
         (void)hipEventDestroy(e1);
         return rc;
     }
-    // the same launch sequence every time for a given aggregation and
-    // operator, so it is captured once as a HIP graph
+    // each part is the same launch sequence every time for a given
+    // aggregation and operator, so it is captured once as a HIP graph
     const bool tangent = op == JFM_OP_TANGENT;
-    hipError_t seq_err = hipSuccess;
-    if (jfem_use_graphs()) {
-        hipGraphExec_t &ex = ctx->coarse_graph[tangent ? 1 : 0];
-        int &gen = ctx->coarse_graph_gen[tangent ? 1 : 0];
+    auto run = [&](hipGraphExec_t &ex, int &gen, auto &&seq) -> hipError_t {
+        if (!jfem_use_graphs()) return seq();
         if (ex && gen != ctx->coarse_gen) {
             (void)hipGraphExecDestroy(ex);
             ex = nullptr;
@@ -1352,7 +1359,7 @@ int jfem_coarse_setup(jfm_ctx *ctx, int op, int coarse_nodes, double *ms) {
             hipError_t le = hipSuccess;
             bool ok = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess;
             if (ok) {
-                le = setup_sequence(ctx, tangent);
+                le = seq();
                 ok = hipStreamEndCapture(s, &graph) == hipSuccess && le == hipSuccess;
             }
             if (ok) ok = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0) == hipSuccess;
@@ -1365,15 +1372,25 @@ int jfem_coarse_setup(jfm_ctx *ctx, int op, int coarse_nodes, double *ms) {
                 gen = ctx->coarse_gen;
             }
         }
-        if (ex) JFM_HIP(ctx, hipGraphLaunch(ex, s));
-        else seq_err = setup_sequence(ctx, tangent);
-    } else {
-        seq_err = setup_sequence(ctx, tangent);
-    }
+        return ex ? hipGraphLaunch(ex, s) : seq();
+    };
+    hipError_t seq_err = run(ctx->coarse_graph[tangent ? 1 : 0], ctx->coarse_graph_gen[tangent ? 1 : 0],
+                             [&] { return setup_matrix_sequence(ctx, tangent); });
+    // the learned blocks beside the inversion, whose steps leave most CUs
+    // idle (beside the element-level products they slowed those by half)
+    int lrc = JFM_OK;
+    if (seq_err == hipSuccess && learned_beside) lrc = jfem_learned_launch(ctx);
+    if (seq_err == hipSuccess)
+        seq_err = run(ctx->coarse_inv_graph, ctx->coarse_inv_gen, [&] { return setup_inverse_sequence(ctx); });
     if (seq_err != hipSuccess) {
         // the probe buffers may no longer be zero: drop the level
+        jfem_learned_drop(ctx);
         jfem_coarse_release(ctx);
         JFM_HIP(ctx, seq_err);
+    }
+    if (lrc) {
+        (void)hipStreamSynchronize(s);
+        return lrc;
     }
     int hflag = 0;
     JFM_HIP(ctx, hipMemcpyAsync(&hflag, ctx->counter + 48, sizeof(int), hipMemcpyDeviceToHost, s));

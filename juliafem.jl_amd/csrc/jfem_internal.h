@@ -254,6 +254,8 @@ struct jfm_ctx {
     int32_t *ecol = nullptr, *ncol = nullptr;        // per colour: probed elements, nodes
     hipGraphExec_t coarse_graph[2] = {nullptr, nullptr};  // probe sequence for K, K_T
     int coarse_graph_gen[2] = {0, 0};
+    hipGraphExec_t coarse_inv_graph = nullptr;  // the inverse and fold sequence
+    int coarse_inv_gen = 0;
     std::vector<int32_t> h_ecol_ptr, h_ncol_ptr, h_cseg;
     int32_t *segb = nullptr, *sega = nullptr;        // colour node lists by aggregate
     int32_t *csub_ptr = nullptr, *csub = nullptr;    // colour node lists: their slots in the colour's elements
@@ -380,7 +382,9 @@ template <typename T>
 hipError_t launch_patch_apply(jfm_ctx *ctx, int op, const T *x, T *y, const T *dotv, double *part, const int *done,
                               int *npart);
 // coarse.hip
-int jfem_coarse_setup(jfm_ctx *ctx, int op, int coarse_nodes, double *ms);
+// With learned_beside, the learned blocks start on the second stream
+// (jfem_learned_launch) once A_c is formed, beside its inversion.
+int jfem_coarse_setup(jfm_ctx *ctx, int op, int coarse_nodes, double *ms, bool learned_beside = false);
 int jfem_coarse_matrix(jfm_ctx *ctx, int op, int coarse_nodes, int64_t *nc, double *Ac, int32_t *node_agg);
 bool jfem_use_graphs();  // false when JFEM_GRAPHS=0 (profilers)
 void jfem_coarse_release(jfm_ctx *ctx);
