@@ -826,7 +826,7 @@ static hipError_t pcg_iteration(jfm_ctx *ctx, int op, int slot, const T *Minv, b
         if (e != hipSuccess) return e;
         e = launch_coarse_gemv_fused(ctx, slot, &np_wz);
         if (e != hipSuccess) return e;
-        return launch_pcg_direction_coarse<T>(ctx, slot, np_rz, np_wz, p, z);
+        return launch_pcg_direction_coarse<T>(ctx, slot, np_rz, np_wz, p, z, x);  // x += alpha p here
     }
     // with the coarse level, the update also writes m o r in aggregate order
     T *rA = coarse ? coarse_rA<T>(ctx) : nullptr;
@@ -836,7 +836,7 @@ static hipError_t pcg_iteration(jfm_ctx *ctx, int op, int slot, const T *Minv, b
         int np_wz = 0;
         e = launch_coarse_product_agg<T>(ctx, rA, &ctx->S->done[slot], &np_wz);
         if (e != hipSuccess) return e;
-        return launch_pcg_direction_coarse<T>(ctx, slot, np_rz, np_wz, p, z);
+        return launch_pcg_direction_coarse<T>(ctx, slot, np_rz, np_wz, p, z, nullptr);
     }
     return launch_pcg_direction<T>(ctx, slot, np_rz, p, z);
 }

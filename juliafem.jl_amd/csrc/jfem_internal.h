@@ -408,7 +408,7 @@ bool jfem_coarse_fused_restriction(const jfm_ctx *ctx);
 hipError_t launch_coarse_gemv_fused(jfm_ctx *ctx, int slot, int *npart_wz);
 // p = z + m o Q zc + beta p, beta from the (r.z, r.r) and w.zc partials
 template <typename T>
-hipError_t launch_pcg_direction_coarse(jfm_ctx *ctx, int slot, int npart_rz, int npart_wz, T *p, const T *z);
+hipError_t launch_pcg_direction_coarse(jfm_ctx *ctx, int slot, int npart_rz, int npart_wz, T *p, const T *z, T *x);
 // heat.hip (kOpHeat: conduction + convection, FP64, row mask ctx->fixed)
 constexpr int kOpHeat = 17;
 hipError_t launch_heat_apply(jfm_ctx *ctx, const double *x, double *y, const double *dotv, double *part, const int *done,

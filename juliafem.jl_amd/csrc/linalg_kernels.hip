@@ -198,20 +198,22 @@ __global__ __launch_bounds__(512) void k_pcg_update(int N, int slot, int npart_p
 
 // Two-level form of the update, with the aggregates' nodes as the blocks'
 // work: block b takes aggregates b per .. (b + 1) per - 1, each one's nodes
-// agg_nodes[agg_ptr[g] .. agg_ptr[g + 1]) (ascending node id).  Besides x, r,
-// z and the partials of (r.Br, r.r) (slot b), it forms the coarse right-hand
+// agg_nodes[agg_ptr[g] .. agg_ptr[g + 1]) (ascending node id).  Besides r, z
+// and the partials of (r.Br, r.r) (slot b), it forms the coarse right-hand
 // side of the new residual from the values in registers, w_g = sum_{a in g}
 // [m_a r_a ; d_a x (m_a r_a)] (cdA: the offsets d in agg_nodes order), in
 // FP64 (w_new) and FP32 (wc, the GEMV's operand).  The first aggregate's
-// first kU nodes per thread are in flight before the alpha reduction.
+// first kU nodes per thread are in flight before the alpha reduction.  x is
+// updated by the direction kernel (k_pcg_dir_coarse), which reads p anyway:
+// this kernel then reads neither x nor p.
 // 512 threads with four FP32 (three FP64) nodes each cover a 1,536-node
 // aggregate in one pass of loads; 1,024 threads with two nodes each measured
 // 4% slower solves.
 constexpr int kAggThreads = 512;
 template <typename T>
 __global__ __launch_bounds__(kAggThreads) void k_pcg_update_agg(
-    int n_agg, int nblk, int per, int slot, int npart_pq, T *__restrict__ x, T *__restrict__ r, T *__restrict__ z,
-    const T *__restrict__ p, const T *__restrict__ q, const T *__restrict__ Minv, const uint8_t *__restrict__ fixed,
+    int n_agg, int nblk, int per, int slot, int npart_pq, T *__restrict__ r, T *__restrict__ z,
+    const T *__restrict__ q, const T *__restrict__ Minv, const uint8_t *__restrict__ fixed,
     const int32_t *__restrict__ agg_ptr, const int32_t *__restrict__ agg_nodes, const T *__restrict__ cdA,
     double *__restrict__ partials, PcgScalars *S, double *__restrict__ w_new, float *__restrict__ wc) {
     const int b = xcd_block();
@@ -219,15 +221,13 @@ __global__ __launch_bounds__(kAggThreads) void k_pcg_update_agg(
     const int g0 = b * per, g1 = min(g0 + per, n_agg);
     constexpr int kU = sizeof(T) == 4 ? 4 : 3;
     const int t = (int)threadIdx.x, nt = (int)blockDim.x;
-    Trip<T> xv[kU], pv[kU], rv[kU], qv[kU], dv[kU];
+    Trip<T> rv[kU], qv[kU], dv[kU];
     Blk9<T> M[kU];
     Trip<uint8_t> fv[kU];
     int av[kU];
     auto load = [&](int u, int k) {
         const int a = agg_nodes[k];
         av[u] = a;
-        xv[u] = ld3(x, a);
-        pv[u] = ld3(p, a);
         rv[u] = ld3(r, a);
         qv[u] = ld3(q, a);
         M[u] = ld9(Minv, a);
@@ -258,7 +258,6 @@ __global__ __launch_bounds__(kAggThreads) void k_pcg_update_agg(
     auto update = [&](int u, double (&w)[6]) {
         const int a = av[u];
         const T rn[3] = {rv[u].a - alpha * qv[u].a, rv[u].b - alpha * qv[u].b, rv[u].c - alpha * qv[u].c};
-        st3(x, a, xv[u].a + alpha * pv[u].a, xv[u].b + alpha * pv[u].b, xv[u].c + alpha * pv[u].c);
         st3(r, a, rn[0], rn[1], rn[2]);
         const T *Mu = M[u].v;
         const T z0 = Mu[0] * rn[0] + Mu[1] * rn[1] + Mu[2] * rn[2];
@@ -346,7 +345,8 @@ __global__ __launch_bounds__(256) void k_pcg_dir(int n3, int slot, int npart_rz,
 // r.Br (update partials) + w.zc (coarse product partials).
 template <typename T>
 __global__ __launch_bounds__(256) void k_pcg_dir_coarse(int N, int slot, int npart_rz, int npart_wz, T *__restrict__ p,
-                                                       const T *__restrict__ z, const int32_t *__restrict__ node_agg,
+                                                       T *__restrict__ x, const T *__restrict__ z,
+                                                       const int32_t *__restrict__ node_agg,
                                                        const T *__restrict__ cd, const uint8_t *__restrict__ fixed,
                                                        const float *__restrict__ zc, const double *__restrict__ partials,
                                                        PcgScalars *S) {
@@ -354,12 +354,13 @@ __global__ __launch_bounds__(256) void k_pcg_dir_coarse(int N, int slot, int npa
     // reductions
     constexpr int kDN = kDirNodes;
     const int a0 = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
-    Trip<T> zv[kDN], pv[kDN], dv[kDN];
+    Trip<T> zv[kDN], pv[kDN], dv[kDN], xv[kDN];
     Trip<uint8_t> fx[kDN];
     Trip<float> yc[kDN][2];  // the node's aggregate's (t, omega), read as soon as its id arrives
     auto load = [&](int u, int a) {
         zv[u] = ld3(z, a);
         pv[u] = ld3(p, a);
+        if (x) xv[u] = ld3(x, a);
         dv[u] = ld3(cd, a);
         fx[u] = ld3(fixed, a);
         const int g = node_agg[a];
@@ -389,7 +390,9 @@ __global__ __launch_bounds__(256) void k_pcg_dir_coarse(int N, int slot, int npa
     const double rz = t[0] + t[2];
     const double beta_d = rz / S->rz[slot];
     const T beta = (T)beta_d;
+    const T alpha = (T)S->alpha;  // this iteration's step, as the update kernel applied it to r
     auto update = [&](int u, int a) {
+        if (x) st3(x, a, xv[u].a + alpha * pv[u].a, xv[u].b + alpha * pv[u].b, xv[u].c + alpha * pv[u].c);
         const Trip<float> y0 = yc[u][0], y1 = yc[u][1];
         const T y[6] = {(T)y0.a, (T)y0.b, (T)y0.c, (T)y1.a, (T)y1.b, (T)y1.c};
         const T d0[3] = {dv[u].a, dv[u].b, dv[u].c};
@@ -528,7 +531,7 @@ hipError_t launch_pcg_update(jfm_ctx *ctx, int slot, int npart_pq, T *x, T *r, T
         const int nb = (n_agg + per - 1) / per;
         *npart = nb;
         k_pcg_update_agg<T><<<xcd_grid((unsigned)nb), kAggThreads, 0, ctx->stream>>>(
-            n_agg, nb, per, slot, npart_pq, x, r, z, p, q, Minv, ctx->fixed, ctx->agg_ptr, ctx->agg_nodes, cdA,
+            n_agg, nb, per, slot, npart_pq, r, z, q, Minv, ctx->fixed, ctx->agg_ptr, ctx->agg_nodes, cdA,
             ctx->partials, ctx->S, ctx->wc64 + (size_t)(slot ^ 1) * ctx->nc, ctx->wc32);
         return hipGetLastError();
     }
@@ -546,7 +549,7 @@ hipError_t launch_pcg_direction(jfm_ctx *ctx, int slot, int npart_rz, T *p, cons
 }
 
 template <typename T>
-hipError_t launch_pcg_direction_coarse(jfm_ctx *ctx, int slot, int npart_rz, int npart_wz, T *p, const T *z) {
+hipError_t launch_pcg_direction_coarse(jfm_ctx *ctx, int slot, int npart_rz, int npart_wz, T *p, const T *z, T *x) {
     const T *cd = sizeof(T) == 8 ? (const T *)ctx->cd64 : (const T *)ctx->cd32;
     // kDirNodes nodes per thread, at most kDirBlocks blocks (no partials are
     // written, so the grid is not capped at kMaxPart): 683 blocks at 349,569
@@ -554,11 +557,11 @@ hipError_t launch_pcg_direction_coarse(jfm_ctx *ctx, int slot, int npart_rz, int
     const int64_t per = (int64_t)kBlock * kDirNodes;
     const unsigned g = (unsigned)std::min<int64_t>(std::max<int64_t>(1, (ctx->N + per - 1) / per), kDirBlocks);
     k_pcg_dir_coarse<T><<<g, kBlock, 0, ctx->stream>>>(
-        (int)ctx->N, slot, npart_rz, npart_wz, p, z, ctx->node_agg, cd, ctx->fixed, ctx->zc32, ctx->partials, ctx->S);
+        (int)ctx->N, slot, npart_rz, npart_wz, p, x, z, ctx->node_agg, cd, ctx->fixed, ctx->zc32, ctx->partials, ctx->S);
     return hipGetLastError();
 }
-template hipError_t launch_pcg_direction_coarse<double>(jfm_ctx *, int, int, int, double *, const double *);
-template hipError_t launch_pcg_direction_coarse<float>(jfm_ctx *, int, int, int, float *, const float *);
+template hipError_t launch_pcg_direction_coarse<double>(jfm_ctx *, int, int, int, double *, const double *, double *);
+template hipError_t launch_pcg_direction_coarse<float>(jfm_ctx *, int, int, int, float *, const float *, float *);
 
 hipError_t launch_finalize(jfm_ctx *ctx, const double *part, int npart, int nv, int fin) {
     k_finalize<<<1, kBlock, 0, ctx->stream>>>(part, npart, nv, fin, ctx->S);
