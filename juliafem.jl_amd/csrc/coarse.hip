@@ -311,11 +311,17 @@ __global__ __launch_bounds__(1024) void k_gal_sum(const int32_t *__restrict__ up
 //   A_ij <- A_ij - (A_iK P^-1) A_Kj          (i, j outside K)
 // After the last step A holds A_c^-1.  A pivot of the small in-LDS sweep that
 // is not positive and finite flags the matrix as not positive definite.
-constexpr int kGJ = 32;  // pivot block
+#ifndef JFEM_GJ_WIDTH
+#define JFEM_GJ_WIDTH 32
+#endif
+constexpr int kGJ = JFEM_GJ_WIDTH;  // pivot block (32 or 64; 64 measured slower: setup 2.52 ms against 1.98 ms)
+static_assert(kGJ == 32 || kGJ == 64, "pivot blocks of 32 or 64");
+constexpr int kGJRows = 256 / kGJ;  // panel rows per 256-thread block
+constexpr int kGJSub = kGJ / 16;    // look-ahead entries per thread and direction
 
 // P^-1 of a bk x bk pivot block (padded to the identity), entry (i, j) read
 // by ld(i, j), by scalar Gauss-Jordan over a block of 256 threads: thread t
-// holds column j = t % kGJ of rows t / kGJ + 8 u (u < 4).  At step k the
+// holds column j = t % kGJ of rows t / kGJ + RG u (RG = 256 / kGJ, u < kGJ / RG).  At step k the
 // threads holding column k (the multipliers) and row k (the pivot row)
 // write them to LDS (double-buffered by step parity: one barrier per step),
 // and every thread updates its four entries.  The result goes to P.
@@ -397,7 +403,7 @@ __global__ __launch_bounds__(256) void k_gj_panel(int n, int k0, int bk, const d
     // earlier, K not), and row K of a column right of it is M_jK' (neither
     // pivoted).
     const int TK = k0 / 64;
-    const int row = blockIdx.x * 8 + threadIdx.x / kGJ, m = threadIdx.x & (kGJ - 1);
+    const int row = blockIdx.x * kGJRows + threadIdx.x / kGJ, m = threadIdx.x & (kGJ - 1);
     if (row < n) {
         double c = 0.0;
         if (row / 64 >= TK) {
@@ -428,11 +434,11 @@ __device__ __forceinline__ void gj_next_pivot(int n, int bk, const double *__res
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
     // the block before this step's update, as the panel kernel copied it (the
     // tiles of this kernel overwrite A meanwhile)
-    double av[2][2];
+    double av[kGJSub][kGJSub];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < kGJSub; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
+        for (int b = 0; b < kGJSub; ++b) {
             const int i = ty + 16 * a, j = tx + 16 * b;
             av[a][b] = (i < nbk && j < nbk) ? Anext[i * kGJ + j] : 0.0;
         }
@@ -442,22 +448,26 @@ __device__ __forceinline__ void gj_next_pivot(int n, int bk, const double *__res
         Rs[a][m] = (a < bk && m < nbk) ? Rbuf[(size_t)a * n + nk0 + m] : 0.0;
     }
     __syncthreads();
-    double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+    double acc[kGJSub][kGJSub];
+#pragma unroll
+    for (int a = 0; a < kGJSub; ++a)
+#pragma unroll
+        for (int b = 0; b < kGJSub; ++b) acc[a][b] = 0.0;
     for (int m = 0; m < bk; ++m) {
-        double cv[2], rv[2];
+        double cv[kGJSub], rv[kGJSub];
 #pragma unroll
-        for (int a = 0; a < 2; ++a) cv[a] = Cs[ty + 16 * a][m];
+        for (int a = 0; a < kGJSub; ++a) cv[a] = Cs[ty + 16 * a][m];
 #pragma unroll
-        for (int b = 0; b < 2; ++b) rv[b] = Rs[m][tx + 16 * b];
+        for (int b = 0; b < kGJSub; ++b) rv[b] = Rs[m][tx + 16 * b];
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
+        for (int a = 0; a < kGJSub; ++a)
 #pragma unroll
-            for (int b = 0; b < 2; ++b) acc[a][b] += cv[a] * rv[b];
+            for (int b = 0; b < kGJSub; ++b) acc[a][b] += cv[a] * rv[b];
     }
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < kGJSub; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) Ps[ty + 16 * a][tx + 16 * b] = av[a][b] - acc[a][b];
+        for (int b = 0; b < kGJSub; ++b) Ps[ty + 16 * a][tx + 16 * b] = av[a][b] - acc[a][b];
     __syncthreads();
     const bool ok = gj_pivot_inverse(nbk, [&](int i, int j) { return Ps[i][j]; }, Cs);
     for (int t = threadIdx.x; t < kGJ * kGJ; t += blockDim.x) Pnext[t] = Cs[t / kGJ][t - kGJ * (t / kGJ)];
@@ -1186,7 +1196,7 @@ static hipError_t setup_inverse_sequence(jfm_ctx *ctx) {
     for (int k0 = 0; k0 < nc; k0 += kGJ) {
         const int bk = std::min(kGJ, nc - k0);
         const int nk0 = k0 + kGJ, nbk = nk0 < nc ? std::min(kGJ, nc - nk0) : 0;
-        k_gj_panel<<<(nc + 7) / 8, 256, 0, s>>>(nc, k0, bk, ctx->Ac, ctx->gjC, ctx->gjR, ctx->gjP,
+        k_gj_panel<<<(nc + kGJRows - 1) / kGJRows, 256, 0, s>>>(nc, k0, bk, ctx->Ac, ctx->gjC, ctx->gjR, ctx->gjP,
                                                 k0 > 0 ? Pnext : nullptr, nk0, nbk, Anext, flag);
         k_gj_update<<<tiles * (tiles + 1) / 2 + (nbk > 0 ? 1 : 0), 256, 0, s>>>(
             nc, k0, bk, ctx->Ac, ctx->gjC, ctx->gjR, ctx->gjP, nk0, nbk, Anext, Pnext, flag);
