@@ -375,7 +375,8 @@ void jfem_patch_release(jfm_ctx *ctx);
 // whether `op` runs in the patch form on this context
 inline bool jfem_patch_form(const jfm_ctx *ctx, int op);
 template <typename T>
-hipError_t launch_patch_phase(jfm_ctx *ctx, int op, const T *x, T *y, const T *dotv, const int *done);
+hipError_t launch_patch_phase(jfm_ctx *ctx, int op, const T *x, T *y, const T *dotv, const int *done,
+                              double *pout = nullptr);
 template <typename T>
 hipError_t launch_patch_combine(jfm_ctx *ctx, T *y, const T *dotv, double *part, const int *done, int *npart);
 template <typename T>

@@ -528,11 +528,12 @@ template <typename T> static bool energy_dot(const jfm_ctx *ctx, int op, const T
     return op == JFM_OP_MATFREE && ctx->affine && ctx->npe == 10 && dotv && dotv == x;
 }
 
+// The block partials of the dot go to pout (ctx->ppart when null).
 template <int NPE, typename T>
-static void patch_kernel(jfm_ctx *ctx, int op, const T *x, T *y, const T *dotv, const int *done) {
+static void patch_kernel(jfm_ctx *ctx, int op, const T *x, T *y, const T *dotv, const int *done, double *pout) {
     const dim3 grid(xcd_grid(ctx->npatch));
     const int n = (int)ctx->n_el;
-    double *pp = dotv ? ctx->ppart : nullptr;
+    double *pp = dotv ? (pout ? pout : ctx->ppart) : nullptr;
     if (energy_dot<T>(ctx, op, x, dotv)) dotv = nullptr;
     auto launch = [&](auto kernel, const T *el) {
         kernel<<<grid, kPatch, 0, ctx->stream>>>(n, ctx->npatch, ctx->npn, ctx->connP, el, x, ctx->pn_ptr, ctx->pn_cnt,
@@ -548,13 +549,14 @@ static void patch_kernel(jfm_ctx *ctx, int op, const T *x, T *y, const T *dotv, 
 }
 
 template <typename T>
-hipError_t launch_patch_phase(jfm_ctx *ctx, int op, const T *x, T *y, const T *dotv, const int *done) {
+hipError_t launch_patch_phase(jfm_ctx *ctx, int op, const T *x, T *y, const T *dotv, const int *done,
+                              double *pout) {
     if (ctx->n_el == 0) return hipSuccess;
     if (!jfem_patch_form(ctx, op)) return hipErrorInvalidValue;
     switch (ctx->npe) {
-    case 10: patch_kernel<10, T>(ctx, op, x, y, dotv, done); break;
-    case 4: patch_kernel<4, T>(ctx, op, x, y, dotv, done); break;
-    case 8: patch_kernel<8, T>(ctx, op, x, y, dotv, done); break;
+    case 10: patch_kernel<10, T>(ctx, op, x, y, dotv, done, pout); break;
+    case 4: patch_kernel<4, T>(ctx, op, x, y, dotv, done, pout); break;
+    case 8: patch_kernel<8, T>(ctx, op, x, y, dotv, done, pout); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -578,14 +580,23 @@ hipError_t launch_patch_combine(jfm_ctx *ctx, T *y, const T *dotv, double *part,
 template <typename T>
 hipError_t launch_patch_apply(jfm_ctx *ctx, int op, const T *x, T *y, const T *dotv, double *part, const int *done,
                               int *npart) {
-    hipError_t e = launch_patch_phase<T>(ctx, op, x, y, dotv, done);
+    const bool energy = energy_dot<T>(ctx, op, x, dotv);
+    if (energy && part && ctx->npatch <= kMaxPart) {
+        // the patches' energy partials are the final ones: straight to part,
+        // and the shared-node kernel only sums nodes
+        hipError_t e = launch_patch_phase<T>(ctx, op, x, y, dotv, done, part);
+        if (e != hipSuccess) return e;
+        if (npart) *npart = ctx->npatch;
+        return launch_patch_combine<T>(ctx, y, nullptr, nullptr, done, nullptr);
+    }
+    hipError_t e = launch_patch_phase<T>(ctx, op, x, y, dotv, done, nullptr);
     if (e != hipSuccess) return e;
     // with the element energies the shared nodes add nothing to the dot
-    return launch_patch_combine<T>(ctx, y, energy_dot<T>(ctx, op, x, dotv) ? nullptr : dotv, part, done, npart);
+    return launch_patch_combine<T>(ctx, y, energy ? nullptr : dotv, part, done, npart);
 }
 
-template hipError_t launch_patch_phase<double>(jfm_ctx *, int, const double *, double *, const double *, const int *);
-template hipError_t launch_patch_phase<float>(jfm_ctx *, int, const float *, float *, const float *, const int *);
+template hipError_t launch_patch_phase<double>(jfm_ctx *, int, const double *, double *, const double *, const int *, double *);
+template hipError_t launch_patch_phase<float>(jfm_ctx *, int, const float *, float *, const float *, const int *, double *);
 template hipError_t launch_patch_combine<double>(jfm_ctx *, double *, const double *, double *, const int *, int *);
 template hipError_t launch_patch_combine<float>(jfm_ctx *, float *, const float *, double *, const int *, int *);
 template hipError_t launch_patch_apply<double>(jfm_ctx *, int, const double *, double *, const double *, double *, const int *, int *);
