@@ -817,12 +817,18 @@ static hipError_t pcg_iteration(jfm_ctx *ctx, int op, int slot, const T *Minv, b
     Workspace &w = ws_of<T>(ctx);
     T *x = (T *)w.x, *r = (T *)w.r, *z = (T *)w.z, *p = (T *)w.p, *q = (T *)w.q;
     int np_pq = 0, np_rz = 0;
-    hipError_t e = op_apply<T>(ctx, op, p, q, p, ctx->partials + part_pq(slot), &ctx->S->done[slot], &np_pq);
+    const bool fused = coarse && jfem_coarse_fused_restriction(ctx);
+    // with the shared nodes' sums left to the update kernel, K p is the patch
+    // kernel alone, its per-patch energies the p.q partials
+    const bool defer = fused && jfem_patch_defer_shared(ctx, op);
+    hipError_t e = defer ? launch_patch_phase<T>(ctx, op, p, q, p, &ctx->S->done[slot], ctx->partials + part_pq(slot))
+                         : op_apply<T>(ctx, op, p, q, p, ctx->partials + part_pq(slot), &ctx->S->done[slot], &np_pq);
     if (e != hipSuccess) return e;
-    if (coarse && jfem_coarse_fused_restriction(ctx)) {
+    if (defer) np_pq = ctx->npatch;
+    if (fused) {
         // the update's blocks, one per aggregate, form w = Q' m r for the GEMV
         int np_wz = 0;
-        e = launch_pcg_update<T>(ctx, slot, np_pq, x, r, z, p, q, Minv, &np_rz, nullptr, true);
+        e = launch_pcg_update<T>(ctx, slot, np_pq, x, r, z, p, q, Minv, &np_rz, nullptr, true, defer);
         if (e != hipSuccess) return e;
         e = launch_coarse_gemv_fused(ctx, slot, &np_wz);
         if (e != hipSuccess) return e;
@@ -925,6 +931,8 @@ static int pcg_run(jfm_ctx *ctx, int op, const T *Minv, double rtol, int maxit, 
     if (int rc = finalize_misc(ctx, np, 1, FIN_BB)) return rc;
     CHECK_LAUNCH(ctx, launch_pcg_init<T>(ctx, r, z, p, Minv, &np));
     if (coarse) CHECK_LAUNCH(ctx, launch_coarse_correct<T>(ctx, r, z, p, ctx->partials + kPartMisc, nullptr, &np));
+    if (coarse && jfem_coarse_fused_restriction(ctx))
+        if (int rc = jfem_patch_qsrc(ctx)) return rc;
     if (int rc = finalize_misc(ctx, np, 2, FIN_INIT)) return rc;
     // Batches of up to check_every iterations; once the residual falls, the
     // next batch is the power of two that covers the iterations its rate

@@ -293,6 +293,9 @@ struct jfm_ctx {
     int32_t *bptr = nullptr, *bnode = nullptr;  // shared nodes, their partial slots
     double *bpart = nullptr;                  // 3 x slots (FP32 or FP64 view)
     double *ppart = nullptr;                  // per-patch partials of y.dotv
+    int bmaxcnt = 0;                          // most slots of one shared node
+    int32_t *qsrc = nullptr;                  // N, aggregate order: -1 or (first slot << 5 | slots)
+    int qsrc_gen = -1;                        // coarse_gen qsrc was built for
     double *pv6 = nullptr, *qv6 = nullptr;  // probe vectors, 6 x 3N (zero outside the probed rows)
     double *ye6 = nullptr;                  // probe incidence slots, 6 x 3 npe n_el (zero outside the probed elements)
     // heat conduction (heat.hip): temperature in component 0
@@ -382,6 +385,15 @@ hipError_t launch_patch_combine(jfm_ctx *ctx, T *y, const T *dotv, double *part,
 template <typename T>
 hipError_t launch_patch_apply(jfm_ctx *ctx, int op, const T *x, T *y, const T *dotv, double *part, const int *done,
                               int *npart);
+// The two-level PCG may leave the shared nodes' sums to its update kernel,
+// which visits the nodes in aggregate order: qsrc[k] is -1 for the k-th node
+// (agg_nodes[k]) when the patch kernel writes it to y, else its first
+// partial slot << 5 | its slot count.  jfem_patch_qsrc builds it for the
+// current aggregation (a no-op when the form is not available);
+// jfem_patch_defer_shared tells whether the PCG's K.p of `op` can use it
+// (JFEM_DEFER_SHARED=0 turns it off).
+int jfem_patch_qsrc(jfm_ctx *ctx);
+bool jfem_patch_defer_shared(const jfm_ctx *ctx, int op);
 // coarse.hip
 // With learned_beside, the learned blocks start on the second stream
 // (jfem_learned_launch) once A_c is formed, beside its inversion.
@@ -428,9 +440,12 @@ hipError_t launch_bsr_spmv(jfm_ctx *ctx, const T *vals, const T *x, T *y, const 
                            const int *done, int *npart);
 template <typename T>
 hipError_t launch_pcg_init(jfm_ctx *ctx, T *r, T *z, T *p, const T *Minv, int *npart);
+// With shared_q (and restrict_q), q holds only what the patch kernel writes
+// and the update sums the shared nodes' partial slots itself (qsrc).
 template <typename T>
 hipError_t launch_pcg_update(jfm_ctx *ctx, int slot, int npart_pq, T *x, T *r, T *z, const T *p, const T *q,
-                             const T *Minv, int *npart, T *rA = nullptr, bool restrict_q = false);
+                             const T *Minv, int *npart, T *rA = nullptr, bool restrict_q = false,
+                             bool shared_q = false);
 template <typename T>
 hipError_t launch_pcg_direction(jfm_ctx *ctx, int slot, int npart_rz, T *p, const T *z);
 hipError_t launch_finalize(jfm_ctx *ctx, const double *part, int npart, int nv, int fin);

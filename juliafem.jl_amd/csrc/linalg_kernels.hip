@@ -206,6 +206,11 @@ __global__ __launch_bounds__(512) void k_pcg_update(int N, int slot, int npart_p
 // first kU nodes per thread are in flight before the alpha reduction.  x is
 // updated by the direction kernel (k_pcg_dir_coarse), which reads p anyway:
 // this kernel then reads neither x nor p.
+// With qsrc, q holds only the nodes the patch kernel finished, and a shared
+// node's q is the sum of its partial slots bpart[s .. s + c) in ascending
+// patch order, masked, as the shared-node kernel of the patch operator forms
+// it (the same sums in the same order, so the same values): the first kQS
+// slots are loaded with the node, further ones kQS at a time.
 // 512 threads with four FP32 (three FP64) nodes each cover a 1,536-node
 // aggregate in one pass of loads; 1,024 threads with two nodes each measured
 // 4% slower solves.
@@ -213,26 +218,66 @@ constexpr int kAggThreads = 512;
 template <typename T>
 __global__ __launch_bounds__(kAggThreads) void k_pcg_update_agg(
     int n_agg, int nblk, int per, int slot, int npart_pq, T *__restrict__ r, T *__restrict__ z,
-    const T *__restrict__ q, const T *__restrict__ Minv, const uint8_t *__restrict__ fixed,
+    const T *__restrict__ q, const int32_t *__restrict__ qsrc, const T *__restrict__ bpart,
+    const T *__restrict__ Minv, const uint8_t *__restrict__ fixed,
     const int32_t *__restrict__ agg_ptr, const int32_t *__restrict__ agg_nodes, const T *__restrict__ cdA,
     double *__restrict__ partials, PcgScalars *S, double *__restrict__ w_new, float *__restrict__ wc) {
     const int b = xcd_block();
     if (b >= nblk) return;  // whole block (grid padded to a multiple of 8)
     const int g0 = b * per, g1 = min(g0 + per, n_agg);
     constexpr int kU = sizeof(T) == 4 ? 4 : 3;
+    constexpr int kQS = sizeof(T) == 4 ? 4 : 2;  // slots loaded with the node (counts are mostly 2 and 4)
     const int t = (int)threadIdx.x, nt = (int)blockDim.x;
-    Trip<T> rv[kU], qv[kU], dv[kU];
+    Trip<T> rv[kU], qv[kU][kQS], dv[kU];
     Blk9<T> M[kU];
     Trip<uint8_t> fv[kU];
-    int av[kU];
+    int av[kU], qc[kU];
+    const T *qp[kU];
     auto load = [&](int u, int k) {
         const int a = agg_nodes[k];
         av[u] = a;
         rv[u] = ld3(r, a);
-        qv[u] = ld3(q, a);
+        if (qsrc) {
+            // a finished node reads q as its single "slot"
+            const int e = qsrc[k];
+            qp[u] = e < 0 ? q + 3 * (size_t)a : bpart + 3 * (size_t)(e >> 5);
+            qc[u] = e < 0 ? 1 : (e & 31);
+#pragma unroll
+            for (int j = 0; j < kQS; ++j) qv[u][j] = j < qc[u] ? ld3(qp[u], j) : Trip<T>{T(0), T(0), T(0)};
+        } else {
+            qv[u][0] = ld3(q, a);
+        }
         M[u] = ld9(Minv, a);
         fv[u] = ld3(fixed, a);
         dv[u] = ld3(cdA, k);
+    };
+    // q of node u: its slots summed in order from zero, masked (with qsrc)
+    auto qsum = [&](int u) -> Trip<T> {
+        if (!qsrc) return qv[u][0];
+        T s0 = T(0), s1 = T(0), s2 = T(0);
+#pragma unroll
+        for (int j = 0; j < kQS; ++j)
+            if (j < qc[u]) {
+                s0 += qv[u][j].a;
+                s1 += qv[u][j].b;
+                s2 += qv[u][j].c;
+            }
+        for (int j0 = kQS; j0 < qc[u]; j0 += kQS) {  // nodes of more than kQS slots
+            Trip<T> w[kQS];
+#pragma unroll
+            for (int j = 0; j < kQS; ++j) w[j] = j0 + j < qc[u] ? ld3(qp[u], j0 + j) : Trip<T>{T(0), T(0), T(0)};
+#pragma unroll
+            for (int j = 0; j < kQS; ++j)
+                if (j0 + j < qc[u]) {
+                    s0 += w[j].a;
+                    s1 += w[j].b;
+                    s2 += w[j].c;
+                }
+        }
+        if (fv[u].a) s0 = T(0);
+        if (fv[u].b) s1 = T(0);
+        if (fv[u].c) s2 = T(0);
+        return Trip<T>{s0, s1, s2};
     };
     int kb = 0, ke = 0;
     auto prefetch = [&](int g) {
@@ -257,7 +302,8 @@ __global__ __launch_bounds__(kAggThreads) void k_pcg_update_agg(
     double v[2] = {0.0, 0.0};
     auto update = [&](int u, double (&w)[6]) {
         const int a = av[u];
-        const T rn[3] = {rv[u].a - alpha * qv[u].a, rv[u].b - alpha * qv[u].b, rv[u].c - alpha * qv[u].c};
+        const Trip<T> qn = qsum(u);
+        const T rn[3] = {rv[u].a - alpha * qn.a, rv[u].b - alpha * qn.b, rv[u].c - alpha * qn.c};
         st3(r, a, rn[0], rn[1], rn[2]);
         const T *Mu = M[u].v;
         const T z0 = Mu[0] * rn[0] + Mu[1] * rn[1] + Mu[2] * rn[2];
@@ -521,7 +567,7 @@ constexpr int kDirBlocks = JFEM_DIR_BLOCKS;  // grid of the PCG direction kernel
 
 template <typename T>
 hipError_t launch_pcg_update(jfm_ctx *ctx, int slot, int npart_pq, T *x, T *r, T *z, const T *p, const T *q,
-                             const T *Minv, int *npart, T *rA, bool restrict_q) {
+                             const T *Minv, int *npart, T *rA, bool restrict_q, bool shared_q) {
     const unsigned g = std::min<unsigned>(capped_grid(ctx->N, kBlock), kVecBlocks);
     *npart = (int)g;
     if (restrict_q) {  // one block per aggregate (or per run of them), w to wc64 by parity
@@ -530,8 +576,10 @@ hipError_t launch_pcg_update(jfm_ctx *ctx, int slot, int npart_pq, T *x, T *r, T
         const int per = (n_agg + kMaxPart - 1) / kMaxPart;
         const int nb = (n_agg + per - 1) / per;
         *npart = nb;
+        if (shared_q && !ctx->qsrc) return hipErrorInvalidValue;
         k_pcg_update_agg<T><<<xcd_grid((unsigned)nb), kAggThreads, 0, ctx->stream>>>(
-            n_agg, nb, per, slot, npart_pq, r, z, q, Minv, ctx->fixed, ctx->agg_ptr, ctx->agg_nodes, cdA,
+            n_agg, nb, per, slot, npart_pq, r, z, q, shared_q ? ctx->qsrc : nullptr,
+            shared_q ? (const T *)ctx->bpart : nullptr, Minv, ctx->fixed, ctx->agg_ptr, ctx->agg_nodes, cdA,
             ctx->partials, ctx->S, ctx->wc64 + (size_t)(slot ^ 1) * ctx->nc, ctx->wc32);
         return hipGetLastError();
     }
@@ -610,8 +658,8 @@ template hipError_t launch_bsr_spmv<double>(jfm_ctx *, const double *, const dou
 template hipError_t launch_bsr_spmv<float>(jfm_ctx *, const float *, const float *, float *, const float *, double *, const int *, int *);
 template hipError_t launch_pcg_init<double>(jfm_ctx *, double *, double *, double *, const double *, int *);
 template hipError_t launch_pcg_init<float>(jfm_ctx *, float *, float *, float *, const float *, int *);
-template hipError_t launch_pcg_update<double>(jfm_ctx *, int, int, double *, double *, double *, const double *, const double *, const double *, int *, double *, bool);
-template hipError_t launch_pcg_update<float>(jfm_ctx *, int, int, float *, float *, float *, const float *, const float *, const float *, int *, float *, bool);
+template hipError_t launch_pcg_update<double>(jfm_ctx *, int, int, double *, double *, double *, const double *, const double *, const double *, int *, double *, bool, bool);
+template hipError_t launch_pcg_update<float>(jfm_ctx *, int, int, float *, float *, float *, const float *, const float *, const float *, int *, float *, bool, bool);
 template hipError_t launch_pcg_direction<double>(jfm_ctx *, int, int, double *, const double *);
 template hipError_t launch_pcg_direction<float>(jfm_ctx *, int, int, float *, const float *);
 template hipError_t launch_convert<double, float>(jfm_ctx *, const double *, float *, int64_t, double);

@@ -309,12 +309,13 @@ void jfem_patch_release(jfm_ctx *ctx) {
     for (void **p : {(void **)&ctx->connP, (void **)&ctx->gfP64, (void **)&ctx->gfP32, (void **)&ctx->pn_ptr,
                      (void **)&ctx->pn_cnt, (void **)&ctx->pn_out, (void **)&ctx->rstart, (void **)&ctx->rbase,
                      (void **)&ctx->linT, (void **)&ctx->bptr, (void **)&ctx->bnode,
-                     (void **)&ctx->bpart, (void **)&ctx->ppart, (void **)&ctx->eperm}) {
+                     (void **)&ctx->bpart, (void **)&ctx->ppart, (void **)&ctx->eperm, (void **)&ctx->qsrc}) {
         jfm_free(ctx, *p);
         *p = nullptr;
     }
     ctx->patch_ready = false;
-    ctx->npatch = ctx->npn = ctx->nbnode = ctx->nslot = 0;
+    ctx->npatch = ctx->npn = ctx->nbnode = ctx->nslot = ctx->bmaxcnt = 0;
+    ctx->qsrc_gen = -1;
     if (ctx->ke_perm) {  // element matrices stored in the released patch order
         jfm_free(ctx, ctx->Ke);
         jfm_free(ctx, ctx->Ke32);
@@ -507,7 +508,64 @@ int jfem_patch_setup(jfm_ctx *ctx) {
     ctx->npn = (int)npn;
     ctx->nbnode = (int)nb;
     ctx->nslot = P.bptr.back();
+    ctx->bmaxcnt = 0;
+    for (int64_t i = 0; i < nb; ++i) ctx->bmaxcnt = std::max(ctx->bmaxcnt, P.bptr[i + 1] - P.bptr[i]);
     ctx->patch_ready = true;
+    return JFM_OK;
+}
+
+// qsrc for the update kernel of the two-level PCG (nodes in aggregate order)
+__global__ __launch_bounds__(256) void k_slot_fill(int N, int32_t *__restrict__ code) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) code[i] = -1;
+}
+
+__global__ __launch_bounds__(256) void k_slot_shared(int nb, const int32_t *__restrict__ bptr,
+                                                    const int32_t *__restrict__ bnode, int32_t *__restrict__ code) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < nb) code[bnode[b]] = bptr[b] << 5 | (bptr[b + 1] - bptr[b]);
+}
+
+__global__ __launch_bounds__(256) void k_slot_gather(int N, const int32_t *__restrict__ agg_nodes,
+                                                    const int32_t *__restrict__ code, int32_t *__restrict__ qsrc) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < N) qsrc[k] = code[agg_nodes[k]];
+}
+
+bool jfem_patch_defer_shared(const jfm_ctx *ctx, int op) {
+    static const bool on = [] {
+        const char *v = std::getenv("JFEM_DEFER_SHARED");
+        return !(v && v[0] == '0');
+    }();
+    // the patch kernel's dot must be the element energies (nothing left for
+    // the shared nodes), its partials final (one per patch)
+    return on && op == JFM_OP_MATFREE && jfem_patch_form(ctx, op) && ctx->affine && ctx->npe == 10 &&
+           ctx->npatch <= kMaxPart && ctx->qsrc && ctx->qsrc_gen == ctx->coarse_gen;
+}
+
+int jfem_patch_qsrc(jfm_ctx *ctx) {
+    if (!ctx->patch_ready || !ctx->agg_nodes || ctx->n_agg == 0) return JFM_OK;
+    if (ctx->qsrc && ctx->qsrc_gen == ctx->coarse_gen) return JFM_OK;
+    ctx->qsrc_gen = -1;
+    // slot codes hold 26 bits of slot index and 5 of count
+    if (ctx->bmaxcnt > 31 || ctx->nslot >= (1 << 26)) return JFM_OK;
+    const int N = (int)ctx->N;
+    if (!ctx->qsrc) ctx->qsrc = jfm_alloc<int32_t>(ctx, std::max(1, N));
+    int32_t *code = jfm_alloc<int32_t>(ctx, std::max(1, N));
+    if (!ctx->qsrc || !code) {
+        jfm_free(ctx, code);
+        return jfm_fail(ctx, JFM_ENOMEM, "shared-node code allocation failed");
+    }
+    hipStream_t s = ctx->stream;
+    const unsigned gN = (unsigned)((N + 255) / 256), gB = (unsigned)((ctx->nbnode + 255) / 256);
+    k_slot_fill<<<std::max(1u, gN), 256, 0, s>>>(N, code);
+    if (ctx->nbnode > 0) k_slot_shared<<<gB, 256, 0, s>>>(ctx->nbnode, ctx->bptr, ctx->bnode, code);
+    k_slot_gather<<<std::max(1u, gN), 256, 0, s>>>(N, ctx->agg_nodes, code, ctx->qsrc);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    jfm_free(ctx, code);
+    if (e != hipSuccess) return jfm_fail(ctx, JFM_EHIP, std::string("shared-node codes: ") + hipGetErrorString(e));
+    ctx->qsrc_gen = ctx->coarse_gen;
     return JFM_OK;
 }
 

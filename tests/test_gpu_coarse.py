@@ -172,6 +172,35 @@ def test_fused_restriction_matches_separate_restriction(cfg0, tmp_path):
         assert np.linalg.norm(u - ud) <= 1e-8 * np.linalg.norm(ud)
 
 
+def test_deferred_shared_sums_match_shared_node_kernel(cfg0, tmp_path):
+    """In the two-level PCG the update kernel sums the shared nodes' patch
+    partials itself (the shared-node kernel of the patch operator does not
+    run).  The sums are the same in the same order, so against the solve
+    with the shared-node kernel (JFEM_DEFER_SHARED=0, a child process) the
+    iterations and the answer are identical, bit for bit, in FP64 PCG and in
+    the FP32 inner solves."""
+    m, ctx, csr, f, ud = cfg0
+    res = {}
+    for meth in (0, 1):
+        say(f"shared sums in the update, method={meth}")
+        u, _, st = ctx.solve(method=meth, op="matfree", op_outer="matfree", rtol=1e-10, inner_rtol=1e-3,
+                             coarse_nodes=32)
+        assert st["converged"] == 1
+        res[meth] = (u, st["iterations"])
+    out = str(tmp_path / "combine.npz")
+    code = _CHILD.format(root=ROOT, pkg=os.path.join(ROOT, "juliafem.jl_amd"), out=out)
+    say("shared-node kernel in a child process (JFEM_DEFER_SHARED=0)")
+    env = dict(os.environ, JFEM_DEFER_SHARED="0")
+    subprocess.run([sys.executable, "-c", code], env=env, check=True, timeout=120)
+    r = np.load(out)
+    for meth in (0, 1):
+        u, it = res[meth]
+        say(f"  method={meth}: iterations deferred={it} kernel={int(r['it%d' % meth])}")
+        assert it == int(r["it%d" % meth])
+        assert np.array_equal(u, r["u%d" % meth])
+        assert np.linalg.norm(u - ud) <= 1e-8 * np.linalg.norm(ud)
+
+
 _CHILD_AC = r"""
 import sys
 import numpy as np
