@@ -74,6 +74,8 @@ __global__ __launch_bounds__(kPatch) void k_ku_patch(int n_el, int npatch, int n
     // the element product so that their latency overlaps it
     const int lb = pn_ptr[p], le = pn_ptr[p + 1], q0 = rstart[p];
     constexpr int kPre = 4, kPreR = sizeof(T) == 4 ? 3 : 1;  // FP64: registers are short
+    constexpr int kExt = 20;  // round 0's incidences kPre .. kPre + kExt - 1, loaded before the barrier
+    const int base0 = lb < le ? rbase[q0] : 0;
     int cnt[kPreR], oo[kPreR], mpre[kPreR][kPre];
     // every index below fits 32 bits: byte offsets from the uniform base
     // pointers keep the per-lane address arithmetic 32-bit
@@ -144,10 +146,29 @@ __global__ __launch_bounds__(kPatch) void k_ku_patch(int n_el, int npatch, int n
         } else
             mf_element<NPE, T, AFF>(n_el, k, connP, el, x, to_lds);
     }
+    // Round 0 holds the patch's nodes of most incidences (up to 24 in a Tet10
+    // box mesh, so one wave would otherwise load 20 of them one after the
+    // other after the barrier): the rest of its incidences, and the fixed
+    // flags of the prefetched rounds' finished nodes, are loaded here, once
+    // the element product has freed its registers, so that their latency
+    // overlaps the barrier.
+    int mext[kExt];
+    Trip<uint8_t> fxp[kPreR];
+    {
+        const int c = cnt[0];  // 0 past the patch's nodes
+#pragma unroll
+        for (int j = 0; j < kExt; ++j)
+            mext[j] = kPre + j < c ? (int)load_at<uint16_t>(linT, 2u * (uint32_t)(base0 + (kPre + j) * kPatch + t)) : 0;
+#pragma unroll
+        for (int r = 0; r < kPreR; ++r)
+            fxp[r] = (fixed && oo[r] >= 0 && cnt[r] > 0) ? load_at<Trip<uint8_t>>(fixed, 3u * (uint32_t)oo[r])
+                                                         : Trip<uint8_t>{0, 0, 0};
+    }
     __syncthreads();
     double dv = en;
     // node l of round r: the sum of its incidences, to y or to its slot
-    auto node = [&](int r, int l, int c, int o, const int (&mp)[kPre]) {
+    // (fx: its fixed flags, when loaded above)
+    auto node = [&](int r, int l, int c, int o, const int (&mp)[kPre], const Trip<uint8_t> *fx) {
         T s0 = T(0), s1 = T(0), s2 = T(0);
 #pragma unroll
         for (int j = 0; j < kPre; ++j)
@@ -157,9 +178,21 @@ __global__ __launch_bounds__(kPatch) void k_ku_patch(int n_el, int npatch, int n
                 s1 += ys[m + kPatch];
                 s2 += ys[m + 2 * kPatch];
             }
-        if (c > kPre) {
+        int j1 = kPre;
+        if (r == 0) {
+#pragma unroll
+            for (int j = 0; j < kExt; ++j)
+                if (kPre + j < c) {
+                    const int m = mext[j];
+                    s0 += ys[m];
+                    s1 += ys[m + kPatch];
+                    s2 += ys[m + 2 * kPatch];
+                }
+            j1 = kPre + kExt;
+        }
+        if (c > j1) {
             const int base = rbase[q0 + r];
-            for (int j = kPre; j < c; ++j) {
+            for (int j = j1; j < c; ++j) {
                 const int m = load_at<uint16_t>(linT, 2u * (uint32_t)(base + j * kPatch + t));
                 s0 += ys[m];
                 s1 += ys[m + kPatch];
@@ -169,10 +202,10 @@ __global__ __launch_bounds__(kPatch) void k_ku_patch(int n_el, int npatch, int n
         if (o >= 0) {
             const uint32_t b = 3u * (uint32_t)o;
             if (fixed) {
-                const Trip<uint8_t> fx = load_at<Trip<uint8_t>>(fixed, b);
-                if (fx.a) s0 = T(0);
-                if (fx.b) s1 = T(0);
-                if (fx.c) s2 = T(0);
+                const Trip<uint8_t> f = fx ? *fx : load_at<Trip<uint8_t>>(fixed, b);
+                if (f.a) s0 = T(0);
+                if (f.b) s1 = T(0);
+                if (f.c) s2 = T(0);
             }
             store_at<Trip<T>>(y, (uint32_t)sizeof(T) * b, Trip<T>{s0, s1, s2});
             if (dotv) {
@@ -186,14 +219,14 @@ __global__ __launch_bounds__(kPatch) void k_ku_patch(int n_el, int npatch, int n
 #pragma unroll
     for (int r = 0; r < kPreR; ++r) {
         const int l = lb + r * kPatch + t;
-        if (l < le) node(r, l, cnt[r], oo[r], mpre[r]);
+        if (l < le) node(r, l, cnt[r], oo[r], mpre[r], &fxp[r]);
     }
     for (int r = kPreR; lb + r * kPatch < le; ++r) {  // patches of more than kPreR rounds
         const int l = lb + r * kPatch + t;
         if (l < le) {
             int c, o, mp[kPre];
             fetch(r, l, c, o, mp);
-            node(r, l, c, o, mp);
+            node(r, l, c, o, mp, nullptr);
         }
     }
     if (ppart) {
