@@ -284,6 +284,8 @@ struct jfm_ctx {
     int npatch = 0, npn = 0, nbnode = 0, nslot = 0;
     int32_t *eperm = nullptr;                 // patch order -> element id
     int32_t *connP = nullptr;                 // npe x n_el, patch order
+    uint16_t *connP16 = nullptr;              // the same as offsets from pbase[patch] (null: ids span too far)
+    int32_t *pbase = nullptr;                 // lowest node id of every patch
     double *gfP64 = nullptr;                  // geometric factors, patch order
     float *gfP32 = nullptr;
     int32_t *pn_ptr = nullptr, *pn_out = nullptr;  // patch nodes: range per patch, target

@@ -55,8 +55,9 @@ constexpr int kPatch = JFEM_PATCH_ELEMS;
 // The element product is the matrix-free one (mf_element.h) from the
 // patch-ordered geometric factors el, or with EBE the stored-matrix one
 // (ebe_element.h) from element matrices el stored in the patch order.
-template <int NPE, typename T, bool AFF, bool EBE>
+template <int NPE, typename T, bool AFF, bool EBE, bool C16>
 __global__ __launch_bounds__(kPatch) void k_ku_patch(int n_el, int npatch, int npn, const int32_t *__restrict__ connP,
+                                                    const uint16_t *__restrict__ connP16, const int32_t *__restrict__ pbase,
                                                     const T *__restrict__ el, const T *__restrict__ x,
                                                     const int32_t *__restrict__ pn_ptr, const uint16_t *__restrict__ pn_cnt,
                                                     const int32_t *__restrict__ pn_out, const int32_t *__restrict__ rstart,
@@ -102,6 +103,21 @@ __global__ __launch_bounds__(kPatch) void k_ku_patch(int n_el, int npatch, int n
     // guarantees x = 0 on fixed DOFs, so x.(masked Kx) = x.Kx)
     constexpr bool kEnergy = !EBE && AFF && NPE == 10;
     double en = 0.0;
+    // local node i's id: 16-bit offsets from the patch's lowest id (C16), or
+    // the 32-bit ids
+    const int pb = C16 ? pbase[p] : 0;  // uniform
+    auto node_of = [&](int i) -> uint32_t {
+        if constexpr (C16)
+            return (uint32_t)pb + load_at<uint16_t>(connP16, 2u * (uint32_t)(i * n_el + k));
+        else
+            return (uint32_t)load_at<int32_t>(connP, 4u * (uint32_t)(i * n_el + k));
+    };
+    auto x_of = [&](int i, T (&v)[3]) {
+        const Trip<T> w = load_at<Trip<T>>(x, (uint32_t)(3 * sizeof(T)) * node_of(i));
+        v[0] = w.a;
+        v[1] = w.b;
+        v[2] = w.c;
+    };
     if (k < n_el) {
         auto to_lds = [&](int i, T f0, T f1, T f2) {
             ys[(3 * i) * kPatch + t] = f0;
@@ -120,13 +136,7 @@ __global__ __launch_bounds__(kPatch) void k_ku_patch(int n_el, int npatch, int n
                 // each node's three products in T, the nodes' sums in FP64
                 T xs[NPE][3];
 #pragma unroll
-                for (int i = 0; i < NPE; ++i) {
-                    const uint32_t a = (uint32_t)load_at<int32_t>(connP, 4u * (uint32_t)(i * n_el + k));
-                    const Trip<T> w = load_at<Trip<T>>(x, (uint32_t)(3 * sizeof(T)) * a);
-                    xs[i][0] = w.a;
-                    xs[i][1] = w.b;
-                    xs[i][2] = w.c;
-                }
+                for (int i = 0; i < NPE; ++i) x_of(i, xs[i]);
                 mf_element_x<NPE, T, AFF>(
                     n_el, k, el,
                     [&](int i, T (&v)[3]) {
@@ -141,10 +151,10 @@ __global__ __launch_bounds__(kPatch) void k_ku_patch(int n_el, int npatch, int n
                                            (xs[i][2] - xs[0][2]) * f2);
                     });
             } else {
-                mf_element<NPE, T, AFF>(n_el, k, connP, el, x, to_lds);
+                mf_element_x<NPE, T, AFF>(n_el, k, el, x_of, to_lds);
             }
         } else
-            mf_element<NPE, T, AFF>(n_el, k, connP, el, x, to_lds);
+            mf_element_x<NPE, T, AFF>(n_el, k, el, x_of, to_lds);
     }
     // Round 0 holds the patch's nodes of most incidences (up to 24 in a Tet10
     // box mesh, so one wave would otherwise load 20 of them one after the
@@ -342,7 +352,8 @@ void jfem_patch_release(jfm_ctx *ctx) {
     for (void **p : {(void **)&ctx->connP, (void **)&ctx->gfP64, (void **)&ctx->gfP32, (void **)&ctx->pn_ptr,
                      (void **)&ctx->pn_cnt, (void **)&ctx->pn_out, (void **)&ctx->rstart, (void **)&ctx->rbase,
                      (void **)&ctx->linT, (void **)&ctx->bptr, (void **)&ctx->bnode,
-                     (void **)&ctx->bpart, (void **)&ctx->ppart, (void **)&ctx->eperm, (void **)&ctx->qsrc}) {
+                     (void **)&ctx->bpart, (void **)&ctx->ppart, (void **)&ctx->eperm, (void **)&ctx->qsrc,
+                     (void **)&ctx->connP16, (void **)&ctx->pbase}) {
         jfm_free(ctx, *p);
         *p = nullptr;
     }
@@ -363,7 +374,8 @@ void jfem_patch_release(jfm_ctx *ctx) {
 // not fit its type.
 struct PatchHost {
     int npatch = 0;
-    std::vector<int32_t> perm, connP, pn_ptr, pn_out, rstart, rbase, bnode, bptr;
+    std::vector<int32_t> perm, connP, pn_ptr, pn_out, rstart, rbase, bnode, bptr, pbase;
+    std::vector<uint16_t> connP16;  // empty when a patch spans more than 65,536 node ids
     std::vector<uint16_t> pn_cnt, linT;
 };
 
@@ -478,6 +490,27 @@ static bool build_patches(const std::vector<int32_t> &conn, const std::vector<do
         for (size_t j = 0; j < shared_l[a].size(); ++j) P.pn_out[shared_l[a][j]] = -(P.bptr.back() + (int32_t)j + 1);
         P.bptr.push_back(P.bptr.back() + (int32_t)shared_l[a].size());
     }
+    // 16-bit connectivity relative to each patch's lowest node id, when every
+    // patch's ids span at most 65,536
+    P.pbase.assign(npatch, 0);
+    bool fits16 = true;
+    for (int p = 0; p < npatch && fits16; ++p) {
+        const int64_t k0 = (int64_t)p * kPatch, k1 = std::min<int64_t>(k0 + kPatch, n_el);
+        int32_t lo = INT32_MAX, hi = -1;
+        for (int i = 0; i < npe; ++i)
+            for (int64_t k = k0; k < k1; ++k) {
+                lo = std::min(lo, P.connP[(size_t)i * n_el + k]);
+                hi = std::max(hi, P.connP[(size_t)i * n_el + k]);
+            }
+        P.pbase[p] = lo;
+        fits16 = hi - lo < 65536;
+    }
+    if (fits16) {
+        P.connP16.assign((size_t)npe * n_el, 0);
+        for (int i = 0; i < npe; ++i)
+            for (int64_t k = 0; k < n_el; ++k)
+                P.connP16[(size_t)i * n_el + k] = (uint16_t)(P.connP[(size_t)i * n_el + k] - P.pbase[k / kPatch]);
+    }
     return 3 * (npe - 1) * kPatch + kPatch <= 65536 && (int64_t)P.pn_out.size() < INT32_MAX / 3;
 }
 
@@ -521,6 +554,10 @@ int jfem_patch_setup(jfm_ctx *ctx) {
     PALLOC(ctx->bnode, int32_t, nb);
     PALLOC(ctx->bpart, double, 3 * (int64_t)P.bptr.back());
     PALLOC(ctx->ppart, double, npatch);
+    if (!P.connP16.empty()) {
+        PALLOC(ctx->connP16, uint16_t, npe * n_el);
+        PALLOC(ctx->pbase, int32_t, npatch);
+    }
 #undef PALLOC
     PUP(ctx->connP, P.connP);
     PUP(ctx->eperm, P.perm);
@@ -532,6 +569,10 @@ int jfem_patch_setup(jfm_ctx *ctx) {
     PUP(ctx->linT, P.linT);
     PUP(ctx->bptr, P.bptr);
     PUP(ctx->bnode, P.bnode);
+    if (!P.connP16.empty()) {
+        PUP(ctx->connP16, P.connP16);
+        PUP(ctx->pbase, P.pbase);
+    }
 #undef PUP
     k_permute_gf<double><<<1024, 256, 0, s>>>(ngfr, (int)n_el, ctx->eperm, ctx->gf64, ctx->gfP64);
     k_permute_gf<float><<<1024, 256, 0, s>>>(ngfr, (int)n_el, ctx->eperm, ctx->gf32, ctx->gfP32);
@@ -627,16 +668,18 @@ static void patch_kernel(jfm_ctx *ctx, int op, const T *x, T *y, const T *dotv, 
     double *pp = dotv ? (pout ? pout : ctx->ppart) : nullptr;
     if (energy_dot<T>(ctx, op, x, dotv)) dotv = nullptr;
     auto launch = [&](auto kernel, const T *el) {
-        kernel<<<grid, kPatch, 0, ctx->stream>>>(n, ctx->npatch, ctx->npn, ctx->connP, el, x, ctx->pn_ptr, ctx->pn_cnt,
+        kernel<<<grid, kPatch, 0, ctx->stream>>>(n, ctx->npatch, ctx->npn, ctx->connP, ctx->connP16, ctx->pbase, el, x,
+                                                 ctx->pn_ptr, ctx->pn_cnt,
                                                  ctx->pn_out, ctx->rstart, ctx->rbase, ctx->linT, ctx->fixed, y,
                                                  (T *)ctx->bpart, dotv, pp, done);
     };
+    const bool c16 = ctx->connP16 != nullptr;
     if (op == JFM_OP_EBE)
-        launch(k_ku_patch<NPE, T, true, true>, keP_of<T>(ctx));
+        launch(k_ku_patch<NPE, T, true, true, false>, keP_of<T>(ctx));
     else if (ctx->affine)
-        launch(k_ku_patch<NPE, T, true, false>, gfP_of<T>(ctx));
+        launch(c16 ? k_ku_patch<NPE, T, true, false, true> : k_ku_patch<NPE, T, true, false, false>, gfP_of<T>(ctx));
     else
-        launch(k_ku_patch<NPE, T, false, false>, gfP_of<T>(ctx));
+        launch(c16 ? k_ku_patch<NPE, T, false, false, true> : k_ku_patch<NPE, T, false, false, false>, gfP_of<T>(ctx));
 }
 
 template <typename T>
