@@ -69,7 +69,8 @@ def kernel_bytes(inf, op, prec):
         spmv = 9 * nnzb * T + 4 * nnzb + 4 * (N + 1) + 2 * 3 * N * T + 3 * N
         return {"spmv": spmv}
     if op in ("matfree", "ebe") and inf.get("n_patches", 0) > 0:
-        # patch kernel: connectivity (int32, npe per element), the geometric
+        # patch kernel: connectivity (int32 or 16-bit patch-relative, npe per
+        # element; the EBE form reads int32), the geometric
         # factors (matrix-free) or the stored upper blocks of K_e (EBE), x
         # once, the patch-local incidences (uint16, npe per element), per
         # patch node its count (uint16) and target (int32); the fixed mask and
@@ -77,7 +78,8 @@ def kernel_bytes(inf, op, prec):
         npn, nb, ns = inf["patch_nodes"], inf["shared_nodes"], inf["shared_slots"]
         interior = npn - ns
         el = 11 * ngf * n_el * T if op == "matfree" else T * 9 * (npe * (npe + 1) // 2) * n_el
-        patch = (4 * npe * n_el + el + 3 * N * T + 2 * npe * n_el + 6 * npn
+        cb = inf.get("patch_conn_bytes", 4) if op == "matfree" else 4
+        patch = (cb * npe * n_el + el + 3 * N * T + 2 * npe * n_el + 6 * npn
                  + 3 * interior + 3 * interior * T + 3 * ns * T)
         # shared-node kernel: slot ranges and node ids, the partials, mask, y
         combine = 8 * nb + 4 + 3 * ns * T + 3 * nb + 3 * nb * T

@@ -110,8 +110,10 @@ typedef struct {
     int64_t device_bytes;  /* bytes currently allocated by the context */
     /* patch form of the matrix-free operator (0 until it is first used):
      * patches of 480 elements, their (patch, node) entries, the nodes shared
-     * between patches and their partial sums */
-    int64_t n_patches, patch_nodes, shared_nodes, shared_slots;
+     * between patches and their partial sums; the bytes per connectivity
+     * entry the patch kernel reads (2: offsets from each patch's lowest node
+     * id, when every patch's ids span at most 65,536; else 4) */
+    int64_t n_patches, patch_nodes, shared_nodes, shared_slots, patch_conn_bytes;
 } jfm_info;
 
 /* ---- context --------------------------------------------------------- */

@@ -152,6 +152,7 @@ extern "C" int jfm_get_info(jfm_ctx *ctx, jfm_info *info) {
     info->patch_nodes = ctx->patch_ready ? ctx->npn : 0;
     info->shared_nodes = ctx->patch_ready ? ctx->nbnode : 0;
     info->shared_slots = ctx->patch_ready ? ctx->nslot : 0;
+    info->patch_conn_bytes = ctx->patch_ready ? (ctx->connP16 ? 2 : 4) : 0;
     return JFM_OK;
 }
 
