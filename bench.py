@@ -288,7 +288,8 @@ def main():
     prec = 1 if method == 1 else 0
     sv = ideal_ku(inf, prec)
     if "patch_ms" in hk:
-        dom, dom_ms, impl_bytes = f"k_ku_patch<10,{tname},AFF>", hk["patch_ms"], hk["bytes"]["patch"]
+        c16 = ",C16" if inf.get("patch_conn_bytes", 4) == 2 else ""  # 16-bit patch-relative connectivity
+        dom, dom_ms, impl_bytes = f"k_ku_patch<10,{tname},AFF{c16}>", hk["patch_ms"], hk["bytes"]["patch"]
     elif hk["element_ms"] >= hk["gather_ms"]:
         dom, dom_ms, impl_bytes = f"k_mf_elem<10,{tname},AFF>", hk["element_ms"], hk["bytes"]["element"]
     else:
