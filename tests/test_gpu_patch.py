@@ -11,7 +11,9 @@ through it):
     whose element count is not a multiple of 480 and that hold nodes of no
     element, with and without the row mask (reactions);
   * the two-phase operator (JFEM_PATCH=0, a separate process): the same
-    product up to the order of the nodal sums, the same PCG answer.
+    product up to the order of the nodal sums, the same PCG answer;
+  * both connectivity widths of the patch kernel (16-bit patch-relative and
+    32-bit) against the oracle.
 """
 import os
 import subprocess
@@ -72,6 +74,31 @@ def test_structure_and_apply(jf, oracle, name):
     assert np.all(y64.reshape(-1, 3)[~used] == 0.0)
     # bitwise repeatable
     assert np.array_equal(ctx.apply(x, op="matfree"), y64)
+
+
+def test_connectivity_width(jf, oracle):
+    """The patch kernel reads 16-bit connectivity relative to each patch's
+    lowest node id when every patch's ids span at most 65,536, else the
+    32-bit ids.  A 120x8x8 Tet10 mesh (69,649 nodes) takes the 16-bit form
+    in its natural numbering and the 32-bit form with its node ids shuffled;
+    both products match the oracle's."""
+    from jfem.mesh import renumber
+    m = M.box_tet10(120, 8, 8, seed=5, jitter=0.1, E_sigma=0.4)
+    sh = renumber(m, np.random.default_rng(4).permutation(m.n_nodes))
+    for name, mm, width in (("natural", m, 2), ("shuffled", sh, 4)):
+        ctx = jf.Context(0)
+        ctx.upload(mm)
+        x = np.random.default_rng(12).standard_normal(mm.n_dofs)
+        y64 = ctx.apply(x, op="matfree")
+        y32 = ctx.apply(x, op="matfree", precision=1)
+        inf = ctx.info()
+        say(f"{name} numbering: {mm.n_nodes} nodes, patch connectivity {inf['patch_conn_bytes']} bytes")
+        assert inf["patch_conn_bytes"] == width
+        ref = oracle.matfree_apply(mm, x, mm.fixed_mask())
+        scale = np.abs(ref).max()
+        assert np.abs(y64 - ref).max() <= 1e-12 * scale
+        assert np.abs(y32 - ref).max() <= 2e-6 * scale
+        ctx.close()
 
 
 def test_reactions_use_unmasked_rows(jf, oracle):
