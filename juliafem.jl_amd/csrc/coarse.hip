@@ -22,20 +22,21 @@
 //    one colour are at distance >= 3, so the restriction of that product to
 //    aggregate g is Q_g' K Q_{p,j} for the single colour-c aggregate p coupled
 //    to g.  No atomics; the result is deterministic.
-//  * a block Gauss-Jordan sweep inverts A_c in place (dense, a few thousand
-//    rows; two launches per 32-row pivot block); the inverse is folded with
+//  * a block Gauss-Jordan sweep inverts A_c (dense, a few thousand rows; one
+//    launch per 32-row pivot block, between two copies of the matrix); the
+//    inverse is folded with
 //    the R_g^-1 into Ahat = R^-1 A_c^-1 R^-T (stored FP32), so the iteration
 //    needs only the raw modes: Q A_c^-1 Q' r = Bm Ahat Bm' r.
 //  * the whole device part is one launch sequence per aggregation and
 //    operator, captured once as a HIP graph and replayed.
 //
 // Per PCG iteration:
-//  k_pcg_update_agg   x, r, z = B r and the partials of r.Br and r.r, one
+//  k_pcg_update_agg   r, z = B r and the partials of r.Br and r.r, one
 //                     block per aggregate over its nodes; the block also
 //                     restricts the new residual, w_g = sum_{a in g}
 //                     [m_a r_a ; d_a x m_a r_a], from the values it holds
 //  k_coarse_gemv      zc = Ahat w, partials of w.zc             (wave per row)
-//  k_pcg_dir_coarse   p_a = z_a + m_a o (t_g + omega_g x d_a) + beta p_a
+//  k_pcg_dir_coarse   x += alpha p, then p_a = z_a + m_a o (t_g + omega_g x d_a) + beta p_a
 // (JFEM_FUSED_RESTRICTION=0: the node-ordered update writes m o r in
 // aggregate order and k_coarse_restrict sums it after the update instead.)
 // The prolongation is folded into the direction update, and r.z of the full
@@ -316,7 +317,6 @@ __global__ __launch_bounds__(1024) void k_gal_sum(const int32_t *__restrict__ up
 #endif
 constexpr int kGJ = JFEM_GJ_WIDTH;  // pivot block (32 or 64; 64 measured slower: setup 2.52 ms against 1.98 ms)
 static_assert(kGJ == 32 || kGJ == 64, "pivot blocks of 32 or 64");
-constexpr int kGJRows = 256 / kGJ;  // panel rows per 256-thread block
 constexpr int kGJSub = kGJ / 16;    // look-ahead entries per thread and direction
 
 // P^-1 of a bk x bk pivot block (padded to the identity), entry (i, j) read
@@ -371,121 +371,134 @@ __device__ __forceinline__ bool gj_pivot_inverse(int bk, Ld &&ld, double (*P)[kG
     return ok;
 }
 
-// Cbuf (n x kGJ) = A_:K P^-1, Rbuf (kGJ x n) = A_K:, Pbuf = P^-1; 8 rows of
-// Cbuf per block of 256 threads.
-// With Ppre, P^-1 was formed by the previous step's update kernel.  With
-// nbk > 0, block 0 also copies the next pivot block (rows and columns nk0 ..
-// nk0 + nbk) to Anext for the update kernel, which overwrites A.
-__global__ __launch_bounds__(256) void k_gj_panel(int n, int k0, int bk, const double *__restrict__ A,
-                                                 double *__restrict__ Cbuf, double *__restrict__ Rbuf,
-                                                 double *__restrict__ Pbuf, const double *__restrict__ Ppre,
-                                                 int nk0, int nbk, double *__restrict__ Anext,
-                                                 int *__restrict__ flag) {
-    if (nbk > 0 && blockIdx.x == 0)
-        for (int t = threadIdx.x; t < nbk * kGJ; t += blockDim.x) {
-            const int i = t / kGJ, j = t - kGJ * (t / kGJ);
-            if (j < nbk) Anext[t] = A[(size_t)(nk0 + i) * n + nk0 + j];
-        }
+// The first step's pivot inverse (rows and columns 0 .. bk), one block.
+__global__ __launch_bounds__(256) void k_gj_first(int n, int bk, const double *__restrict__ A,
+                                                 double *__restrict__ Pout, int *__restrict__ flag) {
     __shared__ double P[kGJ][kGJ + 1];
-    bool ok = true;
-    if (Ppre) {
-        for (int t = threadIdx.x; t < kGJ * kGJ; t += blockDim.x) P[t / kGJ][t - kGJ * (t / kGJ)] = Ppre[t];
-        __syncthreads();
-    } else {
-        ok = gj_pivot_inverse(bk, [&](int i, int j) { return A[(size_t)(k0 + i) * n + k0 + j]; }, P);
-    }
-    if (blockIdx.x == 0) {
-        for (int t = threadIdx.x; t < kGJ * kGJ; t += blockDim.x) Pbuf[t] = P[t / kGJ][t - kGJ * (t / kGJ)];
-        if (threadIdx.x == 0 && !ok) flag[0] = 1;
-    }
-    // Only the tiles on and below the diagonal (64 x 64) are kept, diagonal
-    // tiles whole: M_iK of a row above the pivot's tile is -M_Ki' (i pivoted
-    // earlier, K not), and row K of a column right of it is M_jK' (neither
-    // pivoted).
-    const int TK = k0 / 64;
-    const int row = blockIdx.x * kGJRows + threadIdx.x / kGJ, m = threadIdx.x & (kGJ - 1);
-    if (row < n) {
-        double c = 0.0;
-        if (row / 64 >= TK) {
-            const double *Ar = A + (size_t)row * n + k0;
-            for (int l = 0; l < bk; ++l) c += Ar[l] * P[l][m];
-            if (row / 64 > TK && m < bk) Rbuf[(size_t)m * n + row] = Ar[m];
-        } else {
-            for (int l = 0; l < bk; ++l) c -= A[(size_t)(k0 + l) * n + row] * P[l][m];
-        }
-        Cbuf[(size_t)row * kGJ + m] = c;
-    }
-    const int jmax = min(n, 64 * (TK + 1));  // row K itself up to the end of the pivot's tile
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < (int64_t)bk * jmax;
-         t += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t mm = t / jmax, j = t - mm * jmax;
-        Rbuf[mm * n + j] = A[(size_t)(k0 + mm) * n + j];
-    }
-}
-
-// The next step's pivot block after this step's update (rows and columns
-// nk0 .. nk0 + nbk), with the arithmetic of k_gj_update's tiles (so the same
-// values), and its inverse for the next panel (Pnext).  Run by block 0 of
-// the update, beside the tiles.
-__device__ __forceinline__ void gj_next_pivot(int n, int bk, const double *__restrict__ Anext,
-                                              const double *__restrict__ Cbuf, const double *__restrict__ Rbuf, int nk0,
-                                              int nbk, double *__restrict__ Pnext, int *__restrict__ flag,
-                                              double (*Cs)[kGJ + 1], double (*Rs)[65], double (*Ps)[kGJ + 1]) {
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    // the block before this step's update, as the panel kernel copied it (the
-    // tiles of this kernel overwrite A meanwhile)
-    double av[kGJSub][kGJSub];
-#pragma unroll
-    for (int a = 0; a < kGJSub; ++a)
-#pragma unroll
-        for (int b = 0; b < kGJSub; ++b) {
-            const int i = ty + 16 * a, j = tx + 16 * b;
-            av[a][b] = (i < nbk && j < nbk) ? Anext[i * kGJ + j] : 0.0;
-        }
-    for (int t = threadIdx.x; t < kGJ * kGJ; t += blockDim.x) {
-        const int a = t / kGJ, m = t - kGJ * (t / kGJ);
-        Cs[a][m] = (a < nbk && m < bk) ? Cbuf[(size_t)(nk0 + a) * kGJ + m] : 0.0;
-        Rs[a][m] = (a < bk && m < nbk) ? Rbuf[(size_t)a * n + nk0 + m] : 0.0;
-    }
-    __syncthreads();
-    double acc[kGJSub][kGJSub];
-#pragma unroll
-    for (int a = 0; a < kGJSub; ++a)
-#pragma unroll
-        for (int b = 0; b < kGJSub; ++b) acc[a][b] = 0.0;
-    for (int m = 0; m < bk; ++m) {
-        double cv[kGJSub], rv[kGJSub];
-#pragma unroll
-        for (int a = 0; a < kGJSub; ++a) cv[a] = Cs[ty + 16 * a][m];
-#pragma unroll
-        for (int b = 0; b < kGJSub; ++b) rv[b] = Rs[m][tx + 16 * b];
-#pragma unroll
-        for (int a = 0; a < kGJSub; ++a)
-#pragma unroll
-            for (int b = 0; b < kGJSub; ++b) acc[a][b] += cv[a] * rv[b];
-    }
-#pragma unroll
-    for (int a = 0; a < kGJSub; ++a)
-#pragma unroll
-        for (int b = 0; b < kGJSub; ++b) Ps[ty + 16 * a][tx + 16 * b] = av[a][b] - acc[a][b];
-    __syncthreads();
-    const bool ok = gj_pivot_inverse(nbk, [&](int i, int j) { return Ps[i][j]; }, Cs);
-    for (int t = threadIdx.x; t < kGJ * kGJ; t += blockDim.x) Pnext[t] = Cs[t / kGJ][t - kGJ * (t / kGJ)];
+    const bool ok = gj_pivot_inverse(bk, [&](int i, int j) { return A[(size_t)i * n + j]; }, P);
+    for (int t = threadIdx.x; t < kGJ * kGJ; t += blockDim.x) Pout[t] = P[t / kGJ][t - kGJ * (t / kGJ)];
     if (threadIdx.x == 0 && !ok) flag[0] = 1;
 }
 
-// The update of step K over the 64 x 64 tiles on and below the diagonal,
-// 4 x 4 entries per thread.  With nbk > 0, block 0 forms the next pivot
-// block's inverse instead (gj_next_pivot), beside the tiles and off the
-// critical path of the step.
-__global__ __launch_bounds__(256) void k_gj_update(int n, int k0, int bk, double *__restrict__ A,
-                                                  const double *__restrict__ Cbuf, const double *__restrict__ Rbuf,
-                                                  const double *__restrict__ Pbuf, int nk0, int nbk,
-                                                  const double *__restrict__ Anext, double *__restrict__ Pnext,
-                                                  int *__restrict__ flag) {
-    __shared__ double Cs[64][kGJ + 1], Rs[kGJ][65], Ps[kGJ][kGJ + 1];
+// One block Gauss-Jordan step K (pivot rows and columns k0 .. k0 + bk) from
+// Ao into An, one 64 x 64 tile on or below the diagonal per block (diagonal
+// tiles whole), 4 x 4 entries per thread:
+//   A_ij -= C_i R_j  (i, j not in K),   A_iK = -C_i,   A_Kj = P^-1 R_j,
+//   A_KK = P^-1,
+// with P^-1 the pivot block's inverse (Pbuf, formed by the previous step),
+// C = A_:K P^-1 (every tile forms its own 64 rows of it) and R = row K.  An
+// is a second copy of the matrix, so no tile overwrites what another still
+// reads, and the step needs no panel kernel before it.  The sweep keeps
+// M_ij = +-M_ji' (minus when exactly one of i, j is pivoted), so only the
+// tiles on and below the diagonal are stored and updated: A_:K of a row
+// above the pivot's tile is read as -A_K:', and row K right of the pivot's
+// tile as A_:K'.  With nbk > 0, block 0 instead forms the next pivot block
+// after this update, with the same arithmetic as the tile that holds it (so
+// the same values), and its inverse (Pnext), beside the tiles.
+__global__ __launch_bounds__(256) void k_gj_step(int n, int k0, int bk, const double *__restrict__ Ao,
+                                                double *__restrict__ An, const double *__restrict__ Pbuf, int nk0,
+                                                int nbk, double *__restrict__ Pnext, int *__restrict__ flag) {
+    __shared__ double Cs[64][kGJ + 1], Rs[kGJ][65], Ps[kGJ][kGJ + 1], As[64][kGJ + 1];
+    const int TK = k0 / 64;
+    const int jmax = min(n, 64 * (TK + 1));  // row K is stored as such up to the end of the pivot's tile
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    // the panel's 64 rows into Cs, from As staged by the caller:
+    // Cs[a][m] = sum_l As[a][l] P[l][m] (As holds -A_K:' above the pivot's
+    // tile, so the sum there is c -= A P with A as stored), rows ty + 16 u
+    // and columns tx + 16 v per thread
+    auto panel = [&]() {
+        constexpr int V = kGJ / 16;
+        double c[4][V];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < V; ++v) c[u][v] = 0.0;
+        for (int l = 0; l < bk; ++l) {
+            double av[4], pv[V];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) av[u] = As[ty + 16 * u][l];
+#pragma unroll
+            for (int v = 0; v < V; ++v) pv[v] = Ps[l][tx + 16 * v];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < V; ++v) c[u][v] += av[u] * pv[v];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < V; ++v) Cs[ty + 16 * u][tx + 16 * v] = c[u][v];
+    };
+    // As rows a of rows r0 + a (nr of them): A_:K as stored at or below the
+    // pivot's tile (consecutive threads along a row of A), or -A_K:' above
+    // it (consecutive threads along a row of A_K:)
+    auto stage_col = [&](int r0, int nr) {
+        if (r0 / 64 >= TK) {
+#pragma unroll
+            for (int u = 0; u < 64 * kGJ / 256; ++u) {
+                const int t = (int)threadIdx.x + 256 * u;
+                const int a = t / kGJ, m = t - kGJ * (t / kGJ);
+                As[a][m] = (a < nr && m < bk) ? Ao[(size_t)(r0 + a) * n + k0 + m] : 0.0;
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < 64 * kGJ / 256; ++u) {
+                const int t = (int)threadIdx.x + 256 * u;
+                const int m = t / 64, a = t - 64 * (t / 64);
+                As[a][m] = (a < nr && m < bk) ? -Ao[(size_t)(k0 + m) * n + r0 + a] : 0.0;
+            }
+        }
+    };
+    // row K entry (k0 + m, j): as stored up to the end of the pivot's tile,
+    // else as the transposed entry of column K
+    auto rowK = [&](int m, int j) -> double {
+        return j < jmax ? Ao[(size_t)(k0 + m) * n + j] : Ao[(size_t)j * n + k0 + m];
+    };
     if (nbk > 0 && blockIdx.x == 0) {
-        gj_next_pivot(n, bk, Anext, Cbuf, Rbuf, nk0, nbk, Pnext, flag, Cs, Rs, Ps);
+        // the next pivot block: rows and columns nk0 .. nk0 + nbk, below the
+        // pivot rows, so A_:K is read as stored
+        double av[kGJSub][kGJSub];
+#pragma unroll
+        for (int a = 0; a < kGJSub; ++a)
+#pragma unroll
+            for (int b = 0; b < kGJSub; ++b) {
+                const int i = ty + 16 * a, j = tx + 16 * b;
+                av[a][b] = (i < nbk && j < nbk) ? Ao[(size_t)(nk0 + i) * n + nk0 + j] : 0.0;
+            }
+        stage_col(nk0, nbk);
+        for (int t = threadIdx.x; t < kGJ * kGJ; t += blockDim.x) {
+            const int a = t / kGJ, m = t - kGJ * (t / kGJ);
+            Rs[a][m] = (a < bk && m < nbk) ? rowK(a, nk0 + m) : 0.0;
+            Ps[a][m] = Pbuf[t];
+        }
+        __syncthreads();
+        panel();
+        __syncthreads();
+        double acc[kGJSub][kGJSub];
+#pragma unroll
+        for (int a = 0; a < kGJSub; ++a)
+#pragma unroll
+            for (int b = 0; b < kGJSub; ++b) acc[a][b] = 0.0;
+        for (int m = 0; m < bk; ++m) {
+            double cv[kGJSub], rv[kGJSub];
+#pragma unroll
+            for (int a = 0; a < kGJSub; ++a) cv[a] = Cs[ty + 16 * a][m];
+#pragma unroll
+            for (int b = 0; b < kGJSub; ++b) rv[b] = Rs[m][tx + 16 * b];
+#pragma unroll
+            for (int a = 0; a < kGJSub; ++a)
+#pragma unroll
+                for (int b = 0; b < kGJSub; ++b) acc[a][b] += cv[a] * rv[b];
+        }
+        __syncthreads();  // Ps is overwritten below
+#pragma unroll
+        for (int a = 0; a < kGJSub; ++a)
+#pragma unroll
+            for (int b = 0; b < kGJSub; ++b) Ps[ty + 16 * a][tx + 16 * b] = av[a][b] - acc[a][b];
+        __syncthreads();
+        const bool ok = gj_pivot_inverse(nbk, [&](int i, int j) { return Ps[i][j]; }, As);
+        for (int t = threadIdx.x; t < kGJ * kGJ; t += blockDim.x) Pnext[t] = As[t / kGJ][t - kGJ * (t / kGJ)];
+        if (threadIdx.x == 0 && !ok) flag[0] = 1;
         return;
     }
     // tiles on and below the diagonal: bid = ti (ti + 1) / 2 + tj, tj <= ti
@@ -495,31 +508,28 @@ __global__ __launch_bounds__(256) void k_gj_update(int n, int k0, int bk, double
     while (ti * (ti + 1) / 2 > bid) --ti;
     const int tj = bid - ti * (ti + 1) / 2;
     const int i0 = 64 * ti, j0 = 64 * tj;
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    // this thread's entries of A, in flight while the panels are staged
     double av[4][4];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
             const int i = i0 + ty + 16 * a, j = j0 + tx + 16 * b;
-            av[a][b] = (i < n && j < n) ? A[(size_t)i * n + j] : 0.0;
+            av[a][b] = (i < n && j < n) ? Ao[(size_t)i * n + j] : 0.0;
         }
-    // the panels: a fixed count per thread (256 threads), so that every load
-    // is issued before the first LDS store waits on one
+    stage_col(i0, min(64, n - i0));
 #pragma unroll
     for (int u = 0; u < 64 * kGJ / 256; ++u) {
         const int t = (int)threadIdx.x + 256 * u;
-        const int a = t / kGJ, m = t - kGJ * (t / kGJ);
-        Cs[a][m] = (i0 + a < n && m < bk) ? Cbuf[(size_t)(i0 + a) * kGJ + m] : 0.0;
         const int mm = t / 64, b = t - 64 * (t / 64);
-        Rs[mm][b] = (mm < bk && j0 + b < n) ? Rbuf[(size_t)mm * n + j0 + b] : 0.0;
+        Rs[mm][b] = (mm < bk && j0 + b < n) ? rowK(mm, j0 + b) : 0.0;
     }
 #pragma unroll
     for (int u = 0; u < kGJ * kGJ / 256; ++u) {
         const int t = (int)threadIdx.x + 256 * u;
         Ps[t / kGJ][t - kGJ * (t / kGJ)] = Pbuf[t];
     }
+    __syncthreads();
+    panel();
     __syncthreads();
     double acc[4][4];
 #pragma unroll
@@ -547,7 +557,7 @@ __global__ __launch_bounds__(256) void k_gj_update(int n, int k0, int bk, double
             const int j = j0 + tx + 16 * b;
             if (j >= n) continue;
             const bool inJ = j >= k0 && j < k0 + bk;
-            double *o = A + (size_t)i * n + j;
+            double *o = An + (size_t)i * n + j;
             if (!inI && !inJ) {
                 *o = av[a][b] - acc[a][b];
             } else if (!inI) {
@@ -1083,10 +1093,10 @@ template <typename T> static void free_dev(jfm_ctx *ctx, T *&p) {
 void jfem_coarse_release(jfm_ctx *ctx) {
     free_dev(ctx, ctx->node_agg); free_dev(ctx, ctx->agg_ptr); free_dev(ctx, ctx->agg_nodes);
     free_dev(ctx, ctx->agg_color); free_dev(ctx, ctx->partner); free_dev(ctx, ctx->cd64); free_dev(ctx, ctx->cd32);
-    free_dev(ctx, ctx->Rinv); free_dev(ctx, ctx->Ac); free_dev(ctx, ctx->Ahat32);
+    free_dev(ctx, ctx->Rinv); free_dev(ctx, ctx->Ac); free_dev(ctx, ctx->Ac2); free_dev(ctx, ctx->Ahat32);
     free_dev(ctx, ctx->wc32); free_dev(ctx, ctx->zc32); free_dev(ctx, ctx->wc64);
     free_dev(ctx, ctx->cdA64); free_dev(ctx, ctx->cdA32); free_dev(ctx, ctx->apos); free_dev(ctx, ctx->rA);
-    free_dev(ctx, ctx->gjC); free_dev(ctx, ctx->gjR); free_dev(ctx, ctx->gjP);
+    free_dev(ctx, ctx->gjP);
     free_dev(ctx, ctx->pv6); free_dev(ctx, ctx->qv6); free_dev(ctx, ctx->ye6);
     free_dev(ctx, ctx->gal_el); free_dev(ctx, ctx->gal_gptr); free_dev(ctx, ctx->gal_glist);
     free_dev(ctx, ctx->gal_gq); free_dev(ctx, ctx->gal_gpos); free_dev(ctx, ctx->gal_item);
@@ -1213,17 +1223,21 @@ static hipError_t setup_inverse_sequence(jfm_ctx *ctx) {
     const int tiles = (nc + 63) / 64;
     // gjP: this step's P^-1, the next step's as the update forms it, and the
     // next pivot block before the update
-    double *Pnext = ctx->gjP + kGJ * kGJ, *Anext = ctx->gjP + 2 * kGJ * kGJ;
-    for (int k0 = 0; k0 < nc; k0 += kGJ) {
+    const double *inv = ctx->Ac;  // where the inverse ends
+    // one kernel per step, from one copy of the matrix into the other; P^-1
+    // alternates between two slots of gjP
+    double *Ab[2] = {ctx->Ac, ctx->Ac2}, *Pb[2] = {ctx->gjP, ctx->gjP + kGJ * kGJ};
+    k_gj_first<<<1, 256, 0, s>>>(nc, std::min(kGJ, nc), ctx->Ac, Pb[0], flag);
+    int st = 0;
+    for (int k0 = 0; k0 < nc; k0 += kGJ, ++st) {
         const int bk = std::min(kGJ, nc - k0);
         const int nk0 = k0 + kGJ, nbk = nk0 < nc ? std::min(kGJ, nc - nk0) : 0;
-        k_gj_panel<<<(nc + kGJRows - 1) / kGJRows, 256, 0, s>>>(nc, k0, bk, ctx->Ac, ctx->gjC, ctx->gjR, ctx->gjP,
-                                                k0 > 0 ? Pnext : nullptr, nk0, nbk, Anext, flag);
-        k_gj_update<<<tiles * (tiles + 1) / 2 + (nbk > 0 ? 1 : 0), 256, 0, s>>>(
-            nc, k0, bk, ctx->Ac, ctx->gjC, ctx->gjR, ctx->gjP, nk0, nbk, Anext, Pnext, flag);
+        k_gj_step<<<tiles * (tiles + 1) / 2 + (nbk > 0 ? 1 : 0), 256, 0, s>>>(nc, k0, bk, Ab[st & 1], Ab[(st + 1) & 1],
+                                                                           Pb[st & 1], nk0, nbk, Pb[(st + 1) & 1], flag);
     }
+    inv = Ab[st & 1];
     k_coarse_fold_blocks<<<nblk((int64_t)ctx->n_agg * ctx->n_agg), kBlock, 0, s>>>(
-        (int)ctx->n_agg, nc, (int)ctx->ldc, ctx->Ac, ctx->Rinv, ctx->dropped, ctx->Ahat32);
+        (int)ctx->n_agg, nc, (int)ctx->ldc, inv, ctx->Rinv, ctx->dropped, ctx->Ahat32);
     return hipGetLastError();
 }
 
@@ -1269,9 +1283,8 @@ static int coarse_prepare(jfm_ctx *ctx, int op, int coarse_nodes) {
         CALLOC(ctx->csub, int32_t, std::max<int64_t>(1, (int64_t)A.csub.size()));
         CALLOC(ctx->sega, int32_t, std::max<int64_t>(1, (int64_t)A.sega.size()));
         CALLOC(ctx->Ac, double, nc * nc);
-        CALLOC(ctx->gjC, double, nc * kGJ);
-        CALLOC(ctx->gjR, double, nc * kGJ);
-        CALLOC(ctx->gjP, double, 3 * kGJ * kGJ);
+        CALLOC(ctx->Ac2, double, nc * nc);
+        CALLOC(ctx->gjP, double, 2 * kGJ * kGJ);
         if ((int64_t)A.gal_el.size() >= (int64_t(1) << 27))
             return jfm_fail(ctx, JFM_EINVAL, "too many elements for the coarse level");
         ctx->gal_nitems = (int)A.gal_item.size();

@@ -262,7 +262,8 @@ struct jfm_ctx {
     double *cd64 = nullptr;        // N x 3 offsets from the aggregate centroid
     float *cd32 = nullptr;
     double *Rinv = nullptr;        // n_agg x 36
-    double *Ac = nullptr;          // nc x nc (inverted in place)
+    double *Ac = nullptr;          // nc x nc (inverted through Ac2, a second copy)
+    double *Ac2 = nullptr;
     int32_t *apos = nullptr;       // node -> its position in agg_nodes
     double *rA = nullptr;          // 3 x N: the masked residual in aggregate order (FP64 or FP32 view)
     double *cdA64 = nullptr;       // cd in aggregate order (agg_nodes), for the restriction
@@ -271,7 +272,7 @@ struct jfm_ctx {
     float *Ahat32 = nullptr;       // nc x ldc: R^-1 A_c^-1 R^-T
     float *wc32 = nullptr, *zc32 = nullptr;  // wc32 has ldc entries, zero past nc
     double *wc64 = nullptr;        // 2 x nc: Q' m r by iteration parity (slot 0 at the PCG start)
-    double *gjC = nullptr, *gjR = nullptr, *gjP = nullptr;  // Gauss-Jordan panels
+    double *gjP = nullptr;         // Gauss-Jordan pivot inverses (two, by step parity)
     // A_c element by element (linear operator): listed elements, their
     // aggregates and entry positions, items, pair ranges, unordered pairs,
     // and the entries (36 doubles each)
