@@ -77,7 +77,8 @@ def kernel_bytes(inf, op, prec):
         # y of the interior nodes, the partial sums of the shared entries
         npn, nb, ns = inf["patch_nodes"], inf["shared_nodes"], inf["shared_slots"]
         interior = npn - ns
-        el = 11 * ngf * n_el * T if op == "matfree" else T * 9 * (npe * (npe + 1) // 2) * n_el
+        rows = inf.get("patch_fp32_factor_rows", 11) if T == 4 else 11  # folded FP32 factors: 9
+        el = rows * ngf * n_el * T if op == "matfree" else T * 9 * (npe * (npe + 1) // 2) * n_el
         cb = inf.get("patch_conn_bytes", 4) if op == "matfree" else 4
         patch = (cb * npe * n_el + el + 3 * N * T + 2 * npe * n_el + 6 * npn
                  + 3 * interior + 3 * interior * T + 3 * ns * T)

@@ -112,8 +112,10 @@ typedef struct {
      * patches of 480 elements, their (patch, node) entries, the nodes shared
      * between patches and their partial sums; the bytes per connectivity
      * entry the patch kernel reads (2: offsets from each patch's lowest node
-     * id, when every patch's ids span at most 65,536; else 4) */
-    int64_t n_patches, patch_nodes, shared_nodes, shared_slots, patch_conn_bytes;
+     * id, when every patch's ids span at most 65,536; else 4); the factor
+     * rows per element its FP32 form reads (9: sqrt(mu w detJ / 5) invJ,
+     * affine Tet10 with one Poisson's ratio for all elements; else 11) */
+    int64_t n_patches, patch_nodes, shared_nodes, shared_slots, patch_conn_bytes, patch_fp32_factor_rows;
 } jfm_info;
 
 /* ---- context --------------------------------------------------------- */

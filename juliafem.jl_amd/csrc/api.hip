@@ -153,6 +153,7 @@ extern "C" int jfm_get_info(jfm_ctx *ctx, jfm_info *info) {
     info->shared_nodes = ctx->patch_ready ? ctx->nbnode : 0;
     info->shared_slots = ctx->patch_ready ? ctx->nslot : 0;
     info->patch_conn_bytes = ctx->patch_ready ? (ctx->connP16 ? 2 : 4) : 0;
+    info->patch_fp32_factor_rows = ctx->patch_ready ? (ctx->gfQ32 ? 9 : GF) : 0;
     return JFM_OK;
 }
 
@@ -270,6 +271,9 @@ extern "C" int jfm_upload_mesh(jfm_ctx *ctx, int64_t n_nodes, const double *X, i
     for (int64_t e = 0; e < n_el; ++e)
         if (!(E[e] > 0.0) || !(nu[e] > -1.0 && nu[e] < 0.5))
             return jfm_fail(ctx, JFM_EINVAL, "material out of range (E > 0, -1 < nu < 0.5 required)");
+    ctx->nu_uniform = n_el > 0;
+    for (int64_t e = 1; e < n_el && ctx->nu_uniform; ++e) ctx->nu_uniform = nu[e] == nu[0];
+    ctx->nu0 = n_el > 0 ? nu[0] : 0.0;
 
     const std::vector<int32_t> &hc = ctx->h_conn;
     // node -> incidences (e*npe + i), ascending e  (ext/JuliaFEMCUDAExt.jl:179-193)

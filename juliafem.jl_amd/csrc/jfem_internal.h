@@ -157,6 +157,8 @@ struct jfm_ctx {
     int64_t N = 0, n_el = 0, ndof = 0;
     int npe = 0, ngp = 0, npair = 0;
     bool affine = false, has_mesh = false;
+    bool nu_uniform = false;      // every element has Poisson's ratio nu0
+    double nu0 = 0.0;
     std::vector<int32_t> h_conn;  // n_el x npe (element-major), 0-based
     double *X = nullptr;
     int32_t *conn = nullptr;      // SoA
@@ -289,6 +291,8 @@ struct jfm_ctx {
     int32_t *pbase = nullptr;                 // lowest node id of every patch
     double *gfP64 = nullptr;                  // geometric factors, patch order
     float *gfP32 = nullptr;
+    float *gfQ32 = nullptr;                   // affine Tet10, uniform nu: sqrt(cm) invJ (9 rows), patch order
+    float rho32 = 0.0f;                       // lambda / mu for gfQ32
     int32_t *pn_ptr = nullptr, *pn_out = nullptr;  // patch nodes: range per patch, target
     uint16_t *pn_cnt = nullptr;               // patch node: incidences in the patch
     int32_t *rstart = nullptr, *rbase = nullptr;  // rounds of 256 patch nodes

@@ -28,9 +28,13 @@ namespace jfem {
 //     y_edge (a,b) = 4 (r_b(a) + r_a(b) + S grad L_a + S grad L_b) / 5
 // (the 1/5 is 24 / 120: the rule's weight w = 1/24 sits in cl, cm).  About 40%
 // fewer operations than the Gauss-point form below, same result up to rounding.
-template <typename T, typename XL, typename Out>
+// FOLD: gf holds sqrt(cm) invJ (9 rows) and rho = cl / cm = lambda / mu is
+// the same for every element; then G and sigma carry a factor sqrt(cm) and
+// 1 / sqrt(cm), and P = invJ' sigma is unchanged (two fewer loads and
+// multiplications per element).
+template <typename T, bool FOLD = false, typename XL, typename Out>
 __host__ __device__ __forceinline__ void mf_element_tet10_affine(int n_el, int e, const T *__restrict__ gf, XL &&xl,
-                                                                 Out &&out) {
+                                                                 Out &&out, T rho = T(0)) {
     constexpr int EDG[4][4] = {{-1, 4, 6, 7}, {4, -1, 5, 8}, {6, 5, -1, 9}, {7, 8, 9, -1}};
     T u[10][3];
     T v0[3];
@@ -51,7 +55,7 @@ __host__ __device__ __forceinline__ void mf_element_tet10_affine(int n_el, int e
     };
 #pragma unroll
     for (int k = 0; k < 9; ++k) Ji[k] = gld(k);
-    const T cl = T(0.2) * gld(9), cm = T(0.2) * gld(10);
+    const T cl = FOLD ? rho : T(0.2) * gld(9), cm = FOLD ? T(1) : T(0.2) * gld(10);
     T P[4][9];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -113,10 +117,11 @@ __host__ __device__ __forceinline__ void mf_element_tet10_affine(int n_el, int e
 template <int NPE, typename T, bool AFF, typename XL, typename Out>
 __host__ __device__ __forceinline__ void mf_element_gp(int n_el, int e, const T *__restrict__ gf, XL &&xl, Out &&out);
 
-template <int NPE, typename T, bool AFF, typename XL, typename Out>
-__host__ __device__ __forceinline__ void mf_element_x(int n_el, int e, const T *__restrict__ gf, XL &&xl, Out &&out) {
+template <int NPE, typename T, bool AFF, bool FOLD = false, typename XL, typename Out>
+__host__ __device__ __forceinline__ void mf_element_x(int n_el, int e, const T *__restrict__ gf, XL &&xl, Out &&out,
+                                                      T rho = T(0)) {
     if constexpr (NPE == 10 && AFF)
-        mf_element_tet10_affine<T>(n_el, e, gf, xl, out);
+        mf_element_tet10_affine<T, FOLD>(n_el, e, gf, xl, out, rho);
     else
         mf_element_gp<NPE, T, AFF>(n_el, e, gf, xl, out);
 }

@@ -55,7 +55,7 @@ constexpr int kPatch = JFEM_PATCH_ELEMS;
 // The element product is the matrix-free one (mf_element.h) from the
 // patch-ordered geometric factors el, or with EBE the stored-matrix one
 // (ebe_element.h) from element matrices el stored in the patch order.
-template <int NPE, typename T, bool AFF, bool EBE, bool C16>
+template <int NPE, typename T, bool AFF, bool EBE, bool C16, bool FOLD>
 __global__ __launch_bounds__(kPatch) void k_ku_patch(int n_el, int npatch, int npn, const int32_t *__restrict__ connP,
                                                     const uint16_t *__restrict__ connP16, const int32_t *__restrict__ pbase,
                                                     const T *__restrict__ el, const T *__restrict__ x,
@@ -64,7 +64,8 @@ __global__ __launch_bounds__(kPatch) void k_ku_patch(int n_el, int npatch, int n
                                                     const int32_t *__restrict__ rbase, const uint16_t *__restrict__ linT,
                                                     const uint8_t *__restrict__ fixed, T *__restrict__ y,
                                                     T *__restrict__ bpart, const T *__restrict__ dotv,
-                                                    double *__restrict__ ppart, const int *__restrict__ done) {
+                                                    double *__restrict__ ppart, const int *__restrict__ done,
+                                                    T rho) {
     __shared__ T ys[3 * NPE * kPatch];
     const int p = xcd_block();
     if (p >= npatch) return;  // whole workgroup (grid padded to a multiple of 8)
@@ -137,7 +138,7 @@ __global__ __launch_bounds__(kPatch) void k_ku_patch(int n_el, int npatch, int n
                 T xs[NPE][3];
 #pragma unroll
                 for (int i = 0; i < NPE; ++i) x_of(i, xs[i]);
-                mf_element_x<NPE, T, AFF>(
+                mf_element_x<NPE, T, AFF, FOLD>(
                     n_el, k, el,
                     [&](int i, T (&v)[3]) {
                         v[0] = xs[i][0];
@@ -149,12 +150,13 @@ __global__ __launch_bounds__(kPatch) void k_ku_patch(int n_el, int npatch, int n
                         if (i > 0)
                             en += (double)((xs[i][0] - xs[0][0]) * f0 + (xs[i][1] - xs[0][1]) * f1 +
                                            (xs[i][2] - xs[0][2]) * f2);
-                    });
+                    },
+                    rho);
             } else {
-                mf_element_x<NPE, T, AFF>(n_el, k, el, x_of, to_lds);
+                mf_element_x<NPE, T, AFF, FOLD>(n_el, k, el, x_of, to_lds, rho);
             }
         } else
-            mf_element_x<NPE, T, AFF>(n_el, k, el, x_of, to_lds);
+            mf_element_x<NPE, T, AFF, FOLD>(n_el, k, el, x_of, to_lds, rho);
     }
     // Round 0 holds the patch's nodes of most incidences (up to 24 in a Tet10
     // box mesh, so one wave would otherwise load 20 of them one after the
@@ -325,6 +327,16 @@ __global__ __launch_bounds__(256) void k_permute_gf(int64_t rows, int n_el, cons
     }
 }
 
+// sqrt(cm) invJ in FP32 from the FP64 factors (affine: one set per element),
+// cm = 0.2 mu w detJ as the affine Tet10 product scales it
+__global__ __launch_bounds__(256) void k_fold_gf(int n_el, const double *__restrict__ gf, float *__restrict__ gq) {
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < 9 * (int64_t)n_el;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = idx / n_el, k = idx - r * n_el;
+        gq[idx] = (float)(sqrt(0.2 * gf[10 * (int64_t)n_el + k]) * gf[idx]);
+    }
+}
+
 // ------------------------------------------------------------ host setup ---
 namespace {
 
@@ -349,7 +361,7 @@ bool jfem_patch_enabled() {
 }
 
 void jfem_patch_release(jfm_ctx *ctx) {
-    for (void **p : {(void **)&ctx->connP, (void **)&ctx->gfP64, (void **)&ctx->gfP32, (void **)&ctx->pn_ptr,
+    for (void **p : {(void **)&ctx->connP, (void **)&ctx->gfP64, (void **)&ctx->gfP32, (void **)&ctx->gfQ32, (void **)&ctx->pn_ptr,
                      (void **)&ctx->pn_cnt, (void **)&ctx->pn_out, (void **)&ctx->rstart, (void **)&ctx->rbase,
                      (void **)&ctx->linT, (void **)&ctx->bptr, (void **)&ctx->bnode,
                      (void **)&ctx->bpart, (void **)&ctx->ppart, (void **)&ctx->eperm, (void **)&ctx->qsrc,
@@ -576,6 +588,16 @@ int jfem_patch_setup(jfm_ctx *ctx) {
 #undef PUP
     k_permute_gf<double><<<1024, 256, 0, s>>>(ngfr, (int)n_el, ctx->eperm, ctx->gf64, ctx->gfP64);
     k_permute_gf<float><<<1024, 256, 0, s>>>(ngfr, (int)n_el, ctx->eperm, ctx->gf32, ctx->gfP32);
+    if (ctx->affine && npe == 10 && ctx->nu_uniform) {
+        // FP32 folded factors: sqrt(cm) invJ, cm = mu w detJ / 5, and lambda / mu
+        ctx->gfQ32 = jfm_alloc<float>(ctx, 9 * n_el);
+        if (!ctx->gfQ32) {
+            jfem_patch_release(ctx);
+            return jfm_fail(ctx, JFM_ENOMEM, "patch operator allocation failed");
+        }
+        k_fold_gf<<<1024, 256, 0, s>>>((int)n_el, ctx->gfP64, ctx->gfQ32);
+        ctx->rho32 = (float)(2.0 * ctx->nu0 / (1.0 - 2.0 * ctx->nu0));
+    }
     JFM_HIP(ctx, hipGetLastError());
     JFM_HIP(ctx, hipStreamSynchronize(s));  // host vectors go out of scope
     ctx->npatch = npatch;
@@ -667,19 +689,29 @@ static void patch_kernel(jfm_ctx *ctx, int op, const T *x, T *y, const T *dotv, 
     const int n = (int)ctx->n_el;
     double *pp = dotv ? (pout ? pout : ctx->ppart) : nullptr;
     if (energy_dot<T>(ctx, op, x, dotv)) dotv = nullptr;
-    auto launch = [&](auto kernel, const T *el) {
+    auto launch = [&](auto kernel, const T *el, T rho) {
         kernel<<<grid, kPatch, 0, ctx->stream>>>(n, ctx->npatch, ctx->npn, ctx->connP, ctx->connP16, ctx->pbase, el, x,
                                                  ctx->pn_ptr, ctx->pn_cnt,
                                                  ctx->pn_out, ctx->rstart, ctx->rbase, ctx->linT, ctx->fixed, y,
-                                                 (T *)ctx->bpart, dotv, pp, done);
+                                                 (T *)ctx->bpart, dotv, pp, done, rho);
     };
     const bool c16 = ctx->connP16 != nullptr;
-    if (op == JFM_OP_EBE)
-        launch(k_ku_patch<NPE, T, true, true, false>, keP_of<T>(ctx));
-    else if (ctx->affine)
-        launch(c16 ? k_ku_patch<NPE, T, true, false, true> : k_ku_patch<NPE, T, true, false, false>, gfP_of<T>(ctx));
-    else
-        launch(c16 ? k_ku_patch<NPE, T, false, false, true> : k_ku_patch<NPE, T, false, false, false>, gfP_of<T>(ctx));
+    if (op == JFM_OP_EBE) {
+        launch(k_ku_patch<NPE, T, true, true, false, false>, keP_of<T>(ctx), T(0));
+    } else if (ctx->affine) {
+        if constexpr (sizeof(T) == 4 && NPE == 10) {
+            if (ctx->gfQ32) {  // folded factors (uniform nu)
+                launch(c16 ? k_ku_patch<NPE, T, true, false, true, true> : k_ku_patch<NPE, T, true, false, false, true>,
+                       (const T *)ctx->gfQ32, (T)ctx->rho32);
+                return;
+            }
+        }
+        launch(c16 ? k_ku_patch<NPE, T, true, false, true, false> : k_ku_patch<NPE, T, true, false, false, false>,
+               gfP_of<T>(ctx), T(0));
+    } else {
+        launch(c16 ? k_ku_patch<NPE, T, false, false, true, false> : k_ku_patch<NPE, T, false, false, false, false>,
+               gfP_of<T>(ctx), T(0));
+    }
 }
 
 template <typename T>

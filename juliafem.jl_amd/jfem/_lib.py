@@ -71,7 +71,8 @@ class Info(C.Structure):
     _fields_ = [("n_nodes", C.c_int64), ("n_el", C.c_int64), ("n_dofs", C.c_int64), ("nnz", C.c_int64),
                 ("nnzb", C.c_int64), ("n_faces", C.c_int64), ("npe", C.c_int32), ("affine", C.c_int32),
                 ("device_bytes", C.c_int64), ("n_patches", C.c_int64), ("patch_nodes", C.c_int64),
-                ("shared_nodes", C.c_int64), ("shared_slots", C.c_int64), ("patch_conn_bytes", C.c_int64)]
+                ("shared_nodes", C.c_int64), ("shared_slots", C.c_int64), ("patch_conn_bytes", C.c_int64),
+                ("patch_fp32_factor_rows", C.c_int64)]
 
 
 _lib = None
