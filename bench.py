@@ -290,6 +290,8 @@ def main():
     sv = ideal_ku(inf, prec)
     if "patch_ms" in hk:
         c16 = ",C16" if inf.get("patch_conn_bytes", 4) == 2 else ""  # 16-bit patch-relative connectivity
+        if method == 1 and inf.get("patch_fp32_factor_rows", 11) == 9:
+            c16 += ",F9"  # folded FP32 factors, 9 rows
         dom, dom_ms, impl_bytes = f"k_ku_patch<10,{tname},AFF{c16}>", hk["patch_ms"], hk["bytes"]["patch"]
     elif hk["element_ms"] >= hk["gather_ms"]:
         dom, dom_ms, impl_bytes = f"k_mf_elem<10,{tname},AFF>", hk["element_ms"], hk["bytes"]["element"]
