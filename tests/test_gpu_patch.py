@@ -13,7 +13,8 @@ through it):
   * the two-phase operator (JFEM_PATCH=0, a separate process): the same
     product up to the order of the nodal sums, the same PCG answer;
   * both connectivity widths of the patch kernel (16-bit patch-relative and
-    32-bit) against the oracle.
+    32-bit) and both forms of its FP32 factors (folded, stored) against the
+    oracle.
 """
 import os
 import subprocess
@@ -94,6 +95,31 @@ def test_connectivity_width(jf, oracle):
         inf = ctx.info()
         say(f"{name} numbering: {mm.n_nodes} nodes, patch connectivity {inf['patch_conn_bytes']} bytes")
         assert inf["patch_conn_bytes"] == width
+        ref = oracle.matfree_apply(mm, x, mm.fixed_mask())
+        scale = np.abs(ref).max()
+        assert np.abs(y64 - ref).max() <= 1e-12 * scale
+        assert np.abs(y32 - ref).max() <= 2e-6 * scale
+        ctx.close()
+
+
+def test_fp32_factor_forms(jf, oracle):
+    """With one Poisson's ratio for every element, the FP32 affine Tet10
+    patch kernel reads the inverse Jacobians with the shear factor folded in
+    (9 rows and lambda / mu as an argument); with per-element ratios, the 11
+    stored factors.  Both FP32 products match the oracle's."""
+    import dataclasses
+    m = M.box_tet10(17, 4, 3, seed=9, jitter=0.1, E_sigma=0.4)
+    nu = np.random.default_rng(5).uniform(0.2, 0.35, m.n_el)
+    mv = dataclasses.replace(m, nu=nu)
+    for name, mm, rows in (("uniform nu", m, 9), ("per-element nu", mv, 11)):
+        ctx = jf.Context(0)
+        ctx.upload(mm)
+        x = np.random.default_rng(13).standard_normal(mm.n_dofs)
+        y32 = ctx.apply(x, op="matfree", precision=1)
+        y64 = ctx.apply(x, op="matfree")
+        inf = ctx.info()
+        say(f"{name}: FP32 factor rows {inf['patch_fp32_factor_rows']}")
+        assert inf["patch_fp32_factor_rows"] == rows
         ref = oracle.matfree_apply(mm, x, mm.fixed_mask())
         scale = np.abs(ref).max()
         assert np.abs(y64 - ref).max() <= 1e-12 * scale
