@@ -26,7 +26,7 @@ static inline unsigned nblk(int64_t n, int b = kBlock) { return (unsigned)((n + 
 // nodes per thread of the two-level direction kernel, all loaded before its
 // reductions (its grid is sized for this many)
 #ifndef JFEM_DIR_NODES
-#define JFEM_DIR_NODES 2
+#define JFEM_DIR_NODES 3
 #endif
 constexpr int kDirNodes = JFEM_DIR_NODES;
 
@@ -600,8 +600,9 @@ template <typename T>
 hipError_t launch_pcg_direction_coarse(jfm_ctx *ctx, int slot, int npart_rz, int npart_wz, T *p, const T *z, T *x) {
     const T *cd = sizeof(T) == 8 ? (const T *)ctx->cd64 : (const T *)ctx->cd32;
     // kDirNodes nodes per thread, at most kDirBlocks blocks (no partials are
-    // written, so the grid is not capped at kMaxPart): 683 blocks at 349,569
-    // nodes measured 0.06-0.15 ms faster per solve than 1024
+    // written, so the grid is not capped at kMaxPart): 456 blocks of three
+    // nodes per thread at 349,569 nodes measured 0.12 ms faster per solve
+    // than 683 of two, and those 0.06-0.15 ms faster than 1024
     const int64_t per = (int64_t)kBlock * kDirNodes;
     const unsigned g = (unsigned)std::min<int64_t>(std::max<int64_t>(1, (ctx->N + per - 1) / per), kDirBlocks);
     k_pcg_dir_coarse<T><<<g, kBlock, 0, ctx->stream>>>(
